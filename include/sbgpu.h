@@ -1,0 +1,156 @@
+/*
+ * sbgpu.h - C ABI of the MI355X (gfx950) hot path of a closed-mesh boolean:
+ * LBVH build, tree-vs-tree candidate pairs, Guigue-Devillers triangle-triangle
+ * test with segment construction, and ray-parity inside/outside classification.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status
+ * (SBG_OK = 0, negative on error; sbg_last_error() has the message) and never
+ * throws.  All device work of one context is ordered on that context's HIP
+ * stream; functions that hand results back to the host synchronise it.  One
+ * context (and the meshes / pair lists made from it) must be used by one host
+ * thread at a time; distinct contexts are independent.
+ *
+ * Reference interfaces replaced (huxingyi/solidboolean):
+ *   SolidMesh::setVertices / setTriangles / prepare   src/solidmesh.h:31-32, src/solidmesh.cpp:42-76
+ *   SolidMesh::triangleNormals / triangleAxisAlignedBoundingBoxes  src/solidmesh.h:46-59
+ *   SolidBoolean::searchPotentialIntersectedPairs      src/solidboolean.cpp:94-101
+ *   SolidBoolean::intersectTwoFaces (+ the pair loop)  src/solidboolean.cpp:103-122, 315-341
+ *   tri_tri_intersection_test_3d                       thirdparty/GuigueDevillers03/tri_tri_intersect.h:46-49
+ *   SolidBoolean::isPointInMesh / decideGroupSide      src/solidboolean.cpp:48-92, 482-510
+ */
+#ifndef SBGPU_H
+#define SBGPU_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SBG_OK 0
+#define SBG_ERR_INVALID (-1)   /* bad argument (null handle, index out of range, ...) */
+#define SBG_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define SBG_ERR_OOM (-3)       /* device allocation failed */
+#define SBG_ERR_OVERFLOW (-4)  /* a bounded device list overflowed (see message) */
+#define SBG_ERR_NO_DEVICE (-5) /* no usable GPU */
+#define SBG_ERR_STATE (-6)     /* call out of order (e.g. results before sbg_tri_tri) */
+
+#define SBG_PRECISION_FP64 0          /* the reference's FP64 operation sequence */
+#define SBG_PRECISION_FP32_FILTERED 1 /* FP32 sign certificates, FP64 where uncertain; same results */
+
+typedef struct sbg_context_s *sbg_context;
+typedef struct sbg_mesh_s *sbg_mesh;
+typedef struct sbg_pairs_s *sbg_pairs;
+
+typedef struct {
+    uint64_t tested;         /* candidate pairs tested */
+    uint64_t exit_plane1;    /* rejected by T1 vs plane of T2 (tri_tri_intersect.c:421) */
+    uint64_t exit_plane2;    /* rejected by T2 vs plane of T1 (:438) */
+    uint64_t exit_interval;  /* rejected inside CONSTRUCT_INTERSECTION (:322, :328) */
+    uint64_t hits;           /* intersecting, non-coplanar: a segment was built */
+    uint64_t coplanar;       /* coplanar (intersectTwoFaces returns false, solidboolean.cpp:119) */
+    uint64_t fallback_pairs; /* FP32-filtered mode: pairs re-run in FP64 for an uncertain sign */
+    uint64_t filtered_pairs; /* FP32-filtered mode: pairs decided by FP32 certificates alone */
+} sbg_tri_tri_stats;
+
+/* Stage timers (milliseconds, HIP events on the context stream). */
+enum {
+    SBG_STAGE_RECORDS = 0, /* triangle records, normals, leaf boxes, Morton keys */
+    SBG_STAGE_SORT,        /* Morton radix sort */
+    SBG_STAGE_BVH,         /* Karras hierarchy + refit */
+    SBG_STAGE_TRAVERSE,    /* tree-vs-tree candidate search */
+    SBG_STAGE_PAIR_SORT,   /* canonical (a, b) order of the candidates */
+    SBG_STAGE_TRI_TRI,     /* Guigue-Devillers kernel */
+    SBG_STAGE_COMPACT,     /* hit compaction */
+    SBG_STAGE_CLASSIFY,    /* ray parity + vote */
+    SBG_STAGE_COUNT
+};
+
+/* Last error message of the calling thread ("" if none). */
+const char *sbg_last_error(void);
+
+/* Number of visible HIP devices. */
+int sbg_device_count(int *count);
+
+/* A context binds one device and one HIP stream (default priority). */
+int sbg_context_create(int device, sbg_context *out);
+int sbg_context_destroy(sbg_context ctx);
+int sbg_context_synchronize(sbg_context ctx);
+/* The context's hipStream_t, for callers that order their own work after it. */
+int sbg_context_stream(sbg_context ctx, void **hip_stream);
+
+/* Mesh from host arrays (copied to HBM): xyz is nverts x 3 FP64, tris is
+ * ntris x 3 vertex indices.  Replaces SolidMesh::setVertices/setTriangles
+ * (src/solidmesh.h:31-32).  Indices are validated; ntris < 2^31. */
+int sbg_mesh_create(sbg_context ctx, const double *xyz, size_t nverts, const uint32_t *tris, size_t ntris,
+                    sbg_mesh *out);
+/* Device SolidMesh::prepare (src/solidmesh.cpp:42-76): FP64 normals
+ * (src/vector3.h:155-176), per-triangle boxes and the LBVH.  Asynchronous. */
+int sbg_mesh_prepare(sbg_mesh mesh);
+int sbg_mesh_destroy(sbg_mesh mesh);
+int sbg_mesh_counts(sbg_mesh mesh, size_t *nverts, size_t *ntris);
+/* SolidMesh::triangleNormals (src/solidmesh.h:46-49): ntris x 3 FP64. */
+int sbg_mesh_normals(sbg_mesh mesh, double *out);
+/* SolidMesh::triangleAxisAlignedBoundingBoxes (src/solidmesh.h:56-59):
+ * ntris x 6 FP64 = min xyz, max xyz (exact, as AxisAlignedBoudingBox::update). */
+int sbg_mesh_boxes(sbg_mesh mesh, double *out);
+/* The LBVH for inspection: ntris-1 nodes of 64 B (child boxes, child refs,
+ * parent) into nodes_out (may be NULL), the root reference and FP32 root box. */
+int sbg_mesh_bvh(sbg_mesh mesh, void *nodes_out, uint32_t *root_ref, float *root_box6);
+
+/* SolidBoolean::searchPotentialIntersectedPairs (src/solidboolean.cpp:94-101):
+ * every (a in A, b in B) whose FP64 triangle boxes closed-overlap
+ * (src/axisalignedboundingbox.h:95-105), sorted by (a, b). */
+int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out);
+/* A pair list from host memory: ab = n x (a, b). */
+int sbg_pairs_create(sbg_context ctx, const uint32_t *ab, size_t n, sbg_pairs *out);
+int sbg_pairs_count(sbg_pairs pairs, size_t *n);
+int sbg_pairs_copy(sbg_pairs pairs, uint32_t *ab);
+int sbg_pairs_destroy(sbg_pairs pairs);
+
+/* intersectTwoFaces over every pair (src/solidboolean.cpp:103-122): stores
+ * per-pair status (0 none, 1 intersecting segment, 2 coplanar) and segments in
+ * the pair list.  stats may be NULL (then the call does not synchronise). */
+int sbg_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, int precision, sbg_tri_tri_stats *stats);
+/* Per-pair status (n bytes) and segments (n x 6 FP64: source, target;
+ * meaningful where status == 1).  Either pointer may be NULL. */
+int sbg_pairs_results(sbg_pairs pairs, uint8_t *status, double *seg);
+/* Intersecting pairs only, in pair order.  With ab == NULL only *nhits is set. */
+int sbg_pairs_hits(sbg_pairs pairs, size_t *nhits, uint32_t *ab, double *seg);
+
+/* tri_tri_intersection_test_3d as intersectTwoFaces uses it, on n host
+ * triangle pairs (a9, b9: n x 9 FP64 = p, q, r).  status as above; seg n x 6. */
+int sbg_tri_tri_batch(sbg_context ctx, const double *a9, const double *b9, size_t n, uint8_t *status,
+                      double *seg);
+
+/* isPointInMesh along the three axes of src/solidboolean.cpp:31-35 for n host
+ * points against `target`, and decideGroupSide's vote (:492-508).  axis_inside
+ * (n x 3, may be NULL) gets each ray's parity; inside gets the vote. */
+int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *axis_inside, uint8_t *inside);
+
+/* Every triangle of `self` that no intersecting pair of `pairs` touches is
+ * classified by its centroid (v0 + v1 + v2) / 3.0 against `other`
+ * (src/solidboolean.cpp:497-508).  side = 0 when self is the pairs' first mesh,
+ * 1 when it is the second.  out (ntris bytes): 0 outside, 1 inside,
+ * 2 intersected (not classified).  Requires sbg_tri_tri on `pairs`. */
+int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs pairs, int side, uint8_t *out);
+
+/* Stage timers: when enabled, every stage is bracketed by HIP events.
+ * sbg_context_stage_ms synchronises and returns the milliseconds accumulated
+ * per stage (SBG_STAGE_COUNT doubles) since the last reset. */
+int sbg_context_set_timing(sbg_context ctx, int enable);
+int sbg_context_stage_ms(sbg_context ctx, double *ms, int reset);
+
+/* Runs the tri-tri kernel `iters` times back to back on the context stream
+ * between two HIP events; *avg_ms = elapsed / iters. */
+int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, int precision, int iters, double *avg_ms);
+
+/* Measurement aid (no counterpart in the reference): a plain device-to-device
+ * copy kernel of `bytes` bytes run `iters` times, for calibrating the HBM
+ * traffic counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
+int sbg_calibrate_copy(sbg_context ctx, size_t bytes, int iters, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SBGPU_H */

@@ -1,0 +1,186 @@
+// Stable LSD radix sort, 8-bit digits, for u32 keys (+ u32 values) and u64 keys.
+//
+// Per pass: (1) every block counts the digits of its 4096-key tile into LDS and
+// writes a 256 x nblocks histogram column; (2) one block turns the histogram
+// (digit-major) into exclusive global offsets; (3) every block re-reads its tile
+// in order, 256 keys per round, ranks each key among equal digits with a 64-lane
+// ballot match (stable within the wave), adds the counts of earlier waves and
+// earlier rounds kept in LDS, and scatters.  Per key and pass: read 4 (8) B key
+// (+4 B value) twice, write once.
+#include "sbg_internal.h"
+
+namespace sbg {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortBlock * kSortItems;
+constexpr int kWaves = kSortBlock / 64;
+
+template <typename K>
+__device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask)
+{
+    return (uint32_t)(key >> shift) & mask;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortBlock) void k_hist(const K *__restrict__ keys, size_t n, int shift,
+                                                     uint32_t mask, uint32_t *__restrict__ hist, int nblocks)
+{
+    __shared__ uint32_t s_cnt[256];
+    s_cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+    for (int r = 0; r < kSortItems; ++r) {
+        size_t idx = base + (size_t)r * kSortBlock + threadIdx.x;
+        if (idx < n)
+            atomicAdd(&s_cnt[digit_of(keys[idx], shift, mask)], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = s_cnt[threadIdx.x];
+}
+
+// exclusive scan of m entries in place, one block of 1024 threads
+__global__ __launch_bounds__(1024) void k_scan(uint32_t *__restrict__ data, size_t m)
+{
+    __shared__ uint32_t s_part[1024];
+    const size_t per = (m + 1023) / 1024;
+    const size_t lo = (size_t)threadIdx.x * per;
+    const size_t hi = lo + per < m ? lo + per : m;
+    uint32_t sum = 0;
+    for (size_t i = lo; i < hi; ++i)
+        sum += data[i];
+    s_part[threadIdx.x] = sum;
+    __syncthreads();
+    // Hillis-Steele inclusive scan of the 1024 partials
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)off ? s_part[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? s_part[threadIdx.x - 1] : 0;
+    for (size_t i = lo; i < hi; ++i) {
+        uint32_t v = data[i];
+        data[i] = run;
+        run += v;
+    }
+}
+
+template <typename K, bool HAS_VAL>
+__global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ keys, const uint32_t *__restrict__ vals,
+                                                        K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
+                                                        size_t n, int shift, uint32_t mask,
+                                                        const uint32_t *__restrict__ offsets, int nblocks)
+{
+    __shared__ uint32_t s_base[256];
+    __shared__ uint32_t s_wcnt[kWaves][256];
+    s_base[threadIdx.x] = offsets[(size_t)threadIdx.x * nblocks + blockIdx.x];
+    for (int w = 0; w < kWaves; ++w)
+        s_wcnt[w][threadIdx.x] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const unsigned long long ltmask = (1ull << lane) - 1ull;
+    const size_t base = (size_t)blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        const size_t idx = base + (size_t)r * kSortBlock + threadIdx.x;
+        const bool valid = idx < n;
+        K key = valid ? keys[idx] : (K)0;
+        uint32_t val = (HAS_VAL && valid) ? vals[idx] : 0u;
+        const uint32_t d = digit_of(key, shift, mask);
+        // lanes of this wave holding the same digit
+        unsigned long long peers = __ballot(valid);
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t rank = __popcll(peers & ltmask);
+        if (valid && rank == 0)
+            s_wcnt[wave][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        uint32_t pos = 0;
+        if (valid) {
+            pos = s_base[d] + rank;
+            for (int w = 0; w < wave; ++w)
+                pos += s_wcnt[w][d];
+        }
+        __syncthreads();
+        {
+            uint32_t tot = 0;
+            for (int w = 0; w < kWaves; ++w) {
+                tot += s_wcnt[w][threadIdx.x];
+                s_wcnt[w][threadIdx.x] = 0;
+            }
+            s_base[threadIdx.x] += tot;
+        }
+        __syncthreads();
+        if (valid) {
+            keysOut[pos] = key;
+            if (HAS_VAL)
+                valsOut[pos] = val;
+        }
+    }
+}
+
+void launch_exclusive_scan(uint32_t *data, size_t m, hipStream_t s)
+{
+    if (m == 0)
+        return;
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, data, m);
+}
+
+int sort_blocks(size_t n) { return (int)((n + kSortTile - 1) / kSortTile); }
+size_t sort_hist_entries(size_t n) { return (size_t)256 * (size_t)(sort_blocks(n) > 0 ? sort_blocks(n) : 1); }
+
+template <typename K, bool HAS_VAL>
+static void one_pass(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout, size_t n, int shift,
+                     int bits, uint32_t *hist, hipStream_t s)
+{
+    const int nb = sort_blocks(n);
+    const uint32_t mask = (1u << bits) - 1u;
+    hipLaunchKernelGGL(k_hist<K>, dim3(nb), dim3(kSortBlock), 0, s, kin, n, shift, mask, hist, nb);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, hist, (size_t)256 * nb);
+    hipLaunchKernelGGL((k_scatter<K, HAS_VAL>), dim3(nb), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
+                       shift, mask, hist, nb);
+}
+
+int radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *valsAlt, size_t n,
+                   int beginBit, int endBit, uint32_t *hist, hipStream_t s)
+{
+    if (n <= 1)
+        return 0;
+    int flip = 0;
+    for (int shift = beginBit; shift < endBit; shift += 8) {
+        const int bits = endBit - shift < 8 ? endBit - shift : 8;
+        uint32_t *ki = flip ? keysAlt : keys, *ko = flip ? keys : keysAlt;
+        uint32_t *vi = flip ? valsAlt : vals, *vo = flip ? vals : valsAlt;
+        if (vals)
+            one_pass<uint32_t, true>(ki, vi, ko, vo, n, shift, bits, hist, s);
+        else
+            one_pass<uint32_t, false>(ki, nullptr, ko, nullptr, n, shift, bits, hist, s);
+        flip ^= 1;
+    }
+    return flip;
+}
+
+int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRanges, int nRanges,
+                   uint32_t *hist, hipStream_t s)
+{
+    if (n <= 1)
+        return 0;
+    int flip = 0;
+    for (int r = 0; r < nRanges; ++r) {
+        const int beginBit = bitRanges[2 * r], endBit = bitRanges[2 * r + 1];
+        for (int shift = beginBit; shift < endBit; shift += 8) {
+            const int bits = endBit - shift < 8 ? endBit - shift : 8;
+            uint64_t *ki = flip ? keysAlt : keys, *ko = flip ? keys : keysAlt;
+            one_pass<uint64_t, false>(ki, nullptr, ko, nullptr, n, shift, bits, hist, s);
+            flip ^= 1;
+        }
+    }
+    return flip;
+}
+
+}  // namespace sbg

@@ -1,0 +1,173 @@
+// Tree-vs-tree candidate search: the set the reference's simultaneous DFS emits
+// (src/axisalignedboundingboxtree.h:54-95 via src/solidboolean.cpp:94-101),
+// i.e. every (a, b) whose FP64 triangle boxes closed-overlap.
+//
+// Breadth-first over node pairs: each level's kernel reads the frontier written
+// by the previous one.  A thread loads the two nodes of its pair (64 B each,
+// both child boxes inside), tests the up-to-4 child combinations with the FP32
+// outward-rounded boxes (a superset test), and appends surviving internal pairs
+// to the next frontier.  Leaf-leaf survivors are rechecked with the exact FP64
+// closed box test on the 96-B FP64 records before they are appended to the
+// candidate queue.  Appends are wave-aggregated: one atomic per wave per queue,
+// offsets from a 64-lane prefix sum.  Queues past capacity raise `overflow`;
+// the host grows them and repeats the search, so nothing is ever dropped.
+#include "sbg_device.h"
+#include "sbg_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace sbg {
+
+struct FBox {
+    float lx, ly, lz, hx, hy, hz;
+};
+
+__device__ __forceinline__ bool fbox_overlap(const FBox &a, const FBox &b)
+{
+    return a.lx <= b.hx && a.hx >= b.lx && a.ly <= b.hy && a.hy >= b.ly && a.lz <= b.hz && a.hz >= b.lz;
+}
+
+__device__ __forceinline__ FBox leaf_box(const float *__restrict__ box32, uint32_t tri)
+{
+    const float4 *p = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * tri);
+    const float4 lo = p[0], hi = p[1];
+    return {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+}
+
+// children (1 for a leaf, 2 for an internal node) with their FP32 boxes
+__device__ __forceinline__ int children(const BNode *__restrict__ nodes, const float *__restrict__ box32,
+                                        uint32_t ref, uint32_t c[2], FBox bx[2])
+{
+    if (ref & kLeafBit) {
+        c[0] = ref;
+        bx[0] = leaf_box(box32, ref & ~kLeafBit);
+        return 1;
+    }
+    const BNode *nd = nodes + ref;
+    const float4 a = nd->a, b = nd->b, cc = nd->c;
+    const int4 d = nd->d;
+    c[0] = (uint32_t)d.x;
+    c[1] = (uint32_t)d.y;
+    bx[0] = {a.x, a.y, a.z, a.w, b.x, b.y};
+    bx[1] = {b.z, b.w, cc.x, cc.y, cc.z, cc.w};
+    return 2;
+}
+
+__device__ __forceinline__ Box64 record_box(const double *__restrict__ tri64, uint32_t t)
+{
+    const double2 *p = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * t);
+    const double2 r0 = p[0], r1 = p[1], r2 = p[2], r3 = p[3], r4 = p[4], r5 = p[5];
+    return tri_box64({r0.x, r0.y, r1.x}, {r2.x, r2.y, r3.x}, {r4.x, r4.y, r5.x});
+}
+
+// wave-wide exclusive prefix of `cnt`; returns this lane's offset in the queue
+// (one atomic per wave) or ~0u when the queue would overflow.
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t cnt, uint32_t *counter, uint32_t cap, uint32_t *overflow,
+                                                 uint32_t &waveBaseOut)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t incl = cnt;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off)
+            incl += v;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    uint32_t base = 0;
+    if (lane == 63 && total)
+        base = atomicAdd(counter, total);
+    base = __shfl(base, 63, 64);
+    waveBaseOut = base;
+    if (total && base + total > cap && lane == 0)
+        atomicOr(overflow, 1u);
+    return base + incl - cnt;
+}
+
+__global__ __launch_bounds__(256) void k_expand(TraverseArgs t, const uint2 *__restrict__ in,
+                                                const uint32_t *__restrict__ inCount, uint2 *__restrict__ out,
+                                                uint32_t *outCount, uint32_t outCap, uint64_t *__restrict__ cand,
+                                                uint32_t *candCount, uint32_t candCap, uint32_t *overflow)
+{
+    // both frontier buffers hold outCap pairs; a count past it has already
+    // raised `overflow` and the whole search is repeated with larger queues
+    const uint32_t n = *inCount < outCap ? *inCount : outCap;
+    const uint32_t stride = gridDim.x * 256;
+    for (uint32_t base = blockIdx.x * 256; base < n; base += stride) {
+        const uint32_t i = base + threadIdx.x;
+        const bool valid = i < n;
+        uint32_t ca[2], cb[2];
+        FBox ba[2], bb[2];
+        int na = 0, nb = 0;
+        uint2 pr = make_uint2(0, 0);
+        if (valid) {
+            pr = in[i];
+            na = children(t.nodesA, t.boxA, pr.x, ca, ba);
+            nb = children(t.nodesB, t.boxB, pr.y, cb, bb);
+        }
+        uint2 nxt[4];
+        uint64_t cnd[4];
+        uint32_t nn = 0, nc = 0;
+        for (int u = 0; u < na; ++u)
+            for (int v = 0; v < nb; ++v) {
+                if (!fbox_overlap(ba[u], bb[v]))
+                    continue;
+                if ((ca[u] & kLeafBit) && (cb[v] & kLeafBit)) {
+                    const uint32_t ta = ca[u] & ~kLeafBit, tb = cb[v] & ~kLeafBit;
+                    if (box64_overlap(record_box(t.tri64A, ta), record_box(t.tri64B, tb)))
+                        cnd[nc++] = ((uint64_t)ta << 32) | tb;
+                } else {
+                    nxt[nn++] = make_uint2(ca[u], cb[v]);
+                }
+            }
+        uint32_t wb;
+        const uint32_t on = wave_reserve(nn, outCount, outCap, overflow, wb);
+        for (uint32_t k = 0; k < nn; ++k)
+            if (on + k < outCap)
+                out[on + k] = nxt[k];
+        const uint32_t oc = wave_reserve(nc, candCount, candCap, overflow, wb);
+        for (uint32_t k = 0; k < nc; ++k)
+            if (oc + k < candCap)
+                cand[oc + k] = cnd[k];
+    }
+}
+
+// root pair: both meshes non-empty; a root may itself be a leaf (1 triangle)
+__global__ void k_seed(TraverseArgs t, uint32_t rootA, uint32_t rootB, const float *__restrict__ rootBoxA,
+                       const float *__restrict__ rootBoxB, uint2 *frontier, uint32_t *count, uint64_t *cand,
+                       uint32_t *candCount)
+{
+    if (threadIdx.x != 0)
+        return;
+    FBox a = {rootBoxA[0], rootBoxA[1], rootBoxA[2], rootBoxA[3], rootBoxA[4], rootBoxA[5]};
+    FBox b = {rootBoxB[0], rootBoxB[1], rootBoxB[2], rootBoxB[3], rootBoxB[4], rootBoxB[5]};
+    if (!fbox_overlap(a, b))
+        return;
+    if ((rootA & kLeafBit) && (rootB & kLeafBit)) {
+        const uint32_t ta = rootA & ~kLeafBit, tb = rootB & ~kLeafBit;
+        if (box64_overlap(record_box(t.tri64A, ta), record_box(t.tri64B, tb))) {
+            cand[0] = ((uint64_t)ta << 32) | tb;
+            *candCount = 1;
+        }
+        return;
+    }
+    frontier[0] = make_uint2(rootA, rootB);
+    *count = 1;
+}
+
+void launch_expand(const TraverseArgs &t, const uint2 *in, const uint32_t *inCount, uint2 *out,
+                   uint32_t *outCount, uint32_t outCap, uint64_t *cand, uint32_t *candCount,
+                   uint32_t candCap, uint32_t *overflow, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, s, t, in, inCount, out, outCount, outCap, cand,
+                       candCount, candCap, overflow);
+}
+
+void launch_seed(const TraverseArgs &t, uint32_t rootA, uint32_t rootB, const float *rootBoxA,
+                 const float *rootBoxB, uint2 *frontier, uint32_t *count, uint64_t *cand,
+                 uint32_t *candCount, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_seed, dim3(1), dim3(64), 0, s, t, rootA, rootB, rootBoxA, rootBoxB, frontier, count,
+                       cand, candCount);
+}
+
+}  // namespace sbg

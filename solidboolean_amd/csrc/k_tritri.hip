@@ -1,0 +1,336 @@
+// Batched Guigue-Devillers triangle-triangle test with segment construction
+// (reference: thirdparty/GuigueDevillers03/tri_tri_intersect.c:285-472, called
+// per candidate pair by src/solidboolean.cpp:103-122 and :315-341).
+//
+// One lane per candidate pair.  Two precisions, identical results:
+//  * FP64: the reference's exact operation sequence (sbg_device.h gd_fp64)
+//    on the pair's two 96-B FP64 records.
+//  * FP32-filtered: every orientation predicate the reference would evaluate
+//    is first computed from the 48-B FP32 records with a forward error bound
+//    (fsign below, bound derived in sbg_device.h).  If every sign on the path is certified and the
+//    path ends in a rejection, the pair is decided without touching FP64; a
+//    path that ends in an intersection, or any uncertain sign, re-runs the
+//    whole pair with gd_fp64, so every decision and every segment equals the
+//    FP64 reference's.
+// Per pair and launch: 8 B pair + 1 B status + 2 x 96 B (FP64) or 2 x 48 B
+// (filtered, + 2 x 96 B for hits / fallbacks) + 48 B segment per hit.
+#include "sbg_device.h"
+#include "sbg_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace sbg {
+
+__device__ __forceinline__ void load_tri64(const double *__restrict__ rec, uint32_t t, V3 &p, V3 &q, V3 &r)
+{
+    const double2 *x = reinterpret_cast<const double2 *>(rec + (size_t)kTri64Stride * t);
+    const double2 a = x[0], b = x[1], c = x[2], d = x[3], e = x[4], f = x[5];
+    p = {a.x, a.y, b.x};
+    q = {c.x, c.y, d.x};
+    r = {e.x, e.y, f.x};
+}
+
+__device__ __forceinline__ void load_tri32(const float *__restrict__ rec, uint32_t t, F3 &p, F3 &q, F3 &r)
+{
+    const float4 *x = reinterpret_cast<const float4 *>(rec + (size_t)kTri32Stride * t);
+    const float4 a = x[0], b = x[1], c = x[2];
+    p = {a.x, a.y, a.z};
+    q = {b.x, b.y, b.z};
+    r = {c.x, c.y, c.z};
+}
+
+// Sign of DOT(a, CROSS(b, c)) as the FP64 reference computes it, or 0 when the
+// FP32 evaluation cannot certify it.  eta bounds each FP32 difference's input
+// rounding error (2^-23 * max |coordinate|).  The bound (see the derivation in
+// sbg_device.h) is |d32 - d64| <= 1.5 eta Q + 5 eta^2 S1 + 50 eta^3 + 12 u P.
+__device__ __forceinline__ int fsign(const F3 &a, const F3 &b, const F3 &c, float eta)
+{
+    const float u = 5.9604644775390625e-8f;  // 2^-24
+    const float nx = b.y * c.z - b.z * c.y;
+    const float ny = b.z * c.x - b.x * c.z;
+    const float nz = b.x * c.y - b.y * c.x;
+    const float d = a.x * nx + a.y * ny + a.z * nz;
+    const float ax = __builtin_fabsf(a.x), ay = __builtin_fabsf(a.y), az = __builtin_fabsf(a.z);
+    const float bx = __builtin_fabsf(b.x), by = __builtin_fabsf(b.y), bz = __builtin_fabsf(b.z);
+    const float cx = __builtin_fabsf(c.x), cy = __builtin_fabsf(c.y), cz = __builtin_fabsf(c.z);
+    const float px = by * cz + bz * cy;
+    const float py = bz * cx + bx * cz;
+    const float pz = bx * cy + by * cx;
+    const float P = ax * px + ay * py + az * pz;
+    const float Q = (px + py + pz) + ax * ((by + cz) + (bz + cy)) + ay * ((bz + cx) + (bx + cz)) +
+                    az * ((bx + cy) + (by + cx));
+    const float S1 = (ax + ay + az) + (bx + by + bz) + (cx + cy + cz);
+    const float E = 1.5f * eta * Q + 5.0f * eta * eta * S1 + 50.0f * eta * eta * eta + 12.0f * u * P + 1e-36f;
+    const float ad = __builtin_fabsf(d);
+    // the 2^-99 floor keeps |d64| >= 2^-100, so the reference's FP64 products
+    // dp1 * dq1 (tri_tri_intersect.c:421,438) cannot underflow to zero
+    if (ad > E * 1.0000005f + 1.6e-30f)
+        return d > 0.0f ? 1 : -1;
+    return 0;
+}
+
+// CONSTRUCT_INTERSECTION's predicates (tri_tri_intersect.c:286-296, 324-332):
+// 1 = intersection (segment computed later in FP64), 0 = rejection, -2 = uncertain
+__device__ __forceinline__ int f_construct(const F3 &p1, const F3 &q1, const F3 &r1, const F3 &p2, const F3 &q2,
+                                           const F3 &r2, float eta)
+{
+    const F3 v = fsub(p2, p1);
+    F3 v1 = fsub(q1, p1);
+    F3 v2 = fsub(r2, p1);
+    int s = fsign(v, v1, v2, eta);
+    if (!s)
+        return -2;
+    if (s > 0) {
+        v1 = fsub(r1, p1);
+        s = fsign(v, v1, v2, eta);
+        if (!s)
+            return -2;
+        return s < 0 ? 1 : 0;
+    }
+    v2 = fsub(q2, p1);
+    s = fsign(v, v1, v2, eta);
+    if (!s)
+        return -2;
+    return s < 0 ? 0 : 1;
+}
+
+// TRI_TRI_INTER_3D (tri_tri_intersect.c:360-385) for non-zero signs
+__device__ __forceinline__ int f_inter3d(const F3 &p1, const F3 &q1, const F3 &r1, const F3 &p2, const F3 &q2,
+                                         const F3 &r2, int sp2, int sq2, int sr2, float eta)
+{
+    if (sp2 > 0) {
+        if (sq2 > 0)
+            return f_construct(p1, r1, q1, r2, p2, q2, eta);
+        if (sr2 > 0)
+            return f_construct(p1, r1, q1, q2, r2, p2, eta);
+        return f_construct(p1, q1, r1, p2, q2, r2, eta);
+    }
+    if (sq2 < 0)
+        return f_construct(p1, q1, r1, r2, p2, q2, eta);
+    if (sr2 < 0)
+        return f_construct(p1, q1, r1, q2, r2, p2, eta);
+    return f_construct(p1, r1, q1, p2, q2, r2, eta);
+}
+
+// Returns a certified GdExit rejection (0, 1, 2), GD_HIT when the certified path
+// ends in an intersection, or -1 when some sign is uncertain.
+__device__ __forceinline__ int gd_filter(const F3 &p1, const F3 &q1, const F3 &r1, const F3 &p2, const F3 &q2,
+                                         const F3 &r2)
+{
+    float M = 0.0f;
+    const F3 pts[6] = {p1, q1, r1, p2, q2, r2};
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        M = fmaxf(M, fmaxf(__builtin_fabsf(pts[k].x), fmaxf(__builtin_fabsf(pts[k].y), __builtin_fabsf(pts[k].z))));
+    const float eta = 1.1920928955078125e-7f * M * 1.0000003f;  // 2 u M, rounded up
+    const F3 e1 = fsub(p2, r2), e2 = fsub(q2, r2);
+    const int sp1 = fsign(fsub(p1, r2), e1, e2, eta);
+    const int sq1 = fsign(fsub(q1, r2), e1, e2, eta);
+    const int sr1 = fsign(fsub(r1, r2), e1, e2, eta);
+    if (!sp1 || !sq1 || !sr1)
+        return -1;
+    if (sp1 == sq1 && sp1 == sr1)
+        return GD_REJECT_PLANE1;
+    const F3 f1 = fsub(q1, p1), f2 = fsub(r1, p1);
+    const int sp2 = fsign(fsub(p2, r1), f1, f2, eta);
+    const int sq2 = fsign(fsub(q2, r1), f1, f2, eta);
+    const int sr2 = fsign(fsub(r2, r1), f1, f2, eta);
+    if (!sp2 || !sq2 || !sr2)
+        return -1;
+    if (sp2 == sq2 && sp2 == sr2)
+        return GD_REJECT_PLANE2;
+    int r;
+    if (sp1 > 0) {
+        if (sq1 > 0)
+            r = f_inter3d(r1, p1, q1, p2, r2, q2, sp2, sr2, sq2, eta);
+        else if (sr1 > 0)
+            r = f_inter3d(q1, r1, p1, p2, r2, q2, sp2, sr2, sq2, eta);
+        else
+            r = f_inter3d(p1, q1, r1, p2, q2, r2, sp2, sq2, sr2, eta);
+    } else {
+        if (sq1 < 0)
+            r = f_inter3d(r1, p1, q1, p2, q2, r2, sp2, sq2, sr2, eta);
+        else if (sr1 < 0)
+            r = f_inter3d(q1, r1, p1, p2, q2, r2, sp2, sq2, sr2, eta);
+        else
+            r = f_inter3d(p1, q1, r1, p2, r2, q2, sp2, sr2, sq2, eta);
+    }
+    if (r == -2)
+        return -1;
+    return r ? GD_HIT : GD_REJECT_INTERVAL;
+}
+
+__device__ __forceinline__ void wave_count(bool pred, unsigned long long *counter)
+{
+    const unsigned long long m = __ballot(pred);
+    if ((threadIdx.x & 63) == 0 && m)
+        atomicAdd(counter, (unsigned long long)__popcll(m));
+}
+
+template <int PRECISION>
+__global__ __launch_bounds__(256) void k_tritri(const double *__restrict__ tri64A, const double *__restrict__ tri64B,
+                                                const float *__restrict__ tri32A, const float *__restrict__ tri32B,
+                                                const uint64_t *__restrict__ pairs, uint32_t n,
+                                                uint8_t *__restrict__ status, double *__restrict__ seg,
+                                                GdStats *__restrict__ stats)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n;
+    int exitKind = -1;
+    bool fallback = false;
+    bool filtered = false;
+    if (valid) {
+        const uint64_t pr = pairs[i];
+        const uint32_t a = (uint32_t)(pr >> 32), b = (uint32_t)pr;
+        if (PRECISION == 1) {
+            F3 p1, q1, r1, p2, q2, r2;
+            load_tri32(tri32A, a, p1, q1, r1);
+            load_tri32(tri32B, b, p2, q2, r2);
+            const int f = gd_filter(p1, q1, r1, p2, q2, r2);
+            if (f >= 0 && f <= GD_REJECT_INTERVAL) {
+                exitKind = f;
+                filtered = true;
+            } else {
+                fallback = f < 0;
+            }
+        }
+        if (exitKind < 0) {
+            V3 p1, q1, r1, p2, q2, r2, s, t;
+            load_tri64(tri64A, a, p1, q1, r1);
+            load_tri64(tri64B, b, p2, q2, r2);
+            exitKind = gd_fp64(p1, q1, r1, p2, q2, r2, s, t);
+            if (exitKind == GD_HIT) {
+                double2 *o = reinterpret_cast<double2 *>(seg + 6 * (size_t)i);
+                o[0] = make_double2(s.x, s.y);
+                o[1] = make_double2(s.z, t.x);
+                o[2] = make_double2(t.y, t.z);
+            }
+        }
+        status[i] = exitKind == GD_HIT ? 1 : (exitKind == GD_COPLANAR ? 2 : 0);
+    }
+    if (stats) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            wave_count(valid && exitKind == k, &stats->exits[k]);
+        wave_count(valid && fallback, &stats->fallback);
+        wave_count(valid && filtered, &stats->filtered);
+    }
+}
+
+// host triangle arrays: a9/b9 hold 9 doubles per pair (p, q, r)
+__global__ __launch_bounds__(256) void k_tritri_raw(const double *__restrict__ a9, const double *__restrict__ b9,
+                                                    uint32_t n, uint8_t *__restrict__ status,
+                                                    double *__restrict__ seg)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const double *a = a9 + 9 * (size_t)i;
+    const double *b = b9 + 9 * (size_t)i;
+    V3 p1 = {a[0], a[1], a[2]}, q1 = {a[3], a[4], a[5]}, r1 = {a[6], a[7], a[8]};
+    V3 p2 = {b[0], b[1], b[2]}, q2 = {b[3], b[4], b[5]}, r2 = {b[6], b[7], b[8]};
+    V3 s = {0, 0, 0}, t = {0, 0, 0};
+    const int e = gd_fp64(p1, q1, r1, p2, q2, r2, s, t);
+    status[i] = e == GD_HIT ? 1 : (e == GD_COPLANAR ? 2 : 0);
+    double *o = seg + 6 * (size_t)i;
+    o[0] = s.x;
+    o[1] = s.y;
+    o[2] = s.z;
+    o[3] = t.x;
+    o[4] = t.y;
+    o[5] = t.z;
+}
+
+// ---- hit compaction (pair order preserved) --------------------------------
+constexpr int kCompactBlock = 256;
+
+__global__ __launch_bounds__(kCompactBlock) void k_hit_count(const uint8_t *__restrict__ status, uint32_t n,
+                                                             uint32_t *__restrict__ blockCounts)
+{
+    const uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x;
+    const bool hit = i < n && status[i] == 1;
+    __shared__ uint32_t s_cnt;
+    if (threadIdx.x == 0)
+        s_cnt = 0;
+    __syncthreads();
+    const unsigned long long m = __ballot(hit);
+    if ((threadIdx.x & 63) == 0)
+        atomicAdd(&s_cnt, (uint32_t)__popcll(m));
+    __syncthreads();
+    if (threadIdx.x == 0)
+        blockCounts[blockIdx.x] = s_cnt;
+}
+
+__global__ __launch_bounds__(kCompactBlock) void k_hit_write(const uint64_t *__restrict__ pairs,
+                                                             const uint8_t *__restrict__ status,
+                                                             const double *__restrict__ seg, uint32_t n,
+                                                             const uint32_t *__restrict__ blockOffsets,
+                                                             uint64_t *__restrict__ hitPairs,
+                                                             double *__restrict__ hitSeg, uint32_t *nHits)
+{
+    const uint32_t i = blockIdx.x * kCompactBlock + threadIdx.x;
+    const bool hit = i < n && status[i] == 1;
+    __shared__ uint32_t s_wave[kCompactBlock / 64];
+    const unsigned long long m = __ballot(hit);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        s_wave[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t off = blockOffsets[blockIdx.x];
+    for (int w = 0; w < wave; ++w)
+        off += s_wave[w];
+    off += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (hit) {
+        hitPairs[off] = pairs[i];
+        for (int k = 0; k < 6; ++k)
+            hitSeg[6 * (size_t)off + k] = seg[6 * (size_t)i + k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        uint32_t tot = blockOffsets[blockIdx.x];
+        for (int w = 0; w < kCompactBlock / 64; ++w)
+            tot += s_wave[w];
+        *nHits = tot;
+    }
+}
+
+int compact_blocks(uint32_t n) { return (int)((n + kCompactBlock - 1) / kCompactBlock); }
+
+void launch_tritri(const double *tri64A, const double *tri64B, const float *tri32A, const float *tri32B,
+                   const uint64_t *pairs, uint32_t n, int precision, uint8_t *status, double *seg,
+                   GdStats *stats, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    const dim3 grid((n + 255) / 256);
+    if (precision == 1)
+        hipLaunchKernelGGL(k_tritri<1>, grid, dim3(256), 0, s, tri64A, tri64B, tri32A, tri32B, pairs, n, status,
+                           seg, stats);
+    else
+        hipLaunchKernelGGL(k_tritri<0>, grid, dim3(256), 0, s, tri64A, tri64B, tri32A, tri32B, pairs, n, status,
+                           seg, stats);
+}
+
+void launch_tritri_raw(const double *a9, const double *b9, uint32_t n, uint8_t *status, double *seg,
+                       hipStream_t s)
+{
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_tritri_raw, dim3((n + 255) / 256), dim3(256), 0, s, a9, b9, n, status, seg);
+}
+
+void launch_exclusive_scan(uint32_t *data, size_t m, hipStream_t s);
+
+void launch_compact_hits(const uint64_t *pairs, const uint8_t *status, const double *seg, uint32_t n,
+                         uint32_t *blockCounts, uint64_t *hitPairs, double *hitSeg, uint32_t *nHits,
+                         hipStream_t s)
+{
+    if (n == 0)
+        return;
+    const int nb = compact_blocks(n);
+    hipLaunchKernelGGL(k_hit_count, dim3(nb), dim3(kCompactBlock), 0, s, status, n, blockCounts);
+    launch_exclusive_scan(blockCounts, (size_t)nb, s);
+    hipLaunchKernelGGL(k_hit_write, dim3(nb), dim3(kCompactBlock), 0, s, pairs, status, seg, n, blockCounts,
+                       hitPairs, hitSeg, nHits);
+}
+
+}  // namespace sbg

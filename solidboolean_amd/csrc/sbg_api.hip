@@ -1,0 +1,1042 @@
+// C-ABI implementation (include/sbgpu.h): contexts, meshes, pair lists and the
+// host-side sequencing of the kernels in k_*.hip.
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sbg_internal.h"
+#include "sbgpu.h"
+
+using namespace sbg;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return fail(SBG_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+#define TRY(expr)              \
+    do {                       \
+        int r_ = (expr);       \
+        if (r_ != SBG_OK)      \
+            return r_;         \
+    } while (0)
+
+// growable device buffer (contents are not preserved on growth)
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n)
+    {
+        if (n <= cap && p)
+            return SBG_OK;
+        release();
+        const size_t count = n ? n : 1;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(SBG_ERR_OOM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        cap = count;
+        return SBG_OK;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+constexpr int kMaxLevels = 256;
+constexpr int kExpandGrid = 1024;
+constexpr int kLevelsPerSync = 8;
+
+}  // namespace
+
+struct sbg_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t *pinned = nullptr;  // host-visible counters (kMaxLevels + 8)
+    DevBuf<uint2> front[2];
+    DevBuf<uint32_t> counters;
+    size_t frontCap = 0, candCap = 0;
+    DevBuf<uint32_t> hist;
+    DevBuf<GdStats> stats;
+    DevBuf<double> pts;
+    DevBuf<uint8_t> flags, bits, votes;
+    DevBuf<uint32_t> ovf;
+    DevBuf<double> rawA, rawB, rawSeg;
+    DevBuf<uint8_t> rawStatus;
+    int refs = 1;  // the owner + every live mesh / pair list
+    bool destroyed = false;
+    bool timing = false;
+    struct Ev {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> events;
+    double acc[SBG_STAGE_COUNT] = {0};
+};
+
+struct sbg_mesh_s {
+    sbg_context ctx = nullptr;
+    size_t nv = 0, nt = 0;
+    DevBuf<double> V;
+    DevBuf<uint32_t> T;
+    DevBuf<double> tri64, normals;
+    DevBuf<float> tri32, box32, blockBounds, bounds, rootBox;
+    DevBuf<uint32_t> keys, vals, keysAlt, valsAlt, leafParent, flags;
+    DevBuf<BNode> nodes;
+    uint32_t root = 0;
+    bool prepared = false;
+};
+
+struct sbg_pairs_s {
+    sbg_context ctx = nullptr;
+    size_t n = 0;
+    DevBuf<uint64_t> pairs, alt;
+    DevBuf<uint8_t> status;
+    DevBuf<double> seg, hitSeg;
+    DevBuf<uint64_t> hitPairs;
+    DevBuf<uint32_t> blockCounts, nHits;
+    bool tested = false;
+    // largest first / second index in the list (host-built lists are checked
+    // against the meshes before any kernel indexes them)
+    uint64_t maxA = 0, maxB = 0;
+};
+
+namespace {
+
+int use(sbg_context ctx)
+{
+    if (!ctx)
+        return fail(SBG_ERR_INVALID, "null context");
+    if (ctx->destroyed)
+        return fail(SBG_ERR_STATE, "context already destroyed");
+    HIP_TRY(hipSetDevice(ctx->device));
+    return SBG_OK;
+}
+
+// Meshes and pair lists hold a reference on their context, so destroying a
+// context while they live only marks it; the last release frees it.
+void context_free(sbg_context ctx);
+void context_release(sbg_context ctx)
+{
+    if (--ctx->refs == 0)
+        context_free(ctx);
+}
+
+// HIP-event bracket of one stage (no-op unless timing is enabled)
+struct Stage {
+    sbg_context c;
+    int stage;
+    hipEvent_t a = nullptr;
+    Stage(sbg_context ctx, int s) : c(ctx), stage(s)
+    {
+        if (c->timing && hipEventCreate(&a) == hipSuccess)
+            (void)hipEventRecord(a, c->stream);
+        else
+            a = nullptr;
+    }
+    ~Stage()
+    {
+        if (!a)
+            return;
+        hipEvent_t b = nullptr;
+        if (hipEventCreate(&b) == hipSuccess) {
+            (void)hipEventRecord(b, c->stream);
+            c->events.push_back({stage, a, b});
+        } else {
+            (void)hipEventDestroy(a);
+        }
+    }
+};
+
+int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(SBG_ERR_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
+    return SBG_OK;
+}
+
+int ceil_log2(size_t n)
+{
+    int b = 0;
+    while (((size_t)1 << b) < n)
+        ++b;
+    return b > 0 ? b : 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sbg_last_error(void) { return g_err.c_str(); }
+
+int sbg_device_count(int *count)
+{
+    if (!count)
+        return fail(SBG_ERR_INVALID, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail(SBG_ERR_NO_DEVICE, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *count = n;
+    return SBG_OK;
+}
+
+int sbg_context_create(int device, sbg_context *out)
+{
+    if (!out)
+        return fail(SBG_ERR_INVALID, "null output");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        return fail(SBG_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n)
+        return fail(SBG_ERR_INVALID, "device %d out of range (%d devices)", device, n);
+    HIP_TRY(hipSetDevice(device));
+    sbg_context c = new sbg_context_s;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(SBG_ERR_HIP, "hipStreamCreateWithFlags failed");
+    }
+    if (hipHostMalloc((void **)&c->pinned, sizeof(uint32_t) * (kMaxLevels + 8), hipHostMallocDefault) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(SBG_ERR_OOM, "hipHostMalloc failed");
+    }
+    *out = c;
+    return SBG_OK;
+}
+
+}  // extern "C"
+
+namespace {
+void context_free(sbg_context ctx)
+{
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto &e : ctx->events) {
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    ctx->front[0].release();
+    ctx->front[1].release();
+    ctx->counters.release();
+    ctx->hist.release();
+    ctx->stats.release();
+    ctx->pts.release();
+    ctx->flags.release();
+    ctx->bits.release();
+    ctx->votes.release();
+    ctx->ovf.release();
+    ctx->rawA.release();
+    ctx->rawB.release();
+    ctx->rawSeg.release();
+    ctx->rawStatus.release();
+    (void)hipHostFree(ctx->pinned);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+}  // namespace
+
+extern "C" {
+
+int sbg_context_destroy(sbg_context ctx)
+{
+    if (!ctx || ctx->destroyed)
+        return SBG_OK;
+    ctx->destroyed = true;
+    context_release(ctx);
+    return SBG_OK;
+}
+
+int sbg_context_synchronize(sbg_context ctx)
+{
+    TRY(use(ctx));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SBG_OK;
+}
+
+int sbg_context_stream(sbg_context ctx, void **hip_stream)
+{
+    if (!ctx || !hip_stream)
+        return fail(SBG_ERR_INVALID, "null argument");
+    *hip_stream = (void *)ctx->stream;
+    return SBG_OK;
+}
+
+int sbg_context_set_timing(sbg_context ctx, int enable)
+{
+    TRY(use(ctx));
+    ctx->timing = enable != 0;
+    return SBG_OK;
+}
+
+int sbg_context_stage_ms(sbg_context ctx, double *ms, int reset)
+{
+    TRY(use(ctx));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (auto &e : ctx->events) {
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, e.a, e.b));
+        ctx->acc[e.stage] += t;
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
+    }
+    ctx->events.clear();
+    if (ms)
+        for (int i = 0; i < SBG_STAGE_COUNT; ++i)
+            ms[i] = ctx->acc[i];
+    if (reset)
+        for (int i = 0; i < SBG_STAGE_COUNT; ++i)
+            ctx->acc[i] = 0.0;
+    return SBG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// meshes
+// ---------------------------------------------------------------------------
+int sbg_mesh_create(sbg_context ctx, const double *xyz, size_t nverts, const uint32_t *tris, size_t ntris,
+                    sbg_mesh *out)
+{
+    if (!out)
+        return fail(SBG_ERR_INVALID, "null output");
+    *out = nullptr;
+    TRY(use(ctx));
+    if ((nverts && !xyz) || (ntris && !tris))
+        return fail(SBG_ERR_INVALID, "null vertex or triangle array");
+    if (ntris >= (size_t)kLeafBit)
+        return fail(SBG_ERR_INVALID, "too many triangles (%zu)", ntris);
+    for (size_t i = 0; i < 3 * ntris; ++i)
+        if ((size_t)tris[i] >= nverts)
+            return fail(SBG_ERR_INVALID, "triangle %zu references vertex %u of %zu", i / 3, tris[i], nverts);
+    sbg_mesh m = new sbg_mesh_s;
+    m->ctx = ctx;
+    ++ctx->refs;
+    m->nv = nverts;
+    m->nt = ntris;
+    int r = m->V.ensure(3 * nverts);
+    if (r == SBG_OK)
+        r = m->T.ensure(3 * ntris);
+    if (r != SBG_OK) {
+        sbg_mesh_destroy(m);
+        return r;
+    }
+    if (nverts && hipMemcpy(m->V.p, xyz, sizeof(double) * 3 * nverts, hipMemcpyHostToDevice) != hipSuccess) {
+        sbg_mesh_destroy(m);
+        return fail(SBG_ERR_HIP, "vertex upload failed");
+    }
+    if (ntris && hipMemcpy(m->T.p, tris, sizeof(uint32_t) * 3 * ntris, hipMemcpyHostToDevice) != hipSuccess) {
+        sbg_mesh_destroy(m);
+        return fail(SBG_ERR_HIP, "triangle upload failed");
+    }
+    *out = m;
+    return SBG_OK;
+}
+
+int sbg_mesh_destroy(sbg_mesh m)
+{
+    if (!m)
+        return SBG_OK;
+    (void)hipSetDevice(m->ctx->device);
+    (void)hipStreamSynchronize(m->ctx->stream);
+    m->V.release();
+    m->T.release();
+    m->tri64.release();
+    m->normals.release();
+    m->tri32.release();
+    m->box32.release();
+    m->blockBounds.release();
+    m->bounds.release();
+    m->rootBox.release();
+    m->keys.release();
+    m->vals.release();
+    m->keysAlt.release();
+    m->valsAlt.release();
+    m->leafParent.release();
+    m->flags.release();
+    m->nodes.release();
+    sbg_context c = m->ctx;
+    delete m;
+    context_release(c);
+    return SBG_OK;
+}
+
+int sbg_mesh_counts(sbg_mesh m, size_t *nverts, size_t *ntris)
+{
+    if (!m)
+        return fail(SBG_ERR_INVALID, "null mesh");
+    if (nverts)
+        *nverts = m->nv;
+    if (ntris)
+        *ntris = m->nt;
+    return SBG_OK;
+}
+
+int sbg_mesh_prepare(sbg_mesh m)
+{
+    if (!m)
+        return fail(SBG_ERR_INVALID, "null mesh");
+    sbg_context c = m->ctx;
+    TRY(use(c));
+    const size_t nt = m->nt;
+    const int nb = records_blocks((uint32_t)nt);
+    TRY(m->tri64.ensure((size_t)kTri64Stride * nt));
+    TRY(m->tri32.ensure((size_t)kTri32Stride * nt));
+    TRY(m->normals.ensure(3 * nt));
+    TRY(m->box32.ensure((size_t)kBox32Stride * nt));
+    TRY(m->blockBounds.ensure(6 * (size_t)(nb > 0 ? nb : 1)));
+    TRY(m->bounds.ensure(6));
+    TRY(m->rootBox.ensure(6));
+    TRY(m->keys.ensure(nt));
+    TRY(m->vals.ensure(nt));
+    TRY(m->keysAlt.ensure(nt));
+    TRY(m->valsAlt.ensure(nt));
+    TRY(m->leafParent.ensure(nt));
+    TRY(m->nodes.ensure(nt > 1 ? nt - 1 : 1));
+    TRY(m->flags.ensure(nt > 1 ? nt - 1 : 1));
+    TRY(c->hist.ensure(sort_hist_entries(nt)));
+    m->prepared = false;
+    if (nt == 0) {
+        m->prepared = true;
+        return SBG_OK;
+    }
+    {
+        Stage st(c, SBG_STAGE_RECORDS);
+        launch_records(m->V.p, m->T.p, (uint32_t)nt, m->tri64.p, m->tri32.p, m->normals.p, m->box32.p,
+                       m->blockBounds.p, nb, c->stream);
+        launch_bounds_reduce(m->blockBounds.p, nb, m->bounds.p, c->stream);
+        launch_morton(m->box32.p, m->bounds.p, (uint32_t)nt, m->keys.p, m->vals.p, c->stream);
+        TRY(check_launch("records"));
+    }
+    uint32_t *sk = m->keys.p, *sv = m->vals.p;
+    {
+        Stage st(c, SBG_STAGE_SORT);
+        if (radix_sort_u32(m->keys.p, m->vals.p, m->keysAlt.p, m->valsAlt.p, nt, 0, 30, c->hist.p, c->stream)) {
+            sk = m->keysAlt.p;
+            sv = m->valsAlt.p;
+        }
+        TRY(check_launch("morton sort"));
+    }
+    {
+        Stage st(c, SBG_STAGE_BVH);
+        if (nt == 1) {
+            m->root = kLeafBit;
+            HIP_TRY(hipMemcpyAsync(m->rootBox.p, m->box32.p, 3 * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(m->rootBox.p + 3, m->box32.p + 4, 3 * sizeof(float), hipMemcpyDeviceToDevice,
+                                   c->stream));
+        } else {
+            m->root = 0;
+            launch_hierarchy(sk, sv, (uint32_t)nt, m->nodes.p, m->leafParent.p, c->stream);
+            launch_refit(sv, m->box32.p, (uint32_t)nt, m->nodes.p, m->leafParent.p, m->flags.p, m->rootBox.p,
+                         c->stream);
+        }
+        TRY(check_launch("bvh"));
+    }
+    m->prepared = true;
+    return SBG_OK;
+}
+
+int sbg_mesh_normals(sbg_mesh m, double *out)
+{
+    if (!m || !out)
+        return fail(SBG_ERR_INVALID, "null argument");
+    if (!m->prepared)
+        return fail(SBG_ERR_STATE, "mesh not prepared");
+    TRY(use(m->ctx));
+    if (m->nt) {
+        HIP_TRY(hipMemcpyAsync(out, m->normals.p, sizeof(double) * 3 * m->nt, hipMemcpyDeviceToHost, m->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    }
+    return SBG_OK;
+}
+
+int sbg_mesh_boxes(sbg_mesh m, double *out)
+{
+    if (!m || !out)
+        return fail(SBG_ERR_INVALID, "null argument");
+    if (!m->prepared)
+        return fail(SBG_ERR_STATE, "mesh not prepared");
+    sbg_context c = m->ctx;
+    TRY(use(c));
+    if (!m->nt)
+        return SBG_OK;
+    TRY(c->pts.ensure(6 * m->nt));
+    launch_boxes64(m->tri64.p, (uint32_t)m->nt, c->pts.p, c->stream);
+    TRY(check_launch("boxes64"));
+    HIP_TRY(hipMemcpyAsync(out, c->pts.p, sizeof(double) * 6 * m->nt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SBG_OK;
+}
+
+int sbg_mesh_bvh(sbg_mesh m, void *nodes_out, uint32_t *root_ref, float *root_box6)
+{
+    if (!m)
+        return fail(SBG_ERR_INVALID, "null mesh");
+    if (!m->prepared)
+        return fail(SBG_ERR_STATE, "mesh not prepared");
+    sbg_context c = m->ctx;
+    TRY(use(c));
+    if (root_ref)
+        *root_ref = m->root;
+    if (nodes_out && m->nt > 1)
+        HIP_TRY(hipMemcpyAsync(nodes_out, m->nodes.p, sizeof(BNode) * (m->nt - 1), hipMemcpyDeviceToHost, c->stream));
+    if (root_box6 && m->nt > 0)
+        HIP_TRY(hipMemcpyAsync(root_box6, m->rootBox.p, sizeof(float) * 6, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SBG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// candidate pairs
+// ---------------------------------------------------------------------------
+int sbg_pairs_destroy(sbg_pairs p)
+{
+    if (!p)
+        return SBG_OK;
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->ctx->stream);
+    p->pairs.release();
+    p->alt.release();
+    p->status.release();
+    p->seg.release();
+    p->hitSeg.release();
+    p->hitPairs.release();
+    p->blockCounts.release();
+    p->nHits.release();
+    sbg_context c = p->ctx;
+    delete p;
+    context_release(c);
+    return SBG_OK;
+}
+
+int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
+{
+    if (!out || !a || !b)
+        return fail(SBG_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (a->ctx != b->ctx)
+        return fail(SBG_ERR_INVALID, "meshes belong to different contexts");
+    if (!a->prepared || !b->prepared)
+        return fail(SBG_ERR_STATE, "mesh not prepared");
+    sbg_context c = a->ctx;
+    TRY(use(c));
+    sbg_pairs p = new sbg_pairs_s;
+    p->ctx = c;
+    ++c->refs;
+    if (a->nt == 0 || b->nt == 0) {
+        *out = p;
+        return SBG_OK;
+    }
+    const size_t na = a->nt, nb = b->nt;
+    if (!c->frontCap)
+        c->frontCap = std::max<size_t>((size_t)1 << 18, 2 * (na + nb));
+    if (!c->candCap)
+        c->candCap = std::max<size_t>((size_t)1 << 16, (na + nb) / 4);
+    TraverseArgs t = {a->nodes.p, a->box32.p, a->tri64.p, b->nodes.p, b->box32.p, b->tri64.p};
+    uint32_t nCand = 0;
+    int rc = SBG_OK;
+    for (int attempt = 0; attempt < 16; ++attempt) {
+        if ((rc = c->front[0].ensure(c->frontCap)) != SBG_OK || (rc = c->front[1].ensure(c->frontCap)) != SBG_OK ||
+            (rc = c->counters.ensure(kMaxLevels + 8)) != SBG_OK || (rc = p->pairs.ensure(c->candCap)) != SBG_OK) {
+            sbg_pairs_destroy(p);
+            return rc;
+        }
+        uint32_t *cnt = c->counters.p;                 // [0, kMaxLevels]: frontier sizes
+        uint32_t *candCount = cnt + kMaxLevels + 1;    // candidates
+        uint32_t *overflow = cnt + kMaxLevels + 2;     // overflow flag
+        const uint32_t fcap = (uint32_t)std::min<size_t>(c->frontCap, 0xffffffffu);
+        const uint32_t ccap = (uint32_t)std::min<size_t>(c->candCap, 0xffffffffu);
+        int level = 0;
+        bool done = false, ovf = false;
+        {
+            Stage st(c, SBG_STAGE_TRAVERSE);
+            launch_fill_u32(cnt, 0u, kMaxLevels + 8, c->stream);
+            launch_seed(t, a->root, b->root, a->rootBox.p, b->rootBox.p, c->front[0].p, cnt, p->pairs.p, candCount,
+                        c->stream);
+            while (!done && level < kMaxLevels) {
+                for (int k = 0; k < kLevelsPerSync && level < kMaxLevels; ++k, ++level)
+                    launch_expand(t, c->front[level & 1].p, cnt + level, c->front[(level + 1) & 1].p, cnt + level + 1,
+                                  fcap, p->pairs.p, candCount, ccap, overflow, kExpandGrid, c->stream);
+                if ((rc = check_launch("expand")) != SBG_OK)
+                    break;
+                hipError_t e1 = hipMemcpyAsync(c->pinned, cnt + level, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+                hipError_t e2 = hipMemcpyAsync(c->pinned + 1, overflow, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream);
+                hipError_t e3 = hipStreamSynchronize(c->stream);
+                if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+                    rc = fail(SBG_ERR_HIP, "traversal sync failed: %s", hipGetErrorString(e3));
+                    break;
+                }
+                if (c->pinned[1]) {
+                    ovf = true;
+                    break;
+                }
+                if (c->pinned[0] == 0)
+                    done = true;
+            }
+        }
+        if (rc != SBG_OK) {
+            sbg_pairs_destroy(p);
+            return rc;
+        }
+        if (!done && !ovf) {
+            sbg_pairs_destroy(p);
+            return fail(SBG_ERR_OVERFLOW, "tree-vs-tree search exceeded %d levels", kMaxLevels);
+        }
+        // all counters, to size a retry or read the candidate count
+        hipError_t e = hipMemcpyAsync(c->pinned, cnt, sizeof(uint32_t) * (kMaxLevels + 3), hipMemcpyDeviceToHost,
+                                      c->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) {
+            sbg_pairs_destroy(p);
+            return fail(SBG_ERR_HIP, "counter readback failed: %s", hipGetErrorString(e));
+        }
+        uint32_t maxFront = 0;
+        for (int l = 0; l <= kMaxLevels; ++l)
+            maxFront = std::max(maxFront, c->pinned[l]);
+        nCand = c->pinned[kMaxLevels + 1];
+        if (!ovf && !c->pinned[kMaxLevels + 2])
+            break;
+        if (maxFront > fcap)
+            c->frontCap = std::max<size_t>(2 * c->frontCap, (size_t)maxFront + maxFront / 2);
+        if (nCand > ccap)
+            c->candCap = std::max<size_t>(2 * c->candCap, (size_t)nCand + nCand / 4);
+        if (maxFront <= fcap && nCand <= ccap) {  // overflow flagged at a deeper, unread level
+            c->frontCap *= 2;
+            c->candCap *= 2;
+        }
+        if (attempt == 15) {
+            sbg_pairs_destroy(p);
+            return fail(SBG_ERR_OVERFLOW, "candidate search did not fit after repeated growth");
+        }
+    }
+    p->n = nCand;
+    p->maxA = na - 1;
+    p->maxB = nb - 1;
+    if (nCand > 1) {
+        Stage st(c, SBG_STAGE_PAIR_SORT);
+        int rc2 = p->alt.ensure(nCand);
+        if (rc2 == SBG_OK)
+            rc2 = c->hist.ensure(sort_hist_entries(nCand));
+        if (rc2 != SBG_OK) {
+            sbg_pairs_destroy(p);
+            return rc2;
+        }
+        const int ranges[4] = {0, ceil_log2(nb), 32, 32 + ceil_log2(na)};
+        if (radix_sort_u64(p->pairs.p, p->alt.p, nCand, ranges, 2, c->hist.p, c->stream))
+            std::swap(p->pairs, p->alt);
+        if ((rc2 = check_launch("pair sort")) != SBG_OK) {
+            sbg_pairs_destroy(p);
+            return rc2;
+        }
+    }
+    *out = p;
+    return SBG_OK;
+}
+
+int sbg_pairs_create(sbg_context ctx, const uint32_t *ab, size_t n, sbg_pairs *out)
+{
+    if (!out)
+        return fail(SBG_ERR_INVALID, "null output");
+    *out = nullptr;
+    TRY(use(ctx));
+    if (n && !ab)
+        return fail(SBG_ERR_INVALID, "null pair array");
+    sbg_pairs p = new sbg_pairs_s;
+    p->ctx = ctx;
+    ++ctx->refs;
+    p->n = n;
+    int rc = p->pairs.ensure(n);
+    if (rc != SBG_OK) {
+        sbg_pairs_destroy(p);
+        return rc;
+    }
+    if (n) {
+        std::vector<uint64_t> h(n);
+        for (size_t i = 0; i < n; ++i) {
+            h[i] = ((uint64_t)ab[2 * i] << 32) | ab[2 * i + 1];
+            p->maxA = std::max<uint64_t>(p->maxA, ab[2 * i]);
+            p->maxB = std::max<uint64_t>(p->maxB, ab[2 * i + 1]);
+        }
+        if (hipMemcpy(p->pairs.p, h.data(), sizeof(uint64_t) * n, hipMemcpyHostToDevice) != hipSuccess) {
+            sbg_pairs_destroy(p);
+            return fail(SBG_ERR_HIP, "pair upload failed");
+        }
+    }
+    *out = p;
+    return SBG_OK;
+}
+
+int sbg_pairs_count(sbg_pairs p, size_t *n)
+{
+    if (!p || !n)
+        return fail(SBG_ERR_INVALID, "null argument");
+    *n = p->n;
+    return SBG_OK;
+}
+
+int sbg_pairs_copy(sbg_pairs p, uint32_t *ab)
+{
+    if (!p || (!ab && p->n))
+        return fail(SBG_ERR_INVALID, "null argument");
+    TRY(use(p->ctx));
+    if (!p->n)
+        return SBG_OK;
+    std::vector<uint64_t> h(p->n);
+    HIP_TRY(hipMemcpyAsync(h.data(), p->pairs.p, sizeof(uint64_t) * p->n, hipMemcpyDeviceToHost, p->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    for (size_t i = 0; i < p->n; ++i) {
+        ab[2 * i] = (uint32_t)(h[i] >> 32);
+        ab[2 * i + 1] = (uint32_t)h[i];
+    }
+    return SBG_OK;
+}
+
+static int check_pairs_in_range(sbg_mesh a, sbg_mesh b, sbg_pairs p)
+{
+    if (p->n && (p->maxA >= a->nt || p->maxB >= b->nt))
+        return fail(SBG_ERR_INVALID, "pair indices (max %llu, %llu) out of range for meshes of %zu and %zu triangles",
+                    (unsigned long long)p->maxA, (unsigned long long)p->maxB, a->nt, b->nt);
+    return SBG_OK;
+}
+
+static int run_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, GdStats *dstats)
+{
+    sbg_context c = p->ctx;
+    const uint32_t n = (uint32_t)p->n;
+    TRY(p->status.ensure(p->n));
+    TRY(p->seg.ensure(6 * p->n));
+    TRY(p->hitPairs.ensure(p->n));
+    TRY(p->hitSeg.ensure(6 * p->n));
+    TRY(p->blockCounts.ensure((size_t)compact_blocks(n)));
+    TRY(p->nHits.ensure(1));
+    {
+        Stage st(c, SBG_STAGE_TRI_TRI);
+        launch_tritri(a->tri64.p, b->tri64.p, a->tri32.p, b->tri32.p, p->pairs.p, n, precision, p->status.p,
+                      p->seg.p, dstats, c->stream);
+        TRY(check_launch("tri_tri"));
+    }
+    {
+        Stage st(c, SBG_STAGE_COMPACT);
+        launch_fill_u32(p->nHits.p, 0u, 1, c->stream);
+        launch_compact_hits(p->pairs.p, p->status.p, p->seg.p, n, p->blockCounts.p, p->hitPairs.p, p->hitSeg.p,
+                            p->nHits.p, c->stream);
+        TRY(check_launch("compact"));
+    }
+    return SBG_OK;
+}
+
+int sbg_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, sbg_tri_tri_stats *stats)
+{
+    if (!a || !b || !p)
+        return fail(SBG_ERR_INVALID, "null argument");
+    if (a->ctx != b->ctx || a->ctx != p->ctx)
+        return fail(SBG_ERR_INVALID, "handles belong to different contexts");
+    if (!a->prepared || !b->prepared)
+        return fail(SBG_ERR_STATE, "mesh not prepared");
+    if (precision != SBG_PRECISION_FP64 && precision != SBG_PRECISION_FP32_FILTERED)
+        return fail(SBG_ERR_INVALID, "unknown precision %d", precision);
+    sbg_context c = p->ctx;
+    TRY(use(c));
+    TRY(check_pairs_in_range(a, b, p));
+    GdStats *dstats = nullptr;
+    if (stats) {
+        TRY(c->stats.ensure(1));
+        dstats = c->stats.p;
+        launch_fill_u32((uint32_t *)dstats, 0u, sizeof(GdStats) / 4, c->stream);
+    }
+    TRY(run_tri_tri(a, b, p, precision, dstats));
+    p->tested = true;
+    if (stats) {
+        GdStats h;
+        HIP_TRY(hipMemcpyAsync(&h, dstats, sizeof(GdStats), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        stats->tested = p->n;
+        stats->exit_plane1 = h.exits[0];
+        stats->exit_plane2 = h.exits[1];
+        stats->exit_interval = h.exits[2];
+        stats->hits = h.exits[3];
+        stats->coplanar = h.exits[4];
+        stats->fallback_pairs = h.fallback;
+        stats->filtered_pairs = h.filtered;
+    }
+    return SBG_OK;
+}
+
+int sbg_pairs_results(sbg_pairs p, uint8_t *status, double *seg)
+{
+    if (!p)
+        return fail(SBG_ERR_INVALID, "null pairs");
+    if (!p->tested)
+        return fail(SBG_ERR_STATE, "sbg_tri_tri has not run on this pair list");
+    TRY(use(p->ctx));
+    if (!p->n)
+        return SBG_OK;
+    if (status)
+        HIP_TRY(hipMemcpyAsync(status, p->status.p, p->n, hipMemcpyDeviceToHost, p->ctx->stream));
+    if (seg)
+        HIP_TRY(hipMemcpyAsync(seg, p->seg.p, sizeof(double) * 6 * p->n, hipMemcpyDeviceToHost, p->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    return SBG_OK;
+}
+
+int sbg_pairs_hits(sbg_pairs p, size_t *nhits, uint32_t *ab, double *seg)
+{
+    if (!p || !nhits)
+        return fail(SBG_ERR_INVALID, "null argument");
+    if (!p->tested)
+        return fail(SBG_ERR_STATE, "sbg_tri_tri has not run on this pair list");
+    TRY(use(p->ctx));
+    *nhits = 0;
+    if (!p->n)
+        return SBG_OK;
+    uint32_t h = 0;
+    HIP_TRY(hipMemcpyAsync(&h, p->nHits.p, sizeof(uint32_t), hipMemcpyDeviceToHost, p->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    *nhits = h;
+    if (!h || (!ab && !seg))
+        return SBG_OK;
+    if (ab) {
+        std::vector<uint64_t> v(h);
+        HIP_TRY(hipMemcpyAsync(v.data(), p->hitPairs.p, sizeof(uint64_t) * h, hipMemcpyDeviceToHost, p->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+        for (uint32_t i = 0; i < h; ++i) {
+            ab[2 * i] = (uint32_t)(v[i] >> 32);
+            ab[2 * i + 1] = (uint32_t)v[i];
+        }
+    }
+    if (seg) {
+        HIP_TRY(hipMemcpyAsync(seg, p->hitSeg.p, sizeof(double) * 6 * h, hipMemcpyDeviceToHost, p->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(p->ctx->stream));
+    }
+    return SBG_OK;
+}
+
+int sbg_tri_tri_batch(sbg_context ctx, const double *a9, const double *b9, size_t n, uint8_t *status, double *seg)
+{
+    TRY(use(ctx));
+    if (n && (!a9 || !b9 || !status || !seg))
+        return fail(SBG_ERR_INVALID, "null argument");
+    if (n > 0xffffffffu)
+        return fail(SBG_ERR_INVALID, "too many pairs");
+    if (!n)
+        return SBG_OK;
+    TRY(ctx->rawA.ensure(9 * n));
+    TRY(ctx->rawB.ensure(9 * n));
+    TRY(ctx->rawSeg.ensure(6 * n));
+    TRY(ctx->rawStatus.ensure(n));
+    HIP_TRY(hipMemcpyAsync(ctx->rawA.p, a9, sizeof(double) * 9 * n, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->rawB.p, b9, sizeof(double) * 9 * n, hipMemcpyHostToDevice, ctx->stream));
+    launch_tritri_raw(ctx->rawA.p, ctx->rawB.p, (uint32_t)n, ctx->rawStatus.p, ctx->rawSeg.p, ctx->stream);
+    TRY(check_launch("tri_tri_raw"));
+    HIP_TRY(hipMemcpyAsync(status, ctx->rawStatus.p, n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(seg, ctx->rawSeg.p, sizeof(double) * 6 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return SBG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// classification
+// ---------------------------------------------------------------------------
+static int run_rays(sbg_mesh target, const double *dpts, const uint8_t *dskip, size_t n, uint8_t *dbits)
+{
+    sbg_context c = target->ctx;
+    TRY(c->ovf.ensure(1));
+    launch_fill_u32(c->ovf.p, 0u, 1, c->stream);
+    if (target->nt == 0) {
+        launch_fill_u32((uint32_t *)dbits, 0u, (3 * n + 3) / 4, c->stream);
+        return check_launch("rays");
+    }
+    launch_rays(target->nodes.p, target->root, target->rootBox.p, target->tri64.p, target->normals.p,
+                (uint32_t)target->nt, dpts, dskip, (uint32_t)n, dbits, c->ovf.p, c->stream);
+    return check_launch("rays");
+}
+
+static int check_ray_overflow(sbg_context c)
+{
+    uint32_t h = 0;
+    HIP_TRY(hipMemcpyAsync(&h, c->ovf.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (h)
+        return fail(SBG_ERR_OVERFLOW, "a ray exceeded its traversal stack or distinct-hit list");
+    return SBG_OK;
+}
+
+int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *axis_inside, uint8_t *inside)
+{
+    if (!target || (n && !pts))
+        return fail(SBG_ERR_INVALID, "null argument");
+    if (!target->prepared)
+        return fail(SBG_ERR_STATE, "mesh not prepared");
+    if (n > 0x50000000u)
+        return fail(SBG_ERR_INVALID, "too many points");
+    sbg_context c = target->ctx;
+    TRY(use(c));
+    if (!n)
+        return SBG_OK;
+    TRY(c->pts.ensure(3 * n));
+    TRY(c->bits.ensure(3 * n + 4));
+    TRY(c->votes.ensure(n));
+    HIP_TRY(hipMemcpyAsync(c->pts.p, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    {
+        Stage st(c, SBG_STAGE_CLASSIFY);
+        TRY(run_rays(target, c->pts.p, nullptr, n, c->bits.p));
+        launch_vote(c->bits.p, nullptr, (uint32_t)n, c->votes.p, c->stream);
+        TRY(check_launch("vote"));
+    }
+    TRY(check_ray_overflow(c));
+    if (inside)
+        HIP_TRY(hipMemcpyAsync(inside, c->votes.p, n, hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint8_t> bits;
+    if (axis_inside) {
+        bits.resize(3 * n);
+        HIP_TRY(hipMemcpyAsync(bits.data(), c->bits.p, 3 * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (axis_inside)
+        for (size_t i = 0; i < n; ++i)
+            for (int k = 0; k < 3; ++k)
+                axis_inside[3 * i + k] = bits[(size_t)k * n + i];
+    return SBG_OK;
+}
+
+int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, uint8_t *out)
+{
+    if (!self || !other || !p || !out)
+        return fail(SBG_ERR_INVALID, "null argument");
+    if (self->ctx != other->ctx || self->ctx != p->ctx)
+        return fail(SBG_ERR_INVALID, "handles belong to different contexts");
+    if (!self->prepared || !other->prepared)
+        return fail(SBG_ERR_STATE, "mesh not prepared");
+    if (!p->tested && p->n)
+        return fail(SBG_ERR_STATE, "sbg_tri_tri has not run on this pair list");
+    if (side != 0 && side != 1)
+        return fail(SBG_ERR_INVALID, "side must be 0 or 1");
+    if (p->n && (side == 0 ? p->maxA : p->maxB) >= self->nt)
+        return fail(SBG_ERR_INVALID, "pair list indexes triangle %llu of a %zu-triangle mesh (wrong side?)",
+                    (unsigned long long)(side == 0 ? p->maxA : p->maxB), self->nt);
+    sbg_context c = self->ctx;
+    TRY(use(c));
+    const size_t nt = self->nt;
+    if (!nt)
+        return SBG_OK;
+    TRY(c->flags.ensure(nt + 4));
+    TRY(c->pts.ensure(3 * nt));
+    TRY(c->bits.ensure(3 * nt + 4));
+    TRY(c->votes.ensure(nt));
+    {
+        Stage st(c, SBG_STAGE_CLASSIFY);
+        launch_fill_u32((uint32_t *)c->flags.p, 0u, (nt + 3) / 4, c->stream);
+        launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, c->flags.p, c->stream);
+        launch_centroids(self->tri64.p, c->flags.p, (uint32_t)nt, c->pts.p, c->stream);
+        TRY(run_rays(other, c->pts.p, c->flags.p, nt, c->bits.p));
+        launch_vote(c->bits.p, c->flags.p, (uint32_t)nt, c->votes.p, c->stream);
+        TRY(check_launch("classify"));
+    }
+    TRY(check_ray_overflow(c));
+    HIP_TRY(hipMemcpyAsync(out, c->votes.p, nt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SBG_OK;
+}
+
+int sbg_calibrate_copy(sbg_context ctx, size_t bytes, int iters, double *avg_ms)
+{
+    TRY(use(ctx));
+    if (!avg_ms || iters <= 0 || bytes < 16)
+        return fail(SBG_ERR_INVALID, "bad argument");
+    const size_t n16 = bytes / 16;
+    DevBuf<uint4> src, dst;
+    TRY(src.ensure(n16));
+    int rc = dst.ensure(n16);
+    if (rc != SBG_OK) {
+        src.release();
+        return rc;
+    }
+    launch_fill_u32((uint32_t *)src.p, 1u, 4 * n16, ctx->stream);
+    launch_copy16(src.p, dst.p, n16, ctx->stream);  // warm
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess)
+        e = hipEventCreate(&e1);
+    if (e == hipSuccess)
+        e = hipEventRecord(e0, ctx->stream);
+    for (int i = 0; e == hipSuccess && i < iters; ++i)
+        launch_copy16(src.p, dst.p, n16, ctx->stream);
+    if (e == hipSuccess)
+        e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess)
+        e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess)
+        e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0)
+        (void)hipEventDestroy(e0);
+    if (e1)
+        (void)hipEventDestroy(e1);
+    src.release();
+    dst.release();
+    if (e != hipSuccess)
+        return fail(SBG_ERR_HIP, "calibration copy failed: %s", hipGetErrorString(e));
+    *avg_ms = (double)ms / iters;
+    return check_launch("copy");
+}
+
+int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, int iters, double *avg_ms)
+{
+    if (!a || !b || !p || !avg_ms || iters <= 0)
+        return fail(SBG_ERR_INVALID, "bad argument");
+    if (precision != SBG_PRECISION_FP64 && precision != SBG_PRECISION_FP32_FILTERED)
+        return fail(SBG_ERR_INVALID, "unknown precision %d", precision);
+    sbg_context c = p->ctx;
+    TRY(use(c));
+    TRY(check_pairs_in_range(a, b, p));
+    TRY(p->status.ensure(p->n));
+    TRY(p->seg.ensure(6 * p->n));
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < iters; ++i)
+        launch_tritri(a->tri64.p, b->tri64.p, a->tri32.p, b->tri32.p, p->pairs.p, (uint32_t)p->n, precision,
+                      p->status.p, p->seg.p, nullptr, c->stream);
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / iters;
+    return check_launch("bench tri_tri");
+}
+
+}  // extern "C"

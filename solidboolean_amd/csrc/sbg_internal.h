@@ -1,0 +1,104 @@
+// Internal host-side declarations shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+
+namespace sbg {
+
+// ---- device data layouts (HBM) -------------------------------------------
+// Triangle record, FP64: 3 vertices x (x, y, z, pad) = 96 B, 16-B aligned.
+constexpr int kTri64Stride = 12;  // doubles
+// Triangle record, FP32: 3 vertices x (x, y, z, pad) = 48 B.
+constexpr int kTri32Stride = 12;  // floats
+// Leaf box, FP32 outward-rounded: (minx, miny, minz, 0, maxx, maxy, maxz, 0) = 32 B.
+constexpr int kBox32Stride = 8;  // floats
+
+// BVH2 internal node, 64 B: both children's FP32 boxes + child references.
+// A child reference with the top bit set is a leaf whose low 31 bits are the
+// triangle index; otherwise it is an internal node index.
+struct alignas(16) BNode {
+    float4 a;  // c0.min.x c0.min.y c0.min.z c0.max.x
+    float4 b;  // c0.max.y c0.max.z c1.min.x c1.min.y
+    float4 c;  // c1.min.z c1.max.x c1.max.y c1.max.z
+    int4 d;    // c0 ref, c1 ref, parent, unused
+};
+constexpr uint32_t kLeafBit = 0x80000000u;
+
+// ---- launchers (each .hip file owns its kernels) --------------------------
+// k_prepare.hip
+void launch_records(const double *V, const uint32_t *T, uint32_t nt, double *tri64, float *tri32,
+                    double *normals, float *box32, float *blockBounds, int nblocks, hipStream_t s);
+int records_blocks(uint32_t nt);
+void launch_boxes64(const double *tri64, uint32_t nt, double *out, hipStream_t s);
+void launch_bounds_reduce(const float *blockBounds, int nblocks, float *bounds, hipStream_t s);
+void launch_morton(const float *box32, const float *bounds, uint32_t nt, uint32_t *keys, uint32_t *vals,
+                   hipStream_t s);
+
+// k_sort.hip: stable LSD radix sort; keys (and optional u32 values) ping-pong
+// between the two buffers; returns 1 when the result ended in the alt buffers.
+struct SortTemp {
+    uint32_t *hist = nullptr;  // 256 * nblocks
+    size_t histCap = 0;
+};
+int sort_blocks(size_t n);
+void launch_exclusive_scan(uint32_t *data, size_t m, hipStream_t s);
+size_t sort_hist_entries(size_t n);
+int radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *valsAlt, size_t n,
+                   int beginBit, int endBit, uint32_t *hist, hipStream_t s);
+int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRanges, int nRanges,
+                   uint32_t *hist, hipStream_t s);
+
+// k_lbvh.hip
+void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
+                      uint32_t *leafParent, hipStream_t s);
+void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *nodes,
+                  const uint32_t *leafParent, uint32_t *flags, float *rootBox, hipStream_t s);
+void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+void launch_copy16(const void *src, void *dst, size_t n16, hipStream_t s);
+
+// k_traverse.hip
+struct TraverseArgs {
+    const BNode *nodesA;
+    const float *boxA;
+    const double *tri64A;
+    const BNode *nodesB;
+    const float *boxB;
+    const double *tri64B;
+};
+void launch_expand(const TraverseArgs &t, const uint2 *in, const uint32_t *inCount, uint2 *out,
+                   uint32_t *outCount, uint32_t outCap, uint64_t *cand, uint32_t *candCount,
+                   uint32_t candCap, uint32_t *overflow, int grid, hipStream_t s);
+void launch_seed(const TraverseArgs &t, uint32_t rootA, uint32_t rootB, const float *rootBoxA,
+                 const float *rootBoxB, uint2 *frontier, uint32_t *count, uint64_t *cand,
+                 uint32_t *candCount, hipStream_t s);
+
+// k_tritri.hip
+struct GdStats {
+    unsigned long long exits[5];   // plane-1, plane-2, interval, hit, coplanar
+    unsigned long long fallback;   // pairs re-run in FP64 because a sign was uncertain
+    unsigned long long filtered;   // pairs decided by the FP32 filter alone
+};
+void launch_tritri(const double *tri64A, const double *tri64B, const float *tri32A, const float *tri32B,
+                   const uint64_t *pairs, uint32_t n, int precision, uint8_t *status, double *seg,
+                   GdStats *stats, hipStream_t s);
+void launch_tritri_raw(const double *a9, const double *b9, uint32_t n, uint8_t *status, double *seg,
+                       hipStream_t s);
+// compaction of hits (status == 1) into (pair, seg) lists, in pair order
+void launch_compact_hits(const uint64_t *pairs, const uint8_t *status, const double *seg, uint32_t n,
+                         uint32_t *blockCounts, uint64_t *hitPairs, double *hitSeg, uint32_t *nHits,
+                         hipStream_t s);
+int compact_blocks(uint32_t n);
+
+// k_classify.hip
+void launch_mark_intersected(const uint64_t *pairs, const uint8_t *status, uint32_t n, int side,
+                             uint8_t *flags, hipStream_t s);
+void launch_centroids(const double *tri64, const uint8_t *flags, uint32_t nt, double *pts, hipStream_t s);
+void launch_rays(const BNode *nodes, uint32_t root, const float *rootBox, const double *tri64,
+                 const double *normals, uint32_t ntTarget, const double *pts, const uint8_t *skip,
+                 uint32_t n, uint8_t *axisBits, uint32_t *overflow, hipStream_t s);
+void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);
+
+}  // namespace sbg

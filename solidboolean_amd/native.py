@@ -1,0 +1,310 @@
+"""ctypes binding of libsbgpu.so (include/sbgpu.h).
+
+There is no fallback: if the HIP library is not built, or no GPU is visible,
+every entry point raises.  Arrays cross the boundary as plain host pointers;
+device memory stays inside the library's handles.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libsbgpu.so")
+
+SBG_OK = 0
+SBG_ERR_INVALID = -1
+SBG_ERR_HIP = -2
+SBG_ERR_OOM = -3
+SBG_ERR_OVERFLOW = -4
+SBG_ERR_NO_DEVICE = -5
+SBG_ERR_STATE = -6
+
+FP64 = 0
+FP32_FILTERED = 1
+
+STAGES = ["records", "sort", "bvh", "traverse", "pair_sort", "tri_tri", "compact", "classify"]
+
+# every symbol include/sbgpu.h declares
+EXPORTS = [
+    "sbg_last_error", "sbg_device_count", "sbg_context_create", "sbg_context_destroy", "sbg_context_synchronize",
+    "sbg_context_stream", "sbg_mesh_create", "sbg_mesh_prepare", "sbg_mesh_destroy", "sbg_mesh_counts",
+    "sbg_mesh_normals", "sbg_mesh_boxes", "sbg_mesh_bvh", "sbg_find_pairs", "sbg_pairs_create", "sbg_pairs_count",
+    "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
+    "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_context_set_timing",
+    "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_calibrate_copy",
+]
+
+
+class SbgError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sbgpu error {code}: {msg}")
+        self.code = code
+
+
+class TriTriStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("tested", "exit_plane1", "exit_plane2", "exit_interval", "hits",
+                                          "coplanar", "fallback_pairs", "filtered_pairs")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+_SZ = C.c_size_t
+_PSZ = C.POINTER(C.c_size_t)
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_U32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_F = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads libsbgpu.so; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                           "or `python solidboolean_amd/build.py`")
+    L = C.CDLL(path)
+    L.sbg_last_error.restype = C.c_char_p
+    L.sbg_last_error.argtypes = []
+    sig = {
+        "sbg_device_count": [C.POINTER(C.c_int)],
+        "sbg_context_create": [C.c_int, _PP],
+        "sbg_context_destroy": [_P],
+        "sbg_context_synchronize": [_P],
+        "sbg_context_stream": [_P, _PP],
+        "sbg_mesh_create": [_P, _D, _SZ, _U32, _SZ, _PP],
+        "sbg_mesh_prepare": [_P],
+        "sbg_mesh_destroy": [_P],
+        "sbg_mesh_counts": [_P, _PSZ, _PSZ],
+        "sbg_mesh_normals": [_P, _D],
+        "sbg_mesh_boxes": [_P, _D],
+        "sbg_mesh_bvh": [_P, _P, C.POINTER(C.c_uint32), _F],
+        "sbg_find_pairs": [_P, _P, _PP],
+        "sbg_pairs_create": [_P, _U32, _SZ, _PP],
+        "sbg_pairs_count": [_P, _PSZ],
+        "sbg_pairs_copy": [_P, _U32],
+        "sbg_pairs_destroy": [_P],
+        "sbg_tri_tri": [_P, _P, _P, C.c_int, C.POINTER(TriTriStats)],
+        "sbg_pairs_results": [_P, _P, _P],
+        "sbg_pairs_hits": [_P, _PSZ, _P, _P],
+        "sbg_tri_tri_batch": [_P, _D, _D, _SZ, _U8, _D],
+        "sbg_classify_points": [_P, _D, _SZ, _P, _U8],
+        "sbg_classify_triangles": [_P, _P, _P, C.c_int, _U8],
+        "sbg_context_set_timing": [_P, C.c_int],
+        "sbg_context_stage_ms": [_P, _D, C.c_int],
+        "sbg_bench_tri_tri": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)],
+        "sbg_calibrate_copy": [_P, _SZ, C.c_int, C.POINTER(C.c_double)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    _lib = L
+    return L
+
+
+def _check(code: int):
+    if code != SBG_OK:
+        raise SbgError(code, _lib.sbg_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def device_count() -> int:
+    L = load()
+    n = C.c_int(0)
+    r = L.sbg_device_count(C.byref(n))
+    return n.value if r == SBG_OK else 0
+
+
+class Context:
+    """One device + one HIP stream."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        h = C.c_void_p()
+        _check(self.L.sbg_context_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sbg_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(self.L.sbg_context_synchronize(self.h))
+
+    def set_timing(self, on: bool):
+        _check(self.L.sbg_context_set_timing(self.h, 1 if on else 0))
+
+    def stage_ms(self, reset: bool = True) -> dict:
+        out = np.zeros(len(STAGES), np.float64)
+        _check(self.L.sbg_context_stage_ms(self.h, out, 1 if reset else 0))
+        return dict(zip(STAGES, out.tolist()))
+
+    def calibrate_copy(self, nbytes: int, iters: int) -> float:
+        ms = C.c_double(0)
+        _check(self.L.sbg_calibrate_copy(self.h, nbytes, iters, C.byref(ms)))
+        return ms.value
+
+    def mesh(self, vertices, triangles, prepare: bool = True) -> "Mesh":
+        m = Mesh(self, vertices, triangles)
+        if prepare:
+            m.prepare()
+        return m
+
+    def pairs(self, ab) -> "Pairs":
+        ab = np.ascontiguousarray(ab, np.uint32).reshape(-1, 2)
+        h = C.c_void_p()
+        _check(self.L.sbg_pairs_create(self.h, ab.reshape(-1), ab.shape[0], C.byref(h)))
+        return Pairs(self, h)
+
+    def tri_tri_batch(self, a9, b9):
+        a9 = np.ascontiguousarray(a9, np.float64).reshape(-1, 9)
+        b9 = np.ascontiguousarray(b9, np.float64).reshape(-1, 9)
+        n = a9.shape[0]
+        assert b9.shape[0] == n
+        status = np.zeros(max(n, 1), np.uint8)
+        seg = np.zeros((max(n, 1), 6), np.float64)
+        _check(self.L.sbg_tri_tri_batch(self.h, a9.reshape(-1) if n else np.zeros(9), b9.reshape(-1) if n else np.zeros(9),
+                                        n, status, seg.reshape(-1)))
+        return status[:n], seg[:n]
+
+
+class Mesh:
+    def __init__(self, ctx: Context, vertices, triangles):
+        self.ctx = ctx
+        self.L = ctx.L
+        v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+        t = np.asarray(triangles).reshape(-1, 3)
+        if t.size and (t.min() < 0 or t.max() >= 2 ** 32):
+            raise ValueError("triangle index out of uint32 range")
+        t = np.ascontiguousarray(t, np.uint32)
+        self.nv, self.nt = v.shape[0], t.shape[0]
+        h = C.c_void_p()
+        _check(self.L.sbg_mesh_create(ctx.h, v.reshape(-1) if v.size else np.zeros(3), self.nv,
+                                      t.reshape(-1) if t.size else np.zeros(3, np.uint32), self.nt, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sbg_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare(self):
+        _check(self.L.sbg_mesh_prepare(self.h))
+
+    def normals(self) -> np.ndarray:
+        out = np.zeros((max(self.nt, 1), 3), np.float64)
+        _check(self.L.sbg_mesh_normals(self.h, out.reshape(-1)))
+        return out[: self.nt]
+
+    def boxes(self) -> np.ndarray:
+        out = np.zeros((max(self.nt, 1), 6), np.float64)
+        _check(self.L.sbg_mesh_boxes(self.h, out.reshape(-1)))
+        return out[: self.nt]
+
+    def bvh(self):
+        """(nodes as a structured array, root ref, root box)"""
+        dt = np.dtype([("box0", np.float32, 6), ("box1", np.float32, 6), ("c0", np.uint32), ("c1", np.uint32),
+                       ("parent", np.int32), ("pad", np.int32)])
+        nodes = np.zeros(max(self.nt - 1, 1), dt)
+        root = C.c_uint32(0)
+        rb = np.zeros(6, np.float32)
+        _check(self.L.sbg_mesh_bvh(self.h, nodes.ctypes.data_as(C.c_void_p), C.byref(root), rb))
+        return nodes[: max(self.nt - 1, 0)], root.value, rb
+
+    def find_pairs(self, other: "Mesh") -> "Pairs":
+        h = C.c_void_p()
+        _check(self.L.sbg_find_pairs(self.h, other.h, C.byref(h)))
+        return Pairs(self.ctx, h)
+
+    def classify_points(self, pts):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        n = pts.shape[0]
+        axis = np.zeros((max(n, 1), 3), np.uint8)
+        inside = np.zeros(max(n, 1), np.uint8)
+        _check(self.L.sbg_classify_points(self.h, pts.reshape(-1) if n else np.zeros(3), n, _ptr(axis), inside))
+        return axis[:n], inside[:n]
+
+    def classify_triangles(self, other: "Mesh", pairs: "Pairs", side: int) -> np.ndarray:
+        out = np.zeros(max(self.nt, 1), np.uint8)
+        _check(self.L.sbg_classify_triangles(self.h, other.h, pairs.h, side, out))
+        return out[: self.nt]
+
+
+class Pairs:
+    def __init__(self, ctx: Context, h):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sbg_pairs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        n = C.c_size_t(0)
+        _check(self.L.sbg_pairs_count(self.h, C.byref(n)))
+        return n.value
+
+    def array(self) -> np.ndarray:
+        n = len(self)
+        out = np.zeros((max(n, 1), 2), np.uint32)
+        _check(self.L.sbg_pairs_copy(self.h, out.reshape(-1)))
+        return out[:n]
+
+    def tri_tri(self, a: Mesh, b: Mesh, precision: int = FP64, stats: bool = True):
+        st = TriTriStats()
+        _check(self.L.sbg_tri_tri(a.h, b.h, self.h, precision, C.byref(st) if stats else None))
+        return st.as_dict() if stats else None
+
+    def results(self):
+        n = len(self)
+        status = np.zeros(max(n, 1), np.uint8)
+        seg = np.zeros((max(n, 1), 6), np.float64)
+        _check(self.L.sbg_pairs_results(self.h, _ptr(status), _ptr(seg)))
+        return status[:n], seg[:n]
+
+    def hits(self):
+        nh = C.c_size_t(0)
+        _check(self.L.sbg_pairs_hits(self.h, C.byref(nh), None, None))
+        n = nh.value
+        ab = np.zeros((max(n, 1), 2), np.uint32)
+        seg = np.zeros((max(n, 1), 6), np.float64)
+        _check(self.L.sbg_pairs_hits(self.h, C.byref(nh), _ptr(ab), _ptr(seg)))
+        return ab[:n], seg[:n]
+
+    def bench(self, a: Mesh, b: Mesh, precision: int, iters: int) -> float:
+        ms = C.c_double(0)
+        _check(self.L.sbg_bench_tri_tri(a.h, b.h, self.h, precision, iters, C.byref(ms)))
+        return ms.value
