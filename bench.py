@@ -191,7 +191,14 @@ def main():
         P, _, _ = step()
         ms = P.bench(A, B, N.FP32_FILTERED, args.gd_only)
         ms64 = P.bench(A, B, N.FP64, args.gd_only)
-        log(f"gd-only: {len(P)} pairs, filtered {ms * 1e3:.2f} us, fp64 {ms64 * 1e3:.2f} us per launch")
+        reps = max(1, (1 << 24) // max(1, len(P)))
+        Psat = ctx.pairs(np.tile(P.array(), (reps, 1)))
+        sat = Psat.bench(A, B, N.FP32_FILTERED, 5)
+        sat64 = Psat.bench(A, B, N.FP64, 5)
+        cms = ctx.calibrate_copy(1 << 30, 5)
+        log(f"gd-only: {len(P)} pairs, filtered {ms * 1e3:.2f} us, fp64 {ms64 * 1e3:.2f} us per launch; "
+            f"saturated {len(Psat)} pairs: filtered {sat:.3f} ms, fp64 {sat64:.3f} ms; copy 1 GiB {cms:.3f} ms")
+        del Psat, P
         return
 
     for i in range(args.warmup):

@@ -27,6 +27,9 @@
 #undef private
 #include "tri_tri_intersect.h"
 
+#include <array>
+#include "earcut.hpp"
+
 #define TINYOBJLOADER_IMPLEMENTATION
 #include "tiny_obj_loader.h"
 
@@ -359,6 +362,21 @@ size_t sbref_boolean_new_triangles(void *p, int64_t *out, size_t cap)
         for (int k = 0; k < 3; ++k)
             out[3 * i + k] = (int64_t)t[i][k];
     return t.size();
+}
+
+// The vendored earcut (thirdparty/earcut.hpp) on rings given as xy pairs:
+// ring k has sizes[k] points.  Returns the index count; writes <= cap indices.
+size_t sbref_earcut(const double *xy, const int64_t *sizes, size_t nrings, int64_t *out, size_t cap)
+{
+    std::vector<std::vector<std::array<double, 2>>> rings(nrings);
+    size_t k = 0;
+    for (size_t r = 0; r < nrings; ++r)
+        for (int64_t i = 0; i < sizes[r]; ++i, ++k)
+            rings[r].push_back({xy[2 * k], xy[2 * k + 1]});
+    std::vector<size_t> idx = mapbox::earcut<size_t>(rings);
+    for (size_t i = 0; i < idx.size() && i < cap; ++i)
+        out[i] = (int64_t)idx[i];
+    return idx.size();
 }
 
 // OBJ loading exactly as the reference driver does (test/main.cpp:32-71, tinyobj

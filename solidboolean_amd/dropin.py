@@ -1,0 +1,157 @@
+"""Python view of the drop-in C++ API (libsolidboolean.so): SolidMesh and
+SolidBoolean with the reference's calling pattern
+
+    a = SolidMesh(vertices, triangles); a.prepare()      # GPU
+    b = SolidMesh(...); b.prepare()
+    sb = SolidBoolean(a, b); ok = sb.combine()
+    sb.fetch_union(), sb.fetch_diff(), sb.fetch_intersect(), sb.result_vertices()
+
+Raises if the native libraries are missing or no GPU is usable.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import native
+
+HOST_LIB = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libsolidboolean.so")
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_P = C.c_void_p
+
+_lib = None
+
+STAGES = ["searchPotentialIntersectedPairs", "processPotentialIntersectedPairs", "addUnintersectedTriangles",
+          "reTriangulate", "buildPolygonsFromEdges", "buildFaceGroups", "decideGroupSide"]
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    native.load()  # libsbgpu.so first (the host library links it)
+    if not os.path.exists(HOST_LIB):
+        raise RuntimeError(f"{HOST_LIB} is not built")
+    L = C.CDLL(HOST_LIB)
+    L.sbh_last_error.restype = C.c_char_p
+    L.sbh_mesh_new.restype = _P
+    L.sbh_mesh_new.argtypes = [_D, C.c_size_t, _I, C.c_size_t]
+    L.sbh_mesh_free.argtypes = [_P]
+    L.sbh_boolean_new.restype = C.c_int
+    L.sbh_boolean_new.argtypes = [_P, _P, C.POINTER(C.c_void_p)]
+    L.sbh_boolean_free.argtypes = [_P]
+    L.sbh_boolean_vertices.restype = C.c_size_t
+    L.sbh_boolean_vertices.argtypes = [_P, _D, C.c_size_t]
+    L.sbh_boolean_fetch.restype = C.c_size_t
+    L.sbh_boolean_fetch.argtypes = [_P, C.c_int, _I, C.c_size_t]
+    L.sbh_boolean_triangle_sides.restype = C.c_size_t
+    L.sbh_boolean_triangle_sides.argtypes = [_P, C.c_int, _U8, C.c_size_t]
+    L.sbh_boolean_stage_seconds.argtypes = [_P, _D]
+    L.sbh_earclip.restype = C.c_size_t
+    L.sbh_earclip.argtypes = [_D, _I, C.c_size_t, _I, C.c_size_t]
+    _lib = L
+    return L
+
+
+def earclip(rings) -> np.ndarray:
+    """The host triangulator on rings (outer first) given as (k, 2) arrays."""
+    L = load() if _lib is None else _lib
+    xy = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float64).reshape(-1, 2) for r in rings]))
+    sizes = np.array([len(r) for r in rings], np.int64)
+    n = L.sbh_earclip(xy.reshape(-1), sizes, len(rings), np.zeros(1, np.int64), 0)
+    out = np.zeros(max(n, 1), np.int64)
+    L.sbh_earclip(xy.reshape(-1), sizes, len(rings), out, n)
+    return out[:n]
+
+
+def load_earclip_only():
+    """Loads libsolidboolean.so without touching the GPU library's device
+    functions (the triangulator is pure host code)."""
+    global _lib
+    if _lib is None:
+        L = C.CDLL(HOST_LIB)
+        L.sbh_earclip.restype = C.c_size_t
+        L.sbh_earclip.argtypes = [_D, _I, C.c_size_t, _I, C.c_size_t]
+        return L
+    return _lib
+
+
+class SolidMesh:
+    def __init__(self, vertices, triangles):
+        self.L = load()
+        self.vertices = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+        self.triangles = np.ascontiguousarray(triangles, np.int64).reshape(-1, 3)
+        self.h = None
+
+    def prepare(self):
+        v = self.vertices.reshape(-1) if self.vertices.size else np.zeros(3)
+        t = self.triangles.reshape(-1) if self.triangles.size else np.zeros(3, np.int64)
+        self.h = self.L.sbh_mesh_new(v, self.vertices.shape[0], t, self.triangles.shape[0])
+        if not self.h:
+            raise RuntimeError(self.L.sbh_last_error().decode())
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.sbh_mesh_free(self.h)
+        except Exception:
+            pass
+
+
+class SolidBoolean:
+    def __init__(self, a: SolidMesh, b: SolidMesh):
+        self.L = load()
+        self.a, self.b = a, b
+        self.h = None
+
+    def combine(self) -> bool:
+        h = C.c_void_p()
+        r = self.L.sbh_boolean_new(self.a.h, self.b.h, C.byref(h))
+        self.h = h
+        if r < 0:
+            raise RuntimeError(self.L.sbh_last_error().decode())
+        return r == 1
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.sbh_boolean_free(self.h)
+        except Exception:
+            pass
+
+    def result_vertices(self) -> np.ndarray:
+        n = self.L.sbh_boolean_vertices(self.h, np.zeros(3), 0)
+        out = np.zeros((max(n, 1), 3))
+        self.L.sbh_boolean_vertices(self.h, out.reshape(-1), n)
+        return out[:n]
+
+    def _fetch(self, which):
+        n = self.L.sbh_boolean_fetch(self.h, which, np.zeros(3, np.int64), 0)
+        out = np.zeros((max(n, 1), 3), np.int64)
+        self.L.sbh_boolean_fetch(self.h, which, out.reshape(-1), n)
+        return out[:n]
+
+    def fetch_union(self):
+        return self._fetch(0)
+
+    def fetch_diff(self):
+        return self._fetch(1)
+
+    def fetch_intersect(self):
+        return self._fetch(2)
+
+    def triangle_sides(self, which: int) -> np.ndarray:
+        n = self.L.sbh_boolean_triangle_sides(self.h, which, np.zeros(1, np.uint8), 0)
+        out = np.zeros(max(n, 1), np.uint8)
+        self.L.sbh_boolean_triangle_sides(self.h, which, out, n)
+        return out[:n]
+
+    def stage_seconds(self) -> dict:
+        out = np.zeros(7)
+        self.L.sbh_boolean_stage_seconds(self.h, out)
+        return dict(zip(STAGES, out.tolist()))

@@ -1,0 +1,140 @@
+// C entry points of the host library (libsolidboolean.so) for callers that
+// cannot use the C++ classes directly (ctypes tests, other languages).
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <vector>
+
+#include "earclip.h"
+#include "solidboolean.h"
+#include "solidmesh.h"
+
+namespace {
+
+thread_local std::string g_hostErr;
+
+struct HostMesh {
+    std::vector<Vector3> vertices;
+    std::vector<std::vector<size_t>> triangles;
+    SolidMesh mesh;
+};
+
+struct HostBoolean {
+    SolidBoolean sb;
+    bool ok = false;
+    std::vector<std::vector<size_t>> fetched[3];
+    HostBoolean(const SolidMesh *a, const SolidMesh *b) : sb(a, b) {}
+};
+
+double secs(std::chrono::high_resolution_clock::time_point a, std::chrono::high_resolution_clock::time_point b)
+{
+    return std::chrono::duration<double>(b - a).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sbh_last_error(void) { return g_hostErr.c_str(); }
+
+// SolidMesh over copies of the given arrays, prepared on the GPU; NULL on error.
+void *sbh_mesh_new(const double *xyz, size_t nv, const int64_t *tri, size_t nt)
+{
+    try {
+        HostMesh *m = new HostMesh;
+        m->vertices.resize(nv);
+        for (size_t i = 0; i < nv; ++i)
+            m->vertices[i] = Vector3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        m->triangles.resize(nt);
+        for (size_t i = 0; i < nt; ++i)
+            m->triangles[i] = {(size_t)tri[3 * i], (size_t)tri[3 * i + 1], (size_t)tri[3 * i + 2]};
+        m->mesh.setVertices(&m->vertices);
+        m->mesh.setTriangles(&m->triangles);
+        m->mesh.prepare();
+        return m;
+    } catch (const std::exception &e) {
+        g_hostErr = e.what();
+        return nullptr;
+    }
+}
+
+void sbh_mesh_free(void *p) { delete (HostMesh *)p; }
+
+// SolidBoolean::combine() + fetch*; returns 1 / 0 as combine(), -1 on a GPU error.
+int sbh_boolean_new(void *a, void *b, void **out)
+{
+    HostBoolean *h = new HostBoolean(&((HostMesh *)a)->mesh, &((HostMesh *)b)->mesh);
+    *out = h;
+    try {
+        h->ok = h->sb.combine();
+        if (h->ok) {
+            h->sb.fetchUnion(h->fetched[0]);
+            h->sb.fetchDiff(h->fetched[1]);
+            h->sb.fetchIntersect(h->fetched[2]);
+        }
+        return h->ok ? 1 : 0;
+    } catch (const std::exception &e) {
+        g_hostErr = e.what();
+        return -1;
+    }
+}
+
+void sbh_boolean_free(void *p) { delete (HostBoolean *)p; }
+
+size_t sbh_boolean_vertices(void *p, double *out, size_t cap)
+{
+    const auto &v = ((HostBoolean *)p)->sb.resultVertices();
+    for (size_t i = 0; i < v.size() && i < cap; ++i)
+        for (int k = 0; k < 3; ++k)
+            out[3 * i + k] = v[i][k];
+    return v.size();
+}
+
+size_t sbh_boolean_fetch(void *p, int which, int64_t *out, size_t cap)
+{
+    const auto &f = ((HostBoolean *)p)->fetched[which];
+    for (size_t i = 0; i < f.size() && i < cap; ++i)
+        for (int k = 0; k < 3; ++k)
+            out[3 * i + k] = (int64_t)f[i][k];
+    return f.size();
+}
+
+size_t sbh_boolean_triangle_sides(void *p, int which, uint8_t *out, size_t cap)
+{
+    const auto s = ((HostBoolean *)p)->sb.triangleSides(which);
+    for (size_t i = 0; i < s.size() && i < cap; ++i)
+        out[i] = s[i];
+    return s.size();
+}
+
+// stage durations (s) in the order of the reference's bench time points
+void sbh_boolean_stage_seconds(void *p, double *out)
+{
+    const SolidBoolean &s = ((HostBoolean *)p)->sb;
+    out[0] = secs(s.benchBegin_searchPotentialIntersectedPairs, s.benchEnd_searchPotentialIntersectedPairs);
+    out[1] = secs(s.benchBegin_processPotentialIntersectedPairs, s.benchEnd_processPotentialIntersectedPairs);
+    out[2] = secs(s.benchBegin_addUnintersectedTriangles, s.benchEnd_addUnintersectedTriangles);
+    out[3] = secs(s.benchBegin_reTriangulate, s.benchEnd_reTriangulate);
+    out[4] = secs(s.benchBegin_buildPolygonsFromEdges, s.benchEnd_buildPolygonsFromEdges);
+    out[5] = secs(s.benchBegin_buildFaceGroups, s.benchEnd_buildFaceGroups);
+    out[6] = secs(s.benchBegin_decideGroupSide, s.benchEnd_decideGroupSide);
+}
+
+// Triangulates rings (outer first) given as concatenated xy pairs with
+// sizes[k] points in ring k.  Returns the index count; writes <= cap indices.
+size_t sbh_earclip(const double *xy, const int64_t *sizes, size_t nrings, int64_t *out, size_t cap)
+{
+    std::vector<sbhost::Ring> rings(nrings);
+    size_t k = 0;
+    for (size_t r = 0; r < nrings; ++r)
+        for (int64_t i = 0; i < sizes[r]; ++i, ++k)
+            rings[r].push_back({xy[2 * k], xy[2 * k + 1]});
+    std::vector<size_t> idx = sbhost::earclip(rings);
+    for (size_t i = 0; i < idx.size() && i < cap; ++i)
+        out[i] = (int64_t)idx[i];
+    return idx.size();
+}
+
+}  // extern "C"

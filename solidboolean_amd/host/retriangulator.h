@@ -1,0 +1,44 @@
+// Re-triangulation of one cut triangle (reference: src/retriangulator.h,
+// src/retriangulator.cpp:29-337).  The triangle and the intersection points
+// inside it are projected to the triangle's plane; the cut graph is split into
+// open polylines (ending on the triangle's edges) and closed inner loops; the
+// polylines cut the triangle into polygons, inner loops nest as holes, and
+// every polygon is ear-clipped.  Graph walks use ordered containers so the
+// result does not depend on hash-table iteration order.
+#pragma once
+#include <cstddef>
+#include <map>
+#include <set>
+#include <vector>
+
+#include "vector2.h"
+#include "vector3.h"
+
+class ReTriangulator {
+public:
+    // points: the triangle's three corners; normal: its unit normal
+    ReTriangulator(const std::vector<Vector3> &points, const Vector3 &normal);
+    // points: the cut points (index 3 + k in neighborMapFrom3)
+    void setEdges(const std::vector<Vector3> &points, const std::map<size_t, std::set<size_t>> *neighborMapFrom3);
+    bool reTriangulate();
+    const std::vector<std::vector<size_t>> &polygons() const { return m_polygons; }
+    const std::vector<std::vector<size_t>> &triangles() const { return m_triangles; }
+
+private:
+    Vector3 m_projectAxis, m_projectOrigin, m_projectNormal;
+    std::vector<Vector2> m_points;
+    const std::map<size_t, std::set<size_t>> *m_neighborMap = nullptr;
+    std::vector<std::vector<size_t>> m_polylines;
+    std::vector<std::vector<size_t>> m_innerPolygons;
+    std::vector<std::vector<size_t>> m_polygons;
+    std::map<size_t, std::set<size_t>> m_innerParents;
+    std::map<size_t, std::set<size_t>> m_innerChildren;
+    std::map<size_t, std::vector<size_t>> m_polygonHoles;
+    std::vector<std::vector<size_t>> m_triangles;
+
+    void lookupPolylines();
+    int attachPointToTriangleEdge(const Vector2 &p) const;
+    bool buildPolygons();
+    void buildPolygonHierarchy();
+    void triangulate();
+};

@@ -1,0 +1,19 @@
+// Diagnostics and the shared GPU context of the host library.
+#pragma once
+#include "sbgpu.h"
+
+namespace sbhost {
+
+// Prints a diagnostic line to stderr when SBHOST_VERBOSE is set (the
+// reference prints the same conditions to stdout unconditionally).
+void diag(const char *msg);
+
+// The process-wide sbg_context used by SolidMesh / SolidBoolean: device
+// SBG_DEVICE (default 0), created on first use.  Throws std::runtime_error
+// when no GPU or no libsbgpu is usable: this library has no CPU fallback.
+sbg_context gpu();
+
+// Throws std::runtime_error with the library's message when rc != SBG_OK.
+void check(int rc, const char *what);
+
+}  // namespace sbhost

@@ -1,0 +1,464 @@
+// SolidBoolean; see solidboolean.h.  Stage names and their order follow
+// src/solidboolean.cpp:288-480.
+#include "solidboolean.h"
+
+#include <deque>
+#include <stdexcept>
+
+#include "retriangulator.h"
+#include "sbhost_log.h"
+
+using Clock = std::chrono::high_resolution_clock;
+
+// ---------------------------------------------------------------------------
+// HalfEdgeMap: linear probing, power-of-two capacity, load <= 1/2
+// ---------------------------------------------------------------------------
+static constexpr uint64_t kEmpty = ~0ull;
+
+static inline size_t slot_of(uint64_t key, size_t mask)
+{
+    return (size_t)((key * 0x9E3779B97F4A7C15ull) >> 17) & mask;
+}
+
+void SolidBoolean::HalfEdgeMap::reserve(size_t n)
+{
+    size_t cap = 16;
+    while (cap < 2 * n)
+        cap <<= 1;
+    if (cap <= m_keys.size())
+        return;
+    std::vector<uint64_t> keys(cap, kEmpty);
+    std::vector<size_t> values(cap);
+    const size_t mask = cap - 1;
+    for (size_t i = 0; i < m_keys.size(); ++i)
+        if (m_keys[i] != kEmpty) {
+            size_t s = slot_of(m_keys[i], mask);
+            while (keys[s] != kEmpty)
+                s = (s + 1) & mask;
+            keys[s] = m_keys[i];
+            values[s] = m_values[i];
+        }
+    m_keys.swap(keys);
+    m_values.swap(values);
+    m_mask = mask;
+}
+
+void SolidBoolean::HalfEdgeMap::grow() { reserve(m_keys.empty() ? 8 : m_keys.size()); }
+
+bool SolidBoolean::HalfEdgeMap::insert(uint64_t key, size_t value)
+{
+    if (2 * (m_count + 1) > m_keys.size())
+        grow();
+    size_t s = slot_of(key, m_mask);
+    while (m_keys[s] != kEmpty) {
+        if (m_keys[s] == key)
+            return false;
+        s = (s + 1) & m_mask;
+    }
+    m_keys[s] = key;
+    m_values[s] = value;
+    ++m_count;
+    return true;
+}
+
+const size_t *SolidBoolean::HalfEdgeMap::find(uint64_t key) const
+{
+    if (m_keys.empty())
+        return nullptr;
+    size_t s = slot_of(key, m_mask);
+    while (m_keys[s] != kEmpty) {
+        if (m_keys[s] == key)
+            return &m_values[s];
+        s = (s + 1) & m_mask;
+    }
+    return nullptr;
+}
+
+// ---------------------------------------------------------------------------
+
+SolidBoolean::SolidBoolean(const SolidMesh *firstMesh, const SolidMesh *secondMesh)
+    : m_firstMesh(firstMesh), m_secondMesh(secondMesh)
+{
+}
+
+SolidBoolean::~SolidBoolean() = default;
+
+size_t SolidBoolean::addNewPoint(const Vector3 &position)
+{
+    auto r = m_newPositionMap.insert({PositionKey(position), m_newVertices.size()});
+    if (r.second)
+        m_newVertices.push_back(position);
+    return r.first->second;
+}
+
+bool SolidBoolean::addUnintersectedTriangles(const SolidMesh *mesh, const std::vector<uint8_t> &used,
+                                             HalfEdgeMap *halfEdges)
+{
+    // (reference: src/solidboolean.cpp:250-286; a repeated half-edge stops the copy)
+    const size_t oldVertexCount = m_newVertices.size();
+    const auto &vertices = *mesh->vertices();
+    m_newVertices.insert(m_newVertices.end(), vertices.begin(), vertices.end());
+    const auto &tris = *mesh->triangles();
+    halfEdges->reserve(3 * tris.size());
+    m_newTriangles.reserve(m_newTriangles.size() + tris.size());
+    for (size_t i = 0; i < tris.size(); ++i) {
+        if (used[i])
+            continue;
+        const size_t index = m_newTriangles.size();
+        const size_t a = tris[i][0] + oldVertexCount, b = tris[i][1] + oldVertexCount, c = tris[i][2] + oldVertexCount;
+        m_newTriangles.push_back({a, b, c});
+        if (!halfEdges->insert(halfEdgeKey(a, b), index) || !halfEdges->insert(halfEdgeKey(b, c), index) ||
+            !halfEdges->insert(halfEdgeKey(c, a), index)) {
+            sbhost::diag("Found repeated halfedge");
+            return false;
+        }
+    }
+    return true;
+}
+
+bool SolidBoolean::reTriangulate(const ContextMap &contexts, const SolidMesh *mesh, size_t startOldVertex,
+                                 EdgeMap &edges, HalfEdgeMap &halfEdges)
+{
+    // (reference: src/solidboolean.cpp:352-407)
+    const auto &tris = *mesh->triangles();
+    const auto &verts = *mesh->vertices();
+    const std::vector<Vector3> *normals = contexts.empty() ? nullptr : mesh->triangleNormals();
+    for (const auto &kv : contexts) {
+        const auto &t = tris[kv.first];
+        ReTriangulator rt({verts[t[0]], verts[t[1]], verts[t[2]]}, (*normals)[kv.first]);
+        rt.setEdges(kv.second.points, &kv.second.neighborMap);
+        if (!rt.reTriangulate()) {
+            sbhost::diag("Retriangle failed");
+            return false;
+        }
+        std::vector<size_t> newIndices;
+        newIndices.reserve(3 + kv.second.points.size());
+        newIndices.push_back(startOldVertex + t[0]);
+        newIndices.push_back(startOldVertex + t[1]);
+        newIndices.push_back(startOldVertex + t[2]);
+        for (const auto &p : kv.second.points)
+            newIndices.push_back(addNewPoint(p));
+        for (const auto &nt : rt.triangles()) {
+            const size_t index = m_newTriangles.size();
+            const size_t a = newIndices[nt[0]], b = newIndices[nt[1]], c = newIndices[nt[2]];
+            m_newTriangles.push_back({a, b, c});
+            if (!halfEdges.insert(halfEdgeKey(a, b), index))
+                sbhost::diag("Found repeated halfedge");
+            if (!halfEdges.insert(halfEdgeKey(b, c), index))
+                sbhost::diag("Found repeated halfedge");
+            if (!halfEdges.insert(halfEdgeKey(c, a), index))
+                sbhost::diag("Found repeated halfedge");
+        }
+        for (const auto &nb : kv.second.neighborMap) {
+            const size_t from = newIndices[nb.first];
+            for (size_t k : nb.second) {
+                const size_t to = newIndices[k];
+                edges[from].insert(to);
+                edges[to].insert(from);
+            }
+        }
+    }
+    return true;
+}
+
+bool SolidBoolean::buildPolygonsFromEdges(const EdgeMap &edges, std::vector<std::vector<size_t>> &polygons)
+{
+    // (reference: src/solidboolean.cpp:124-165)
+    std::set<size_t> visited;
+    for (const auto &kv : edges) {
+        const size_t start = kv.first;
+        if (visited.count(start))
+            continue;
+        std::vector<size_t> loop;
+        size_t cur = start;
+        for (;;) {
+            visited.insert(cur);
+            loop.push_back(cur);
+            auto it = edges.find(cur);
+            if (it == edges.end())
+                break;
+            bool stepped = false;
+            for (size_t nb : it->second)
+                if (!visited.count(nb)) {
+                    cur = nb;
+                    stepped = true;
+                    break;
+                }
+            if (!stepped)
+                break;
+        }
+        if (loop.size() <= 2) {
+            sbhost::diag("buildPolygonsFromEdges failed, too short");
+            return false;
+        }
+        auto last = edges.find(loop.back());
+        if (!last->second.count(start)) {
+            sbhost::diag("buildPolygonsFromEdges failed, could not form a ring");
+            return false;
+        }
+        polygons.push_back(loop);
+    }
+    return true;
+}
+
+void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &intersections,
+                                   const HalfEdgeMap &halfEdges, size_t remainingStart, size_t remainingCount,
+                                   std::vector<std::vector<size_t>> &groups)
+{
+    // (reference: src/solidboolean.cpp:167-239) seeds on both sides of every
+    // loop edge, flood fill across half-edges that are not loop edges
+    HalfEdgeMap groupOf;
+    size_t groupIndex = 0;
+    std::deque<std::pair<size_t, size_t>> queue;
+    for (const auto &loop : intersections) {
+        for (size_t i = 0; i < loop.size(); ++i) {
+            const size_t j = (i + 1) % loop.size();
+            const uint64_t fwd = halfEdgeKey(loop[i], loop[j]);
+            groupOf.insert(fwd, groupIndex);
+            if (const size_t *t = halfEdges.find(fwd))
+                queue.push_back({*t, groupIndex});
+            const uint64_t rev = halfEdgeKey(loop[j], loop[i]);
+            groupOf.insert(rev, groupIndex + 1);
+            if (const size_t *t = halfEdges.find(rev))
+                queue.push_back({*t, groupIndex + 1});
+        }
+        groupIndex += 2;
+    }
+    groups.resize(groupIndex);
+    std::vector<uint8_t> visited(m_newTriangles.size(), 0);
+    auto drain = [&]() {
+        while (!queue.empty()) {
+            const auto tg = queue.front();
+            queue.pop_front();
+            if (visited[tg.first])
+                continue;
+            visited[tg.first] = 1;
+            groups[tg.second].push_back(tg.first);
+            const auto &t = m_newTriangles[tg.first];
+            for (size_t i = 0; i < 3; ++i) {
+                const size_t j = (i + 1) % 3;
+                const uint64_t he = halfEdgeKey(t[i], t[j]);
+                if (groupOf.find(he))
+                    continue;
+                groupOf.insert(he, tg.second);
+                if (const size_t *o = halfEdges.find(halfEdgeKey(t[j], t[i])))
+                    queue.push_back({*o, tg.second});
+            }
+        }
+    };
+    drain();
+    const size_t end = remainingStart + remainingCount;
+    for (size_t t = remainingStart; t < end; ++t) {
+        if (visited[t])
+            continue;
+        groups.push_back({});
+        queue.push_back({t, groupIndex});
+        ++groupIndex;
+        drain();
+    }
+}
+
+void SolidBoolean::decideGroupSide(const std::vector<std::vector<size_t>> &groups, const SolidMesh *mesh,
+                                   std::vector<bool> &groupSides)
+{
+    // (reference: src/solidboolean.cpp:482-510) one representative per group:
+    // the centroid of its first triangle, voted by three rays on the GPU
+    groupSides.assign(groups.size(), false);
+    std::vector<double> pts;
+    std::vector<size_t> which;
+    for (size_t i = 0; i < groups.size(); ++i) {
+        if (groups[i].empty())
+            continue;
+        const auto &t = m_newTriangles[groups[i][0]];
+        const Vector3 c = (m_newVertices[t[0]] + m_newVertices[t[1]] + m_newVertices[t[2]]) / 3.0;
+        pts.push_back(c.x());
+        pts.push_back(c.y());
+        pts.push_back(c.z());
+        which.push_back(i);
+    }
+    if (which.empty())
+        return;
+    std::vector<uint8_t> inside(which.size());
+    sbhost::check(sbg_classify_points(mesh->gpuMesh(), pts.data(), which.size(), nullptr, inside.data()),
+                  "sbg_classify_points");
+    for (size_t k = 0; k < which.size(); ++k)
+        groupSides[which[k]] = inside[k] != 0;
+}
+
+bool SolidBoolean::combine()
+{
+    if (!m_firstMesh->gpuMesh() || !m_secondMesh->gpuMesh())
+        throw std::logic_error("SolidBoolean::combine: both meshes must be prepared");
+    const size_t na = m_firstMesh->triangles()->size();
+    const size_t nb = m_secondMesh->triangles()->size();
+
+    benchBegin_searchPotentialIntersectedPairs = Clock::now();
+    sbg_pairs pairs = nullptr;
+    sbhost::check(sbg_find_pairs(m_firstMesh->gpuMesh(), m_secondMesh->gpuMesh(), &pairs), "sbg_find_pairs");
+    benchEnd_searchPotentialIntersectedPairs = Clock::now();
+
+    benchBegin_processPotentialIntersectedPairs = Clock::now();
+    std::vector<uint32_t> ab;
+    std::vector<double> seg;
+    {
+        int rc = sbg_tri_tri(m_firstMesh->gpuMesh(), m_secondMesh->gpuMesh(), pairs, SBG_PRECISION_FP32_FILTERED,
+                             nullptr);
+        size_t nh = 0;
+        if (rc == SBG_OK)
+            rc = sbg_pairs_hits(pairs, &nh, nullptr, nullptr);
+        if (rc == SBG_OK) {
+            ab.resize(2 * nh);
+            seg.resize(6 * nh);
+            rc = sbg_pairs_hits(pairs, &nh, ab.data(), seg.data());
+        }
+        sbg_pairs_destroy(pairs);
+        sbhost::check(rc, "sbg_tri_tri");
+    }
+    m_firstIntersected.assign(na, 0);
+    m_secondIntersected.assign(nb, 0);
+    ContextMap firstCtx, secondCtx;
+    auto addPoint = [](IntersectedContext &c, const Vector3 &p) {
+        auto r = c.positionMap.insert({PositionKey(p), c.points.size()});
+        if (r.second)
+            c.points.push_back(p);
+        return r.first->second;
+    };
+    auto addSegment = [&](IntersectedContext &c, const Vector3 &s, const Vector3 &t) {
+        const size_t i = 3 + addPoint(c, s);
+        const size_t j = 3 + addPoint(c, t);
+        if (i != j) {
+            c.neighborMap[i].insert(j);
+            c.neighborMap[j].insert(i);
+        }
+    };
+    const size_t nh = ab.size() / 2;
+    for (size_t h = 0; h < nh; ++h) {
+        const size_t a = ab[2 * h], b = ab[2 * h + 1];
+        const Vector3 s(seg[6 * h], seg[6 * h + 1], seg[6 * h + 2]);
+        const Vector3 t(seg[6 * h + 3], seg[6 * h + 4], seg[6 * h + 5]);
+        m_firstIntersected[a] = 1;
+        m_secondIntersected[b] = 1;
+        addSegment(firstCtx[a], s, t);
+        addSegment(secondCtx[b], s, t);
+    }
+    benchEnd_processPotentialIntersectedPairs = Clock::now();
+
+    EdgeMap firstEdges, secondEdges;
+    std::vector<std::vector<size_t>> firstIntersections;
+    HalfEdgeMap firstHalfEdges, secondHalfEdges;
+
+    benchBegin_addUnintersectedTriangles = Clock::now();
+    m_firstRemainingStart = m_newTriangles.size();
+    if (!addUnintersectedTriangles(m_firstMesh, m_firstIntersected, &firstHalfEdges))
+        sbhost::diag("Add first mesh remaining triangles failed");
+    const size_t firstRemainingCount = m_newTriangles.size() - m_firstRemainingStart;
+    m_firstRemainingCount = firstRemainingCount;
+    m_secondRemainingStart = m_newTriangles.size();
+    if (!addUnintersectedTriangles(m_secondMesh, m_secondIntersected, &secondHalfEdges))
+        sbhost::diag("Add second mesh remaining triangles failed");
+    const size_t secondRemainingCount = m_newTriangles.size() - m_secondRemainingStart;
+    m_secondRemainingCount = secondRemainingCount;
+    benchEnd_addUnintersectedTriangles = Clock::now();
+
+    benchBegin_reTriangulate = Clock::now();
+    if (!reTriangulate(firstCtx, m_firstMesh, 0, firstEdges, firstHalfEdges)) {
+        sbhost::diag("Retriangulate first mesh failed");
+        return false;
+    }
+    if (!reTriangulate(secondCtx, m_secondMesh, m_firstMesh->vertices()->size(), secondEdges, secondHalfEdges)) {
+        sbhost::diag("Retriangulate second mesh failed");
+        return false;
+    }
+    benchEnd_reTriangulate = Clock::now();
+
+    benchBegin_buildPolygonsFromEdges = Clock::now();
+    if (!buildPolygonsFromEdges(firstEdges, firstIntersections)) {
+        sbhost::diag("Build polygons from edges failed");
+        return false;
+    }
+    benchEnd_buildPolygonsFromEdges = Clock::now();
+
+    // both meshes are grouped by the first mesh's loops, as the reference does
+    // (src/solidboolean.cpp:451-462)
+    benchBegin_buildFaceGroups = Clock::now();
+    buildFaceGroups(firstIntersections, firstHalfEdges, m_firstRemainingStart, firstRemainingCount,
+                    m_firstTriangleGroups);
+    buildFaceGroups(firstIntersections, secondHalfEdges, m_secondRemainingStart, secondRemainingCount,
+                    m_secondTriangleGroups);
+    benchEnd_buildFaceGroups = Clock::now();
+
+    benchBegin_decideGroupSide = Clock::now();
+    decideGroupSide(m_firstTriangleGroups, m_secondMesh, m_firstGroupSides);
+    decideGroupSide(m_secondTriangleGroups, m_firstMesh, m_secondGroupSides);
+    benchEnd_decideGroupSide = Clock::now();
+    return true;
+}
+
+void SolidBoolean::fetchUnion(std::vector<std::vector<size_t>> &out)
+{
+    for (size_t i = 0; i < m_firstGroupSides.size(); ++i)
+        if (!m_firstGroupSides[i])
+            for (size_t t : m_firstTriangleGroups[i])
+                out.push_back(m_newTriangles[t]);
+    for (size_t i = 0; i < m_secondGroupSides.size(); ++i)
+        if (!m_secondGroupSides[i])
+            for (size_t t : m_secondTriangleGroups[i])
+                out.push_back(m_newTriangles[t]);
+}
+
+void SolidBoolean::fetchDiff(std::vector<std::vector<size_t>> &out)
+{
+    for (size_t i = 0; i < m_firstGroupSides.size(); ++i)
+        if (!m_firstGroupSides[i])
+            for (size_t t : m_firstTriangleGroups[i])
+                out.push_back(m_newTriangles[t]);
+    for (size_t i = 0; i < m_secondGroupSides.size(); ++i)
+        if (m_secondGroupSides[i])
+            for (size_t t : m_secondTriangleGroups[i]) {
+                const auto &tri = m_newTriangles[t];
+                out.push_back({tri[2], tri[1], tri[0]});  // B's inside, reversed (:541-546)
+            }
+}
+
+void SolidBoolean::fetchIntersect(std::vector<std::vector<size_t>> &out)
+{
+    for (size_t i = 0; i < m_firstGroupSides.size(); ++i)
+        if (m_firstGroupSides[i])
+            for (size_t t : m_firstTriangleGroups[i])
+                out.push_back(m_newTriangles[t]);
+    for (size_t i = 0; i < m_secondGroupSides.size(); ++i)
+        if (m_secondGroupSides[i])
+            for (size_t t : m_secondTriangleGroups[i])
+                out.push_back(m_newTriangles[t]);
+}
+
+const std::vector<Vector3> &SolidBoolean::resultVertices() { return m_newVertices; }
+
+std::vector<size_t> SolidBoolean::intersectedFaces(int which) const
+{
+    const auto &f = which == 0 ? m_firstIntersected : m_secondIntersected;
+    std::vector<size_t> out;
+    for (size_t i = 0; i < f.size(); ++i)
+        if (f[i])
+            out.push_back(i);
+    return out;
+}
+
+std::vector<uint8_t> SolidBoolean::triangleSides(int which) const
+{
+    const auto &flags = which == 0 ? m_firstIntersected : m_secondIntersected;
+    const auto &groups = which == 0 ? m_firstTriangleGroups : m_secondTriangleGroups;
+    const auto &sides = which == 0 ? m_firstGroupSides : m_secondGroupSides;
+    const size_t start = which == 0 ? m_firstRemainingStart : m_secondRemainingStart;
+    const size_t count = which == 0 ? m_firstRemainingCount : m_secondRemainingCount;
+    std::vector<size_t> kept;
+    for (size_t i = 0; i < flags.size() && kept.size() < count; ++i)
+        if (!flags[i])
+            kept.push_back(i);
+    std::vector<uint8_t> out(flags.size(), 2);
+    for (size_t g = 0; g < groups.size() && g < sides.size(); ++g)
+        for (size_t t : groups[g])
+            if (t >= start && t < start + kept.size())
+                out[kept[t - start]] = sides[g] ? 1 : 0;
+    return out;
+}

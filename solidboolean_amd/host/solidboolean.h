@@ -1,0 +1,100 @@
+// SolidBoolean: union / difference / intersection of two closed meshes, with
+// the reference's interface (src/solidboolean.h:33-108).
+//
+// combine() runs the data-parallel stages on the GPU through libsbgpu
+// (candidate pairs, Guigue-Devillers segments, ray-parity group sides) and the
+// topological stages on the host with the reference's rules: PositionKey
+// welding of intersection points (first inserted wins), re-triangulation of
+// cut triangles, intersection loops, half-edge flood fill into groups.  Host
+// maps are ordered (or keyed hash tables whose iteration order is never
+// observed), so results do not depend on hash iteration order; candidate pairs
+// are processed in (a, b) order.
+#pragma once
+#include <chrono>
+#include <cstdint>
+#include <map>
+#include <set>
+#include <vector>
+
+#include "positionkey.h"
+#include "solidmesh.h"
+#include "vector3.h"
+
+class SolidBoolean {
+public:
+    SolidBoolean(const SolidMesh *firstMesh, const SolidMesh *secondMesh);
+    ~SolidBoolean();
+    bool combine();
+    void fetchUnion(std::vector<std::vector<size_t>> &resultTriangles);
+    void fetchDiff(std::vector<std::vector<size_t>> &resultTriangles);
+    void fetchIntersect(std::vector<std::vector<size_t>> &resultTriangles);
+    const std::vector<Vector3> &resultVertices();
+
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchBegin_searchPotentialIntersectedPairs;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchEnd_searchPotentialIntersectedPairs;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchBegin_processPotentialIntersectedPairs;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchEnd_processPotentialIntersectedPairs;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchBegin_addUnintersectedTriangles;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchEnd_addUnintersectedTriangles;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchBegin_reTriangulate;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchEnd_reTriangulate;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchBegin_buildPolygonsFromEdges;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchEnd_buildPolygonsFromEdges;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchBegin_buildFaceGroups;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchEnd_buildFaceGroups;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchBegin_decideGroupSide;
+    std::chrono::time_point<std::chrono::high_resolution_clock> benchEnd_decideGroupSide;
+
+    // Inspection (no counterpart in the reference's public API): the
+    // intersected triangles of mesh 0 / 1, and for every original triangle of
+    // mesh 0 / 1 the side of the group it landed in (0 outside, 1 inside) or
+    // 2 when it was intersected.
+    std::vector<size_t> intersectedFaces(int which) const;
+    std::vector<uint8_t> triangleSides(int which) const;
+
+    // open-addressed map from a half-edge key (first << 32 | second) to a triangle
+    class HalfEdgeMap {
+    public:
+        void reserve(size_t n);
+        bool insert(uint64_t key, size_t value);  // false when the key is present
+        const size_t *find(uint64_t key) const;
+
+    private:
+        std::vector<uint64_t> m_keys;
+        std::vector<size_t> m_values;
+        size_t m_count = 0;
+        size_t m_mask = 0;
+        void grow();
+    };
+
+private:
+    struct IntersectedContext {
+        std::vector<Vector3> points;
+        std::map<PositionKey, size_t> positionMap;
+        std::map<size_t, std::set<size_t>> neighborMap;
+    };
+    using ContextMap = std::map<size_t, IntersectedContext>;
+    using EdgeMap = std::map<size_t, std::set<size_t>>;
+
+    const SolidMesh *m_firstMesh;
+    const SolidMesh *m_secondMesh;
+    std::vector<Vector3> m_newVertices;
+    std::vector<std::vector<size_t>> m_newTriangles;
+    std::map<PositionKey, size_t> m_newPositionMap;
+    std::vector<std::vector<size_t>> m_firstTriangleGroups, m_secondTriangleGroups;
+    std::vector<bool> m_firstGroupSides, m_secondGroupSides;
+    std::vector<uint8_t> m_firstIntersected, m_secondIntersected;
+    size_t m_firstRemainingStart = 0, m_secondRemainingStart = 0;
+    size_t m_firstRemainingCount = 0, m_secondRemainingCount = 0;
+
+    static uint64_t halfEdgeKey(size_t first, size_t second) { return ((uint64_t)first << 32) | (uint64_t)second; }
+    size_t addNewPoint(const Vector3 &position);
+    bool addUnintersectedTriangles(const SolidMesh *mesh, const std::vector<uint8_t> &used, HalfEdgeMap *halfEdges);
+    bool reTriangulate(const ContextMap &contexts, const SolidMesh *mesh, size_t startOldVertex, EdgeMap &edges,
+                       HalfEdgeMap &halfEdges);
+    bool buildPolygonsFromEdges(const EdgeMap &edges, std::vector<std::vector<size_t>> &polygons);
+    void buildFaceGroups(const std::vector<std::vector<size_t>> &intersections, const HalfEdgeMap &halfEdges,
+                         size_t remainingStart, size_t remainingCount, std::vector<std::vector<size_t>> &groups);
+    void decideGroupSide(const std::vector<std::vector<size_t>> &groups, const SolidMesh *mesh,
+                         std::vector<bool> &groupSides);
+};

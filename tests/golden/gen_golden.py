@@ -197,6 +197,46 @@ def gd_vectors(L: RefLib) -> dict:
     return dict(a9=A, b9=B, raw=ret, coplanar=cop, status=status, seg=seg)
 
 
+def earcut_vectors(L: RefLib, n: int = 1500) -> dict:
+    """Polygons with holes, collinear and duplicate points (the shapes the
+    retriangulator hands over) and the vendored earcut's index lists."""
+    rng = mg.SplitMix64(2024)
+
+    def star(k, r0, r1, cx=0.0, cy=0.0):
+        ang = np.sort(rng.uniform(k) * 2 * np.pi)
+        rad = r0 + (r1 - r0) * rng.uniform(k)
+        return np.stack([cx + rad * np.cos(ang), cy + rad * np.sin(ang)], 1)
+
+    xy, sizes, counts, idx = [], [], [], []
+    for t in range(n):
+        kind = t % 4
+        k = 3 + int(rng.next_u64(1)[0] % np.uint64(60))
+        if kind == 0:
+            rings = [star(k, 0.5, 1.0)]
+        elif kind == 1:
+            rings = [star(k + 60, 0.7, 1.0), star(5, 0.05, 0.15, -0.3, 0.0), star(4, 0.05, 0.12, 0.3, 0.1)]
+        elif kind == 2:
+            tri = np.array([[0, 0], [1, 0], [0.3, 0.9]])
+            pts = []
+            for i in range(3):
+                a, b = tri[i], tri[(i + 1) % 3]
+                pts.append(a)
+                for s in np.sort(rng.uniform(int(rng.next_u64(1)[0] % np.uint64(4)))):
+                    pts.append(a + s * (b - a))
+            rings = [np.array(pts), star(5, 0.02, 0.06, 0.4, 0.3)]
+        else:
+            g = (rng.next_u64(2 * k) % np.uint64(6)).astype(np.float64).reshape(k, 2)
+            rings = [g]
+        out = L.earcut(rings)
+        for r in rings:
+            xy.append(np.asarray(r, np.float64))
+            sizes.append(len(r))
+        counts.append(len(rings))
+        idx.append(out)
+    return dict(xy=np.concatenate(xy), ring_sizes=np.array(sizes), rings_per_case=np.array(counts),
+                index_counts=np.array([len(i) for i in idx]), indices=np.concatenate(idx).astype(np.int64))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--small", action="store_true", help="skip the million-triangle cases")
@@ -205,6 +245,8 @@ def main():
     out = HERE
     print("GD vectors")
     np.savez_compressed(os.path.join(out, "gd_vectors.npz"), **gd_vectors(L))
+    print("earcut vectors")
+    np.savez_compressed(os.path.join(out, "earcut_vectors.npz"), **earcut_vectors(L))
     print("subdivision-3 icosphere pairs")
     for seed in (1, 2, 3):
         a, b = mg.small_pair(seed)

@@ -59,6 +59,8 @@ class RefLib:
         L.sbref_boolean_group_members.argtypes = [_P, C.c_int, _I, C.c_size_t]
         L.sbref_boolean_new_triangles.restype = C.c_size_t
         L.sbref_boolean_new_triangles.argtypes = [_P, _I, C.c_size_t]
+        L.sbref_earcut.restype = C.c_size_t
+        L.sbref_earcut.argtypes = [_D, _I, C.c_size_t, _I, C.c_size_t]
         L.sbref_load_obj.restype = C.c_int
         L.sbref_load_obj.argtypes = [C.c_char_p, _D, C.c_size_t, C.POINTER(C.c_size_t), _I,
                                      C.c_size_t, C.POINTER(C.c_size_t)]
@@ -78,6 +80,15 @@ class RefLib:
         t = np.zeros((nt.value, 3), np.int64)
         self.L.sbref_load_obj(path.encode(), v, nv.value, C.byref(nv), t, nt.value, C.byref(nt))
         return v, t
+
+    def earcut(self, rings):
+        """rings: list of (k, 2) arrays, outer first.  Returns the index list."""
+        xy = np.ascontiguousarray(np.concatenate([np.asarray(r, np.float64).reshape(-1, 2) for r in rings]))
+        sizes = np.array([len(r) for r in rings], np.int64)
+        n = self.L.sbref_earcut(xy.reshape(-1), sizes, len(rings), np.zeros(1, np.int64), 0)
+        out = np.zeros(max(n, 1), np.int64)
+        self.L.sbref_earcut(xy.reshape(-1), sizes, len(rings), out, n)
+        return out[:n]
 
     def tri_tri(self, a9: np.ndarray, b9: np.ndarray):
         cop = C.c_int(0)

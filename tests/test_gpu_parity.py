@@ -220,8 +220,9 @@ def test_disjoint_meshes(gpu):
 
 def test_identical_meshes_grow_queues(gpu):
     """A mesh against itself: every triangle pairs with all its neighbours
-    (more candidates than the initial queue holds) and every self pair is
-    coplanar."""
+    (more candidates than the initial queue holds); shared vertices and
+    self pairs put many orientations at or near zero (a triangle against
+    itself is exactly coplanar only when its FP64 triple products round to 0)."""
     m = mg.icosphere(5)
     A, B = gpu.mesh(m.vertices, m.triangles), gpu.mesh(m.vertices, m.triangles)
     OA = OracleMesh(m.vertices, m.triangles)
@@ -232,7 +233,8 @@ def test_identical_meshes_grow_queues(gpu):
     status, _ = P.results()
     ost, _ = OA.tri_tri_pairs(OracleMesh(m.vertices, m.triangles), ref)
     assert np.array_equal(status, ost)
-    assert st["coplanar"] >= m.nt
+    assert st["coplanar"] == int((ost == 2).sum()) > 0
+    assert st["fallback_pairs"] > 0  # exact zeros cannot be certified in FP32
 
 
 def test_degenerate_triangles(gpu):
