@@ -24,9 +24,15 @@ __device__ __forceinline__ int delta(const uint32_t *__restrict__ codes, int n, 
     return __clz(a ^ b);
 }
 
+// Leaves are refit in tiles of kRefitTile sorted positions; a node whose leaf
+// range crosses a tile boundary is a "top" node, finished by k_refit_top.
+constexpr int kRefitTile = 1024;
+constexpr int kRefitBlock = 256;
+
 __global__ __launch_bounds__(256) void k_hierarchy(const uint32_t *__restrict__ codes,
                                                    const uint32_t *__restrict__ ids, int n,
-                                                   BNode *__restrict__ nodes, uint32_t *__restrict__ leafParent)
+                                                   BNode *__restrict__ nodes, uint32_t *__restrict__ leafParent,
+                                                   uint32_t *__restrict__ topList, uint32_t *topCount)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n - 1)
@@ -69,8 +75,11 @@ __global__ __launch_bounds__(256) void k_hierarchy(const uint32_t *__restrict__ 
     }
     nodes[i].d.x = c0;
     nodes[i].d.y = c1;
+    nodes[i].d.w = j;  // the node covers sorted leaves [min(i, j), max(i, j)]
     if (i == 0)
         nodes[0].d.z = -1;
+    if (lo / kRefitTile != hi / kRefitTile)
+        topList[atomicAdd(topCount, 1u)] = (uint32_t)i;
 }
 
 __device__ __forceinline__ void store_child_box(BNode *nd, int slot, float4 lo, float4 hi)
@@ -86,46 +95,126 @@ __device__ __forceinline__ void store_child_box(BNode *nd, int slot, float4 lo, 
     }
 }
 
-// one thread per leaf position; the second thread to reach a node merges it
-__global__ __launch_bounds__(256) void k_refit(const uint32_t *__restrict__ ids, const float *__restrict__ box32,
-                                               int n, BNode *nodes, const uint32_t *__restrict__ leafParent,
-                                               uint32_t *flags, float *rootBox)
+__device__ __forceinline__ void write_root(float *rootBox, float4 lo, float4 hi)
 {
-    const int pos = blockIdx.x * 256 + threadIdx.x;
-    if (pos >= n)
-        return;
-    const uint32_t tri = ids[pos];
-    float4 lo = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * tri)[0];
-    float4 hi = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * tri)[1];
-    int node = (int)leafParent[pos];
-    int child = (int)(kLeafBit | tri);
-    // a Karras tree over n leaves is at most n - 1 deep; the bound keeps every
-    // lane's walk finite whatever the input
-    for (int step = 0; node >= 0 && step < n; ++step) {
-        BNode *nd = nodes + node;
-        const int slot = (nd->d.x == child) ? 0 : 1;
-        store_child_box(nd, slot, lo, hi);
-        __threadfence();
-        const uint32_t prev = __hip_atomic_fetch_add(flags + node, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (prev == 0)
-            return;  // the sibling's thread finishes this node
-        __threadfence();
-        // the sibling's box was written by another lane, possibly on another
-        // XCD: read it through volatile loads after the acquire fence
-        const volatile float *f = reinterpret_cast<const volatile float *>(nd);
-        lo = make_float4(fminf(f[0], f[6]), fminf(f[1], f[7]), fminf(f[2], f[8]), 0.0f);
-        hi = make_float4(fmaxf(f[3], f[9]), fmaxf(f[4], f[10]), fmaxf(f[5], f[11]), 0.0f);
-        child = node;
-        node = nd->d.z;
-    }
-    if (node >= 0)
-        return;
     rootBox[0] = lo.x;
     rootBox[1] = lo.y;
     rootBox[2] = lo.z;
     rootBox[3] = hi.x;
     rootBox[4] = hi.y;
     rootBox[5] = hi.z;
+}
+
+// Refit, stage 1: one workgroup per tile of kRefitTile sorted leaves.  Every
+// leaf walks up; at a node whose leaf range lies inside the tile the first of
+// its two children to arrive leaves its box in LDS and stops, the second
+// merges and continues.  All hand-offs stay inside the workgroup (LDS +
+// workgroup-scope fences), so no lane ever waits and no device-wide fence is
+// needed.  At a node whose range leaves the tile the lane stores its box in
+// the node, counts the arrival in topFlags and stops.
+__global__ __launch_bounds__(kRefitBlock) void k_refit_tiles(const uint32_t *__restrict__ ids,
+                                                             const float *__restrict__ box32, int n,
+                                                             BNode *__restrict__ nodes,
+                                                             const uint32_t *__restrict__ leafParent,
+                                                             uint32_t *__restrict__ topFlags, float *rootBox)
+{
+    __shared__ float s_box[kRefitTile][2][6];
+    __shared__ uint32_t s_flag[kRefitTile];
+    const int tile0 = blockIdx.x * kRefitTile;
+    const int tile1 = min(tile0 + kRefitTile, n);
+    for (int k = threadIdx.x; k < kRefitTile; k += kRefitBlock)
+        s_flag[k] = 0;
+    __syncthreads();
+    for (int pos = tile0 + threadIdx.x; pos < tile1; pos += kRefitBlock) {
+        const uint32_t tri = ids[pos];
+        float4 lo = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * tri)[0];
+        float4 hi = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * tri)[1];
+        int node = (int)leafParent[pos];
+        int child = (int)(kLeafBit | tri);
+        // each step moves to a strictly larger leaf range, so a walk ends
+        // within the tile's depth; the bound only guards malformed input
+        for (int step = 0; step < kRefitTile; ++step) {
+            BNode *nd = nodes + node;
+            const int4 d = nd->d;
+            const int slot = (d.x == child) ? 0 : 1;
+            store_child_box(nd, slot, lo, hi);
+            const int lo_leaf = min(node, d.w), hi_leaf = max(node, d.w);
+            if (lo_leaf < tile0 || hi_leaf >= tile1 || (lo_leaf / kRefitTile) != (hi_leaf / kRefitTile)) {
+                atomicAdd(topFlags + node, 1u);
+                break;
+            }
+            const int local = node - tile0;
+            float *mine = s_box[local][slot];
+            mine[0] = lo.x;
+            mine[1] = lo.y;
+            mine[2] = lo.z;
+            mine[3] = hi.x;
+            mine[4] = hi.y;
+            mine[5] = hi.z;
+            __threadfence_block();
+            if (atomicAdd(&s_flag[local], 1u) == 0)
+                break;
+            __threadfence_block();
+            const volatile float *o = s_box[local][1 - slot];
+            lo = make_float4(fminf(lo.x, o[0]), fminf(lo.y, o[1]), fminf(lo.z, o[2]), 0.0f);
+            hi = make_float4(fmaxf(hi.x, o[3]), fmaxf(hi.y, o[4]), fmaxf(hi.z, o[5]), 0.0f);
+            child = node;
+            node = d.z;
+            if (node < 0) {  // the whole tree fits in this tile
+                write_root(rootBox, lo, hi);
+                break;
+            }
+        }
+    }
+}
+
+// Refit, stage 2: one workgroup finishes the top nodes (those spanning tiles)
+// in rounds.  Mark phase: a node whose two children are complete
+// (topFlags == 2) is marked 4.  Merge phase (after a barrier): every marked
+// node merges its child boxes into its parent's slot, counts itself in the
+// parent's flag and becomes 3 (done).  Slots read in a merge phase were all
+// written before its barrier.  Every round finishes at least the lowest
+// unfinished top node, so the loop ends within ntop + 1 rounds.
+__global__ __launch_bounds__(1024) void k_refit_top(BNode *__restrict__ nodes, const uint32_t *__restrict__ topList,
+                                                    const uint32_t *__restrict__ topCount, uint32_t *topFlags,
+                                                    float *rootBox)
+{
+    __shared__ int s_progress;
+    const uint32_t ntop = *topCount;
+    for (uint32_t round = 0; round <= ntop; ++round) {
+        if (threadIdx.x == 0)
+            s_progress = 0;
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < ntop; k += 1024) {
+            const uint32_t p = topList[k];
+            if (__hip_atomic_load(topFlags + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 2) {
+                __hip_atomic_store(topFlags + p, 4u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                s_progress = 1;
+            }
+        }
+        __syncthreads();
+        if (!s_progress)
+            break;
+        for (uint32_t k = threadIdx.x; k < ntop; k += 1024) {
+            const uint32_t p = topList[k];
+            if (__hip_atomic_load(topFlags + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 4)
+                continue;
+            const volatile float *b = reinterpret_cast<const volatile float *>(nodes + p);
+            const float4 lo = make_float4(fminf(b[0], b[6]), fminf(b[1], b[7]), fminf(b[2], b[8]), 0.0f);
+            const float4 hi = make_float4(fmaxf(b[3], b[9]), fmaxf(b[4], b[10]), fmaxf(b[5], b[11]), 0.0f);
+            __hip_atomic_store(topFlags + p, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const int parent = nodes[p].d.z;
+            if (parent < 0) {
+                write_root(rootBox, lo, hi);
+            } else {
+                BNode *pn = nodes + parent;
+                store_child_box(pn, pn->d.x == (int)p ? 0 : 1, lo, hi);
+                __threadfence_block();
+                atomicAdd(topFlags + parent, 1u);
+            }
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n)
@@ -157,23 +246,26 @@ void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, value, n);
 }
 
+// work: 2n + 1 u32 = top flags [0, n-1), top list [n, 2n-1), top count [2n]
 void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
-                      uint32_t *leafParent, hipStream_t s)
+                      uint32_t *leafParent, uint32_t *work, hipStream_t s)
 {
     if (n < 2)
         return;
+    launch_fill_u32(work, 0u, (size_t)2 * n + 1, s);
     hipLaunchKernelGGL(k_hierarchy, dim3((n - 1 + 255) / 256), dim3(256), 0, s, codes, ids, (int)n, nodes,
-                       leafParent);
+                       leafParent, work + n, work + 2 * (size_t)n);
 }
 
 void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *nodes,
-                  const uint32_t *leafParent, uint32_t *flags, float *rootBox, hipStream_t s)
+                  const uint32_t *leafParent, uint32_t *work, float *rootBox, hipStream_t s)
 {
     if (n < 2)
         return;
-    launch_fill_u32(flags, 0u, n - 1, s);
-    hipLaunchKernelGGL(k_refit, dim3((n + 255) / 256), dim3(256), 0, s, ids, box32, (int)n, nodes, leafParent,
-                       flags, rootBox);
+    hipLaunchKernelGGL(k_refit_tiles, dim3((n + kRefitTile - 1) / kRefitTile), dim3(kRefitBlock), 0, s, ids, box32,
+                       (int)n, nodes, leafParent, work, rootBox);
+    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, nodes, work + n, work + 2 * (size_t)n, work,
+                       rootBox);
 }
 
 }  // namespace sbg

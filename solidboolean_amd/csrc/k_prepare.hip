@@ -36,16 +36,22 @@ __global__ __launch_bounds__(kRecBlock) void k_records(const double *__restrict_
         r64[3] = make_double2(b.z, 0.0);
         r64[4] = make_double2(c.x, c.y);
         r64[5] = make_double2(c.z, 0.0);
-        float4 *r32 = reinterpret_cast<float4 *>(tri32 + (size_t)kTri32Stride * i);
-        r32[0] = make_float4((float)a.x, (float)a.y, (float)a.z, 0.0f);
-        r32[1] = make_float4((float)b.x, (float)b.y, (float)b.z, 0.0f);
-        r32[2] = make_float4((float)c.x, (float)c.y, (float)c.z, 0.0f);
         V3 n = tri_normal(a, b, c);
         normals[3 * (size_t)i] = n.x;
         normals[3 * (size_t)i + 1] = n.y;
         normals[3 * (size_t)i + 2] = n.z;
         // exact FP64 min/max (AxisAlignedBoudingBox::update), then rounded outward
         Box64 bb = tri_box64(a, b, c);
+        // FP32 record; its pad slots carry upper bounds of the triangle's
+        // largest box extent and of its largest |coordinate| (the FP32
+        // filter's error bound is built from them)
+        const double ext = fmax(fmax(bb.mx.x - bb.mn.x, bb.mx.y - bb.mn.y), bb.mx.z - bb.mn.z);
+        const double mag = fmax(fmax(fmax(fabs(bb.mn.x), fabs(bb.mx.x)), fmax(fabs(bb.mn.y), fabs(bb.mx.y))),
+                                fmax(fabs(bb.mn.z), fabs(bb.mx.z)));
+        float4 *r32 = reinterpret_cast<float4 *>(tri32 + (size_t)kTri32Stride * i);
+        r32[0] = make_float4((float)a.x, (float)a.y, (float)a.z, f_up(ext * (1.0 + 0x1p-40)));
+        r32[1] = make_float4((float)b.x, (float)b.y, (float)b.z, f_up(mag));
+        r32[2] = make_float4((float)c.x, (float)c.y, (float)c.z, 0.0f);
         float4 lo = make_float4(f_down(bb.mn.x), f_down(bb.mn.y), f_down(bb.mn.z), 0.0f);
         float4 hi = make_float4(f_up(bb.mx.x), f_up(bb.mx.y), f_up(bb.mx.z), 0.0f);
         float4 *bx = reinterpret_cast<float4 *>(box32 + (size_t)kBox32Stride * i);

@@ -67,15 +67,59 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t *__restrict__ data, size
     }
 }
 
+// exclusive scan of each digit's row of the histogram (one workgroup per
+// digit); the row's total goes to rowTotal[digit]
+__global__ __launch_bounds__(256) void k_scan_rows(uint32_t *__restrict__ hist, int nblocks,
+                                                   uint32_t *__restrict__ rowTotal)
+{
+    __shared__ uint32_t s[256];
+    uint32_t *row = hist + (size_t)blockIdx.x * nblocks;
+    const int per = (nblocks + 255) / 256;
+    const int lo = threadIdx.x * per;
+    const int hi = lo + per < nblocks ? lo + per : nblocks;
+    uint32_t sum = 0;
+    for (int i = lo; i < hi; ++i)
+        sum += row[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = threadIdx.x ? s[threadIdx.x - 1] : 0;
+    for (int i = lo; i < hi; ++i) {
+        const uint32_t v = row[i];
+        row[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 255)
+        rowTotal[blockIdx.x] = s[255];
+}
+
 template <typename K, bool HAS_VAL>
 __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ keys, const uint32_t *__restrict__ vals,
                                                         K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
                                                         size_t n, int shift, uint32_t mask,
-                                                        const uint32_t *__restrict__ offsets, int nblocks)
+                                                        const uint32_t *__restrict__ offsets, int nblocks,
+                                                        const uint32_t *__restrict__ rowTotal)
 {
     __shared__ uint32_t s_base[256];
     __shared__ uint32_t s_wcnt[kWaves][256];
-    s_base[threadIdx.x] = offsets[(size_t)threadIdx.x * nblocks + blockIdx.x];
+    // digit bases: exclusive scan of the 256 row totals
+    const uint32_t total = rowTotal[threadIdx.x];
+    s_base[threadIdx.x] = total;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t v = threadIdx.x >= (unsigned)off ? s_base[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_base[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint32_t digitBase = s_base[threadIdx.x] - total;
+    __syncthreads();
+    s_base[threadIdx.x] = digitBase + offsets[(size_t)threadIdx.x * nblocks + blockIdx.x];
     for (int w = 0; w < kWaves; ++w)
         s_wcnt[w][threadIdx.x] = 0;
     __syncthreads();
@@ -132,7 +176,8 @@ void launch_exclusive_scan(uint32_t *data, size_t m, hipStream_t s)
 }
 
 int sort_blocks(size_t n) { return (int)((n + kSortTile - 1) / kSortTile); }
-size_t sort_hist_entries(size_t n) { return (size_t)256 * (size_t)(sort_blocks(n) > 0 ? sort_blocks(n) : 1); }
+// 256 x nblocks histogram + 256 row totals
+size_t sort_hist_entries(size_t n) { return (size_t)256 * (size_t)(sort_blocks(n) > 0 ? sort_blocks(n) : 1) + 256; }
 
 template <typename K, bool HAS_VAL>
 static void one_pass(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout, size_t n, int shift,
@@ -140,10 +185,11 @@ static void one_pass(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
 {
     const int nb = sort_blocks(n);
     const uint32_t mask = (1u << bits) - 1u;
+    uint32_t *rowTotal = hist + (size_t)256 * nb;
     hipLaunchKernelGGL(k_hist<K>, dim3(nb), dim3(kSortBlock), 0, s, kin, n, shift, mask, hist, nb);
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, hist, (size_t)256 * nb);
+    hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(256), 0, s, hist, nb, rowTotal);
     hipLaunchKernelGGL((k_scatter<K, HAS_VAL>), dim3(nb), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
-                       shift, mask, hist, nb);
+                       shift, mask, hist, nb, rowTotal);
 }
 
 int radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *valsAlt, size_t n,

@@ -5,15 +5,17 @@
 // One lane per candidate pair.  Two precisions, identical results:
 //  * FP64: the reference's exact operation sequence (sbg_device.h gd_fp64)
 //    on the pair's two 96-B FP64 records.
-//  * FP32-filtered: every orientation predicate the reference would evaluate
-//    is first computed from the 48-B FP32 records with a forward error bound
-//    (fsign below, bound derived in sbg_device.h).  If every sign on the path is certified and the
-//    path ends in a rejection, the pair is decided without touching FP64; a
-//    path that ends in an intersection, or any uncertain sign, re-runs the
-//    whole pair with gd_fp64, so every decision and every segment equals the
-//    FP64 reference's.
+//  * FP32-filtered, two kernels: k_gd_filter evaluates every orientation
+//    predicate the reference would evaluate from the 48-B FP32 records and
+//    certifies its sign against a per-pair forward error bound.  A pair whose
+//    certified path ends in a rejection is decided there; every other pair (a
+//    certified intersection, or any uncertain sign) is appended to a list that
+//    k_gd_list re-runs with the full FP64 sequence, so every status and every
+//    segment equals the FP64 reference's.  Splitting the two keeps waves from
+//    running both paths for the few lanes that hit.
 // Per pair and launch: 8 B pair + 1 B status + 2 x 96 B (FP64) or 2 x 48 B
-// (filtered, + 2 x 96 B for hits / fallbacks) + 48 B segment per hit.
+// (filtered, + 2 x 96 B + 4 B list entry for hits / fallbacks) + 48 B segment
+// per hit.
 #include "sbg_device.h"
 #include "sbg_internal.h"
 
@@ -30,65 +32,85 @@ __device__ __forceinline__ void load_tri64(const double *__restrict__ rec, uint3
     r = {e.x, e.y, f.x};
 }
 
-__device__ __forceinline__ void load_tri32(const float *__restrict__ rec, uint32_t t, F3 &p, F3 &q, F3 &r)
+// FP32 record: vertices + (largest box extent, largest |coordinate|) upper bounds
+__device__ __forceinline__ void load_tri32(const float *__restrict__ rec, uint32_t t, F3 &p, F3 &q, F3 &r,
+                                           float &ext, float &mag)
 {
     const float4 *x = reinterpret_cast<const float4 *>(rec + (size_t)kTri32Stride * t);
     const float4 a = x[0], b = x[1], c = x[2];
     p = {a.x, a.y, a.z};
     q = {b.x, b.y, b.z};
     r = {c.x, c.y, c.z};
+    ext = a.w;
+    mag = b.w;
 }
 
-// Sign of DOT(a, CROSS(b, c)) as the FP64 reference computes it, or 0 when the
-// FP32 evaluation cannot certify it.  eta bounds each FP32 difference's input
-// rounding error (2^-23 * max |coordinate|).  The bound (see the derivation in
-// sbg_device.h) is |d32 - d64| <= 1.5 eta Q + 5 eta^2 S1 + 50 eta^3 + 12 u P.
-__device__ __forceinline__ int fsign(const F3 &a, const F3 &b, const F3 &c, float eta)
+// Per-pair bound on |d32 - d64| for every predicate d = DOT(a, CROSS(b, c)) of
+// the pair.  Every vertex of A lies within extA of p1 and every vertex of B
+// within extB of p2 on each axis (extents of the FP64 boxes, rounded up), and
+// FP32 rounding moves each coordinate by at most uM, so every coordinate
+// difference among the six vertices, as computed in FP32, is at most
+//   D = (max_axis |p1 - p2| + extA + extB + 2 eta)(1 + 4u),
+// with eta = 2uM(1 + 2u) >= the input-rounding error of a difference and M
+// the larger |coordinate|.  This holds for any pair, overlapping or not.  With
+// P <= 6D^3, Q <= 18D^2 and S1 <= 9D in the per-predicate bound derived in
+// sbg_device.h,
+//   |d32 - d64| <= 27 eta D^2 + 45 eta^2 D + 50 eta^3 + 72 u D^3,
+// taken here with a further 1e-4 relative margin for the rounding of this
+// very expression and 1e-36 absolute for FP32 gradual underflow.
+__device__ __forceinline__ float pair_bound(const F3 &p1, const F3 &p2, float extA, float extB, float magA,
+                                            float magB)
 {
     const float u = 5.9604644775390625e-8f;  // 2^-24
-    const float nx = b.y * c.z - b.z * c.y;
-    const float ny = b.z * c.x - b.x * c.z;
-    const float nz = b.x * c.y - b.y * c.x;
-    const float d = a.x * nx + a.y * ny + a.z * nz;
-    const float ax = __builtin_fabsf(a.x), ay = __builtin_fabsf(a.y), az = __builtin_fabsf(a.z);
-    const float bx = __builtin_fabsf(b.x), by = __builtin_fabsf(b.y), bz = __builtin_fabsf(b.z);
-    const float cx = __builtin_fabsf(c.x), cy = __builtin_fabsf(c.y), cz = __builtin_fabsf(c.z);
-    const float px = by * cz + bz * cy;
-    const float py = bz * cx + bx * cz;
-    const float pz = bx * cy + by * cx;
-    const float P = ax * px + ay * py + az * pz;
-    const float Q = (px + py + pz) + ax * ((by + cz) + (bz + cy)) + ay * ((bz + cx) + (bx + cz)) +
-                    az * ((bx + cy) + (by + cx));
-    const float S1 = (ax + ay + az) + (bx + by + bz) + (cx + cy + cz);
-    const float E = 1.5f * eta * Q + 5.0f * eta * eta * S1 + 50.0f * eta * eta * eta + 12.0f * u * P + 1e-36f;
-    const float ad = __builtin_fabsf(d);
-    // the 2^-99 floor keeps |d64| >= 2^-100, so the reference's FP64 products
-    // dp1 * dq1 (tri_tri_intersect.c:421,438) cannot underflow to zero
-    if (ad > E * 1.0000005f + 1.6e-30f)
+    const float eta = 2.0f * u * fmaxf(magA, magB) * (1.0f + 2.0f * u);
+    const float gap = fmaxf(fmaxf(__builtin_fabsf(p1.x - p2.x), __builtin_fabsf(p1.y - p2.y)),
+                            __builtin_fabsf(p1.z - p2.z));
+    const float D = (gap + extA + extB + 2.0f * eta) * (1.0f + 4.0f * u);
+    const float E = 27.0f * eta * D * D + 45.0f * eta * eta * D + 50.0f * eta * eta * eta + 72.0f * u * D * D * D;
+    return E * 1.0001f + 1e-36f;
+}
+
+// certified sign of DOT(a, n) with n = CROSS(b, c) precomputed, or 0.  The
+// 2^-99 floor keeps |d64| >= 2^-100, so the reference's FP64 products
+// dp1 * dq1 (tri_tri_intersect.c:421,438) cannot underflow to zero.
+__device__ __forceinline__ int fsign_n(const F3 &a, const F3 &n, float E)
+{
+    const float d = a.x * n.x + a.y * n.y + a.z * n.z;
+    if (__builtin_fabsf(d) > E + 1.6e-30f)
         return d > 0.0f ? 1 : -1;
     return 0;
+}
+
+__device__ __forceinline__ F3 fcross(const F3 &b, const F3 &c)
+{
+    return {b.y * c.z - b.z * c.y, b.z * c.x - b.x * c.z, b.x * c.y - b.y * c.x};
+}
+
+__device__ __forceinline__ int fsign(const F3 &a, const F3 &b, const F3 &c, float E)
+{
+    return fsign_n(a, fcross(b, c), E);
 }
 
 // CONSTRUCT_INTERSECTION's predicates (tri_tri_intersect.c:286-296, 324-332):
 // 1 = intersection (segment computed later in FP64), 0 = rejection, -2 = uncertain
 __device__ __forceinline__ int f_construct(const F3 &p1, const F3 &q1, const F3 &r1, const F3 &p2, const F3 &q2,
-                                           const F3 &r2, float eta)
+                                           const F3 &r2, float E)
 {
     const F3 v = fsub(p2, p1);
     F3 v1 = fsub(q1, p1);
     F3 v2 = fsub(r2, p1);
-    int s = fsign(v, v1, v2, eta);
+    int s = fsign(v, v1, v2, E);
     if (!s)
         return -2;
     if (s > 0) {
         v1 = fsub(r1, p1);
-        s = fsign(v, v1, v2, eta);
+        s = fsign(v, v1, v2, E);
         if (!s)
             return -2;
         return s < 0 ? 1 : 0;
     }
     v2 = fsub(q2, p1);
-    s = fsign(v, v1, v2, eta);
+    s = fsign(v, v1, v2, E);
     if (!s)
         return -2;
     return s < 0 ? 0 : 1;
@@ -96,45 +118,39 @@ __device__ __forceinline__ int f_construct(const F3 &p1, const F3 &q1, const F3 
 
 // TRI_TRI_INTER_3D (tri_tri_intersect.c:360-385) for non-zero signs
 __device__ __forceinline__ int f_inter3d(const F3 &p1, const F3 &q1, const F3 &r1, const F3 &p2, const F3 &q2,
-                                         const F3 &r2, int sp2, int sq2, int sr2, float eta)
+                                         const F3 &r2, int sp2, int sq2, int sr2, float E)
 {
     if (sp2 > 0) {
         if (sq2 > 0)
-            return f_construct(p1, r1, q1, r2, p2, q2, eta);
+            return f_construct(p1, r1, q1, r2, p2, q2, E);
         if (sr2 > 0)
-            return f_construct(p1, r1, q1, q2, r2, p2, eta);
-        return f_construct(p1, q1, r1, p2, q2, r2, eta);
+            return f_construct(p1, r1, q1, q2, r2, p2, E);
+        return f_construct(p1, q1, r1, p2, q2, r2, E);
     }
     if (sq2 < 0)
-        return f_construct(p1, q1, r1, r2, p2, q2, eta);
+        return f_construct(p1, q1, r1, r2, p2, q2, E);
     if (sr2 < 0)
-        return f_construct(p1, q1, r1, q2, r2, p2, eta);
-    return f_construct(p1, r1, q1, p2, q2, r2, eta);
+        return f_construct(p1, q1, r1, q2, r2, p2, E);
+    return f_construct(p1, r1, q1, p2, q2, r2, E);
 }
 
 // Returns a certified GdExit rejection (0, 1, 2), GD_HIT when the certified path
 // ends in an intersection, or -1 when some sign is uncertain.
 __device__ __forceinline__ int gd_filter(const F3 &p1, const F3 &q1, const F3 &r1, const F3 &p2, const F3 &q2,
-                                         const F3 &r2)
+                                         const F3 &r2, float E)
 {
-    float M = 0.0f;
-    const F3 pts[6] = {p1, q1, r1, p2, q2, r2};
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-        M = fmaxf(M, fmaxf(__builtin_fabsf(pts[k].x), fmaxf(__builtin_fabsf(pts[k].y), __builtin_fabsf(pts[k].z))));
-    const float eta = 1.1920928955078125e-7f * M * 1.0000003f;  // 2 u M, rounded up
-    const F3 e1 = fsub(p2, r2), e2 = fsub(q2, r2);
-    const int sp1 = fsign(fsub(p1, r2), e1, e2, eta);
-    const int sq1 = fsign(fsub(q1, r2), e1, e2, eta);
-    const int sr1 = fsign(fsub(r1, r2), e1, e2, eta);
+    const F3 N2 = fcross(fsub(p2, r2), fsub(q2, r2));
+    const int sp1 = fsign_n(fsub(p1, r2), N2, E);
+    const int sq1 = fsign_n(fsub(q1, r2), N2, E);
+    const int sr1 = fsign_n(fsub(r1, r2), N2, E);
     if (!sp1 || !sq1 || !sr1)
         return -1;
     if (sp1 == sq1 && sp1 == sr1)
         return GD_REJECT_PLANE1;
-    const F3 f1 = fsub(q1, p1), f2 = fsub(r1, p1);
-    const int sp2 = fsign(fsub(p2, r1), f1, f2, eta);
-    const int sq2 = fsign(fsub(q2, r1), f1, f2, eta);
-    const int sr2 = fsign(fsub(r2, r1), f1, f2, eta);
+    const F3 N1 = fcross(fsub(q1, p1), fsub(r1, p1));
+    const int sp2 = fsign_n(fsub(p2, r1), N1, E);
+    const int sq2 = fsign_n(fsub(q2, r1), N1, E);
+    const int sr2 = fsign_n(fsub(r2, r1), N1, E);
     if (!sp2 || !sq2 || !sr2)
         return -1;
     if (sp2 == sq2 && sp2 == sr2)
@@ -142,18 +158,18 @@ __device__ __forceinline__ int gd_filter(const F3 &p1, const F3 &q1, const F3 &r
     int r;
     if (sp1 > 0) {
         if (sq1 > 0)
-            r = f_inter3d(r1, p1, q1, p2, r2, q2, sp2, sr2, sq2, eta);
+            r = f_inter3d(r1, p1, q1, p2, r2, q2, sp2, sr2, sq2, E);
         else if (sr1 > 0)
-            r = f_inter3d(q1, r1, p1, p2, r2, q2, sp2, sr2, sq2, eta);
+            r = f_inter3d(q1, r1, p1, p2, r2, q2, sp2, sr2, sq2, E);
         else
-            r = f_inter3d(p1, q1, r1, p2, q2, r2, sp2, sq2, sr2, eta);
+            r = f_inter3d(p1, q1, r1, p2, q2, r2, sp2, sq2, sr2, E);
     } else {
         if (sq1 < 0)
-            r = f_inter3d(r1, p1, q1, p2, q2, r2, sp2, sq2, sr2, eta);
+            r = f_inter3d(r1, p1, q1, p2, q2, r2, sp2, sq2, sr2, E);
         else if (sr1 < 0)
-            r = f_inter3d(q1, r1, p1, p2, q2, r2, sp2, sq2, sr2, eta);
+            r = f_inter3d(q1, r1, p1, p2, q2, r2, sp2, sq2, sr2, E);
         else
-            r = f_inter3d(p1, q1, r1, p2, r2, q2, sp2, sr2, sq2, eta);
+            r = f_inter3d(p1, q1, r1, p2, r2, q2, sp2, sr2, sq2, E);
     }
     if (r == -2)
         return -1;
@@ -167,53 +183,100 @@ __device__ __forceinline__ void wave_count(bool pred, unsigned long long *counte
         atomicAdd(counter, (unsigned long long)__popcll(m));
 }
 
-template <int PRECISION>
-__global__ __launch_bounds__(256) void k_tritri(const double *__restrict__ tri64A, const double *__restrict__ tri64B,
-                                                const float *__restrict__ tri32A, const float *__restrict__ tri32B,
-                                                const uint64_t *__restrict__ pairs, uint32_t n,
-                                                uint8_t *__restrict__ status, double *__restrict__ seg,
-                                                GdStats *__restrict__ stats)
+__device__ __forceinline__ void gd_pair_fp64(const double *__restrict__ tri64A, const double *__restrict__ tri64B,
+                                             uint64_t pr, uint32_t i, uint8_t *__restrict__ status,
+                                             double *__restrict__ seg, int &exitKind)
+{
+    const uint32_t a = (uint32_t)(pr >> 32), b = (uint32_t)pr;
+    V3 p1, q1, r1, p2, q2, r2, s, t;
+    load_tri64(tri64A, a, p1, q1, r1);
+    load_tri64(tri64B, b, p2, q2, r2);
+    exitKind = gd_fp64(p1, q1, r1, p2, q2, r2, s, t);
+    if (exitKind == GD_HIT) {
+        double2 *o = reinterpret_cast<double2 *>(seg + 6 * (size_t)i);
+        o[0] = make_double2(s.x, s.y);
+        o[1] = make_double2(s.z, t.x);
+        o[2] = make_double2(t.y, t.z);
+    }
+    status[i] = exitKind == GD_HIT ? 1 : (exitKind == GD_COPLANAR ? 2 : 0);
+}
+
+// plain FP64 over every pair
+__global__ __launch_bounds__(256) void k_gd_fp64(const double *__restrict__ tri64A, const double *__restrict__ tri64B,
+                                                 const uint64_t *__restrict__ pairs, uint32_t n,
+                                                 uint8_t *__restrict__ status, double *__restrict__ seg,
+                                                 GdStats *__restrict__ stats)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     const bool valid = i < n;
     int exitKind = -1;
-    bool fallback = false;
-    bool filtered = false;
-    if (valid) {
-        const uint64_t pr = pairs[i];
-        const uint32_t a = (uint32_t)(pr >> 32), b = (uint32_t)pr;
-        if (PRECISION == 1) {
-            F3 p1, q1, r1, p2, q2, r2;
-            load_tri32(tri32A, a, p1, q1, r1);
-            load_tri32(tri32B, b, p2, q2, r2);
-            const int f = gd_filter(p1, q1, r1, p2, q2, r2);
-            if (f >= 0 && f <= GD_REJECT_INTERVAL) {
-                exitKind = f;
-                filtered = true;
-            } else {
-                fallback = f < 0;
-            }
-        }
-        if (exitKind < 0) {
-            V3 p1, q1, r1, p2, q2, r2, s, t;
-            load_tri64(tri64A, a, p1, q1, r1);
-            load_tri64(tri64B, b, p2, q2, r2);
-            exitKind = gd_fp64(p1, q1, r1, p2, q2, r2, s, t);
-            if (exitKind == GD_HIT) {
-                double2 *o = reinterpret_cast<double2 *>(seg + 6 * (size_t)i);
-                o[0] = make_double2(s.x, s.y);
-                o[1] = make_double2(s.z, t.x);
-                o[2] = make_double2(t.y, t.z);
-            }
-        }
-        status[i] = exitKind == GD_HIT ? 1 : (exitKind == GD_COPLANAR ? 2 : 0);
-    }
+    if (valid)
+        gd_pair_fp64(tri64A, tri64B, pairs[i], i, status, seg, exitKind);
     if (stats) {
 #pragma unroll
         for (int k = 0; k < 5; ++k)
             wave_count(valid && exitKind == k, &stats->exits[k]);
-        wave_count(valid && fallback, &stats->fallback);
-        wave_count(valid && filtered, &stats->filtered);
+    }
+}
+
+// FP32 certificates; undecided pairs go to list (count in *listCount)
+__global__ __launch_bounds__(256) void k_gd_filter(const float *__restrict__ tri32A, const float *__restrict__ tri32B,
+                                                   const uint64_t *__restrict__ pairs, uint32_t n,
+                                                   uint8_t *__restrict__ status, uint32_t *__restrict__ list,
+                                                   uint32_t *listCount, GdStats *__restrict__ stats)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    const bool valid = i < n;
+    int f = -3;
+    if (valid) {
+        const uint64_t pr = pairs[i];
+        F3 p1, q1, r1, p2, q2, r2;
+        float extA, magA, extB, magB;
+        load_tri32(tri32A, (uint32_t)(pr >> 32), p1, q1, r1, extA, magA);
+        load_tri32(tri32B, (uint32_t)pr, p2, q2, r2, extB, magB);
+        f = gd_filter(p1, q1, r1, p2, q2, r2, pair_bound(p1, p2, extA, extB, magA, magB));
+        if (f >= 0 && f <= GD_REJECT_INTERVAL)
+            status[i] = 0;
+    }
+    const bool pass = valid && (f < 0 || f == GD_HIT);
+    // wave-aggregated append of the undecided pairs
+    const unsigned long long m = __ballot(pass);
+    const int lane = threadIdx.x & 63;
+    uint32_t base = 0;
+    if (lane == 0 && m)
+        base = atomicAdd(listCount, (uint32_t)__popcll(m));
+    base = __shfl(base, 0, 64);
+    if (pass)
+        list[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
+    if (stats) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            wave_count(valid && f == k, &stats->exits[k]);
+        wave_count(valid && f < 0, &stats->fallback);
+        wave_count(valid && f >= 0 && f <= GD_REJECT_INTERVAL, &stats->filtered);
+    }
+}
+
+// FP64 on the pairs the filter left undecided
+__global__ __launch_bounds__(256) void k_gd_list(const double *__restrict__ tri64A, const double *__restrict__ tri64B,
+                                                 const uint64_t *__restrict__ pairs, const uint32_t *__restrict__ list,
+                                                 const uint32_t *__restrict__ listCount, uint8_t *__restrict__ status,
+                                                 double *__restrict__ seg, GdStats *__restrict__ stats)
+{
+    const uint32_t total = *listCount;
+    for (uint32_t base = blockIdx.x * 256; base < total; base += gridDim.x * 256) {
+        const uint32_t k = base + threadIdx.x;
+        const bool valid = k < total;
+        int exitKind = -1;
+        if (valid) {
+            const uint32_t i = list[k];
+            gd_pair_fp64(tri64A, tri64B, pairs[i], i, status, seg, exitKind);
+        }
+        if (stats) {
+#pragma unroll
+            for (int e = 0; e < 5; ++e)
+                wave_count(valid && exitKind == e, &stats->exits[e]);
+        }
     }
 }
 
@@ -295,19 +358,26 @@ __global__ __launch_bounds__(kCompactBlock) void k_hit_write(const uint64_t *__r
 
 int compact_blocks(uint32_t n) { return (int)((n + kCompactBlock - 1) / kCompactBlock); }
 
+void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
+
+// work: n + 1 u32 (the filter's list and its count) for precision 1
 void launch_tritri(const double *tri64A, const double *tri64B, const float *tri32A, const float *tri32B,
-                   const uint64_t *pairs, uint32_t n, int precision, uint8_t *status, double *seg,
+                   const uint64_t *pairs, uint32_t n, int precision, uint8_t *status, double *seg, uint32_t *work,
                    GdStats *stats, hipStream_t s)
 {
     if (n == 0)
         return;
     const dim3 grid((n + 255) / 256);
-    if (precision == 1)
-        hipLaunchKernelGGL(k_tritri<1>, grid, dim3(256), 0, s, tri64A, tri64B, tri32A, tri32B, pairs, n, status,
+    if (precision == 1) {
+        uint32_t *count = work + n;
+        launch_fill_u32(count, 0u, 1, s);
+        hipLaunchKernelGGL(k_gd_filter, grid, dim3(256), 0, s, tri32A, tri32B, pairs, n, status, work, count, stats);
+        const unsigned listGrid = grid.x < 1024u ? grid.x : 1024u;
+        hipLaunchKernelGGL(k_gd_list, dim3(listGrid), dim3(256), 0, s, tri64A, tri64B, pairs, work, count, status,
                            seg, stats);
-    else
-        hipLaunchKernelGGL(k_tritri<0>, grid, dim3(256), 0, s, tri64A, tri64B, tri32A, tri32B, pairs, n, status,
-                           seg, stats);
+    } else {
+        hipLaunchKernelGGL(k_gd_fp64, grid, dim3(256), 0, s, tri64A, tri64B, pairs, n, status, seg, stats);
+    }
 }
 
 void launch_tritri_raw(const double *a9, const double *b9, uint32_t n, uint8_t *status, double *seg,

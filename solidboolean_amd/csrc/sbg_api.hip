@@ -87,7 +87,7 @@ struct sbg_context_s {
     DevBuf<GdStats> stats;
     DevBuf<double> pts;
     DevBuf<uint8_t> flags, bits, votes;
-    DevBuf<uint32_t> ovf;
+    DevBuf<uint32_t> ovf, rayList;
     DevBuf<double> rawA, rawB, rawSeg;
     DevBuf<uint8_t> rawStatus;
     int refs = 1;  // the owner + every live mesh / pair list
@@ -111,6 +111,7 @@ struct sbg_mesh_s {
     DevBuf<uint32_t> keys, vals, keysAlt, valsAlt, leafParent, flags;
     DevBuf<BNode> nodes;
     uint32_t root = 0;
+    const uint32_t *sortedIds = nullptr;  // triangles in Morton order (vals or valsAlt)
     bool prepared = false;
 };
 
@@ -121,7 +122,7 @@ struct sbg_pairs_s {
     DevBuf<uint8_t> status;
     DevBuf<double> seg, hitSeg;
     DevBuf<uint64_t> hitPairs;
-    DevBuf<uint32_t> blockCounts, nHits;
+    DevBuf<uint32_t> blockCounts, nHits, work;
     bool tested = false;
     // largest first / second index in the list (host-built lists are checked
     // against the meshes before any kernel indexes them)
@@ -258,6 +259,7 @@ void context_free(sbg_context ctx)
     ctx->bits.release();
     ctx->votes.release();
     ctx->ovf.release();
+    ctx->rayList.release();
     ctx->rawA.release();
     ctx->rawB.release();
     ctx->rawSeg.release();
@@ -423,7 +425,7 @@ int sbg_mesh_prepare(sbg_mesh m)
     TRY(m->valsAlt.ensure(nt));
     TRY(m->leafParent.ensure(nt));
     TRY(m->nodes.ensure(nt > 1 ? nt - 1 : 1));
-    TRY(m->flags.ensure(nt > 1 ? nt - 1 : 1));
+    TRY(m->flags.ensure(2 * nt + 1));
     TRY(c->hist.ensure(sort_hist_entries(nt)));
     m->prepared = false;
     if (nt == 0) {
@@ -447,6 +449,7 @@ int sbg_mesh_prepare(sbg_mesh m)
         }
         TRY(check_launch("morton sort"));
     }
+    m->sortedIds = sv;
     {
         Stage st(c, SBG_STAGE_BVH);
         if (nt == 1) {
@@ -456,7 +459,7 @@ int sbg_mesh_prepare(sbg_mesh m)
                                    c->stream));
         } else {
             m->root = 0;
-            launch_hierarchy(sk, sv, (uint32_t)nt, m->nodes.p, m->leafParent.p, c->stream);
+            launch_hierarchy(sk, sv, (uint32_t)nt, m->nodes.p, m->leafParent.p, m->flags.p, c->stream);
             launch_refit(sv, m->box32.p, (uint32_t)nt, m->nodes.p, m->leafParent.p, m->flags.p, m->rootBox.p,
                          c->stream);
         }
@@ -533,6 +536,7 @@ int sbg_pairs_destroy(sbg_pairs p)
     p->hitPairs.release();
     p->blockCounts.release();
     p->nHits.release();
+    p->work.release();
     sbg_context c = p->ctx;
     delete p;
     context_release(c);
@@ -740,10 +744,11 @@ static int run_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, GdSta
     TRY(p->hitSeg.ensure(6 * p->n));
     TRY(p->blockCounts.ensure((size_t)compact_blocks(n)));
     TRY(p->nHits.ensure(1));
+    TRY(p->work.ensure(p->n + 1));
     {
         Stage st(c, SBG_STAGE_TRI_TRI);
         launch_tritri(a->tri64.p, b->tri64.p, a->tri32.p, b->tri32.p, p->pairs.p, n, precision, p->status.p,
-                      p->seg.p, dstats, c->stream);
+                      p->seg.p, p->work.p, dstats, c->stream);
         TRY(check_launch("tri_tri"));
     }
     {
@@ -868,17 +873,21 @@ int sbg_tri_tri_batch(sbg_context ctx, const double *a9, const double *b9, size_
 // ---------------------------------------------------------------------------
 // classification
 // ---------------------------------------------------------------------------
-static int run_rays(sbg_mesh target, const double *dpts, const uint8_t *dskip, size_t n, uint8_t *dbits)
+// ovf[0]: overflow flag, ovf[1]: number of rays handed to the list kernel
+static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder, const uint8_t *dskip, size_t n,
+                    uint8_t *dbits)
 {
     sbg_context c = target->ctx;
-    TRY(c->ovf.ensure(1));
-    launch_fill_u32(c->ovf.p, 0u, 1, c->stream);
+    TRY(c->ovf.ensure(2));
+    TRY(c->rayList.ensure(3 * n));
+    launch_fill_u32(c->ovf.p, 0u, 2, c->stream);
     if (target->nt == 0) {
         launch_fill_u32((uint32_t *)dbits, 0u, (3 * n + 3) / 4, c->stream);
         return check_launch("rays");
     }
     launch_rays(target->nodes.p, target->root, target->rootBox.p, target->tri64.p, target->normals.p,
-                (uint32_t)target->nt, dpts, dskip, (uint32_t)n, dbits, c->ovf.p, c->stream);
+                (uint32_t)target->nt, dpts, dorder, dskip, (uint32_t)n, dbits, c->rayList.p, c->ovf.p + 1, c->ovf.p,
+                c->stream);
     return check_launch("rays");
 }
 
@@ -910,7 +919,7 @@ int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *a
     HIP_TRY(hipMemcpyAsync(c->pts.p, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
     {
         Stage st(c, SBG_STAGE_CLASSIFY);
-        TRY(run_rays(target, c->pts.p, nullptr, n, c->bits.p));
+        TRY(run_rays(target, c->pts.p, nullptr, nullptr, n, c->bits.p));
         launch_vote(c->bits.p, nullptr, (uint32_t)n, c->votes.p, c->stream);
         TRY(check_launch("vote"));
     }
@@ -959,7 +968,7 @@ int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side,
         launch_fill_u32((uint32_t *)c->flags.p, 0u, (nt + 3) / 4, c->stream);
         launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, c->flags.p, c->stream);
         launch_centroids(self->tri64.p, c->flags.p, (uint32_t)nt, c->pts.p, c->stream);
-        TRY(run_rays(other, c->pts.p, c->flags.p, nt, c->bits.p));
+        TRY(run_rays(other, c->pts.p, self->sortedIds, c->flags.p, nt, c->bits.p));
         launch_vote(c->bits.p, c->flags.p, (uint32_t)nt, c->votes.p, c->stream);
         TRY(check_launch("classify"));
     }
@@ -1022,13 +1031,14 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, int it
     TRY(check_pairs_in_range(a, b, p));
     TRY(p->status.ensure(p->n));
     TRY(p->seg.ensure(6 * p->n));
+    TRY(p->work.ensure(p->n + 1));
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, c->stream));
     for (int i = 0; i < iters; ++i)
         launch_tritri(a->tri64.p, b->tri64.p, a->tri32.p, b->tri32.p, p->pairs.p, (uint32_t)p->n, precision,
-                      p->status.p, p->seg.p, nullptr, c->stream);
+                      p->status.p, p->seg.p, p->work.p, nullptr, c->stream);
     HIP_TRY(hipEventRecord(e1, c->stream));
     HIP_TRY(hipEventSynchronize(e1));
     float ms = 0.0f;

@@ -52,10 +52,11 @@ int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRa
                    uint32_t *hist, hipStream_t s);
 
 // k_lbvh.hip
+// work: 2n + 1 u32 of scratch shared by the two (top-node flags, list, count)
 void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
-                      uint32_t *leafParent, hipStream_t s);
+                      uint32_t *leafParent, uint32_t *work, hipStream_t s);
 void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *nodes,
-                  const uint32_t *leafParent, uint32_t *flags, float *rootBox, hipStream_t s);
+                  const uint32_t *leafParent, uint32_t *work, float *rootBox, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 void launch_copy16(const void *src, void *dst, size_t n16, hipStream_t s);
 
@@ -81,8 +82,9 @@ struct GdStats {
     unsigned long long fallback;   // pairs re-run in FP64 because a sign was uncertain
     unsigned long long filtered;   // pairs decided by the FP32 filter alone
 };
+// work: n + 1 u32 of scratch (the FP32 filter's undecided list and its count)
 void launch_tritri(const double *tri64A, const double *tri64B, const float *tri32A, const float *tri32B,
-                   const uint64_t *pairs, uint32_t n, int precision, uint8_t *status, double *seg,
+                   const uint64_t *pairs, uint32_t n, int precision, uint8_t *status, double *seg, uint32_t *work,
                    GdStats *stats, hipStream_t s);
 void launch_tritri_raw(const double *a9, const double *b9, uint32_t n, uint8_t *status, double *seg,
                        hipStream_t s);
@@ -96,9 +98,12 @@ int compact_blocks(uint32_t n);
 void launch_mark_intersected(const uint64_t *pairs, const uint8_t *status, uint32_t n, int side,
                              uint8_t *flags, hipStream_t s);
 void launch_centroids(const double *tri64, const uint8_t *flags, uint32_t nt, double *pts, hipStream_t s);
+// order (may be null): query q uses point order[q]; ovfList holds 3n entries,
+// *ovfCount must be zero on entry
 void launch_rays(const BNode *nodes, uint32_t root, const float *rootBox, const double *tri64,
-                 const double *normals, uint32_t ntTarget, const double *pts, const uint8_t *skip,
-                 uint32_t n, uint8_t *axisBits, uint32_t *overflow, hipStream_t s);
+                 const double *normals, uint32_t ntTarget, const double *pts, const uint32_t *order,
+                 const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList, uint32_t *ovfCount,
+                 uint32_t *overflow, hipStream_t s);
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);
 
 }  // namespace sbg
