@@ -246,7 +246,7 @@ def main():
     tests_per_s = n_pairs / (stages["tri_tri"] * 1e-3) if stages["tri_tri"] > 0 else n_pairs / (gd_ms * 1e-3)
 
     # roofline of the dominant kernel of the step
-    dom = max(("records", "sort", "bvh", "traverse", "pair_sort", "tri_tri", "compact", "classify"),
+    dom = max(("records", "sort", "bvh", "traverse", "pair_sort", "tri_tri", "compact", "classify", "grid"),
               key=lambda k: stages[k])
     sat_bytes = gd_bytes(n_sat, hits * reps, fb * reps, 1)
     roof_gd = {

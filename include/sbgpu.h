@@ -63,6 +63,7 @@ enum {
     SBG_STAGE_TRI_TRI,     /* Guigue-Devillers kernel */
     SBG_STAGE_COMPACT,     /* hit compaction */
     SBG_STAGE_CLASSIFY,    /* ray parity + vote */
+    SBG_STAGE_GRID,        /* per-axis ray grid of a target mesh */
     SBG_STAGE_COUNT
 };
 

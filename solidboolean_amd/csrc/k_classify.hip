@@ -6,28 +6,30 @@
 // the querying mesh's Morton order, so neighbouring lanes trace parallel rays
 // from neighbouring points.  The ray is the segment p -> p + axis with
 // axis = (DBL_MAX, eps, eps) and permutations; its FP64 box is rounded outward
-// to FP32 to prune the target's LBVH; every leaf whose exact FP64 box
-// closed-overlaps the segment's box (the reference's candidate set, :55-63)
+// to FP32 for a first cut; every triangle whose exact FP64 box closed-overlaps
+// the segment's box (the reference's candidate set, :55-63)
 // gets the FP64 plane test (src/vector3.h:264-280) and the three-normal inside
 // test (:78-86) in the reference's operation order.  Hits are deduplicated by
 // PositionKey (src/positionkey.cpp:25-37); the lane's answer is the parity of
 // the distinct keys (:89).
 //
-// k_rays_fast keeps its traversal stack in LDS and up to 4 distinct keys in
-// registers; a lane that needs more appends its (point, axis) to an overflow
-// list, which k_rays_list finishes with a 64-deep private stack and 32 keys.
-// A lane past those limits raises `overflow` and the call fails: nothing is
-// guessed.
+// The candidates come from a per-axis grid over the target's projected
+// triangle boxes instead of a tree walk: a ray's candidates are the entries of
+// the one cell holding its origin, a superset of the reference's set that the
+// exact FP64 box test then cuts down to it.  k_rays_grid keeps up to 4
+// distinct keys in registers; a lane that needs more appends its (point,
+// axis) to an overflow list, which k_rays_list finishes with 32 keys.  A lane
+// past that raises `overflow` and the call fails: nothing is guessed.
 #include "sbg_device.h"
 #include "sbg_internal.h"
+
+#include <cmath>
 
 #pragma clang fp contract(off)
 
 namespace sbg {
 
-constexpr int kFastDepth = 40;
 constexpr int kFastKeys = 4;
-constexpr int kListStack = 64;
 constexpr int kListKeys = 32;
 
 struct RBox {
@@ -139,17 +141,151 @@ __device__ __forceinline__ void ray_leaf(uint32_t t, const double *__restrict__ 
     keys.add(position_key(hit));
 }
 
-__global__ __launch_bounds__(256) void k_rays_fast(const BNode *__restrict__ nodes, uint32_t root,
-                                                   const float *__restrict__ rootBox, const double *__restrict__ tri64,
+// ---- per-axis projection grids of the target mesh -------------------------
+// For a ray along `axis`, (u, v) are the other two coordinates.  The target's
+// root box is split into G x G cells in (u, v); cell(x) is monotone in x, so a
+// triangle filed under every cell its slack-widened (u, v) box spans is filed
+// under the cell of every ray origin whose segment box it can overlap: the
+// segment covers [p.u, p.u + eps] and the slack (1e-15 (1 + |lo|)) exceeds eps
+// plus the rounding of lo - slack.  Each entry holds the triangle's FP32
+// outward-rounded (u, v) box and its upper bound along `axis`, so most of a
+// cell's entries are rejected from 24 B without touching the FP64 record.
+struct GridFrame {
+    double lu, lv, iu, iv;
+};
+
+__device__ __forceinline__ int axis_u(int axis) { return axis == 0 ? 1 : 0; }
+__device__ __forceinline__ int axis_v(int axis) { return axis == 2 ? 1 : 2; }
+
+__device__ __forceinline__ GridFrame grid_frame(const float *__restrict__ rootBox, int axis, uint32_t G)
+{
+    const int u = axis_u(axis), v = axis_v(axis);
+    GridFrame f;
+    f.lu = (double)rootBox[u];
+    f.lv = (double)rootBox[v];
+    const double su = (double)rootBox[3 + u] - f.lu, sv = (double)rootBox[3 + v] - f.lv;
+    f.iu = su > 0.0 ? (double)G / su : 0.0;
+    f.iv = sv > 0.0 ? (double)G / sv : 0.0;
+    return f;
+}
+
+__device__ __forceinline__ uint32_t grid_cell(double x, double lo, double inv, uint32_t G)
+{
+    const double t = (x - lo) * inv;
+    if (!(t >= 0.0))
+        return 0;
+    return t < (double)G ? (uint32_t)t : G - 1u;
+}
+
+__device__ __forceinline__ double comp(const V3 &a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+struct GridSpan {
+    uint32_t u0, u1, v0, v1;
+    float4 box;   // FP32 (lo_u, hi_u, lo_v, hi_v), outward
+    float hiAxis; // FP32 upper bound of the box along the ray axis
+};
+
+__device__ __forceinline__ GridSpan grid_span(const double *__restrict__ tri64, uint32_t t, int axis,
+                                              const GridFrame &f, uint32_t G)
+{
+    const double2 *x = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * t);
+    const double2 r0 = x[0], r1 = x[1], r2 = x[2], r3 = x[3], r4 = x[4], r5 = x[5];
+    const Box64 b = tri_box64({r0.x, r0.y, r1.x}, {r2.x, r2.y, r3.x}, {r4.x, r4.y, r5.x});
+    const int u = axis_u(axis), v = axis_v(axis);
+    const double lu = comp(b.mn, u), hu = comp(b.mx, u), lv = comp(b.mn, v), hv = comp(b.mx, v);
+    GridSpan s;
+    s.u0 = grid_cell(lu - 1e-15 * (1.0 + __builtin_fabs(lu)), f.lu, f.iu, G);
+    s.u1 = grid_cell(hu, f.lu, f.iu, G);
+    s.v0 = grid_cell(lv - 1e-15 * (1.0 + __builtin_fabs(lv)), f.lv, f.iv, G);
+    s.v1 = grid_cell(hv, f.lv, f.iv, G);
+    s.box = make_float4(f_down(lu), f_up(hu), f_down(lv), f_up(hv));
+    s.hiAxis = f_up(comp(b.mx, axis));
+    return s;
+}
+
+// cells: 3 G^2 + 1 u32, zero on entry; entries of cell c are counted in cells[c + 1]
+__global__ __launch_bounds__(256) void k_grid_count(const double *__restrict__ tri64, uint32_t nt,
+                                                    const float *__restrict__ rootBox, uint32_t G,
+                                                    uint32_t *__restrict__ cells)
+{
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= 3u * nt)
+        return;
+    const int axis = (int)(g / nt);
+    const uint32_t t = g - (uint32_t)axis * nt;
+    const GridFrame f = grid_frame(rootBox, axis, G);
+    const GridSpan s = grid_span(tri64, t, axis, f, G);
+    const size_t base = (size_t)axis * G * G + 1;
+    for (uint32_t cu = s.u0; cu <= s.u1; ++cu)
+        for (uint32_t cv = s.v0; cv <= s.v1; ++cv)
+            atomicAdd(&cells[base + (size_t)cu * G + cv], 1u);
+}
+
+// after the exclusive scan, cells[c + 1] is the start of cell c; each entry
+// advances it, so afterwards cell c spans [cells[c], cells[c + 1])
+__global__ __launch_bounds__(256) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt,
+                                                   const float *__restrict__ rootBox, uint32_t G,
+                                                   uint32_t *__restrict__ cells, float4 *__restrict__ gbox,
+                                                   uint2 *__restrict__ gref, uint32_t cap)
+{
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= 3u * nt)
+        return;
+    const int axis = (int)(g / nt);
+    const uint32_t t = g - (uint32_t)axis * nt;
+    const GridFrame f = grid_frame(rootBox, axis, G);
+    const GridSpan s = grid_span(tri64, t, axis, f, G);
+    const size_t base = (size_t)axis * G * G + 1;
+    for (uint32_t cu = s.u0; cu <= s.u1; ++cu)
+        for (uint32_t cv = s.v0; cv <= s.v1; ++cv) {
+            const uint32_t slot = atomicAdd(&cells[base + (size_t)cu * G + cv], 1u);
+            if (slot < cap) {
+                gbox[slot] = s.box;
+                gref[slot] = make_uint2(t, __float_as_uint(s.hiAxis));
+            }
+        }
+}
+
+// every candidate of the ray's cell, until the key set is full
+template <class Keys>
+__device__ __forceinline__ void ray_grid(const GridArgs &ga, const double *__restrict__ tri64,
+                                         const double *__restrict__ normals, const Ray &r, int axis, Keys &keys)
+{
+    const GridFrame f = grid_frame(ga.rootBox, axis, ga.G);
+    const int u = axis_u(axis), v = axis_v(axis);
+    const uint32_t cu = grid_cell(comp(r.p, u), f.lu, f.iu, ga.G);
+    const uint32_t cv = grid_cell(comp(r.p, v), f.lv, f.iv, ga.G);
+    const size_t c = (size_t)axis * ga.G * ga.G + (size_t)cu * ga.G + cv;
+    const uint32_t k0 = ga.cells[c], k1 = ga.cells[c + 1];
+    const float rlu = u == 0 ? r.rf.lx : r.rf.ly, rhu = u == 0 ? r.rf.hx : r.rf.hy;
+    const float rlv = v == 1 ? r.rf.ly : r.rf.lz, rhv = v == 1 ? r.rf.hy : r.rf.hz;
+    const float rla = axis == 0 ? r.rf.lx : (axis == 1 ? r.rf.ly : r.rf.lz);
+    for (uint32_t k = k0; k < k1 && !keys.full; ++k) {
+        const float4 e = ga.gbox[k];
+        if (!(e.x <= rhu && e.y >= rlu && e.z <= rhv && e.w >= rlv))
+            continue;
+        const uint2 ref = ga.gref[k];
+        if (__uint_as_float(ref.y) >= rla)
+            ray_leaf(ref.x, tri64, normals, r, keys);
+    }
+}
+
+// One lane per (axis, query) with up to kFastKeys distinct keys; a lane that
+// needs more hands (axis, query) to the list kernel.
+__global__ __launch_bounds__(256) void k_rays_grid(GridArgs ga, const double *__restrict__ tri64,
                                                    const double *__restrict__ normals, const double *__restrict__ pts,
                                                    const uint32_t *__restrict__ order, const uint8_t *__restrict__ skip,
                                                    uint32_t n, uint8_t *__restrict__ bits, uint32_t *__restrict__ ovfList,
-                                                   uint32_t *ovfCount, uint32_t maxVisits)
+                                                   uint32_t *ovfCount, uint32_t *overflow)
 {
-    __shared__ uint32_t s_stack[kFastDepth][256];
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
     if (g >= 3u * n)
         return;
+    if (ga.cells[3 * (size_t)ga.G * ga.G] > ga.cap) {  // grid truncated: the host rebuilds it larger
+        if ((threadIdx.x & 63) == 0)
+            atomicOr(overflow, 2u);
+        return;
+    }
     const uint32_t axis = g / n;
     const uint32_t q = g - axis * n;
     const uint32_t i = order ? order[q] : q;
@@ -157,62 +293,8 @@ __global__ __launch_bounds__(256) void k_rays_fast(const BNode *__restrict__ nod
         return;
     const Ray r = ray_setup(pts, i, axis);
     KeySet<kFastKeys> keys;
-    bool ovf = false;
-    if (rbox_overlap(rootBox[0], rootBox[1], rootBox[2], rootBox[3], rootBox[4], rootBox[5], r.rf)) {
-        if (root & kLeafBit) {
-            ray_leaf(root & ~kLeafBit, tri64, normals, r, keys);
-        } else {
-            uint32_t *stack = &s_stack[0][threadIdx.x];  // entry k at stack[k * 256]
-            int sp = 0;
-            uint32_t cur = root;
-            uint32_t visits = 0;
-            // visit cur; descend into an overlapping child, push the other
-            for (;;) {
-                if (++visits > maxVisits || keys.full) {
-                    ovf = true;
-                    break;
-                }
-                const BNode *nd = nodes + cur;
-                const float4 a = nd->a, b = nd->b, c = nd->c;
-                const int4 d = nd->d;
-                const bool o0 = rbox_overlap(a.x, a.y, a.z, a.w, b.x, b.y, r.rf);
-                const bool o1 = rbox_overlap(b.z, b.w, c.x, c.y, c.z, c.w, r.rf);
-                const uint32_t c0 = (uint32_t)d.x, c1 = (uint32_t)d.y;
-                bool go0 = false, go1 = false;
-                if (o0) {
-                    if (c0 & kLeafBit)
-                        ray_leaf(c0 & ~kLeafBit, tri64, normals, r, keys);
-                    else
-                        go0 = true;
-                }
-                if (o1) {
-                    if (c1 & kLeafBit)
-                        ray_leaf(c1 & ~kLeafBit, tri64, normals, r, keys);
-                    else
-                        go1 = true;
-                }
-                if (go0 && go1) {
-                    if (sp == kFastDepth) {
-                        ovf = true;
-                        break;
-                    }
-                    stack[sp * 256] = c1;
-                    ++sp;
-                    cur = c0;
-                } else if (go0) {
-                    cur = c0;
-                } else if (go1) {
-                    cur = c1;
-                } else if (sp > 0) {
-                    --sp;
-                    cur = stack[sp * 256];
-                } else {
-                    break;
-                }
-            }
-        }
-    }
-    if (ovf || keys.full) {
+    ray_grid(ga, tri64, normals, r, (int)axis, keys);
+    if (keys.full) {
         const uint32_t slot = atomicAdd(ovfCount, 1u);
         ovfList[slot] = g;  // slot < 3n: every lane appends at most once
         return;
@@ -220,14 +302,12 @@ __global__ __launch_bounds__(256) void k_rays_fast(const BNode *__restrict__ nod
     bits[(size_t)axis * n + i] = (uint8_t)(keys.n & 1);
 }
 
-// lanes the fast kernel handed over: (axis, query) encoded as g = axis * n + q
-__global__ __launch_bounds__(256) void k_rays_list(const BNode *__restrict__ nodes, uint32_t root,
-                                                   const double *__restrict__ tri64,
+// lanes handed over by k_rays_grid, g = axis * n + q, with kListKeys keys
+__global__ __launch_bounds__(256) void k_rays_list(GridArgs ga, const double *__restrict__ tri64,
                                                    const double *__restrict__ normals, const double *__restrict__ pts,
                                                    const uint32_t *__restrict__ order, uint32_t n,
                                                    uint8_t *__restrict__ bits, const uint32_t *__restrict__ list,
-                                                   const uint32_t *__restrict__ count, uint32_t *overflow,
-                                                   uint32_t maxVisits)
+                                                   const uint32_t *__restrict__ count, uint32_t *overflow)
 {
     const uint32_t total = *count;
     for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < total; k += gridDim.x * 256) {
@@ -237,41 +317,8 @@ __global__ __launch_bounds__(256) void k_rays_list(const BNode *__restrict__ nod
         const uint32_t i = order ? order[q] : q;
         const Ray r = ray_setup(pts, i, axis);
         KeySet<kListKeys> keys;
-        bool ovf = false;
-        uint32_t stack[kListStack];
-        int sp = 0;
-        if (root & kLeafBit)
-            ray_leaf(root & ~kLeafBit, tri64, normals, r, keys);
-        else
-            stack[sp++] = root;
-        uint32_t visits = 0;
-        while (sp > 0) {
-            if (++visits > maxVisits) {
-                ovf = true;
-                break;
-            }
-            const BNode *nd = nodes + stack[--sp];
-            const float4 a = nd->a, b = nd->b, c = nd->c;
-            const int4 d = nd->d;
-            const uint32_t c0 = (uint32_t)d.x, c1 = (uint32_t)d.y;
-            if (rbox_overlap(b.z, b.w, c.x, c.y, c.z, c.w, r.rf)) {
-                if (c1 & kLeafBit)
-                    ray_leaf(c1 & ~kLeafBit, tri64, normals, r, keys);
-                else if (sp < kListStack)
-                    stack[sp++] = c1;
-                else
-                    ovf = true;
-            }
-            if (rbox_overlap(a.x, a.y, a.z, a.w, b.x, b.y, r.rf)) {
-                if (c0 & kLeafBit)
-                    ray_leaf(c0 & ~kLeafBit, tri64, normals, r, keys);
-                else if (sp < kListStack)
-                    stack[sp++] = c0;
-                else
-                    ovf = true;
-            }
-        }
-        if (ovf || keys.full)
+        ray_grid(ga, tri64, normals, r, (int)axis, keys);
+        if (keys.full)
             atomicOr(overflow, 1u);
         bits[(size_t)axis * n + i] = (uint8_t)(keys.n & 1);
     }
@@ -308,18 +355,38 @@ void launch_centroids(const double *tri64, const uint8_t *flags, uint32_t nt, do
     hipLaunchKernelGGL(k_centroids, dim3((nt + 255) / 256), dim3(256), 0, s, tri64, flags, nt, pts);
 }
 
-void launch_rays(const BNode *nodes, uint32_t root, const float *rootBox, const double *tri64,
-                 const double *normals, uint32_t ntTarget, const double *pts, const uint32_t *order,
-                 const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList, uint32_t *ovfCount,
-                 uint32_t *overflow, hipStream_t s)
+uint32_t grid_dim(uint32_t nt)
+{
+    uint32_t G = (uint32_t)std::ceil(std::sqrt((double)nt / 4.0));
+    return G < 1u ? 1u : (G > 1024u ? 1024u : G);
+}
+
+size_t grid_cells(uint32_t G) { return 3 * (size_t)G * G + 1; }
+
+void launch_grid_build(const double *tri64, uint32_t nt, const float *rootBox, uint32_t G, uint32_t *cells,
+                       uint32_t *scanTemp, float4 *gbox, uint2 *gref, uint32_t cap, hipStream_t s)
+{
+    const size_t m = grid_cells(G);
+    launch_fill_u32(cells, 0u, m, s);
+    if (nt == 0)
+        return;
+    const unsigned blocks = (unsigned)((3ull * nt + 255) / 256);
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks), dim3(256), 0, s, tri64, nt, rootBox, G, cells);
+    launch_scan_large(cells, m, scanTemp, s);
+    hipLaunchKernelGGL(k_grid_fill, dim3(blocks), dim3(256), 0, s, tri64, nt, rootBox, G, cells, gbox, gref, cap);
+}
+
+void launch_rays(const GridArgs &ga, const double *tri64, const double *normals, const double *pts,
+                 const uint32_t *order, const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList,
+                 uint32_t *ovfCount, uint32_t *overflow, hipStream_t s)
 {
     if (n == 0)
         return;
     const uint64_t lanes = 3ull * n;
-    hipLaunchKernelGGL(k_rays_fast, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, nodes, root, rootBox,
-                       tri64, normals, pts, order, skip, n, axisBits, ovfList, ovfCount, ntTarget + 1u);
-    hipLaunchKernelGGL(k_rays_list, dim3(256), dim3(256), 0, s, nodes, root, tri64, normals, pts, order, n, axisBits,
-                       ovfList, ovfCount, overflow, ntTarget + 1u);
+    hipLaunchKernelGGL(k_rays_grid, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, ga, tri64, normals, pts,
+                       order, skip, n, axisBits, ovfList, ovfCount, overflow);
+    hipLaunchKernelGGL(k_rays_list, dim3(256), dim3(256), 0, s, ga, tri64, normals, pts, order, n, axisBits, ovfList,
+                       ovfCount, overflow);
 }
 
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s)

@@ -169,51 +169,59 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_tiles(const uint32_t *__r
 }
 
 // Refit, stage 2: one workgroup finishes the top nodes (those spanning tiles)
-// in rounds.  Mark phase: a node whose two children are complete
-// (topFlags == 2) is marked 4.  Merge phase (after a barrier): every marked
-// node merges its child boxes into its parent's slot, counts itself in the
-// parent's flag and becomes 3 (done).  Slots read in a merge phase were all
-// written before its barrier.  Every round finishes at least the lowest
-// unfinished top node, so the loop ends within ntop + 1 rounds.
+// from a ready list.  A top node is ready once both children are complete
+// (topFlags == 2).  Each round merges every ready node's two child boxes into
+// its parent's slot and counts the arrival in the parent's flag; the arrival
+// that brings a parent to 2 appends it to the next round's list.  Every node
+// is merged once, so the work is O(ntop) and the number of rounds is the
+// height of the top tree.  All hand-offs stay inside the workgroup (barriers
+// and workgroup-scope fences); the loop ends when a round's list is empty,
+// which the root's merge guarantees within ntop + 1 rounds.
 __global__ __launch_bounds__(1024) void k_refit_top(BNode *__restrict__ nodes, const uint32_t *__restrict__ topList,
                                                     const uint32_t *__restrict__ topCount, uint32_t *topFlags,
+                                                    uint32_t *__restrict__ readyA, uint32_t *__restrict__ readyB,
                                                     float *rootBox)
 {
-    __shared__ int s_progress;
+    __shared__ uint32_t s_n[2];
     const uint32_t ntop = *topCount;
+    if (threadIdx.x == 0) {
+        s_n[0] = 0;
+        s_n[1] = 0;
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < ntop; k += 1024) {
+        const uint32_t p = topList[k];
+        if (topFlags[p] == 2)
+            readyA[atomicAdd(&s_n[0], 1u)] = p;
+    }
+    int cur = 0;
     for (uint32_t round = 0; round <= ntop; ++round) {
-        if (threadIdx.x == 0)
-            s_progress = 0;
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < ntop; k += 1024) {
-            const uint32_t p = topList[k];
-            if (__hip_atomic_load(topFlags + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 2) {
-                __hip_atomic_store(topFlags + p, 4u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                s_progress = 1;
-            }
-        }
-        __syncthreads();
-        if (!s_progress)
+        const uint32_t m = s_n[cur];
+        if (m == 0)
             break;
-        for (uint32_t k = threadIdx.x; k < ntop; k += 1024) {
-            const uint32_t p = topList[k];
-            if (__hip_atomic_load(topFlags + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != 4)
-                continue;
+        uint32_t *in = cur ? readyB : readyA;
+        uint32_t *out = cur ? readyA : readyB;
+        if (threadIdx.x == 0)
+            s_n[cur ^ 1] = 0;
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < m; k += 1024) {
+            const uint32_t p = in[k];
             const volatile float *b = reinterpret_cast<const volatile float *>(nodes + p);
             const float4 lo = make_float4(fminf(b[0], b[6]), fminf(b[1], b[7]), fminf(b[2], b[8]), 0.0f);
             const float4 hi = make_float4(fmaxf(b[3], b[9]), fmaxf(b[4], b[10]), fmaxf(b[5], b[11]), 0.0f);
-            __hip_atomic_store(topFlags + p, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             const int parent = nodes[p].d.z;
             if (parent < 0) {
                 write_root(rootBox, lo, hi);
-            } else {
-                BNode *pn = nodes + parent;
-                store_child_box(pn, pn->d.x == (int)p ? 0 : 1, lo, hi);
-                __threadfence_block();
-                atomicAdd(topFlags + parent, 1u);
+                continue;
             }
+            BNode *pn = nodes + parent;
+            store_child_box(pn, pn->d.x == (int)p ? 0 : 1, lo, hi);
+            __threadfence_block();
+            if (atomicAdd(topFlags + parent, 1u) == 1u)
+                out[atomicAdd(&s_n[cur ^ 1], 1u)] = (uint32_t)parent;
         }
-        __syncthreads();
+        cur ^= 1;
     }
 }
 
@@ -246,7 +254,8 @@ void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, value, n);
 }
 
-// work: 2n + 1 u32 = top flags [0, n-1), top list [n, 2n-1), top count [2n]
+// work: 4n + 1 u32 = top flags [0, n-1), top list [n, 2n-1), top count [2n],
+// two ready lists of the top refit [2n + 1, 3n + 1) and [3n + 1, 4n + 1)
 void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
                       uint32_t *leafParent, uint32_t *work, hipStream_t s)
 {
@@ -265,7 +274,7 @@ void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *no
     hipLaunchKernelGGL(k_refit_tiles, dim3((n + kRefitTile - 1) / kRefitTile), dim3(kRefitBlock), 0, s, ids, box32,
                        (int)n, nodes, leafParent, work, rootBox);
     hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, nodes, work + n, work + 2 * (size_t)n, work,
-                       rootBox);
+                       work + 2 * (size_t)n + 1, work + 3 * (size_t)n + 1, rootBox);
 }
 
 }  // namespace sbg

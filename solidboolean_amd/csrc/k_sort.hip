@@ -67,6 +67,75 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t *__restrict__ data, size
     }
 }
 
+// ---- exclusive scan of a long u32 array, three launches --------------------
+// Tiles of 4096 entries (16 per thread): tile sums, a one-block scan of the
+// sums, then each tile rescans its entries from its base.
+constexpr int kScanTile = 4096;
+
+__global__ __launch_bounds__(256) void k_scan_tile_sum(const uint32_t *__restrict__ data, size_t m,
+                                                       uint32_t *__restrict__ tileSum)
+{
+    __shared__ uint32_t s[256];
+    const size_t base = (size_t)blockIdx.x * kScanTile;
+    uint32_t sum = 0;
+    for (int k = 0; k < kScanTile / 256; ++k) {
+        const size_t i = base + (size_t)k * 256 + threadIdx.x;
+        if (i < m)
+            sum += data[i];
+    }
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            s[threadIdx.x] += s[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        tileSum[blockIdx.x] = s[0];
+}
+
+__global__ __launch_bounds__(256) void k_scan_tile_apply(uint32_t *__restrict__ data, size_t m,
+                                                         const uint32_t *__restrict__ tileBase)
+{
+    __shared__ uint32_t s[256];
+    constexpr int per = kScanTile / 256;
+    const size_t lo = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * per;
+    uint32_t v[per];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        v[k] = lo + k < m ? data[lo + k] : 0u;
+        sum += v[k];
+    }
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = tileBase[blockIdx.x] + (threadIdx.x ? s[threadIdx.x - 1] : 0u);
+#pragma unroll
+    for (int k = 0; k < per; ++k) {
+        if (lo + k < m)
+            data[lo + k] = run;
+        run += v[k];
+    }
+}
+
+size_t scan_large_temp(size_t m) { return (m + kScanTile - 1) / kScanTile + 1; }
+
+void launch_scan_large(uint32_t *data, size_t m, uint32_t *temp, hipStream_t s)
+{
+    if (m == 0)
+        return;
+    const size_t tiles = (m + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_tile_sum, dim3((unsigned)tiles), dim3(256), 0, s, data, m, temp);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, temp, tiles);
+    hipLaunchKernelGGL(k_scan_tile_apply, dim3((unsigned)tiles), dim3(256), 0, s, data, m, temp);
+}
+
 // exclusive scan of each digit's row of the histogram (one workgroup per
 // digit); the row's total goes to rowTotal[digit]
 __global__ __launch_bounds__(256) void k_scan_rows(uint32_t *__restrict__ hist, int nblocks,

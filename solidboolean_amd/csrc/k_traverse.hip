@@ -131,6 +131,126 @@ __global__ __launch_bounds__(256) void k_expand(TraverseArgs t, const uint2 *__r
     }
 }
 
+// Persistent depth-first stage.  One wave per workgroup keeps a stack of node
+// pairs in LDS (spilling to a per-wave region of global memory when full).
+// Each step pops up to 64 pairs (one per lane), expands them as k_expand
+// does, appends candidates with one atomic per wave, and pushes the surviving
+// child pairs back with a 64-lane prefix sum.  An empty wave claims the next
+// few seed pairs (the breadth-first frontier) from a shared counter.  Waves never wait for
+// each other: a wave leaves when its own stacks are empty and no seed is left.
+// Spill overflow or the iteration cap raise `overflow` (the host then repeats
+// the search breadth-first), so nothing is dropped silently.
+constexpr int kDfsLds = 1024;
+constexpr uint32_t kDfsMaxIter = 1u << 24;
+
+__global__ __launch_bounds__(64) void k_traverse_dfs(TraverseArgs t, const uint2 *__restrict__ seeds,
+                                                     const uint32_t *__restrict__ seedCount, uint32_t seedCap,
+                                                     uint32_t *head, uint2 *__restrict__ spill, uint32_t spillPerWave,
+                                                     uint64_t *__restrict__ cand, uint32_t *candCount,
+                                                     uint32_t candCap, uint32_t *overflow)
+{
+    __shared__ uint2 s_stack[kDfsLds];
+    const int lane = threadIdx.x;
+    const uint32_t nseed = *seedCount < seedCap ? *seedCount : seedCap;
+    // seeds per claim: enough for every wave to start busy, at most one per lane
+    uint32_t grab = nseed / gridDim.x;
+    grab = grab < 1u ? 1u : (grab > 64u ? 64u : grab);
+    uint2 *gstack = spill + (size_t)blockIdx.x * spillPerWave;
+    int sp = 0;          // LDS entries (same value in every lane)
+    uint32_t gsp = 0;    // spilled entries (same value in every lane)
+    uint32_t iter = 0;
+    for (; iter < kDfsMaxIter; ++iter) {
+        if (sp == 0 && gsp > 0) {
+            const uint32_t m = gsp < (uint32_t)(kDfsLds / 2) ? gsp : (uint32_t)(kDfsLds / 2);
+            for (uint32_t k = lane; k < m; k += 64)
+                s_stack[k] = gstack[gsp - m + k];
+            gsp -= m;
+            sp = (int)m;
+            __syncthreads();
+        }
+        if (sp == 0) {
+            uint32_t start = 0;
+            if (lane == 0)
+                start = atomicAdd(head, grab);
+            start = __shfl(start, 0, 64);
+            if (start >= nseed)
+                break;
+            const uint32_t m = nseed - start < grab ? nseed - start : grab;
+            if ((uint32_t)lane < m)
+                s_stack[lane] = seeds[start + lane];
+            sp = (int)m;
+            __syncthreads();
+        }
+        const int take = sp < 64 ? sp : 64;
+        const bool valid = lane < take;
+        uint2 pr = make_uint2(0, 0);
+        if (valid)
+            pr = s_stack[sp - take + lane];
+        sp -= take;
+        __syncthreads();
+        uint32_t ca[2], cb[2];
+        FBox ba[2], bb[2];
+        int na = 0, nb = 0;
+        if (valid) {
+            na = children(t.nodesA, t.boxA, pr.x, ca, ba);
+            nb = children(t.nodesB, t.boxB, pr.y, cb, bb);
+        }
+        uint2 nxt[4];
+        uint64_t cnd[4];
+        uint32_t nn = 0, nc = 0;
+        for (int u = 0; u < na; ++u)
+            for (int v = 0; v < nb; ++v) {
+                if (!fbox_overlap(ba[u], bb[v]))
+                    continue;
+                if ((ca[u] & kLeafBit) && (cb[v] & kLeafBit)) {
+                    const uint32_t ta = ca[u] & ~kLeafBit, tb = cb[v] & ~kLeafBit;
+                    if (box64_overlap(record_box(t.tri64A, ta), record_box(t.tri64B, tb)))
+                        cnd[nc++] = ((uint64_t)ta << 32) | tb;
+                } else {
+                    nxt[nn++] = make_uint2(ca[u], cb[v]);
+                }
+            }
+        uint32_t wb;
+        const uint32_t oc = wave_reserve(nc, candCount, candCap, overflow, wb);
+        for (uint32_t k = 0; k < nc; ++k)
+            if (oc + k < candCap)
+                cand[oc + k] = cnd[k];
+        // push children: LDS when they fit, else the wave's spill region
+        uint32_t incl = nn;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t v = __shfl_up(incl, off, 64);
+            if (lane >= off)
+                incl += v;
+        }
+        const uint32_t total = __shfl(incl, 63, 64);
+        const uint32_t mine = incl - nn;
+        if (sp + (int)total <= kDfsLds) {
+            for (uint32_t k = 0; k < nn; ++k)
+                s_stack[sp + mine + k] = nxt[k];
+            sp += (int)total;
+        } else if (gsp + total <= spillPerWave) {
+            for (uint32_t k = 0; k < nn; ++k)
+                gstack[gsp + mine + k] = nxt[k];
+            gsp += total;
+        } else {
+            if (lane == 0)
+                atomicOr(overflow, 4u);
+            break;
+        }
+        __syncthreads();
+    }
+    if (iter == kDfsMaxIter && lane == 0)
+        atomicOr(overflow, 8u);
+}
+
+void launch_traverse_dfs(const TraverseArgs &t, const uint2 *seeds, const uint32_t *seedCount, uint32_t seedCap,
+                         uint32_t *head, uint2 *spill, uint32_t spillPerWave, int waves, uint64_t *cand,
+                         uint32_t *candCount, uint32_t candCap, uint32_t *overflow, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_traverse_dfs, dim3(waves), dim3(64), 0, s, t, seeds, seedCount, seedCap, head, spill,
+                       spillPerWave, cand, candCount, candCap, overflow);
+}
+
 // root pair: both meshes non-empty; a root may itself be a leaf (1 triangle)
 __global__ void k_seed(TraverseArgs t, uint32_t rootA, uint32_t rootB, const float *__restrict__ rootBoxA,
                        const float *__restrict__ rootBoxB, uint2 *frontier, uint32_t *count, uint64_t *cand,

@@ -9,10 +9,11 @@
 //    predicate the reference would evaluate from the 48-B FP32 records and
 //    certifies its sign against a per-pair forward error bound.  A pair whose
 //    certified path ends in a rejection is decided there; every other pair (a
-//    certified intersection, or any uncertain sign) is appended to a list that
-//    k_gd_list re-runs with the full FP64 sequence, so every status and every
-//    segment equals the FP64 reference's.  Splitting the two keeps waves from
-//    running both paths for the few lanes that hit.
+//    certified intersection, or any uncertain sign) is written to its wave
+//    chunk's slice of a list that k_gd_list re-runs with the full FP64
+//    sequence, so every status and every segment equals the FP64 reference's.
+//    Splitting the two keeps waves from running both paths for the few lanes
+//    that hit; per-chunk slices keep the list free of shared atomics.
 // Per pair and launch: 8 B pair + 1 B status + 2 x 96 B (FP64) or 2 x 48 B
 // (filtered, + 2 x 96 B + 4 B list entry for hits / fallbacks) + 48 B segment
 // per hit.
@@ -176,11 +177,22 @@ __device__ __forceinline__ int gd_filter(const F3 &p1, const F3 &q1, const F3 &r
     return r ? GD_HIT : GD_REJECT_INTERVAL;
 }
 
-__device__ __forceinline__ void wave_count(bool pred, unsigned long long *counter)
+// ---- wave chunks ------------------------------------------------------------
+// Pairs are processed in wave chunks of 64 * items consecutive pairs, one wave
+// per chunk.  The filter writes the chunk's undecided pairs into the chunk's
+// own slice of the list (list[base, base + count)), so no queue counter is
+// shared between waves and nothing needs an atomic; k_gd_list then gives each
+// chunk's slice to one wave.  Statistics are kept as wave-uniform counts and
+// written once per chunk (8 u32: five exits, fallback, filtered, spare) for a
+// one-block reduction.
+constexpr int kStatSlots = 8;
+
+__device__ __forceinline__ uint32_t wave_popc(bool pred) { return (uint32_t)__popcll(__ballot(pred)); }
+
+// ex[k] for k < 5, else 0, without indexing a register array by a lane value
+__device__ __forceinline__ uint32_t pick5(const uint32_t ex[5], int k)
 {
-    const unsigned long long m = __ballot(pred);
-    if ((threadIdx.x & 63) == 0 && m)
-        atomicAdd(counter, (unsigned long long)__popcll(m));
+    return k == 0 ? ex[0] : k == 1 ? ex[1] : k == 2 ? ex[2] : k == 3 ? ex[3] : k == 4 ? ex[4] : 0u;
 }
 
 __device__ __forceinline__ void gd_pair_fp64(const double *__restrict__ tri64A, const double *__restrict__ tri64B,
@@ -203,80 +215,142 @@ __device__ __forceinline__ void gd_pair_fp64(const double *__restrict__ tri64A, 
 
 // plain FP64 over every pair
 __global__ __launch_bounds__(256) void k_gd_fp64(const double *__restrict__ tri64A, const double *__restrict__ tri64B,
-                                                 const uint64_t *__restrict__ pairs, uint32_t n,
+                                                 const uint64_t *__restrict__ pairs, uint32_t n, uint32_t items,
                                                  uint8_t *__restrict__ status, double *__restrict__ seg,
-                                                 GdStats *__restrict__ stats)
+                                                 uint32_t *__restrict__ partial)
 {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const bool valid = i < n;
-    int exitKind = -1;
-    if (valid)
-        gd_pair_fp64(tri64A, tri64B, pairs[i], i, status, seg, exitKind);
-    if (stats) {
-#pragma unroll
-        for (int k = 0; k < 5; ++k)
-            wave_count(valid && exitKind == k, &stats->exits[k]);
-    }
-}
-
-// FP32 certificates; undecided pairs go to list (count in *listCount)
-__global__ __launch_bounds__(256) void k_gd_filter(const float *__restrict__ tri32A, const float *__restrict__ tri32B,
-                                                   const uint64_t *__restrict__ pairs, uint32_t n,
-                                                   uint8_t *__restrict__ status, uint32_t *__restrict__ list,
-                                                   uint32_t *listCount, GdStats *__restrict__ stats)
-{
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    const bool valid = i < n;
-    int f = -3;
-    if (valid) {
-        const uint64_t pr = pairs[i];
-        F3 p1, q1, r1, p2, q2, r2;
-        float extA, magA, extB, magB;
-        load_tri32(tri32A, (uint32_t)(pr >> 32), p1, q1, r1, extA, magA);
-        load_tri32(tri32B, (uint32_t)pr, p2, q2, r2, extB, magB);
-        f = gd_filter(p1, q1, r1, p2, q2, r2, pair_bound(p1, p2, extA, extB, magA, magB));
-        if (f >= 0 && f <= GD_REJECT_INTERVAL)
-            status[i] = 0;
-    }
-    const bool pass = valid && (f < 0 || f == GD_HIT);
-    // wave-aggregated append of the undecided pairs
-    const unsigned long long m = __ballot(pass);
     const int lane = threadIdx.x & 63;
-    uint32_t base = 0;
-    if (lane == 0 && m)
-        base = atomicAdd(listCount, (uint32_t)__popcll(m));
-    base = __shfl(base, 0, 64);
-    if (pass)
-        list[base + __popcll(m & ((1ull << lane) - 1ull))] = i;
-    if (stats) {
+    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t base = (uint64_t)w * 64u * items;
+    if (base >= n)
+        return;
+    uint32_t ex[5] = {0, 0, 0, 0, 0};
+    for (uint32_t it = 0; it < items; ++it) {
+        const uint64_t i = base + 64u * it + lane;
+        const bool valid = i < n;
+        int exitKind = -1;
+        if (valid)
+            gd_pair_fp64(tri64A, tri64B, pairs[i], (uint32_t)i, status, seg, exitKind);
+        if (partial) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            wave_count(valid && f == k, &stats->exits[k]);
-        wave_count(valid && f < 0, &stats->fallback);
-        wave_count(valid && f >= 0 && f <= GD_REJECT_INTERVAL, &stats->filtered);
+            for (int k = 0; k < 5; ++k)
+                ex[k] += wave_popc(valid && exitKind == k);
+        }
+    }
+    if (partial && lane < kStatSlots)
+        partial[(size_t)w * kStatSlots + lane] = pick5(ex, lane);
+}
+
+// FP32 certificates; each chunk's undecided pairs go to its slice of `list`
+__global__ __launch_bounds__(256) void k_gd_filter(const float *__restrict__ tri32A, const float *__restrict__ tri32B,
+                                                   const uint64_t *__restrict__ pairs, uint32_t n, uint32_t items,
+                                                   uint8_t *__restrict__ status, uint32_t *__restrict__ list,
+                                                   uint32_t *__restrict__ chunkCount, uint32_t *__restrict__ partial)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t base = (uint64_t)w * 64u * items;
+    if (base >= n)
+        return;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    uint32_t cnt = 0, e0 = 0, e1 = 0, e2 = 0, fb = 0, ft = 0;
+    for (uint32_t it = 0; it < items; ++it) {
+        const uint64_t i = base + 64u * it + lane;
+        const bool valid = i < n;
+        int f = -3;
+        if (valid) {
+            const uint64_t pr = pairs[i];
+            F3 p1, q1, r1, p2, q2, r2;
+            float extA, magA, extB, magB;
+            load_tri32(tri32A, (uint32_t)(pr >> 32), p1, q1, r1, extA, magA);
+            load_tri32(tri32B, (uint32_t)pr, p2, q2, r2, extB, magB);
+            f = gd_filter(p1, q1, r1, p2, q2, r2, pair_bound(p1, p2, extA, extB, magA, magB));
+            if (f >= 0 && f <= GD_REJECT_INTERVAL)
+                status[i] = 0;
+        }
+        const bool pass = valid && (f < 0 || f == GD_HIT);
+        const unsigned long long m = __ballot(pass);
+        if (pass)
+            list[base + cnt + (uint32_t)__popcll(m & below)] = (uint32_t)i;
+        cnt += (uint32_t)__popcll(m);
+        if (partial) {
+            e0 += wave_popc(valid && f == 0);
+            e1 += wave_popc(valid && f == 1);
+            e2 += wave_popc(valid && f == 2);
+            fb += wave_popc(valid && f < 0);
+            ft += wave_popc(valid && f >= 0 && f <= GD_REJECT_INTERVAL);
+        }
+    }
+    if (lane == 0)
+        chunkCount[w] = cnt;
+    if (partial && lane < kStatSlots) {
+        const uint32_t v = lane == 0 ? e0 : lane == 1 ? e1 : lane == 2 ? e2 : lane == 5 ? fb : lane == 6 ? ft : 0u;
+        partial[(size_t)w * kStatSlots + lane] = v;
     }
 }
 
-// FP64 on the pairs the filter left undecided
+// FP64 on each chunk's undecided pairs; adds its exits to the chunk's counts
 __global__ __launch_bounds__(256) void k_gd_list(const double *__restrict__ tri64A, const double *__restrict__ tri64B,
-                                                 const uint64_t *__restrict__ pairs, const uint32_t *__restrict__ list,
-                                                 const uint32_t *__restrict__ listCount, uint8_t *__restrict__ status,
-                                                 double *__restrict__ seg, GdStats *__restrict__ stats)
+                                                 const uint64_t *__restrict__ pairs, uint32_t nChunks, uint32_t items,
+                                                 const uint32_t *__restrict__ list,
+                                                 const uint32_t *__restrict__ chunkCount,
+                                                 uint8_t *__restrict__ status, double *__restrict__ seg,
+                                                 uint32_t *__restrict__ partial)
 {
-    const uint32_t total = *listCount;
-    for (uint32_t base = blockIdx.x * 256; base < total; base += gridDim.x * 256) {
-        const uint32_t k = base + threadIdx.x;
-        const bool valid = k < total;
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w >= nChunks)
+        return;
+    const uint64_t base = (uint64_t)w * 64u * items;
+    const uint32_t m = chunkCount[w];
+    uint32_t ex[5] = {0, 0, 0, 0, 0};
+    for (uint32_t k0 = 0; k0 < m; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        const bool valid = k < m;
         int exitKind = -1;
         if (valid) {
-            const uint32_t i = list[k];
+            const uint32_t i = list[base + k];
             gd_pair_fp64(tri64A, tri64B, pairs[i], i, status, seg, exitKind);
         }
-        if (stats) {
+        if (partial) {
 #pragma unroll
             for (int e = 0; e < 5; ++e)
-                wave_count(valid && exitKind == e, &stats->exits[e]);
+                ex[e] += wave_popc(valid && exitKind == e);
         }
+    }
+    if (partial && lane < 5)
+        partial[(size_t)w * kStatSlots + lane] += pick5(ex, lane);
+}
+
+// one block: sums the per-chunk counts into the launch's statistics
+__global__ __launch_bounds__(256) void k_gd_stats(const uint32_t *__restrict__ partial, uint32_t nChunks,
+                                                  GdStats *__restrict__ stats)
+{
+    __shared__ unsigned long long s_sum[256];
+    unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t w = threadIdx.x; w < nChunks; w += 256) {
+        const uint32_t *p = partial + (size_t)w * kStatSlots;
+#pragma unroll
+        for (int k = 0; k < 7; ++k)
+            acc[k] += p[k];
+    }
+    for (int k = 0; k < 7; ++k) {
+        s_sum[threadIdx.x] = acc[k];
+        __syncthreads();
+        for (int h = 128; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h)
+                s_sum[threadIdx.x] += s_sum[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            if (k < 5)
+                stats->exits[k] += s_sum[0];
+            else if (k == 5)
+                stats->fallback += s_sum[0];
+            else
+                stats->filtered += s_sum[0];
+        }
+        __syncthreads();
     }
 }
 
@@ -360,24 +434,45 @@ int compact_blocks(uint32_t n) { return (int)((n + kCompactBlock - 1) / kCompact
 
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 
-// work: n + 1 u32 (the filter's list and its count) for precision 1
+// pairs per wave chunk / 64: one chunk per wave until about 8192 waves are
+// available, then up to 16 pairs per lane
+static uint32_t gd_items(uint32_t n)
+{
+    const uint32_t per = (uint32_t)((n + 64ull * 8192u - 1) / (64ull * 8192u));
+    return per < 1u ? 1u : (per > 16u ? 16u : per);
+}
+
+static uint32_t gd_chunks(uint32_t n, uint32_t items) { return (uint32_t)((n + 64ull * items - 1) / (64ull * items)); }
+
+size_t tritri_work_u32(uint32_t n)
+{
+    const uint32_t chunks = gd_chunks(n, gd_items(n));
+    return (size_t)n + (size_t)chunks * (1 + kStatSlots) + 4;
+}
+
+// work: tritri_work_u32(n) u32 (list slices, chunk counts, per-chunk statistics)
 void launch_tritri(const double *tri64A, const double *tri64B, const float *tri32A, const float *tri32B,
                    const uint64_t *pairs, uint32_t n, int precision, uint8_t *status, double *seg, uint32_t *work,
                    GdStats *stats, hipStream_t s)
 {
     if (n == 0)
         return;
-    const dim3 grid((n + 255) / 256);
+    const uint32_t items = gd_items(n);
+    const uint32_t chunks = gd_chunks(n, items);
+    const dim3 grid((chunks + 3) / 4);
+    uint32_t *list = work;
+    uint32_t *chunkCount = work + n;
+    uint32_t *partial = stats ? chunkCount + chunks : nullptr;
     if (precision == 1) {
-        uint32_t *count = work + n;
-        launch_fill_u32(count, 0u, 1, s);
-        hipLaunchKernelGGL(k_gd_filter, grid, dim3(256), 0, s, tri32A, tri32B, pairs, n, status, work, count, stats);
-        const unsigned listGrid = grid.x < 1024u ? grid.x : 1024u;
-        hipLaunchKernelGGL(k_gd_list, dim3(listGrid), dim3(256), 0, s, tri64A, tri64B, pairs, work, count, status,
-                           seg, stats);
+        hipLaunchKernelGGL(k_gd_filter, grid, dim3(256), 0, s, tri32A, tri32B, pairs, n, items, status, list,
+                           chunkCount, partial);
+        hipLaunchKernelGGL(k_gd_list, grid, dim3(256), 0, s, tri64A, tri64B, pairs, chunks, items, list, chunkCount,
+                           status, seg, partial);
     } else {
-        hipLaunchKernelGGL(k_gd_fp64, grid, dim3(256), 0, s, tri64A, tri64B, pairs, n, status, seg, stats);
+        hipLaunchKernelGGL(k_gd_fp64, grid, dim3(256), 0, s, tri64A, tri64B, pairs, n, items, status, seg, partial);
     }
+    if (stats)
+        hipLaunchKernelGGL(k_gd_stats, dim3(1), dim3(256), 0, s, partial, chunks, stats);
 }
 
 void launch_tritri_raw(const double *a9, const double *b9, uint32_t n, uint8_t *status, double *seg,

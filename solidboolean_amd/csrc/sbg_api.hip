@@ -73,6 +73,11 @@ struct DevBuf {
 constexpr int kMaxLevels = 256;
 constexpr int kExpandGrid = 1024;
 constexpr int kLevelsPerSync = 8;
+// breadth-first levels before the persistent depth-first kernel takes the
+// frontier, its wave count, and the initial / largest spill region per wave
+constexpr int kBfsLevels = 10;
+constexpr int kDfsWaves = 2048;
+constexpr uint32_t kSpillInit = 2048, kSpillMax = 1u << 15;
 
 }  // namespace
 
@@ -81,6 +86,9 @@ struct sbg_context_s {
     hipStream_t stream = nullptr;
     uint32_t *pinned = nullptr;  // host-visible counters (kMaxLevels + 8)
     DevBuf<uint2> front[2];
+    DevBuf<uint2> spill;              // depth-first spill regions
+    uint32_t spillPerWave = kSpillInit;
+    bool depthFirst = true;           // false after the spill regions proved too small
     DevBuf<uint32_t> counters;
     size_t frontCap = 0, candCap = 0;
     DevBuf<uint32_t> hist;
@@ -113,6 +121,13 @@ struct sbg_mesh_s {
     uint32_t root = 0;
     const uint32_t *sortedIds = nullptr;  // triangles in Morton order (vals or valsAlt)
     bool prepared = false;
+    // per-axis ray grid, built on first use as a classification target
+    DevBuf<uint32_t> gridCells, gridTemp;
+    DevBuf<float4> gridBox;
+    DevBuf<uint2> gridRef;
+    size_t gridCap = 0;
+    uint32_t gridG = 0;
+    bool gridReady = false;
 };
 
 struct sbg_pairs_s {
@@ -251,6 +266,7 @@ void context_free(sbg_context ctx)
     }
     ctx->front[0].release();
     ctx->front[1].release();
+    ctx->spill.release();
     ctx->counters.release();
     ctx->hist.release();
     ctx->stats.release();
@@ -387,6 +403,10 @@ int sbg_mesh_destroy(sbg_mesh m)
     m->leafParent.release();
     m->flags.release();
     m->nodes.release();
+    m->gridCells.release();
+    m->gridTemp.release();
+    m->gridBox.release();
+    m->gridRef.release();
     sbg_context c = m->ctx;
     delete m;
     context_release(c);
@@ -425,9 +445,10 @@ int sbg_mesh_prepare(sbg_mesh m)
     TRY(m->valsAlt.ensure(nt));
     TRY(m->leafParent.ensure(nt));
     TRY(m->nodes.ensure(nt > 1 ? nt - 1 : 1));
-    TRY(m->flags.ensure(2 * nt + 1));
+    TRY(m->flags.ensure(4 * nt + 1));
     TRY(c->hist.ensure(sort_hist_entries(nt)));
     m->prepared = false;
+    m->gridReady = false;
     if (nt == 0) {
         m->prepared = true;
         return SBG_OK;
@@ -569,6 +590,7 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
     TraverseArgs t = {a->nodes.p, a->box32.p, a->tri64.p, b->nodes.p, b->box32.p, b->tri64.p};
     uint32_t nCand = 0;
     int rc = SBG_OK;
+    bool found = false;
     for (int attempt = 0; attempt < 16; ++attempt) {
         if ((rc = c->front[0].ensure(c->frontCap)) != SBG_OK || (rc = c->front[1].ensure(c->frontCap)) != SBG_OK ||
             (rc = c->counters.ensure(kMaxLevels + 8)) != SBG_OK || (rc = p->pairs.ensure(c->candCap)) != SBG_OK) {
@@ -577,9 +599,15 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
         }
         uint32_t *cnt = c->counters.p;                 // [0, kMaxLevels]: frontier sizes
         uint32_t *candCount = cnt + kMaxLevels + 1;    // candidates
-        uint32_t *overflow = cnt + kMaxLevels + 2;     // overflow flag
+        uint32_t *overflow = cnt + kMaxLevels + 2;     // overflow flags
+        uint32_t *head = cnt + kMaxLevels + 3;         // depth-first seed claims
         const uint32_t fcap = (uint32_t)std::min<size_t>(c->frontCap, 0xffffffffu);
         const uint32_t ccap = (uint32_t)std::min<size_t>(c->candCap, 0xffffffffu);
+        const bool dfs = c->depthFirst;
+        if (dfs && (rc = c->spill.ensure((size_t)kDfsWaves * c->spillPerWave)) != SBG_OK) {
+            sbg_pairs_destroy(p);
+            return rc;
+        }
         int level = 0;
         bool done = false, ovf = false;
         {
@@ -587,7 +615,18 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
             launch_fill_u32(cnt, 0u, kMaxLevels + 8, c->stream);
             launch_seed(t, a->root, b->root, a->rootBox.p, b->rootBox.p, c->front[0].p, cnt, p->pairs.p, candCount,
                         c->stream);
-            while (!done && level < kMaxLevels) {
+            if (dfs) {
+                // a fixed number of levels with no host round trip, then one
+                // persistent launch over the frontier, then one sync
+                for (; level < kBfsLevels; ++level)
+                    launch_expand(t, c->front[level & 1].p, cnt + level, c->front[(level + 1) & 1].p, cnt + level + 1,
+                                  fcap, p->pairs.p, candCount, ccap, overflow, kExpandGrid, c->stream);
+                launch_traverse_dfs(t, c->front[level & 1].p, cnt + level, fcap, head, c->spill.p, c->spillPerWave,
+                                    kDfsWaves, p->pairs.p, candCount, ccap, overflow, c->stream);
+                rc = check_launch("depth-first search");
+                done = true;
+            }
+            while (!dfs && !done && level < kMaxLevels) {
                 for (int k = 0; k < kLevelsPerSync && level < kMaxLevels; ++k, ++level)
                     launch_expand(t, c->front[level & 1].p, cnt + level, c->front[(level + 1) & 1].p, cnt + level + 1,
                                   fcap, p->pairs.p, candCount, ccap, overflow, kExpandGrid, c->stream);
@@ -629,8 +668,19 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
         for (int l = 0; l <= kMaxLevels; ++l)
             maxFront = std::max(maxFront, c->pinned[l]);
         nCand = c->pinned[kMaxLevels + 1];
-        if (!ovf && !c->pinned[kMaxLevels + 2])
+        const uint32_t flags = c->pinned[kMaxLevels + 2];
+        if (!ovf && !flags) {
+            found = true;
             break;
+        }
+        if (flags & 12u) {  // depth-first spill or iteration cap: larger regions, then breadth-first only
+            if (c->spillPerWave < kSpillMax)
+                c->spillPerWave *= 4;
+            else
+                c->depthFirst = false;
+            if (!(flags & 1u))
+                continue;
+        }
         if (maxFront > fcap)
             c->frontCap = std::max<size_t>(2 * c->frontCap, (size_t)maxFront + maxFront / 2);
         if (nCand > ccap)
@@ -639,10 +689,10 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
             c->frontCap *= 2;
             c->candCap *= 2;
         }
-        if (attempt == 15) {
-            sbg_pairs_destroy(p);
-            return fail(SBG_ERR_OVERFLOW, "candidate search did not fit after repeated growth");
-        }
+    }
+    if (!found) {
+        sbg_pairs_destroy(p);
+        return fail(SBG_ERR_OVERFLOW, "candidate search did not fit after repeated growth");
     }
     p->n = nCand;
     p->maxA = na - 1;
@@ -744,7 +794,7 @@ static int run_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, GdSta
     TRY(p->hitSeg.ensure(6 * p->n));
     TRY(p->blockCounts.ensure((size_t)compact_blocks(n)));
     TRY(p->nHits.ensure(1));
-    TRY(p->work.ensure(p->n + 1));
+    TRY(p->work.ensure(tritri_work_u32((uint32_t)p->n)));
     {
         Stage st(c, SBG_STAGE_TRI_TRI);
         launch_tritri(a->tri64.p, b->tri64.p, a->tri32.p, b->tri32.p, p->pairs.p, n, precision, p->status.p,
@@ -873,32 +923,73 @@ int sbg_tri_tri_batch(sbg_context ctx, const double *a9, const double *b9, size_
 // ---------------------------------------------------------------------------
 // classification
 // ---------------------------------------------------------------------------
-// ovf[0]: overflow flag, ovf[1]: number of rays handed to the list kernel
-static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder, const uint8_t *dskip, size_t n,
-                    uint8_t *dbits)
+static int ensure_grid(sbg_mesh m)
+{
+    if (m->gridReady)
+        return SBG_OK;
+    sbg_context c = m->ctx;
+    const uint32_t G = grid_dim((uint32_t)m->nt);
+    if (!m->gridCap)
+        m->gridCap = 6 * m->nt + 1024;
+    if (m->gridCap > 0xffffffffu)
+        return fail(SBG_ERR_OVERFLOW, "ray grid needs %zu entries", m->gridCap);
+    TRY(m->gridCells.ensure(grid_cells(G)));
+    TRY(m->gridTemp.ensure(scan_large_temp(grid_cells(G))));
+    TRY(m->gridBox.ensure(m->gridCap));
+    TRY(m->gridRef.ensure(m->gridCap));
+    {
+        Stage st(c, SBG_STAGE_GRID);
+        launch_grid_build(m->tri64.p, (uint32_t)m->nt, m->rootBox.p, G, m->gridCells.p, m->gridTemp.p, m->gridBox.p,
+                          m->gridRef.p, (uint32_t)m->gridCap, c->stream);
+        TRY(check_launch("ray grid"));
+    }
+    m->gridG = G;
+    m->gridReady = true;
+    return SBG_OK;
+}
+
+// Rays from n points against `target` and the 2-of-3 vote into c->votes;
+// ovf[0]: overflow flags, ovf[1]: number of rays handed to the list kernel.
+// A grid that turned out too small is rebuilt at its exact size and the rays
+// run again.
+static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder, const uint8_t *dskip, size_t n)
 {
     sbg_context c = target->ctx;
     TRY(c->ovf.ensure(2));
     TRY(c->rayList.ensure(3 * n));
-    launch_fill_u32(c->ovf.p, 0u, 2, c->stream);
     if (target->nt == 0) {
-        launch_fill_u32((uint32_t *)dbits, 0u, (3 * n + 3) / 4, c->stream);
+        Stage st(c, SBG_STAGE_CLASSIFY);
+        launch_fill_u32((uint32_t *)c->bits.p, 0u, (3 * n + 3) / 4, c->stream);
+        launch_vote(c->bits.p, dskip, (uint32_t)n, c->votes.p, c->stream);
         return check_launch("rays");
     }
-    launch_rays(target->nodes.p, target->root, target->rootBox.p, target->tri64.p, target->normals.p,
-                (uint32_t)target->nt, dpts, dorder, dskip, (uint32_t)n, dbits, c->rayList.p, c->ovf.p + 1, c->ovf.p,
-                c->stream);
-    return check_launch("rays");
-}
-
-static int check_ray_overflow(sbg_context c)
-{
-    uint32_t h = 0;
-    HIP_TRY(hipMemcpyAsync(&h, c->ovf.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    if (h)
-        return fail(SBG_ERR_OVERFLOW, "a ray exceeded its traversal stack or distinct-hit list");
-    return SBG_OK;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        TRY(ensure_grid(target));
+        {
+            Stage st(c, SBG_STAGE_CLASSIFY);
+            launch_fill_u32(c->ovf.p, 0u, 2, c->stream);
+            const GridArgs ga = {target->gridCells.p, target->gridBox.p, target->gridRef.p, target->rootBox.p,
+                                 target->gridG, (uint32_t)target->gridCap};
+            launch_rays(ga, target->tri64.p, target->normals.p, dpts, dorder, dskip, (uint32_t)n, c->bits.p,
+                        c->rayList.p, c->ovf.p + 1, c->ovf.p, c->stream);
+            launch_vote(c->bits.p, dskip, (uint32_t)n, c->votes.p, c->stream);
+            TRY(check_launch("rays"));
+        }
+        uint32_t h[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(h, c->ovf.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipMemcpyAsync(h + 1, target->gridCells.p + 3 * (size_t)target->gridG * target->gridG,
+                               sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (h[0] & 2u) {
+            target->gridCap = (size_t)h[1] + h[1] / 16 + 1024;
+            target->gridReady = false;
+            continue;
+        }
+        if (h[0])
+            return fail(SBG_ERR_OVERFLOW, "a ray has more distinct hits than its key list holds");
+        return SBG_OK;
+    }
+    return fail(SBG_ERR_OVERFLOW, "ray grid did not fit after a resize");
 }
 
 int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *axis_inside, uint8_t *inside)
@@ -917,13 +1008,7 @@ int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *a
     TRY(c->bits.ensure(3 * n + 4));
     TRY(c->votes.ensure(n));
     HIP_TRY(hipMemcpyAsync(c->pts.p, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-    {
-        Stage st(c, SBG_STAGE_CLASSIFY);
-        TRY(run_rays(target, c->pts.p, nullptr, nullptr, n, c->bits.p));
-        launch_vote(c->bits.p, nullptr, (uint32_t)n, c->votes.p, c->stream);
-        TRY(check_launch("vote"));
-    }
-    TRY(check_ray_overflow(c));
+    TRY(run_rays(target, c->pts.p, nullptr, nullptr, n));
     if (inside)
         HIP_TRY(hipMemcpyAsync(inside, c->votes.p, n, hipMemcpyDeviceToHost, c->stream));
     std::vector<uint8_t> bits;
@@ -968,11 +1053,9 @@ int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side,
         launch_fill_u32((uint32_t *)c->flags.p, 0u, (nt + 3) / 4, c->stream);
         launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, c->flags.p, c->stream);
         launch_centroids(self->tri64.p, c->flags.p, (uint32_t)nt, c->pts.p, c->stream);
-        TRY(run_rays(other, c->pts.p, self->sortedIds, c->flags.p, nt, c->bits.p));
-        launch_vote(c->bits.p, c->flags.p, (uint32_t)nt, c->votes.p, c->stream);
         TRY(check_launch("classify"));
     }
-    TRY(check_ray_overflow(c));
+    TRY(run_rays(other, c->pts.p, self->sortedIds, c->flags.p, nt));
     HIP_TRY(hipMemcpyAsync(out, c->votes.p, nt, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return SBG_OK;
@@ -1031,7 +1114,7 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, int it
     TRY(check_pairs_in_range(a, b, p));
     TRY(p->status.ensure(p->n));
     TRY(p->seg.ensure(6 * p->n));
-    TRY(p->work.ensure(p->n + 1));
+    TRY(p->work.ensure(tritri_work_u32((uint32_t)p->n)));
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));

@@ -46,13 +46,17 @@ struct SortTemp {
 int sort_blocks(size_t n);
 void launch_exclusive_scan(uint32_t *data, size_t m, hipStream_t s);
 size_t sort_hist_entries(size_t n);
+// exclusive scan of m u32 in place; temp holds scan_large_temp(m) u32
+size_t scan_large_temp(size_t m);
+void launch_scan_large(uint32_t *data, size_t m, uint32_t *temp, hipStream_t s);
 int radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *valsAlt, size_t n,
                    int beginBit, int endBit, uint32_t *hist, hipStream_t s);
 int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRanges, int nRanges,
                    uint32_t *hist, hipStream_t s);
 
 // k_lbvh.hip
-// work: 2n + 1 u32 of scratch shared by the two (top-node flags, list, count)
+// work: 4n + 1 u32 of scratch shared by the two (top-node flags, list, count,
+// the top refit's two ready lists)
 void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
                       uint32_t *leafParent, uint32_t *work, hipStream_t s);
 void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *nodes,
@@ -72,6 +76,12 @@ struct TraverseArgs {
 void launch_expand(const TraverseArgs &t, const uint2 *in, const uint32_t *inCount, uint2 *out,
                    uint32_t *outCount, uint32_t outCap, uint64_t *cand, uint32_t *candCount,
                    uint32_t candCap, uint32_t *overflow, int grid, hipStream_t s);
+// persistent depth-first search from the frontier seeds[0, *seedCount); *head
+// zero on entry; spill holds waves * spillPerWave pairs; overflow bit 2: a
+// wave's spill region filled up, bit 3: iteration cap (the host retries)
+void launch_traverse_dfs(const TraverseArgs &t, const uint2 *seeds, const uint32_t *seedCount, uint32_t seedCap,
+                         uint32_t *head, uint2 *spill, uint32_t spillPerWave, int waves, uint64_t *cand,
+                         uint32_t *candCount, uint32_t candCap, uint32_t *overflow, hipStream_t s);
 void launch_seed(const TraverseArgs &t, uint32_t rootA, uint32_t rootB, const float *rootBoxA,
                  const float *rootBoxB, uint2 *frontier, uint32_t *count, uint64_t *cand,
                  uint32_t *candCount, hipStream_t s);
@@ -82,7 +92,9 @@ struct GdStats {
     unsigned long long fallback;   // pairs re-run in FP64 because a sign was uncertain
     unsigned long long filtered;   // pairs decided by the FP32 filter alone
 };
-// work: n + 1 u32 of scratch (the FP32 filter's undecided list and its count)
+// work: tritri_work_u32(n) u32 of scratch (undecided-pair slices, per-chunk
+// counts and statistics)
+size_t tritri_work_u32(uint32_t n);
 void launch_tritri(const double *tri64A, const double *tri64B, const float *tri32A, const float *tri32B,
                    const uint64_t *pairs, uint32_t n, int precision, uint8_t *status, double *seg, uint32_t *work,
                    GdStats *stats, hipStream_t s);
@@ -98,12 +110,25 @@ int compact_blocks(uint32_t n);
 void launch_mark_intersected(const uint64_t *pairs, const uint8_t *status, uint32_t n, int side,
                              uint8_t *flags, hipStream_t s);
 void launch_centroids(const double *tri64, const uint8_t *flags, uint32_t nt, double *pts, hipStream_t s);
+// per-axis projection grid of a target mesh: cells holds grid_cells(G) u32,
+// entries gbox / gref hold cap entries each
+struct GridArgs {
+    const uint32_t *cells;
+    const float4 *gbox;
+    const uint2 *gref;
+    const float *rootBox;
+    uint32_t G, cap;
+};
+uint32_t grid_dim(uint32_t nt);
+size_t grid_cells(uint32_t G);
+void launch_grid_build(const double *tri64, uint32_t nt, const float *rootBox, uint32_t G, uint32_t *cells,
+                       uint32_t *scanTemp, float4 *gbox, uint2 *gref, uint32_t cap, hipStream_t s);
 // order (may be null): query q uses point order[q]; ovfList holds 3n entries,
-// *ovfCount must be zero on entry
-void launch_rays(const BNode *nodes, uint32_t root, const float *rootBox, const double *tri64,
-                 const double *normals, uint32_t ntTarget, const double *pts, const uint32_t *order,
-                 const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList, uint32_t *ovfCount,
-                 uint32_t *overflow, hipStream_t s);
+// *ovfCount must be zero on entry; overflow bit 0: a ray has too many distinct
+// hits, bit 1: the grid was truncated (cells[3 G^2] > cap)
+void launch_rays(const GridArgs &ga, const double *tri64, const double *normals, const double *pts,
+                 const uint32_t *order, const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList,
+                 uint32_t *ovfCount, uint32_t *overflow, hipStream_t s);
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);
 
 }  // namespace sbg

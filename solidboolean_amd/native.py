@@ -24,7 +24,7 @@ SBG_ERR_STATE = -6
 FP64 = 0
 FP32_FILTERED = 1
 
-STAGES = ["records", "sort", "bvh", "traverse", "pair_sort", "tri_tri", "compact", "classify"]
+STAGES = ["records", "sort", "bvh", "traverse", "pair_sort", "tri_tri", "compact", "classify", "grid"]
 
 # every symbol include/sbgpu.h declares
 EXPORTS = [
