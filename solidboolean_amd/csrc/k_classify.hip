@@ -16,10 +16,11 @@
 // The candidates come from a per-axis grid over the target's projected
 // triangle boxes instead of a tree walk: a ray's candidates are the entries of
 // the one cell holding its origin, a superset of the reference's set that the
-// exact FP64 box test then cuts down to it.  k_rays_grid keeps up to 4
-// distinct keys in registers; a lane that needs more appends its (point,
-// axis) to an overflow list, which k_rays_list finishes with 32 keys.  A lane
-// past that raises `overflow` and the call fails: nothing is guessed.
+// exact FP64 box test then cuts down to it.  k_rays_grid scans a cell of up
+// to 48 entries with up to 4 distinct keys in registers; a ray with a crowded
+// cell or more hits goes to k_rays_wave, where one wave splits the cell and
+// counts distinct keys in LDS (up to 256 hits).  A ray past that raises
+// `overflow` and the call fails: nothing is guessed.
 #include "sbg_device.h"
 #include "sbg_internal.h"
 
@@ -29,8 +30,9 @@
 
 namespace sbg {
 
-constexpr int kFastKeys = 4;
-constexpr int kListKeys = 32;
+constexpr int kFastKeys = 4;     // distinct keys a fast lane keeps
+constexpr int kFastCands = 48;   // cell entries a fast lane scans alone
+constexpr int kWaveHits = 256;   // hits a wave gathers for one deferred ray
 
 struct RBox {
     float lx, ly, lz, hx, hy, hz;
@@ -88,8 +90,9 @@ __device__ __forceinline__ Ray ray_setup(const double *__restrict__ pts, uint32_
     return r;
 }
 
-// distinct PositionKeys of one ray; CAP is small enough to stay in registers
-// when every index below is a compile-time constant
+// Distinct PositionKeys of one ray, CAP of them in registers.  New keys enter
+// at slot 0 and shift the others up, so every slot index is a compile-time
+// constant and the set never lands in scratch memory.
 template <int CAP>
 struct KeySet {
     Key3 k[CAP];
@@ -106,50 +109,53 @@ struct KeySet {
             return;
         }
 #pragma unroll
-        for (int j = 0; j < CAP; ++j)
-            if (j == n)
-                k[j] = key;
+        for (int j = CAP - 1; j > 0; --j)
+            k[j] = k[j - 1];
+        k[0] = key;
         ++n;
     }
 };
 
-// candidate triangle t: FP64 box test, plane hit, three-normal inside test
-template <class Keys>
-__device__ __forceinline__ void ray_leaf(uint32_t t, const double *__restrict__ tri64,
-                                         const double *__restrict__ normals, const Ray &r, Keys &keys)
+// candidate triangle t: FP64 box test, plane hit, three-normal inside test;
+// on a hit, `key` is the hit point's PositionKey
+__device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ tri64,
+                                        const double *__restrict__ normals, const Ray &r, Key3 &key)
 {
     const double2 *x = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * t);
     const double2 r0 = x[0], r1 = x[1], r2 = x[2], r3 = x[3], r4 = x[4], r5 = x[5];
     const V3 v0 = {r0.x, r0.y, r1.x}, v1 = {r2.x, r2.y, r3.x}, v2 = {r4.x, r4.y, r5.x};
     if (!box64_overlap(tri_box64(v0, v1, v2), r.rb))
-        return;
+        return false;
     const V3 n = {normals[3 * (size_t)t], normals[3 * (size_t)t + 1], normals[3 * (size_t)t + 2]};
     const V3 w = vsub(r.p, v0);
     const double d = n.x * r.u.x + n.y * r.u.y + n.z * r.u.z;
     const double num = (-n.x) * w.x + (-n.y) * w.y + (-n.z) * w.z;
     if (__builtin_fabs(d) <= 2.220446049250313080847e-16)
-        return;
+        return false;
     const double s = num / d;
     if (s < 0 || s > 1 || __builtin_isnan(s) || __builtin_isinf(s))
-        return;
+        return false;
     const V3 hit = {r.p.x + s * r.u.x, r.p.y + s * r.u.y, r.p.z + s * r.u.z};
     const V3 n0 = tri_normal(hit, v0, v1);
     const V3 n1 = tri_normal(hit, v1, v2);
     const V3 n2 = tri_normal(hit, v2, v0);
     if (!(vdot(n0, n1) > 0 && vdot(n0, n2) > 0))
-        return;
-    keys.add(position_key(hit));
+        return false;
+    key = position_key(hit);
+    return true;
 }
 
-// ---- per-axis projection grids of the target mesh -------------------------
-// For a ray along `axis`, (u, v) are the other two coordinates.  The target's
-// root box is split into G x G cells in (u, v); cell(x) is monotone in x, so a
-// triangle filed under every cell its slack-widened (u, v) box spans is filed
-// under the cell of every ray origin whose segment box it can overlap: the
-// segment covers [p.u, p.u + eps] and the slack (1e-15 (1 + |lo|)) exceeds eps
-// plus the rounding of lo - slack.  Each entry holds the triangle's FP32
-// outward-rounded (u, v) box and its upper bound along `axis`, so most of a
-// cell's entries are rejected from 24 B without touching the FP64 record.
+// ---- per-axis projection grids of the target meshes ------------------------
+// For a ray along `axis`, (u, v) are the other two coordinates.  A target
+// mesh's root box is split into G x G cells in (u, v); cell(x) is monotone in
+// x, so a triangle filed under every cell its slack-widened (u, v) box spans
+// is filed under the cell of every ray origin whose segment box it can
+// overlap: the segment covers [p.u, p.u + eps] and the slack
+// (1e-15 (1 + |lo|)) exceeds eps plus the rounding of lo - slack.  Each entry
+// holds the triangle's FP32 outward-rounded (u, v) box and its upper bound
+// along `axis`, so most of a cell's entries are rejected from 24 B without
+// touching the FP64 record.  Several meshes (a batch) share one cell array:
+// mesh m owns 3 G_m^2 cells from its first cell on.
 struct GridFrame {
     double lu, lv, iu, iv;
 };
@@ -157,13 +163,13 @@ struct GridFrame {
 __device__ __forceinline__ int axis_u(int axis) { return axis == 0 ? 1 : 0; }
 __device__ __forceinline__ int axis_v(int axis) { return axis == 2 ? 1 : 2; }
 
-__device__ __forceinline__ GridFrame grid_frame(const float *__restrict__ rootBox, int axis, uint32_t G)
+__device__ __forceinline__ GridFrame grid_frame(const float *__restrict__ box, int axis, uint32_t G)
 {
     const int u = axis_u(axis), v = axis_v(axis);
     GridFrame f;
-    f.lu = (double)rootBox[u];
-    f.lv = (double)rootBox[v];
-    const double su = (double)rootBox[3 + u] - f.lu, sv = (double)rootBox[3 + v] - f.lv;
+    f.lu = (double)box[u];
+    f.lv = (double)box[v];
+    const double su = (double)box[3 + u] - f.lu, sv = (double)box[3 + v] - f.lv;
     f.iu = su > 0.0 ? (double)G / su : 0.0;
     f.iv = sv > 0.0 ? (double)G / sv : 0.0;
     return f;
@@ -178,6 +184,12 @@ __device__ __forceinline__ uint32_t grid_cell(double x, double lo, double inv, u
 }
 
 __device__ __forceinline__ double comp(const V3 &a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// (G, first cell) of mesh m
+__device__ __forceinline__ uint2 grid_desc(const GridSet &gs, uint32_t m)
+{
+    return gs.desc ? gs.desc[m] : make_uint2(gs.G0, 0u);
+}
 
 struct GridSpan {
     uint32_t u0, u1, v0, v1;
@@ -203,19 +215,23 @@ __device__ __forceinline__ GridSpan grid_span(const double *__restrict__ tri64, 
     return s;
 }
 
-// cells: 3 G^2 + 1 u32, zero on entry; entries of cell c are counted in cells[c + 1]
-__global__ __launch_bounds__(256) void k_grid_count(const double *__restrict__ tri64, uint32_t nt,
-                                                    const float *__restrict__ rootBox, uint32_t G,
+// cells: ncells + 1 u32, zero on entry; entries of cell c are counted in cells[c + 1]
+__global__ __launch_bounds__(256) void k_grid_count(const double *__restrict__ tri64, uint32_t nt, GridSet gs,
                                                     uint32_t *__restrict__ cells)
 {
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
     if (g >= 3u * nt)
         return;
     const int axis = (int)(g / nt);
-    const uint32_t t = g - (uint32_t)axis * nt;
-    const GridFrame f = grid_frame(rootBox, axis, G);
+    // a stride coprime with nt spreads a wave's triangles over the mesh, so its
+    // atomics land on different cells
+    const uint32_t t = (uint32_t)(((uint64_t)(g - (uint32_t)axis * nt) * gs.mix) % nt);
+    const uint32_t m = gs.meshOf ? gs.meshOf[t] : 0u;
+    const uint2 d = grid_desc(gs, m);
+    const uint32_t G = d.x;
+    const GridFrame f = grid_frame(gs.boxes + 6 * (size_t)m, axis, G);
     const GridSpan s = grid_span(tri64, t, axis, f, G);
-    const size_t base = (size_t)axis * G * G + 1;
+    const size_t base = (size_t)d.y + (size_t)axis * G * G + 1;
     for (uint32_t cu = s.u0; cu <= s.u1; ++cu)
         for (uint32_t cv = s.v0; cv <= s.v1; ++cv)
             atomicAdd(&cells[base + (size_t)cu * G + cv], 1u);
@@ -223,56 +239,67 @@ __global__ __launch_bounds__(256) void k_grid_count(const double *__restrict__ t
 
 // after the exclusive scan, cells[c + 1] is the start of cell c; each entry
 // advances it, so afterwards cell c spans [cells[c], cells[c + 1])
-__global__ __launch_bounds__(256) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt,
-                                                   const float *__restrict__ rootBox, uint32_t G,
+__global__ __launch_bounds__(256) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt, GridSet gs,
                                                    uint32_t *__restrict__ cells, float4 *__restrict__ gbox,
-                                                   uint2 *__restrict__ gref, uint32_t cap)
+                                                   uint2 *__restrict__ gref)
 {
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
     if (g >= 3u * nt)
         return;
     const int axis = (int)(g / nt);
-    const uint32_t t = g - (uint32_t)axis * nt;
-    const GridFrame f = grid_frame(rootBox, axis, G);
+    // a stride coprime with nt spreads a wave's triangles over the mesh, so its
+    // atomics land on different cells
+    const uint32_t t = (uint32_t)(((uint64_t)(g - (uint32_t)axis * nt) * gs.mix) % nt);
+    const uint32_t m = gs.meshOf ? gs.meshOf[t] : 0u;
+    const uint2 d = grid_desc(gs, m);
+    const uint32_t G = d.x;
+    const GridFrame f = grid_frame(gs.boxes + 6 * (size_t)m, axis, G);
     const GridSpan s = grid_span(tri64, t, axis, f, G);
-    const size_t base = (size_t)axis * G * G + 1;
+    const size_t base = (size_t)d.y + (size_t)axis * G * G + 1;
     for (uint32_t cu = s.u0; cu <= s.u1; ++cu)
         for (uint32_t cv = s.v0; cv <= s.v1; ++cv) {
             const uint32_t slot = atomicAdd(&cells[base + (size_t)cu * G + cv], 1u);
-            if (slot < cap) {
+            if (slot < gs.cap) {
                 gbox[slot] = s.box;
                 gref[slot] = make_uint2(t, __float_as_uint(s.hiAxis));
             }
         }
 }
 
-// every candidate of the ray's cell, until the key set is full
-template <class Keys>
-__device__ __forceinline__ void ray_grid(const GridArgs &ga, const double *__restrict__ tri64,
-                                         const double *__restrict__ normals, const Ray &r, int axis, Keys &keys)
+// candidate range of the ray's cell in target mesh m
+__device__ __forceinline__ uint2 ray_cell(const GridSet &gs, uint32_t m, const Ray &r, int axis)
 {
-    const GridFrame f = grid_frame(ga.rootBox, axis, ga.G);
+    const uint2 d = grid_desc(gs, m);
+    const uint32_t G = d.x;
+    const GridFrame f = grid_frame(gs.boxes + 6 * (size_t)m, axis, G);
+    const uint32_t cu = grid_cell(comp(r.p, axis_u(axis)), f.lu, f.iu, G);
+    const uint32_t cv = grid_cell(comp(r.p, axis_v(axis)), f.lv, f.iv, G);
+    const size_t c = (size_t)d.y + (size_t)axis * G * G + (size_t)cu * G + cv;
+    return make_uint2(gs.cells[c], gs.cells[c + 1]);
+}
+
+// FP32 pre-test of grid entry k against the ray (a superset of the FP64 box
+// test); on success `t` is the triangle
+__device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, const Ray &r, int axis, uint32_t &t)
+{
     const int u = axis_u(axis), v = axis_v(axis);
-    const uint32_t cu = grid_cell(comp(r.p, u), f.lu, f.iu, ga.G);
-    const uint32_t cv = grid_cell(comp(r.p, v), f.lv, f.iv, ga.G);
-    const size_t c = (size_t)axis * ga.G * ga.G + (size_t)cu * ga.G + cv;
-    const uint32_t k0 = ga.cells[c], k1 = ga.cells[c + 1];
     const float rlu = u == 0 ? r.rf.lx : r.rf.ly, rhu = u == 0 ? r.rf.hx : r.rf.hy;
     const float rlv = v == 1 ? r.rf.ly : r.rf.lz, rhv = v == 1 ? r.rf.hy : r.rf.hz;
     const float rla = axis == 0 ? r.rf.lx : (axis == 1 ? r.rf.ly : r.rf.lz);
-    for (uint32_t k = k0; k < k1 && !keys.full; ++k) {
-        const float4 e = ga.gbox[k];
-        if (!(e.x <= rhu && e.y >= rlu && e.z <= rhv && e.w >= rlv))
-            continue;
-        const uint2 ref = ga.gref[k];
-        if (__uint_as_float(ref.y) >= rla)
-            ray_leaf(ref.x, tri64, normals, r, keys);
-    }
+    const float4 e = gs.gbox[k];
+    if (!(e.x <= rhu && e.y >= rlu && e.z <= rhv && e.w >= rlv))
+        return false;
+    const uint2 ref = gs.gref[k];
+    t = ref.x;
+    return __uint_as_float(ref.y) >= rla;
 }
 
-// One lane per (axis, query) with up to kFastKeys distinct keys; a lane that
-// needs more hands (axis, query) to the list kernel.
-__global__ __launch_bounds__(256) void k_rays_grid(GridArgs ga, const double *__restrict__ tri64,
+// One lane per (axis, query).  A lane whose cell holds at most kFastCands
+// entries scans them with up to kFastKeys distinct keys in registers; any other
+// lane (a crowded cell, or more distinct hits) hands (axis, query) to
+// k_rays_wave.  Query i's rays go to mesh target[i] (mesh 0 without a target
+// list); a query without a target mesh (~0u) counts as outside.
+__global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__restrict__ tri64,
                                                    const double *__restrict__ normals, const double *__restrict__ pts,
                                                    const uint32_t *__restrict__ order, const uint8_t *__restrict__ skip,
                                                    uint32_t n, uint8_t *__restrict__ bits, uint32_t *__restrict__ ovfList,
@@ -281,7 +308,7 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridArgs ga, const double *__
     const uint32_t g = blockIdx.x * 256 + threadIdx.x;
     if (g >= 3u * n)
         return;
-    if (ga.cells[3 * (size_t)ga.G * ga.G] > ga.cap) {  // grid truncated: the host rebuilds it larger
+    if (gs.cells[gs.ncells] > gs.cap) {  // grid truncated: the host rebuilds it larger
         if ((threadIdx.x & 63) == 0)
             atomicOr(overflow, 2u);
         return;
@@ -291,10 +318,24 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridArgs ga, const double *__
     const uint32_t i = order ? order[q] : q;
     if (skip && skip[i])
         return;
+    const uint32_t m = gs.target ? gs.target[i] : 0u;
+    if (m == ~0u) {
+        bits[(size_t)axis * n + i] = 0;
+        return;
+    }
     const Ray r = ray_setup(pts, i, axis);
+    const uint2 range = ray_cell(gs, m, r, (int)axis);
+    bool defer = range.y - range.x > (uint32_t)kFastCands;
     KeySet<kFastKeys> keys;
-    ray_grid(ga, tri64, normals, r, (int)axis, keys);
-    if (keys.full) {
+    for (uint32_t k = range.x; !defer && k < range.y; ++k) {
+        uint32_t t;
+        Key3 key;
+        if (entry_may_hit(gs, k, r, (int)axis, t) && ray_hit(t, tri64, normals, r, key)) {
+            keys.add(key);
+            defer = keys.full;
+        }
+    }
+    if (defer) {
         const uint32_t slot = atomicAdd(ovfCount, 1u);
         ovfList[slot] = g;  // slot < 3n: every lane appends at most once
         return;
@@ -302,25 +343,61 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridArgs ga, const double *__
     bits[(size_t)axis * n + i] = (uint8_t)(keys.n & 1);
 }
 
-// lanes handed over by k_rays_grid, g = axis * n + q, with kListKeys keys
-__global__ __launch_bounds__(256) void k_rays_list(GridArgs ga, const double *__restrict__ tri64,
-                                                   const double *__restrict__ normals, const double *__restrict__ pts,
-                                                   const uint32_t *__restrict__ order, uint32_t n,
-                                                   uint8_t *__restrict__ bits, const uint32_t *__restrict__ list,
-                                                   const uint32_t *__restrict__ count, uint32_t *overflow)
+// One wave per deferred (axis, query), g = axis * n + q: the 64 lanes split
+// the cell's entries, hit keys are gathered in LDS (up to kWaveHits) and the
+// distinct ones counted; more hits than that raise `overflow`.
+__global__ __launch_bounds__(64) void k_rays_wave(GridSet gs, const double *__restrict__ tri64,
+                                                  const double *__restrict__ normals, const double *__restrict__ pts,
+                                                  const uint32_t *__restrict__ order, uint32_t n,
+                                                  uint8_t *__restrict__ bits, const uint32_t *__restrict__ list,
+                                                  const uint32_t *__restrict__ count, uint32_t *overflow)
 {
+    __shared__ long long s_key[3][kWaveHits];
+    const int lane = threadIdx.x;
+    const unsigned long long below = (1ull << lane) - 1ull;
     const uint32_t total = *count;
-    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < total; k += gridDim.x * 256) {
+    for (uint32_t k = blockIdx.x; k < total; k += gridDim.x) {
         const uint32_t g = list[k];
         const uint32_t axis = g / n;
         const uint32_t q = g - axis * n;
         const uint32_t i = order ? order[q] : q;
+        const uint32_t m = gs.target ? gs.target[i] : 0u;
         const Ray r = ray_setup(pts, i, axis);
-        KeySet<kListKeys> keys;
-        ray_grid(ga, tri64, normals, r, (int)axis, keys);
-        if (keys.full)
-            atomicOr(overflow, 1u);
-        bits[(size_t)axis * n + i] = (uint8_t)(keys.n & 1);
+        const uint2 range = ray_cell(gs, m, r, (int)axis);
+        uint32_t nh = 0;
+        for (uint32_t k0 = range.x; k0 < range.y; k0 += 64) {
+            const uint32_t e = k0 + lane;
+            uint32_t t;
+            Key3 key;
+            const bool hit = e < range.y && entry_may_hit(gs, e, r, (int)axis, t) && ray_hit(t, tri64, normals, r, key);
+            const unsigned long long mask = __ballot(hit);
+            const uint32_t pos = nh + (uint32_t)__popcll(mask & below);
+            if (hit && pos < (uint32_t)kWaveHits) {
+                s_key[0][pos] = key.x;
+                s_key[1][pos] = key.y;
+                s_key[2][pos] = key.z;
+            }
+            nh += (uint32_t)__popcll(mask);
+        }
+        __syncthreads();
+        if (nh > (uint32_t)kWaveHits) {
+            if (lane == 0)
+                atomicOr(overflow, 1u);
+        } else {
+            // a key counts where it first occurs
+            uint32_t distinct = 0;
+            for (uint32_t a = 0; a < nh; a += 64) {
+                const uint32_t x = a + lane;
+                bool first = x < nh;
+                for (uint32_t b = 0; first && b < x; ++b)
+                    if (s_key[0][b] == s_key[0][x] && s_key[1][b] == s_key[1][x] && s_key[2][b] == s_key[2][x])
+                        first = false;
+                distinct += (uint32_t)__popcll(__ballot(first));
+            }
+            if (lane == 0)
+                bits[(size_t)axis * n + i] = (uint8_t)(distinct & 1);
+        }
+        __syncthreads();
     }
 }
 
@@ -363,29 +440,46 @@ uint32_t grid_dim(uint32_t nt)
 
 size_t grid_cells(uint32_t G) { return 3 * (size_t)G * G + 1; }
 
-void launch_grid_build(const double *tri64, uint32_t nt, const float *rootBox, uint32_t G, uint32_t *cells,
-                       uint32_t *scanTemp, float4 *gbox, uint2 *gref, uint32_t cap, hipStream_t s)
+uint32_t grid_mix(uint32_t nt)
 {
-    const size_t m = grid_cells(G);
+    if (nt < 2)
+        return 1;
+    uint32_t s = (uint32_t)(0.6180339887 * nt) | 1u;
+    for (;; s += 2) {
+        uint32_t a = s % nt, b = nt;
+        while (a) {
+            const uint32_t r = b % a;
+            b = a;
+            a = r;
+        }
+        if (b == 1)
+            return s % nt ? s % nt : 1;
+    }
+}
+
+void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
+                       float4 *gbox, uint2 *gref, hipStream_t s)
+{
+    const size_t m = (size_t)gs.ncells + 1;
     launch_fill_u32(cells, 0u, m, s);
     if (nt == 0)
         return;
     const unsigned blocks = (unsigned)((3ull * nt + 255) / 256);
-    hipLaunchKernelGGL(k_grid_count, dim3(blocks), dim3(256), 0, s, tri64, nt, rootBox, G, cells);
+    hipLaunchKernelGGL(k_grid_count, dim3(blocks), dim3(256), 0, s, tri64, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
-    hipLaunchKernelGGL(k_grid_fill, dim3(blocks), dim3(256), 0, s, tri64, nt, rootBox, G, cells, gbox, gref, cap);
+    hipLaunchKernelGGL(k_grid_fill, dim3(blocks), dim3(256), 0, s, tri64, nt, gs, cells, gbox, gref);
 }
 
-void launch_rays(const GridArgs &ga, const double *tri64, const double *normals, const double *pts,
+void launch_rays(const GridSet &gs, const double *tri64, const double *normals, const double *pts,
                  const uint32_t *order, const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList,
                  uint32_t *ovfCount, uint32_t *overflow, hipStream_t s)
 {
     if (n == 0)
         return;
     const uint64_t lanes = 3ull * n;
-    hipLaunchKernelGGL(k_rays_grid, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, ga, tri64, normals, pts,
+    hipLaunchKernelGGL(k_rays_grid, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, gs, tri64, normals, pts,
                        order, skip, n, axisBits, ovfList, ovfCount, overflow);
-    hipLaunchKernelGGL(k_rays_list, dim3(256), dim3(256), 0, s, ga, tri64, normals, pts, order, n, axisBits, ovfList,
+    hipLaunchKernelGGL(k_rays_wave, dim3(2048), dim3(64), 0, s, gs, tri64, normals, pts, order, n, axisBits, ovfList,
                        ovfCount, overflow);
 }
 

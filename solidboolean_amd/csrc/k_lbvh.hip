@@ -24,12 +24,24 @@ __device__ __forceinline__ int delta(const uint32_t *__restrict__ codes, int n, 
     return __clz(a ^ b);
 }
 
+// 64-bit keys (mesh id above the Morton code, for the combined tree of a batch)
+__device__ __forceinline__ int delta(const uint64_t *__restrict__ codes, int n, int i, int j)
+{
+    if (j < 0 || j >= n)
+        return -1;
+    const uint64_t a = codes[i], b = codes[j];
+    if (a == b)
+        return 64 + __clz((uint32_t)(i ^ j));
+    return __clzll(a ^ b);
+}
+
 // Leaves are refit in tiles of kRefitTile sorted positions; a node whose leaf
 // range crosses a tile boundary is a "top" node, finished by k_refit_top.
 constexpr int kRefitTile = 1024;
 constexpr int kRefitBlock = 256;
 
-__global__ __launch_bounds__(256) void k_hierarchy(const uint32_t *__restrict__ codes,
+template <typename K>
+__global__ __launch_bounds__(256) void k_hierarchy(const K *__restrict__ codes,
                                                    const uint32_t *__restrict__ ids, int n,
                                                    BNode *__restrict__ nodes, uint32_t *__restrict__ leafParent,
                                                    uint32_t *__restrict__ topList, uint32_t *topCount)
@@ -262,7 +274,17 @@ void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BN
     if (n < 2)
         return;
     launch_fill_u32(work, 0u, (size_t)2 * n + 1, s);
-    hipLaunchKernelGGL(k_hierarchy, dim3((n - 1 + 255) / 256), dim3(256), 0, s, codes, ids, (int)n, nodes,
+    hipLaunchKernelGGL(k_hierarchy<uint32_t>, dim3((n - 1 + 255) / 256), dim3(256), 0, s, codes, ids, (int)n, nodes,
+                       leafParent, work + n, work + 2 * (size_t)n);
+}
+
+void launch_hierarchy64(const uint64_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
+                        uint32_t *leafParent, uint32_t *work, hipStream_t s)
+{
+    if (n < 2)
+        return;
+    launch_fill_u32(work, 0u, (size_t)2 * n + 1, s);
+    hipLaunchKernelGGL(k_hierarchy<uint64_t>, dim3((n - 1 + 255) / 256), dim3(256), 0, s, codes, ids, (int)n, nodes,
                        leafParent, work + n, work + 2 * (size_t)n);
 }
 

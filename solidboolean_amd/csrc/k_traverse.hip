@@ -34,25 +34,6 @@ __device__ __forceinline__ FBox leaf_box(const float *__restrict__ box32, uint32
     return {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
 }
 
-// children (1 for a leaf, 2 for an internal node) with their FP32 boxes
-__device__ __forceinline__ int children(const BNode *__restrict__ nodes, const float *__restrict__ box32,
-                                        uint32_t ref, uint32_t c[2], FBox bx[2])
-{
-    if (ref & kLeafBit) {
-        c[0] = ref;
-        bx[0] = leaf_box(box32, ref & ~kLeafBit);
-        return 1;
-    }
-    const BNode *nd = nodes + ref;
-    const float4 a = nd->a, b = nd->b, cc = nd->c;
-    const int4 d = nd->d;
-    c[0] = (uint32_t)d.x;
-    c[1] = (uint32_t)d.y;
-    bx[0] = {a.x, a.y, a.z, a.w, b.x, b.y};
-    bx[1] = {b.z, b.w, cc.x, cc.y, cc.z, cc.w};
-    return 2;
-}
-
 __device__ __forceinline__ Box64 record_box(const double *__restrict__ tri64, uint32_t t)
 {
     const double2 *p = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * t);
@@ -83,6 +64,85 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t cnt, uint32_t *counter
     return base + incl - cnt;
 }
 
+// Children of a node reference: one (a leaf) or two, with their FP32 boxes.
+struct Kids {
+    uint32_t c0, c1;
+    FBox b0, b1;
+    bool two;
+};
+
+__device__ __forceinline__ Kids kids_of(const BNode *__restrict__ nodes, const float *__restrict__ box32, uint32_t ref)
+{
+    Kids k;
+    if (ref & kLeafBit) {
+        k.c0 = k.c1 = ref;
+        k.b0 = k.b1 = leaf_box(box32, ref & ~kLeafBit);
+        k.two = false;
+        return k;
+    }
+    const BNode *nd = nodes + ref;
+    const float4 a = nd->a, b = nd->b, cc = nd->c;
+    const int4 d = nd->d;
+    k.c0 = (uint32_t)d.x;
+    k.c1 = (uint32_t)d.y;
+    k.b0 = {a.x, a.y, a.z, a.w, b.x, b.y};
+    k.b1 = {b.z, b.w, cc.x, cc.y, cc.z, cc.w};
+    k.two = true;
+    return k;
+}
+
+// Expansion of one node pair: child pair s = 2u + v (u over A's children, v
+// over B's) goes to the frontier (bit s of next) or, when both are leaves
+// whose exact FP64 boxes overlap, to the candidates (bit s of cand).  Every
+// field is a named register, so nothing lands in scratch memory.
+struct Expansion {
+    uint32_t a0, a1, b0, b1;
+    uint32_t next, cand;
+};
+
+// 0: pruned, 1: to the frontier, 2: candidate
+__device__ __forceinline__ uint32_t expand_one(const TraverseArgs &t, uint32_t ca, uint32_t cb, const FBox &ba,
+                                               const FBox &bb)
+{
+    if (!fbox_overlap(ba, bb))
+        return 0u;
+    if ((ca & kLeafBit) && (cb & kLeafBit))
+        return box64_overlap(record_box(t.tri64A, ca & ~kLeafBit), record_box(t.tri64B, cb & ~kLeafBit)) ? 2u : 0u;
+    return 1u;
+}
+
+__device__ __forceinline__ Expansion expand_pair(const TraverseArgs &t, uint2 pr, bool valid)
+{
+    Expansion x = {0, 0, 0, 0, 0, 0};
+    if (!valid)
+        return x;
+    const Kids ka = kids_of(t.nodesA, t.boxA, pr.x);
+    const Kids kb = kids_of(t.nodesB, t.boxB, pr.y);
+    x.a0 = ka.c0;
+    x.a1 = ka.c1;
+    x.b0 = kb.c0;
+    x.b1 = kb.c1;
+    const uint32_t r0 = expand_one(t, ka.c0, kb.c0, ka.b0, kb.b0);
+    const uint32_t r1 = kb.two ? expand_one(t, ka.c0, kb.c1, ka.b0, kb.b1) : 0u;
+    const uint32_t r2 = ka.two ? expand_one(t, ka.c1, kb.c0, ka.b1, kb.b0) : 0u;
+    const uint32_t r3 = ka.two && kb.two ? expand_one(t, ka.c1, kb.c1, ka.b1, kb.b1) : 0u;
+    x.next = (r0 == 1u ? 1u : 0u) | (r1 == 1u ? 2u : 0u) | (r2 == 1u ? 4u : 0u) | (r3 == 1u ? 8u : 0u);
+    x.cand = (r0 == 2u ? 1u : 0u) | (r1 == 2u ? 2u : 0u) | (r2 == 2u ? 4u : 0u) | (r3 == 2u ? 8u : 0u);
+    return x;
+}
+
+// child pair of slot s (a compile-time constant at every use)
+__device__ __forceinline__ uint2 slot_pair(const Expansion &x, int s)
+{
+    return make_uint2(s < 2 ? x.a0 : x.a1, (s & 1) ? x.b1 : x.b0);
+}
+
+__device__ __forceinline__ uint64_t cand_key(const Expansion &x, int s)
+{
+    const uint2 p = slot_pair(x, s);
+    return ((uint64_t)(p.x & ~kLeafBit) << 32) | (p.y & ~kLeafBit);
+}
+
 __global__ __launch_bounds__(256) void k_expand(TraverseArgs t, const uint2 *__restrict__ in,
                                                 const uint32_t *__restrict__ inCount, uint2 *__restrict__ out,
                                                 uint32_t *outCount, uint32_t outCap, uint64_t *__restrict__ cand,
@@ -95,39 +155,24 @@ __global__ __launch_bounds__(256) void k_expand(TraverseArgs t, const uint2 *__r
     for (uint32_t base = blockIdx.x * 256; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < n;
-        uint32_t ca[2], cb[2];
-        FBox ba[2], bb[2];
-        int na = 0, nb = 0;
-        uint2 pr = make_uint2(0, 0);
-        if (valid) {
-            pr = in[i];
-            na = children(t.nodesA, t.boxA, pr.x, ca, ba);
-            nb = children(t.nodesB, t.boxB, pr.y, cb, bb);
-        }
-        uint2 nxt[4];
-        uint64_t cnd[4];
-        uint32_t nn = 0, nc = 0;
-        for (int u = 0; u < na; ++u)
-            for (int v = 0; v < nb; ++v) {
-                if (!fbox_overlap(ba[u], bb[v]))
-                    continue;
-                if ((ca[u] & kLeafBit) && (cb[v] & kLeafBit)) {
-                    const uint32_t ta = ca[u] & ~kLeafBit, tb = cb[v] & ~kLeafBit;
-                    if (box64_overlap(record_box(t.tri64A, ta), record_box(t.tri64B, tb)))
-                        cnd[nc++] = ((uint64_t)ta << 32) | tb;
-                } else {
-                    nxt[nn++] = make_uint2(ca[u], cb[v]);
-                }
-            }
+        const Expansion x = expand_pair(t, valid ? in[i] : make_uint2(0, 0), valid);
         uint32_t wb;
-        const uint32_t on = wave_reserve(nn, outCount, outCap, overflow, wb);
-        for (uint32_t k = 0; k < nn; ++k)
-            if (on + k < outCap)
-                out[on + k] = nxt[k];
-        const uint32_t oc = wave_reserve(nc, candCount, candCap, overflow, wb);
-        for (uint32_t k = 0; k < nc; ++k)
-            if (oc + k < candCap)
-                cand[oc + k] = cnd[k];
+        const uint32_t on = wave_reserve(__popc(x.next), outCount, outCap, overflow, wb);
+        const uint32_t oc = wave_reserve(__popc(x.cand), candCount, candCap, overflow, wb);
+        uint32_t kn = 0, kc = 0;
+#pragma unroll
+        for (int sl = 0; sl < 4; ++sl) {
+            if ((x.next >> sl) & 1u) {
+                if (on + kn < outCap)
+                    out[on + kn] = slot_pair(x, sl);
+                ++kn;
+            }
+            if ((x.cand >> sl) & 1u) {
+                if (oc + kc < candCap)
+                    cand[oc + kc] = cand_key(x, sl);
+                ++kc;
+            }
+        }
     }
 }
 
@@ -188,33 +233,18 @@ __global__ __launch_bounds__(64) void k_traverse_dfs(TraverseArgs t, const uint2
             pr = s_stack[sp - take + lane];
         sp -= take;
         __syncthreads();
-        uint32_t ca[2], cb[2];
-        FBox ba[2], bb[2];
-        int na = 0, nb = 0;
-        if (valid) {
-            na = children(t.nodesA, t.boxA, pr.x, ca, ba);
-            nb = children(t.nodesB, t.boxB, pr.y, cb, bb);
-        }
-        uint2 nxt[4];
-        uint64_t cnd[4];
-        uint32_t nn = 0, nc = 0;
-        for (int u = 0; u < na; ++u)
-            for (int v = 0; v < nb; ++v) {
-                if (!fbox_overlap(ba[u], bb[v]))
-                    continue;
-                if ((ca[u] & kLeafBit) && (cb[v] & kLeafBit)) {
-                    const uint32_t ta = ca[u] & ~kLeafBit, tb = cb[v] & ~kLeafBit;
-                    if (box64_overlap(record_box(t.tri64A, ta), record_box(t.tri64B, tb)))
-                        cnd[nc++] = ((uint64_t)ta << 32) | tb;
-                } else {
-                    nxt[nn++] = make_uint2(ca[u], cb[v]);
-                }
-            }
+        const Expansion x = expand_pair(t, pr, valid);
+        const uint32_t nn = __popc(x.next);
         uint32_t wb;
-        const uint32_t oc = wave_reserve(nc, candCount, candCap, overflow, wb);
-        for (uint32_t k = 0; k < nc; ++k)
-            if (oc + k < candCap)
-                cand[oc + k] = cnd[k];
+        const uint32_t oc = wave_reserve(__popc(x.cand), candCount, candCap, overflow, wb);
+        uint32_t kc = 0;
+#pragma unroll
+        for (int sl = 0; sl < 4; ++sl)
+            if ((x.cand >> sl) & 1u) {
+                if (oc + kc < candCap)
+                    cand[oc + kc] = cand_key(x, sl);
+                ++kc;
+            }
         // push children: LDS when they fit, else the wave's spill region
         uint32_t incl = nn;
         for (int off = 1; off < 64; off <<= 1) {
@@ -225,12 +255,18 @@ __global__ __launch_bounds__(64) void k_traverse_dfs(TraverseArgs t, const uint2
         const uint32_t total = __shfl(incl, 63, 64);
         const uint32_t mine = incl - nn;
         if (sp + (int)total <= kDfsLds) {
-            for (uint32_t k = 0; k < nn; ++k)
-                s_stack[sp + mine + k] = nxt[k];
+            uint32_t kn = 0;
+#pragma unroll
+            for (int sl = 0; sl < 4; ++sl)
+                if ((x.next >> sl) & 1u)
+                    s_stack[sp + mine + kn++] = slot_pair(x, sl);
             sp += (int)total;
         } else if (gsp + total <= spillPerWave) {
-            for (uint32_t k = 0; k < nn; ++k)
-                gstack[gsp + mine + k] = nxt[k];
+            uint32_t kn = 0;
+#pragma unroll
+            for (int sl = 0; sl < 4; ++sl)
+                if ((x.next >> sl) & 1u)
+                    gstack[gsp + mine + kn++] = slot_pair(x, sl);
             gsp += total;
         } else {
             if (lane == 0)

@@ -215,17 +215,19 @@ __device__ __forceinline__ void gd_pair_fp64(const double *__restrict__ tri64A, 
 
 // plain FP64 over every pair
 __global__ __launch_bounds__(256) void k_gd_fp64(const double *__restrict__ tri64A, const double *__restrict__ tri64B,
-                                                 const uint64_t *__restrict__ pairs, uint32_t n, uint32_t items,
+                                                 const uint64_t *__restrict__ pairs, uint32_t cap,
+                                                 const uint32_t *__restrict__ count, uint32_t items,
                                                  uint8_t *__restrict__ status, double *__restrict__ seg,
                                                  uint32_t *__restrict__ partial)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint64_t base = (uint64_t)w * 64u * items;
-    if (base >= n)
+    if (base >= cap)
         return;
+    const uint32_t n = count && *count < cap ? *count : cap;
     uint32_t ex[5] = {0, 0, 0, 0, 0};
-    for (uint32_t it = 0; it < items; ++it) {
+    for (uint32_t it = 0; it < items && base + 64u * it < n; ++it) {
         const uint64_t i = base + 64u * it + lane;
         const bool valid = i < n;
         int exitKind = -1;
@@ -243,18 +245,21 @@ __global__ __launch_bounds__(256) void k_gd_fp64(const double *__restrict__ tri6
 
 // FP32 certificates; each chunk's undecided pairs go to its slice of `list`
 __global__ __launch_bounds__(256) void k_gd_filter(const float *__restrict__ tri32A, const float *__restrict__ tri32B,
-                                                   const uint64_t *__restrict__ pairs, uint32_t n, uint32_t items,
+                                                   const uint64_t *__restrict__ pairs, uint32_t cap,
+                                                   const uint32_t *__restrict__ count, uint32_t items,
                                                    uint8_t *__restrict__ status, uint32_t *__restrict__ list,
                                                    uint32_t *__restrict__ chunkCount, uint32_t *__restrict__ partial)
 {
     const int lane = threadIdx.x & 63;
     const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint64_t base = (uint64_t)w * 64u * items;
-    if (base >= n)
+    if (base >= cap)
         return;
+    // with a device-side count, chunks past it still clear their slice count
+    const uint32_t n = count && *count < cap ? *count : cap;
     const unsigned long long below = (1ull << lane) - 1ull;
     uint32_t cnt = 0, e0 = 0, e1 = 0, e2 = 0, fb = 0, ft = 0;
-    for (uint32_t it = 0; it < items; ++it) {
+    for (uint32_t it = 0; it < items && base + 64u * it < n; ++it) {
         const uint64_t i = base + 64u * it + lane;
         const bool valid = i < n;
         int f = -3;
@@ -450,10 +455,11 @@ size_t tritri_work_u32(uint32_t n)
     return (size_t)n + (size_t)chunks * (1 + kStatSlots) + 4;
 }
 
-// work: tritri_work_u32(n) u32 (list slices, chunk counts, per-chunk statistics)
+// work: tritri_work_u32(n) u32 (list slices, chunk counts, per-chunk statistics);
+// count (may be null): device-side number of pairs, at most n, read at run time
 void launch_tritri(const double *tri64A, const double *tri64B, const float *tri32A, const float *tri32B,
-                   const uint64_t *pairs, uint32_t n, int precision, uint8_t *status, double *seg, uint32_t *work,
-                   GdStats *stats, hipStream_t s)
+                   const uint64_t *pairs, uint32_t n, const uint32_t *count, int precision, uint8_t *status,
+                   double *seg, uint32_t *work, GdStats *stats, hipStream_t s)
 {
     if (n == 0)
         return;
@@ -464,12 +470,13 @@ void launch_tritri(const double *tri64A, const double *tri64B, const float *tri3
     uint32_t *chunkCount = work + n;
     uint32_t *partial = stats ? chunkCount + chunks : nullptr;
     if (precision == 1) {
-        hipLaunchKernelGGL(k_gd_filter, grid, dim3(256), 0, s, tri32A, tri32B, pairs, n, items, status, list,
+        hipLaunchKernelGGL(k_gd_filter, grid, dim3(256), 0, s, tri32A, tri32B, pairs, n, count, items, status, list,
                            chunkCount, partial);
         hipLaunchKernelGGL(k_gd_list, grid, dim3(256), 0, s, tri64A, tri64B, pairs, chunks, items, list, chunkCount,
                            status, seg, partial);
     } else {
-        hipLaunchKernelGGL(k_gd_fp64, grid, dim3(256), 0, s, tri64A, tri64B, pairs, n, items, status, seg, partial);
+        hipLaunchKernelGGL(k_gd_fp64, grid, dim3(256), 0, s, tri64A, tri64B, pairs, n, count, items, status, seg,
+                           partial);
     }
     if (stats)
         hipLaunchKernelGGL(k_gd_stats, dim3(1), dim3(256), 0, s, partial, chunks, stats);

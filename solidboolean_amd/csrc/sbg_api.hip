@@ -7,12 +7,11 @@
 #include <string>
 #include <vector>
 
-#include "sbg_internal.h"
-#include "sbgpu.h"
+#include "sbg_host.h"
 
 using namespace sbg;
 
-namespace {
+namespace sbg {
 
 thread_local std::string g_err;
 
@@ -27,87 +26,40 @@ int fail(int code, const char *fmt, ...)
     return code;
 }
 
-#define HIP_TRY(expr)                                                                                     \
-    do {                                                                                                  \
-        hipError_t e_ = (expr);                                                                           \
-        if (e_ != hipSuccess)                                                                             \
-            return fail(SBG_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
-    } while (0)
+int use(sbg_context ctx)
+{
+    if (!ctx)
+        return fail(SBG_ERR_INVALID, "null context");
+    if (ctx->destroyed)
+        return fail(SBG_ERR_STATE, "context already destroyed");
+    HIP_TRY(hipSetDevice(ctx->device));
+    return SBG_OK;
+}
 
-#define TRY(expr)              \
-    do {                       \
-        int r_ = (expr);       \
-        if (r_ != SBG_OK)      \
-            return r_;         \
-    } while (0)
 
-// growable device buffer (contents are not preserved on growth)
-template <typename T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t cap = 0;
-    int ensure(size_t n)
-    {
-        if (n <= cap && p)
-            return SBG_OK;
-        release();
-        const size_t count = n ? n : 1;
-        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            (void)hipGetLastError();
-            return fail(SBG_ERR_OOM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
-        }
-        cap = count;
-        return SBG_OK;
-    }
-    void release()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        cap = 0;
-    }
-};
+void context_release(sbg_context ctx)
+{
+    if (--ctx->refs == 0)
+        context_free(ctx);
+}
 
-constexpr int kMaxLevels = 256;
-constexpr int kExpandGrid = 1024;
-constexpr int kLevelsPerSync = 8;
-// breadth-first levels before the persistent depth-first kernel takes the
-// frontier, its wave count, and the initial / largest spill region per wave
-constexpr int kBfsLevels = 10;
-constexpr int kDfsWaves = 2048;
-constexpr uint32_t kSpillInit = 2048, kSpillMax = 1u << 15;
+int check_launch(const char *what)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(SBG_ERR_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
+    return SBG_OK;
+}
 
-}  // namespace
+int ceil_log2(size_t n)
+{
+    int b = 0;
+    while (((size_t)1 << b) < n)
+        ++b;
+    return b > 0 ? b : 1;
+}
 
-struct sbg_context_s {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t *pinned = nullptr;  // host-visible counters (kMaxLevels + 8)
-    DevBuf<uint2> front[2];
-    DevBuf<uint2> spill;              // depth-first spill regions
-    uint32_t spillPerWave = kSpillInit;
-    bool depthFirst = true;           // false after the spill regions proved too small
-    DevBuf<uint32_t> counters;
-    size_t frontCap = 0, candCap = 0;
-    DevBuf<uint32_t> hist;
-    DevBuf<GdStats> stats;
-    DevBuf<double> pts;
-    DevBuf<uint8_t> flags, bits, votes;
-    DevBuf<uint32_t> ovf, rayList;
-    DevBuf<double> rawA, rawB, rawSeg;
-    DevBuf<uint8_t> rawStatus;
-    int refs = 1;  // the owner + every live mesh / pair list
-    bool destroyed = false;
-    bool timing = false;
-    struct Ev {
-        int stage;
-        hipEvent_t a, b;
-    };
-    std::vector<Ev> events;
-    double acc[SBG_STAGE_COUNT] = {0};
-};
+}  // namespace sbg
 
 struct sbg_mesh_s {
     sbg_context ctx = nullptr;
@@ -143,71 +95,6 @@ struct sbg_pairs_s {
     // against the meshes before any kernel indexes them)
     uint64_t maxA = 0, maxB = 0;
 };
-
-namespace {
-
-int use(sbg_context ctx)
-{
-    if (!ctx)
-        return fail(SBG_ERR_INVALID, "null context");
-    if (ctx->destroyed)
-        return fail(SBG_ERR_STATE, "context already destroyed");
-    HIP_TRY(hipSetDevice(ctx->device));
-    return SBG_OK;
-}
-
-// Meshes and pair lists hold a reference on their context, so destroying a
-// context while they live only marks it; the last release frees it.
-void context_free(sbg_context ctx);
-void context_release(sbg_context ctx)
-{
-    if (--ctx->refs == 0)
-        context_free(ctx);
-}
-
-// HIP-event bracket of one stage (no-op unless timing is enabled)
-struct Stage {
-    sbg_context c;
-    int stage;
-    hipEvent_t a = nullptr;
-    Stage(sbg_context ctx, int s) : c(ctx), stage(s)
-    {
-        if (c->timing && hipEventCreate(&a) == hipSuccess)
-            (void)hipEventRecord(a, c->stream);
-        else
-            a = nullptr;
-    }
-    ~Stage()
-    {
-        if (!a)
-            return;
-        hipEvent_t b = nullptr;
-        if (hipEventCreate(&b) == hipSuccess) {
-            (void)hipEventRecord(b, c->stream);
-            c->events.push_back({stage, a, b});
-        } else {
-            (void)hipEventDestroy(a);
-        }
-    }
-};
-
-int check_launch(const char *what)
-{
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess)
-        return fail(SBG_ERR_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
-    return SBG_OK;
-}
-
-int ceil_log2(size_t n)
-{
-    int b = 0;
-    while (((size_t)1 << b) < n)
-        ++b;
-    return b > 0 ? b : 1;
-}
-
-}  // namespace
 
 extern "C" {
 
@@ -255,7 +142,7 @@ int sbg_context_create(int device, sbg_context *out)
 
 }  // extern "C"
 
-namespace {
+namespace sbg {
 void context_free(sbg_context ctx)
 {
     (void)hipSetDevice(ctx->device);
@@ -284,7 +171,7 @@ void context_free(sbg_context ctx)
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
-}  // namespace
+}  // namespace sbg
 
 extern "C" {
 
@@ -797,7 +684,7 @@ static int run_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, GdSta
     TRY(p->work.ensure(tritri_work_u32((uint32_t)p->n)));
     {
         Stage st(c, SBG_STAGE_TRI_TRI);
-        launch_tritri(a->tri64.p, b->tri64.p, a->tri32.p, b->tri32.p, p->pairs.p, n, precision, p->status.p,
+        launch_tritri(a->tri64.p, b->tri64.p, a->tri32.p, b->tri32.p, p->pairs.p, n, nullptr, precision, p->status.p,
                       p->seg.p, p->work.p, dstats, c->stream);
         TRY(check_launch("tri_tri"));
     }
@@ -923,6 +810,20 @@ int sbg_tri_tri_batch(sbg_context ctx, const double *a9, const double *b9, size_
 // ---------------------------------------------------------------------------
 // classification
 // ---------------------------------------------------------------------------
+static GridSet grid_set(sbg_mesh m)
+{
+    GridSet gs = {};
+    gs.boxes = m->rootBox.p;
+    gs.cells = m->gridCells.p;
+    gs.gbox = m->gridBox.p;
+    gs.gref = m->gridRef.p;
+    gs.G0 = m->gridG;
+    gs.cap = (uint32_t)m->gridCap;
+    gs.ncells = (uint32_t)(grid_cells(m->gridG) - 1);
+    gs.mix = grid_mix((uint32_t)m->nt);
+    return gs;
+}
+
 static int ensure_grid(sbg_mesh m)
 {
     if (m->gridReady)
@@ -937,13 +838,13 @@ static int ensure_grid(sbg_mesh m)
     TRY(m->gridTemp.ensure(scan_large_temp(grid_cells(G))));
     TRY(m->gridBox.ensure(m->gridCap));
     TRY(m->gridRef.ensure(m->gridCap));
+    m->gridG = G;
     {
         Stage st(c, SBG_STAGE_GRID);
-        launch_grid_build(m->tri64.p, (uint32_t)m->nt, m->rootBox.p, G, m->gridCells.p, m->gridTemp.p, m->gridBox.p,
-                          m->gridRef.p, (uint32_t)m->gridCap, c->stream);
+        launch_grid_build(m->tri64.p, (uint32_t)m->nt, grid_set(m), m->gridCells.p, m->gridTemp.p, m->gridBox.p,
+                          m->gridRef.p, c->stream);
         TRY(check_launch("ray grid"));
     }
-    m->gridG = G;
     m->gridReady = true;
     return SBG_OK;
 }
@@ -968,16 +869,14 @@ static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder,
         {
             Stage st(c, SBG_STAGE_CLASSIFY);
             launch_fill_u32(c->ovf.p, 0u, 2, c->stream);
-            const GridArgs ga = {target->gridCells.p, target->gridBox.p, target->gridRef.p, target->rootBox.p,
-                                 target->gridG, (uint32_t)target->gridCap};
-            launch_rays(ga, target->tri64.p, target->normals.p, dpts, dorder, dskip, (uint32_t)n, c->bits.p,
+            launch_rays(grid_set(target), target->tri64.p, target->normals.p, dpts, dorder, dskip, (uint32_t)n, c->bits.p,
                         c->rayList.p, c->ovf.p + 1, c->ovf.p, c->stream);
             launch_vote(c->bits.p, dskip, (uint32_t)n, c->votes.p, c->stream);
             TRY(check_launch("rays"));
         }
         uint32_t h[2] = {0, 0};
         HIP_TRY(hipMemcpyAsync(h, c->ovf.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(h + 1, target->gridCells.p + 3 * (size_t)target->gridG * target->gridG,
+        HIP_TRY(hipMemcpyAsync(h + 1, target->gridCells.p + (grid_cells(target->gridG) - 1),
                                sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (h[0] & 2u) {
@@ -1120,7 +1019,7 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, int it
     HIP_TRY(hipEventCreate(&e1));
     HIP_TRY(hipEventRecord(e0, c->stream));
     for (int i = 0; i < iters; ++i)
-        launch_tritri(a->tri64.p, b->tri64.p, a->tri32.p, b->tri32.p, p->pairs.p, (uint32_t)p->n, precision,
+        launch_tritri(a->tri64.p, b->tri64.p, a->tri32.p, b->tri32.p, p->pairs.p, (uint32_t)p->n, nullptr, precision,
                       p->status.p, p->seg.p, p->work.p, nullptr, c->stream);
     HIP_TRY(hipEventRecord(e1, c->stream));
     HIP_TRY(hipEventSynchronize(e1));

@@ -1,0 +1,140 @@
+// Host-side internals shared by the C-ABI translation units: error
+// reporting, growable device buffers, the context record and stage timing.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "sbg_internal.h"
+#include "sbgpu.h"
+
+namespace sbg {
+
+// records the message for sbg_last_error() and returns `code`
+int fail(int code, const char *fmt, ...);
+
+#define HIP_TRY(expr)                                                                                     \
+    do {                                                                                                  \
+        hipError_t e_ = (expr);                                                                           \
+        if (e_ != hipSuccess)                                                                             \
+            return fail(SBG_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    } while (0)
+
+#define TRY(expr)              \
+    do {                       \
+        int r_ = (expr);       \
+        if (r_ != SBG_OK)      \
+            return r_;         \
+    } while (0)
+
+// growable device buffer (contents are not preserved on growth)
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n)
+    {
+        if (n <= cap && p)
+            return SBG_OK;
+        release();
+        const size_t count = n ? n : 1;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(SBG_ERR_OOM, "hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+        }
+        cap = count;
+        return SBG_OK;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+constexpr int kMaxLevels = 256;
+constexpr int kExpandGrid = 1024;
+constexpr int kLevelsPerSync = 8;
+// breadth-first levels before the persistent depth-first kernel takes the
+// frontier, its wave count, and the initial / largest spill region per wave
+constexpr int kBfsLevels = 10;
+constexpr int kDfsWaves = 2048;
+constexpr uint32_t kSpillInit = 2048, kSpillMax = 1u << 15;
+
+}  // namespace sbg
+
+using namespace sbg;
+
+struct sbg_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t *pinned = nullptr;  // host-visible counters (kMaxLevels + 8)
+    DevBuf<uint2> front[2];
+    DevBuf<uint2> spill;              // depth-first spill regions
+    uint32_t spillPerWave = kSpillInit;
+    bool depthFirst = true;           // false after the spill regions proved too small
+    DevBuf<uint32_t> counters;
+    size_t frontCap = 0, candCap = 0;
+    DevBuf<uint32_t> hist;
+    DevBuf<GdStats> stats;
+    DevBuf<double> pts;
+    DevBuf<uint8_t> flags, bits, votes;
+    DevBuf<uint32_t> ovf, rayList;
+    DevBuf<double> rawA, rawB, rawSeg;
+    DevBuf<uint8_t> rawStatus;
+    int refs = 1;  // the owner + every live mesh / pair list
+    bool destroyed = false;
+    bool timing = false;
+    struct Ev {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Ev> events;
+    double acc[SBG_STAGE_COUNT] = {0};
+};
+
+namespace sbg {
+
+// validates the handle and makes its device current
+int use(sbg_context ctx);
+// Meshes and pair lists hold a reference on their context, so destroying a
+// context while they live only marks it; the last release frees it.
+void context_release(sbg_context ctx);
+void context_free(sbg_context ctx);
+
+// HIP-event bracket of one stage (no-op unless timing is enabled)
+struct Stage {
+    sbg_context c;
+    int stage;
+    hipEvent_t a = nullptr;
+    Stage(sbg_context ctx, int s) : c(ctx), stage(s)
+    {
+        if (c->timing && hipEventCreate(&a) == hipSuccess)
+            (void)hipEventRecord(a, c->stream);
+        else
+            a = nullptr;
+    }
+    ~Stage()
+    {
+        if (!a)
+            return;
+        hipEvent_t b = nullptr;
+        if (hipEventCreate(&b) == hipSuccess) {
+            (void)hipEventRecord(b, c->stream);
+            c->events.push_back({stage, a, b});
+        } else {
+            (void)hipEventDestroy(a);
+        }
+    }
+};
+
+int check_launch(const char *what);
+int ceil_log2(size_t n);
+
+}  // namespace sbg

@@ -59,6 +59,8 @@ int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRa
 // the top refit's two ready lists)
 void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
                       uint32_t *leafParent, uint32_t *work, hipStream_t s);
+void launch_hierarchy64(const uint64_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
+                        uint32_t *leafParent, uint32_t *work, hipStream_t s);
 void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *nodes,
                   const uint32_t *leafParent, uint32_t *work, float *rootBox, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
@@ -95,9 +97,10 @@ struct GdStats {
 // work: tritri_work_u32(n) u32 of scratch (undecided-pair slices, per-chunk
 // counts and statistics)
 size_t tritri_work_u32(uint32_t n);
+// count (may be null): device-side number of pairs (at most n), read at run time
 void launch_tritri(const double *tri64A, const double *tri64B, const float *tri32A, const float *tri32B,
-                   const uint64_t *pairs, uint32_t n, int precision, uint8_t *status, double *seg, uint32_t *work,
-                   GdStats *stats, hipStream_t s);
+                   const uint64_t *pairs, uint32_t n, const uint32_t *count, int precision, uint8_t *status,
+                   double *seg, uint32_t *work, GdStats *stats, hipStream_t s);
 void launch_tritri_raw(const double *a9, const double *b9, uint32_t n, uint8_t *status, double *seg,
                        hipStream_t s);
 // compaction of hits (status == 1) into (pair, seg) lists, in pair order
@@ -110,23 +113,28 @@ int compact_blocks(uint32_t n);
 void launch_mark_intersected(const uint64_t *pairs, const uint8_t *status, uint32_t n, int side,
                              uint8_t *flags, hipStream_t s);
 void launch_centroids(const double *tri64, const uint8_t *flags, uint32_t nt, double *pts, hipStream_t s);
-// per-axis projection grid of a target mesh: cells holds grid_cells(G) u32,
-// entries gbox / gref hold cap entries each
-struct GridArgs {
+// Per-axis projection grids of one or more target meshes sharing one cell
+// array (ncells cells + 1 total) and one entry array (cap entries).
+struct GridSet {
+    const uint32_t *meshOf;  // triangle -> mesh (null: a single mesh, id 0)
+    const uint32_t *target;  // query -> target mesh, ~0u for none (null: mesh 0)
+    const uint2 *desc;       // mesh -> (G, first cell) (null: (G0, 0))
+    const float *boxes;      // mesh -> FP32 root box (lo xyz, hi xyz)
     const uint32_t *cells;
     const float4 *gbox;
     const uint2 *gref;
-    const float *rootBox;
-    uint32_t G, cap;
+    uint32_t G0, cap, ncells;
+    uint32_t mix;  // triangle visiting stride of the grid build, coprime with the triangle count
 };
+uint32_t grid_mix(uint32_t nt);
 uint32_t grid_dim(uint32_t nt);
-size_t grid_cells(uint32_t G);
-void launch_grid_build(const double *tri64, uint32_t nt, const float *rootBox, uint32_t G, uint32_t *cells,
-                       uint32_t *scanTemp, float4 *gbox, uint2 *gref, uint32_t cap, hipStream_t s);
+size_t grid_cells(uint32_t G);  // 3 G^2 + 1
+void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
+                       float4 *gbox, uint2 *gref, hipStream_t s);
 // order (may be null): query q uses point order[q]; ovfList holds 3n entries,
 // *ovfCount must be zero on entry; overflow bit 0: a ray has too many distinct
-// hits, bit 1: the grid was truncated (cells[3 G^2] > cap)
-void launch_rays(const GridArgs &ga, const double *tri64, const double *normals, const double *pts,
+// hits, bit 1: the grid was truncated (cells[ncells] > cap)
+void launch_rays(const GridSet &gs, const double *tri64, const double *normals, const double *pts,
                  const uint32_t *order, const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList,
                  uint32_t *ovfCount, uint32_t *overflow, hipStream_t s);
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);
