@@ -142,6 +142,74 @@ def cpu_baseline(a, b, pairs_ref, budget_s: float = 12.0):
     }
 
 
+def fp64_vs_batched(ctx, a, b, step, N, steps: int) -> dict:
+    """The 1.31M-triangle pair end to end two ways: the plain FP64 path (single
+    pair API, FP64 Guigue-Devillers) and the FP32-filtered batched path (the
+    batch API with this one pair, replaying its captured graph)."""
+    for _ in range(2):
+        step(N.FP64)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(N.FP64)
+    ctx.synchronize()
+    plain = (time.perf_counter() - t0) / steps * 1e3
+    batch = ctx.batch([(a, b)])
+    batch.run(N.FP32_FILTERED)
+    batch.run(N.FP32_FILTERED)
+    dev, t0 = [], time.perf_counter()
+    for _ in range(steps):
+        dev.append(batch.run(N.FP32_FILTERED))
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    out = {"plain_fp64_ms": plain, "filtered_batched_ms": wall, "filtered_batched_device_ms": float(np.mean(dev)),
+           "speedup": plain / wall}
+    batch.close()
+    log(f"1.31M pair: plain FP64 {plain:.2f} ms, filtered batched {wall:.2f} ms (device {out['filtered_batched_device_ms']:.2f})")
+    return out
+
+
+def batch_section(ctx, mg, N, steps: int) -> dict:
+    """4,096 independent icosphere pairs (levels 1-6, P ~ 4^-level, radii
+    U[0.5, 2], offsets U[-0.8, 0.8]^3 x radius, seeded rotations) as one batch:
+    autotuned, then timed as graph replays."""
+    t0 = time.perf_counter()
+    pairs = mg.batch_pairs(4096, seed=1)
+    batch = ctx.batch(pairs)
+    log(f"batch: {len(pairs)} pairs, {batch.nt} triangles, built in {time.perf_counter() - t0:.1f}s")
+    batch.run(N.FP32_FILTERED)
+    tuned, tuned_ms = batch.autotune(N.FP32_FILTERED, reps=2)
+    log(f"batch autotune: {tuned} ({tuned_ms:.3f} ms)")
+    res = {}
+    for name, prec in (("filtered", N.FP32_FILTERED), ("fp64", N.FP64)):
+        batch.run(prec)
+        batch.run(prec)
+        dev, t0 = [], time.perf_counter()
+        for _ in range(steps):
+            dev.append(batch.run(prec))
+        wall = (time.perf_counter() - t0) / steps * 1e3
+        res[name] = (wall, float(np.mean(dev)))
+    ncand, nhits = batch.counts()
+    wall, dev = res["filtered"]
+    out = {
+        "workload": "4,096 icosphere pairs, levels 1-6 with P ~ 4^-level, radii U[0.5, 2], offsets U[-0.8, 0.8]^3 "
+                    "x radius, seeded rotations (seed 1), one captured graph per run",
+        "pairs": len(pairs),
+        "triangles": batch.nt,
+        "candidate_pairs": ncand,
+        "intersecting_pairs": nhits,
+        "ms_per_batch": wall,
+        "device_ms_per_batch": dev,
+        "mesh_pairs_per_s": len(pairs) / (wall * 1e-3),
+        "tri_pair_tests_per_s": ncand / (wall * 1e-3),
+        "fp64_ms_per_batch": res["fp64"][0],
+        "filtered_speedup_vs_fp64": res["fp64"][0] / wall,
+        "tuning": tuned,
+    }
+    batch.close()
+    log(f"batch: {wall:.2f} ms per batch (device {dev:.2f}), fp64 {res['fp64'][0]:.2f} ms")
+    return out
+
+
 def _cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -161,6 +229,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 4,096-pair batch and the FP64 comparison")
     ap.add_argument("--gd-only", type=int, default=0, help="profiling aid: only N back-to-back GD launches")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -305,11 +374,14 @@ def main():
         "copy_kernel_gbs": copy_gbs,
         "parity": parity,
     }
+    if not args.no_batch:
+        line["fp64_vs_filtered_batched"] = fp64_vs_batched(ctx, a, b, step, N, args.steps)
+        line["batch"] = batch_section(ctx, mg, N, args.steps)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(a, b, None)
     print(json.dumps(line), flush=True)
-    del Psat, P, A, B
+    del Psat, A, B
     ctx.close()
 
 

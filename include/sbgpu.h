@@ -151,6 +151,47 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, int precision, in
  * traffic counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
 int sbg_calibrate_copy(sbg_context ctx, size_t bytes, int iters, double *avg_ms);
 
+/* ---- batched booleans ------------------------------------------------------
+ * Many independent mesh pairs in one structure.  There are nmeshes = 2 *
+ * npairs meshes in concatenated arrays: mesh m owns vertices
+ * [vert_begin[m], vert_begin[m + 1]) of xyz and triangles
+ * [tri_begin[m], tri_begin[m + 1]) of tris, whose indices are local to the
+ * mesh's own vertices; pair k is (mesh 2k, mesh 2k + 1).  One run is, for
+ * every pair, the hot path of one SolidBoolean::combine
+ * (src/solidboolean.cpp:94-122 candidate search + Guigue-Devillers, :482-510
+ * ray classification of every non-intersected triangle against the partner
+ * mesh), all pairs sharing every launch.  The reference runs pairs one by one
+ * and has no batched call. */
+typedef struct sbg_batch_s *sbg_batch;
+
+typedef struct sbg_batch_tuning {
+    int bfs_levels;   /* breadth-first levels before the persistent depth-first search (0..32) */
+    int dfs_waves;    /* one-wave workgroups of the depth-first search (64..16384) */
+    int grid_density; /* ray-grid cells per axis plane = triangles * grid_density / 16 (1..64) */
+    int use_graph;    /* nonzero: once sized, each run replays one captured hipGraph */
+} sbg_batch_tuning;
+
+int sbg_batch_create(sbg_context ctx, const double *xyz, const uint32_t *tris, const uint64_t *vert_begin,
+                     const uint64_t *tri_begin, size_t nmeshes, sbg_batch *out);
+/* new vertex coordinates (same count and topology); the captured graph stays valid */
+int sbg_batch_set_vertices(sbg_batch b, const double *xyz);
+int sbg_batch_set_tuning(sbg_batch b, const sbg_batch_tuning *t);
+int sbg_batch_get_tuning(sbg_batch b, sbg_batch_tuning *t);
+/* one run at SBG_PRECISION_*; returns when its results are ready */
+int sbg_batch_run(sbg_batch b, int precision);
+/* device time of the last run's work (HIP events), in ms */
+int sbg_batch_last_ms(sbg_batch b, double *ms);
+int sbg_batch_counts(sbg_batch b, size_t *candidates, size_t *hits);
+/* candidates of the last run sorted by (a, b) as global triangle indices
+ * (2 per candidate), with status (1 hit, 2 coplanar, 0 none) and, for hits,
+ * the segment (6 doubles per candidate); any output may be null */
+int sbg_batch_candidates(sbg_batch b, uint32_t *ab, uint8_t *status, double *seg);
+/* per triangle: 0 outside / 1 inside its pair partner (3-ray vote), 2 intersected */
+int sbg_batch_sides(sbg_batch b, uint8_t *side);
+/* coordinate search over the tuning knobs, `reps` timed runs per setting; leaves the best set */
+int sbg_batch_autotune(sbg_batch b, int precision, int reps, sbg_batch_tuning *best, double *best_ms);
+int sbg_batch_destroy(sbg_batch b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -259,6 +259,19 @@ void launch_copy16(const void *src, void *dst, size_t n16, hipStream_t s)
     hipLaunchKernelGGL(k_copy16, dim3(4096), dim3(256), 0, s, (const uint4 *)src, (uint4 *)dst, n16);
 }
 
+__global__ void k_copy_u32(const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        dst[i] = src[i];
+}
+
+void launch_copy_u32(const uint32_t *src, uint32_t *dst, size_t n, hipStream_t s)
+{
+    if (n)
+        hipLaunchKernelGGL(k_copy_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+}
+
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s)
 {
     if (n == 0)

@@ -108,17 +108,6 @@ __global__ __launch_bounds__(256) void k_bounds_reduce(const float *__restrict__
         bounds[threadIdx.x] = s[threadIdx.x][0];
 }
 
-// spread the low 10 bits of x to every third bit
-__device__ __forceinline__ uint32_t spread3(uint32_t x)
-{
-    x &= 0x3ffu;
-    x = (x | (x << 16)) & 0x030000ffu;
-    x = (x | (x << 8)) & 0x0300f00fu;
-    x = (x | (x << 4)) & 0x030c30c3u;
-    x = (x | (x << 2)) & 0x09249249u;
-    return x;
-}
-
 __global__ __launch_bounds__(256) void k_morton(const float *__restrict__ box32, const float *__restrict__ bounds,
                                                 uint32_t nt, uint32_t *__restrict__ keys,
                                                 uint32_t *__restrict__ vals)
@@ -126,17 +115,7 @@ __global__ __launch_bounds__(256) void k_morton(const float *__restrict__ box32,
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nt)
         return;
-    const float4 lo = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * i)[0];
-    const float4 hi = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * i)[1];
-    float c[3] = {0.5f * lo.x + 0.5f * hi.x, 0.5f * lo.y + 0.5f * hi.y, 0.5f * lo.z + 0.5f * hi.z};
-    uint32_t q[3];
-    for (int k = 0; k < 3; ++k) {
-        float ext = bounds[3 + k] - bounds[k];
-        float t = ext > 0.0f ? (c[k] - bounds[k]) / ext : 0.0f;
-        t = fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f);
-        q[k] = (uint32_t)t;
-    }
-    keys[i] = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+    keys[i] = morton30(box_centre(box32, i), bounds);
     vals[i] = i;
 }
 

@@ -253,6 +253,41 @@ __device__ __forceinline__ bool box64_overlap(const Box64 &a, const Box64 &b)
 __device__ __forceinline__ float f_down(double x) { return __double2float_rd(x); }
 __device__ __forceinline__ float f_up(double x) { return __double2float_ru(x); }
 
+// ---- Morton codes ----------------------------------------------------------
+// spread the low 10 bits of x to every third bit
+__device__ __forceinline__ uint32_t spread3(uint32_t x)
+{
+    x &= 0x3ffu;
+    x = (x | (x << 16)) & 0x030000ffu;
+    x = (x | (x << 8)) & 0x0300f00fu;
+    x = (x | (x << 4)) & 0x030c30c3u;
+    x = (x | (x << 2)) & 0x09249249u;
+    return x;
+}
+
+// centre of triangle i's FP32 leaf box (8 floats: lo xyz, pad, hi xyz, pad)
+__device__ __forceinline__ float3 box_centre(const float *__restrict__ box32, uint32_t i)
+{
+    const float4 lo = reinterpret_cast<const float4 *>(box32 + (size_t)8 * i)[0];
+    const float4 hi = reinterpret_cast<const float4 *>(box32 + (size_t)8 * i)[1];
+    return make_float3(0.5f * lo.x + 0.5f * hi.x, 0.5f * lo.y + 0.5f * hi.y, 0.5f * lo.z + 0.5f * hi.z);
+}
+
+// 30-bit Morton code of c within bounds (lo xyz, hi xyz), 10 bits per axis
+__device__ __forceinline__ uint32_t morton30(float3 c, const float *bounds)
+{
+    const float cc[3] = {c.x, c.y, c.z};
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float ext = bounds[3 + k] - bounds[k];
+        float t = ext > 0.0f ? (cc[k] - bounds[k]) / ext : 0.0f;
+        t = fminf(fmaxf(t * 1024.0f, 0.0f), 1023.0f);
+        q[k] = (uint32_t)t;
+    }
+    return (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+}
+
 // PositionKey (src/positionkey.cpp:25-37): (long)(x * 100000), truncation.
 struct Key3 {
     long long x, y, z;

@@ -65,6 +65,7 @@ void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *no
                   const uint32_t *leafParent, uint32_t *work, float *rootBox, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 void launch_copy16(const void *src, void *dst, size_t n16, hipStream_t s);
+void launch_copy_u32(const uint32_t *src, uint32_t *dst, size_t n, hipStream_t s);
 
 // k_traverse.hip
 struct TraverseArgs {
@@ -138,5 +139,21 @@ void launch_rays(const GridSet &gs, const double *tri64, const double *normals, 
                  const uint32_t *order, const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList,
                  uint32_t *ovfCount, uint32_t *overflow, hipStream_t s);
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);
+
+// k_batch.hip
+void launch_batch_bounds_init(int *mb, uint32_t nmeshes, hipStream_t s);
+void launch_batch_bounds(const float *box32, const uint32_t *meshOf, uint32_t nt, int *mb, hipStream_t s);
+void launch_batch_keys(const float *box32, const uint32_t *meshOf, const uint32_t *triBegin, const int *mb,
+                       uint32_t nt, int localBits, int mortonBits, uint64_t *keys, hipStream_t s);
+void launch_batch_codes(const uint64_t *keys, uint32_t n, int localBits, int mortonBits, const uint32_t *triBegin,
+                        uint64_t *codes, uint32_t *ids, hipStream_t s);
+void launch_mesh_roots(const BNode *nodes, const uint64_t *codes, const uint32_t *ids, uint32_t n, int mortonBits,
+                       const float *box32, const float *rootBox, uint32_t *meshRoot, float *meshBox, hipStream_t s);
+void launch_batch_seed(const TraverseArgs &t, const uint32_t *meshRoot, const float *meshBox, uint32_t npairs,
+                       uint2 *frontier, uint32_t *frontCount, uint32_t frontCap, uint64_t *cand, uint32_t *candCount,
+                       uint32_t candCap, uint32_t *overflow, hipStream_t s);
+void launch_pad_pairs(uint64_t *pairs, const uint32_t *count, uint32_t cap, hipStream_t s);
+void launch_mark_both(const uint64_t *pairs, const uint8_t *status, const uint32_t *count, uint32_t cap,
+                      uint8_t *flags, hipStream_t s);
 
 }  // namespace sbg

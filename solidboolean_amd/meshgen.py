@@ -277,6 +277,34 @@ def cube_pair(n: int = 512):
     return a, b
 
 
+BATCH_LEVELS = (1, 2, 3, 4, 5, 6)
+
+
+def batch_pairs(npairs: int = 4096, seed: int = 1, levels=BATCH_LEVELS):
+    """Independent icosphere pairs of uneven sizes, all draws from one
+    SplitMix64(seed) stream, per pair in this order: the subdivision level
+    (P(level) proportional to 4^-level over `levels`), radius A and radius B
+    (uniform in [0.5, 2]), B's offset (uniform in [-0.8, 0.8]^3 times radius A),
+    then B's rotation (4 Gaussians).  Both spheres of a pair share the level."""
+    rng = SplitMix64(seed)
+    lv = np.asarray(levels, np.int64)
+    p = 4.0 ** (-lv.astype(np.float64))
+    cdf = np.cumsum(p / p.sum())
+    base = {int(l): icosphere(int(l)) for l in lv}
+    out = []
+    for _ in range(npairs):
+        u = float(rng.uniform(1)[0])
+        level = int(lv[min(int(np.searchsorted(cdf, u, side="right")), len(lv) - 1)])
+        ra, rb = (0.5 + 1.5 * rng.uniform(2)).tolist()
+        off = ((2.0 * rng.uniform(3) - 1.0) * 0.8 * ra).tolist()
+        rot = random_rotation(rng)
+        s = base[level]
+        a = Mesh(s.vertices * ra, s.triangles)
+        b = transform(Mesh(s.vertices * rb, s.triangles), rot, off)
+        out.append((a, b))
+    return out
+
+
 def mesh_digest(mesh: Mesh) -> str:
     import hashlib
     h = hashlib.sha256()

@@ -34,6 +34,9 @@ EXPORTS = [
     "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
     "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_context_set_timing",
     "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_calibrate_copy",
+    "sbg_batch_create", "sbg_batch_set_vertices", "sbg_batch_set_tuning", "sbg_batch_get_tuning", "sbg_batch_run",
+    "sbg_batch_last_ms", "sbg_batch_counts", "sbg_batch_candidates", "sbg_batch_sides", "sbg_batch_autotune",
+    "sbg_batch_destroy",
 ]
 
 
@@ -41,6 +44,13 @@ class SbgError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"sbgpu error {code}: {msg}")
         self.code = code
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("bfs_levels", C.c_int), ("dfs_waves", C.c_int), ("grid_density", C.c_int), ("use_graph", C.c_int)]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
 class TriTriStats(C.Structure):
@@ -59,6 +69,7 @@ _D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _U32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
 _U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 _F = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_U64 = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
 
 _lib = None
 
@@ -102,6 +113,17 @@ def load(path: str = LIB_PATH):
         "sbg_context_stage_ms": [_P, _D, C.c_int],
         "sbg_bench_tri_tri": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)],
         "sbg_calibrate_copy": [_P, _SZ, C.c_int, C.POINTER(C.c_double)],
+        "sbg_batch_create": [_P, _D, _U32, _U64, _U64, _SZ, _PP],
+        "sbg_batch_set_vertices": [_P, _D],
+        "sbg_batch_set_tuning": [_P, C.POINTER(BatchTuning)],
+        "sbg_batch_get_tuning": [_P, C.POINTER(BatchTuning)],
+        "sbg_batch_run": [_P, C.c_int],
+        "sbg_batch_last_ms": [_P, C.POINTER(C.c_double)],
+        "sbg_batch_counts": [_P, _PSZ, _PSZ],
+        "sbg_batch_candidates": [_P, _P, _P, _P],
+        "sbg_batch_sides": [_P, _U8],
+        "sbg_batch_autotune": [_P, C.c_int, C.c_int, C.POINTER(BatchTuning), C.POINTER(C.c_double)],
+        "sbg_batch_destroy": [_P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -162,6 +184,11 @@ class Context:
         ms = C.c_double(0)
         _check(self.L.sbg_calibrate_copy(self.h, nbytes, iters, C.byref(ms)))
         return ms.value
+
+    def batch(self, pairs) -> "Batch":
+        """pairs: sequence of (mesh A, mesh B), each mesh an object with
+        `vertices` (n x 3) and `triangles` (m x 3) or a (vertices, triangles) tuple."""
+        return Batch(self, pairs)
 
     def mesh(self, vertices, triangles, prepare: bool = True) -> "Mesh":
         m = Mesh(self, vertices, triangles)
@@ -308,3 +335,91 @@ class Pairs:
         ms = C.c_double(0)
         _check(self.L.sbg_bench_tri_tri(a.h, b.h, self.h, precision, iters, C.byref(ms)))
         return ms.value
+
+
+def _vt(m):
+    if hasattr(m, "vertices"):
+        return m.vertices, m.triangles
+    return m[0], m[1]
+
+
+class Batch:
+    """Many independent mesh pairs run as one (sbg_batch_*)."""
+
+    def __init__(self, ctx: Context, pairs):
+        self.ctx = ctx
+        self.L = ctx.L
+        meshes = []
+        for a, b in pairs:
+            meshes.append(_vt(a))
+            meshes.append(_vt(b))
+        vb = np.zeros(len(meshes) + 1, np.uint64)
+        tb = np.zeros(len(meshes) + 1, np.uint64)
+        for i, (v, t) in enumerate(meshes):
+            vb[i + 1] = vb[i] + len(v)
+            tb[i + 1] = tb[i] + len(t)
+        xyz = np.ascontiguousarray(np.concatenate([np.asarray(v, np.float64).reshape(-1, 3) for v, _ in meshes]))
+        tris = np.ascontiguousarray(np.concatenate([np.asarray(t).reshape(-1, 3) for _, t in meshes]), np.uint32)
+        self.vert_begin, self.tri_begin = vb, tb
+        self.nmeshes = len(meshes)
+        self.nt = int(tb[-1])
+        h = C.c_void_p()
+        _check(self.L.sbg_batch_create(ctx.h, xyz.reshape(-1), tris.reshape(-1), vb, tb, self.nmeshes, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.sbg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_vertices(self, xyz):
+        xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1)
+        _check(self.L.sbg_batch_set_vertices(self.h, xyz))
+
+    def tuning(self) -> dict:
+        t = BatchTuning()
+        _check(self.L.sbg_batch_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = BatchTuning()
+        _check(self.L.sbg_batch_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        _check(self.L.sbg_batch_set_tuning(self.h, C.byref(t)))
+
+    def run(self, precision: int = FP32_FILTERED) -> float:
+        _check(self.L.sbg_batch_run(self.h, precision))
+        ms = C.c_double(0)
+        _check(self.L.sbg_batch_last_ms(self.h, C.byref(ms)))
+        return ms.value
+
+    def counts(self):
+        c, h = C.c_size_t(0), C.c_size_t(0)
+        _check(self.L.sbg_batch_counts(self.h, C.byref(c), C.byref(h)))
+        return c.value, h.value
+
+    def candidates(self):
+        n, _ = self.counts()
+        ab = np.zeros((max(n, 1), 2), np.uint32)
+        st = np.zeros(max(n, 1), np.uint8)
+        seg = np.zeros((max(n, 1), 6), np.float64)
+        _check(self.L.sbg_batch_candidates(self.h, _ptr(ab), _ptr(st), _ptr(seg)))
+        return ab[:n], st[:n], seg[:n]
+
+    def sides(self) -> np.ndarray:
+        out = np.zeros(max(self.nt, 1), np.uint8)
+        _check(self.L.sbg_batch_sides(self.h, out))
+        return out[: self.nt]
+
+    def autotune(self, precision: int = FP32_FILTERED, reps: int = 3):
+        t = BatchTuning()
+        ms = C.c_double(0)
+        _check(self.L.sbg_batch_autotune(self.h, precision, reps, C.byref(t), C.byref(ms)))
+        return t.as_dict(), ms.value
