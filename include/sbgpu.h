@@ -151,6 +151,21 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, int precision, in
  * traffic counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
 int sbg_calibrate_copy(sbg_context ctx, size_t bytes, int iters, double *avg_ms);
 
+/* ---- topology-fixed adjoint ------------------------------------------------
+ * No reference counterpart (the reference computes no derivatives).  nin
+ * input vertices xyz_in; nd derived vertices, derived d being the point where
+ * the line through inputs src[5d], src[5d + 1] meets the plane through inputs
+ * src[5d + 2 .. 5d + 4] (how every intersection-segment endpoint is made,
+ * thirdparty/GuigueDevillers03/tri_tri_intersect.c:297-356); faces (nf x 3)
+ * index [inputs | derived] and should form a closed, consistently oriented
+ * surface (e.g. a union).  Writes the derived positions (nd x 3) and the
+ * faces' signed volume, and, when grad_in is non-null,
+ *   grad_in = vol_weight * d(volume)/d(xyz_in) + J^T grad_derived,
+ * J being d(derived)/d(xyz_in); grad_derived may be null.  Outputs may be null. */
+int sbg_adjoint(sbg_context ctx, const double *xyz_in, size_t nin, const uint32_t *src, size_t nd,
+                const uint32_t *faces, size_t nf, double vol_weight, const double *grad_derived, double *derived_xyz,
+                double *volume, double *grad_in);
+
 /* ---- batched booleans ------------------------------------------------------
  * Many independent mesh pairs in one structure.  There are nmeshes = 2 *
  * npairs meshes in concatenated arrays: mesh m owns vertices

@@ -167,6 +167,13 @@ void context_free(sbg_context ctx)
     ctx->rawB.release();
     ctx->rawSeg.release();
     ctx->rawStatus.release();
+    ctx->adjXyz.release();
+    ctx->adjAll.release();
+    ctx->adjGradD.release();
+    ctx->adjGall.release();
+    ctx->adjVol.release();
+    ctx->adjSrc.release();
+    ctx->adjFaces.release();
     (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -957,6 +964,57 @@ int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side,
     TRY(run_rays(other, c->pts.p, self->sortedIds, c->flags.p, nt));
     HIP_TRY(hipMemcpyAsync(out, c->votes.p, nt, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return SBG_OK;
+}
+
+int sbg_adjoint(sbg_context ctx, const double *xyz_in, size_t nin, const uint32_t *src, size_t nd,
+                const uint32_t *faces, size_t nf, double vol_weight, const double *grad_derived, double *derived_xyz,
+                double *volume, double *grad_in)
+{
+    TRY(use(ctx));
+    if ((nin && !xyz_in) || (nd && !src) || (nf && !faces))
+        return fail(SBG_ERR_INVALID, "null input array");
+    if (nin + nd >= 0x7fffffffu || nf >= 0x7fffffffu)
+        return fail(SBG_ERR_INVALID, "too many vertices or faces");
+    for (size_t i = 0; i < 5 * nd; ++i)
+        if (src[i] >= nin)
+            return fail(SBG_ERR_INVALID, "derived vertex %zu names input %u of %zu", i / 5, src[i], nin);
+    for (size_t i = 0; i < 3 * nf; ++i)
+        if (faces[i] >= nin + nd)
+            return fail(SBG_ERR_INVALID, "face %zu names vertex %u of %zu", i / 3, faces[i], nin + nd);
+    const size_t nall = nin + nd;
+    TRY(ctx->adjXyz.ensure(3 * nin + 3));
+    TRY(ctx->adjAll.ensure(3 * nall + 3));
+    TRY(ctx->adjGall.ensure(3 * nall + 3));
+    TRY(ctx->adjGradD.ensure(3 * nd + 3));
+    TRY(ctx->adjVol.ensure(1));
+    TRY(ctx->adjSrc.ensure(5 * nd + 5));
+    TRY(ctx->adjFaces.ensure(3 * nf + 3));
+    hipStream_t s = ctx->stream;
+    if (nin)
+        HIP_TRY(hipMemcpyAsync(ctx->adjXyz.p, xyz_in, sizeof(double) * 3 * nin, hipMemcpyHostToDevice, s));
+    if (nd)
+        HIP_TRY(hipMemcpyAsync(ctx->adjSrc.p, src, sizeof(uint32_t) * 5 * nd, hipMemcpyHostToDevice, s));
+    if (nf)
+        HIP_TRY(hipMemcpyAsync(ctx->adjFaces.p, faces, sizeof(uint32_t) * 3 * nf, hipMemcpyHostToDevice, s));
+    const bool wantGrad = grad_in != nullptr;
+    if (wantGrad && grad_derived && nd)
+        HIP_TRY(hipMemcpyAsync(ctx->adjGradD.p, grad_derived, sizeof(double) * 3 * nd, hipMemcpyHostToDevice, s));
+    launch_fill_u32(reinterpret_cast<uint32_t *>(ctx->adjVol.p), 0u, 2, s);
+    if (wantGrad)
+        launch_fill_u32(reinterpret_cast<uint32_t *>(ctx->adjGall.p), 0u, 6 * nall, s);
+    launch_adjoint(ctx->adjXyz.p, (uint32_t)nin, ctx->adjSrc.p, (uint32_t)nd, ctx->adjFaces.p, (uint32_t)nf,
+                   vol_weight, (wantGrad && grad_derived) ? ctx->adjGradD.p : nullptr, ctx->adjAll.p, ctx->adjVol.p,
+                   wantGrad ? ctx->adjGall.p : nullptr, s);
+    TRY(check_launch("adjoint"));
+    if (derived_xyz && nd)
+        HIP_TRY(hipMemcpyAsync(derived_xyz, ctx->adjAll.p + 3 * nin, sizeof(double) * 3 * nd, hipMemcpyDeviceToHost,
+                               s));
+    if (volume)
+        HIP_TRY(hipMemcpyAsync(volume, ctx->adjVol.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (wantGrad && nin)
+        HIP_TRY(hipMemcpyAsync(grad_in, ctx->adjGall.p, sizeof(double) * 3 * nin, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return SBG_OK;
 }
 

@@ -88,6 +88,8 @@ struct sbg_context_s {
     DevBuf<uint32_t> ovf, rayList;
     DevBuf<double> rawA, rawB, rawSeg;
     DevBuf<uint8_t> rawStatus;
+    DevBuf<double> adjXyz, adjAll, adjGradD, adjGall, adjVol;  // sbg_adjoint
+    DevBuf<uint32_t> adjSrc, adjFaces;
     int refs = 1;  // the owner + every live mesh / pair list
     bool destroyed = false;
     bool timing = false;

@@ -140,6 +140,11 @@ void launch_rays(const GridSet &gs, const double *tri64, const double *normals, 
                  uint32_t *ovfCount, uint32_t *overflow, hipStream_t s);
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);
 
+// k_adjoint.hip
+void launch_adjoint(const double *xyz, uint32_t nin, const uint32_t *src, uint32_t nd, const uint32_t *faces,
+                    uint32_t nf, double weight, const double *gradDerived, double *all, double *vol, double *gall,
+                    hipStream_t s);
+
 // k_batch.hip
 void launch_batch_bounds_init(int *mb, uint32_t nmeshes, hipStream_t s);
 void launch_batch_bounds(const float *box32, const uint32_t *meshOf, uint32_t nt, int *mb, hipStream_t s);

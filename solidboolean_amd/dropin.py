@@ -52,6 +52,9 @@ def load():
     L.sbh_boolean_triangle_sides.restype = C.c_size_t
     L.sbh_boolean_triangle_sides.argtypes = [_P, C.c_int, _U8, C.c_size_t]
     L.sbh_boolean_stage_seconds.argtypes = [_P, _D]
+    L.sbh_boolean_vertex_sources.restype = C.c_size_t
+    L.sbh_boolean_vertex_sources.argtypes = [_P, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
+                                             C.c_size_t]
     L.sbh_earclip.restype = C.c_size_t
     L.sbh_earclip.argtypes = [_D, _I, C.c_size_t, _I, C.c_size_t]
     _lib = L
@@ -149,6 +152,17 @@ class SolidBoolean:
         n = self.L.sbh_boolean_triangle_sides(self.h, which, np.zeros(1, np.uint8), 0)
         out = np.zeros(max(n, 1), np.uint8)
         self.L.sbh_boolean_triangle_sides(self.h, which, out, n)
+        return out[:n]
+
+    def vertex_sources(self) -> np.ndarray:
+        """(n_result_vertices, 5) indices into the input vertices (first mesh's,
+        then second's): (v, ~0, ~0, ~0, ~0) for an input vertex, (x, y, p0, p1,
+        p2) for a new vertex = line x-y meets plane p0-p1-p2."""
+        n = self.L.sbh_boolean_vertex_sources(self.h, np.zeros(5, np.uint32), 0)
+        if n == 2 ** 64 - 1:
+            raise RuntimeError("a segment endpoint matched no edge-plane intersection")
+        out = np.zeros((max(n, 1), 5), np.uint32)
+        self.L.sbh_boolean_vertex_sources(self.h, out.reshape(-1), n)
         return out[:n]
 
     def stage_seconds(self) -> dict:

@@ -109,6 +109,18 @@ size_t sbh_boolean_triangle_sides(void *p, int which, uint8_t *out, size_t cap)
     return s.size();
 }
 
+// 5 input-vertex indices per result vertex (SolidBoolean::vertexSources);
+// returns the result vertex count, or (size_t)-1 if a source was not found
+size_t sbh_boolean_vertex_sources(void *p, uint32_t *out, size_t cap)
+{
+    std::vector<uint32_t> src;
+    if (!((HostBoolean *)p)->sb.vertexSources(src))
+        return (size_t)-1;
+    for (size_t i = 0; i < src.size() && i < 5 * cap; ++i)
+        out[i] = src[i];
+    return src.size() / 5;
+}
+
 // stage durations (s) in the order of the reference's bench time points
 void sbh_boolean_stage_seconds(void *p, double *out)
 {

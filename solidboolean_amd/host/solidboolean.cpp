@@ -2,6 +2,8 @@
 // src/solidboolean.cpp:288-480.
 #include "solidboolean.h"
 
+#include <cstring>
+
 #include <deque>
 #include <stdexcept>
 
@@ -83,21 +85,25 @@ SolidBoolean::SolidBoolean(const SolidMesh *firstMesh, const SolidMesh *secondMe
 
 SolidBoolean::~SolidBoolean() = default;
 
-size_t SolidBoolean::addNewPoint(const Vector3 &position)
+size_t SolidBoolean::addNewPoint(const Vector3 &position, int64_t origin)
 {
     auto r = m_newPositionMap.insert({PositionKey(position), m_newVertices.size()});
-    if (r.second)
+    if (r.second) {
         m_newVertices.push_back(position);
+        m_vertexSource.push_back(-origin - 1);
+    }
     return r.first->second;
 }
 
 bool SolidBoolean::addUnintersectedTriangles(const SolidMesh *mesh, const std::vector<uint8_t> &used,
-                                             HalfEdgeMap *halfEdges)
+                                             HalfEdgeMap *halfEdges, size_t inputOffset)
 {
     // (reference: src/solidboolean.cpp:250-286; a repeated half-edge stops the copy)
     const size_t oldVertexCount = m_newVertices.size();
     const auto &vertices = *mesh->vertices();
     m_newVertices.insert(m_newVertices.end(), vertices.begin(), vertices.end());
+    for (size_t i = 0; i < vertices.size(); ++i)
+        m_vertexSource.push_back((int64_t)(inputOffset + i));
     const auto &tris = *mesh->triangles();
     halfEdges->reserve(3 * tris.size());
     m_newTriangles.reserve(m_newTriangles.size() + tris.size());
@@ -136,8 +142,8 @@ bool SolidBoolean::reTriangulate(const ContextMap &contexts, const SolidMesh *me
         newIndices.push_back(startOldVertex + t[0]);
         newIndices.push_back(startOldVertex + t[1]);
         newIndices.push_back(startOldVertex + t[2]);
-        for (const auto &p : kv.second.points)
-            newIndices.push_back(addNewPoint(p));
+        for (size_t k = 0; k < kv.second.points.size(); ++k)
+            newIndices.push_back(addNewPoint(kv.second.points[k], kv.second.origin[k]));
         for (const auto &nt : rt.triangles()) {
             const size_t index = m_newTriangles.size();
             const size_t a = newIndices[nt[0]], b = newIndices[nt[1]], c = newIndices[nt[2]];
@@ -317,15 +323,17 @@ bool SolidBoolean::combine()
     m_firstIntersected.assign(na, 0);
     m_secondIntersected.assign(nb, 0);
     ContextMap firstCtx, secondCtx;
-    auto addPoint = [](IntersectedContext &c, const Vector3 &p) {
+    auto addPoint = [](IntersectedContext &c, const Vector3 &p, int64_t origin) {
         auto r = c.positionMap.insert({PositionKey(p), c.points.size()});
-        if (r.second)
+        if (r.second) {
             c.points.push_back(p);
+            c.origin.push_back(origin);
+        }
         return r.first->second;
     };
-    auto addSegment = [&](IntersectedContext &c, const Vector3 &s, const Vector3 &t) {
-        const size_t i = 3 + addPoint(c, s);
-        const size_t j = 3 + addPoint(c, t);
+    auto addSegment = [&](IntersectedContext &c, const Vector3 &s, const Vector3 &t, size_t hit) {
+        const size_t i = 3 + addPoint(c, s, 2 * (int64_t)hit);
+        const size_t j = 3 + addPoint(c, t, 2 * (int64_t)hit + 1);
         if (i != j) {
             c.neighborMap[i].insert(j);
             c.neighborMap[j].insert(i);
@@ -338,9 +346,11 @@ bool SolidBoolean::combine()
         const Vector3 t(seg[6 * h + 3], seg[6 * h + 4], seg[6 * h + 5]);
         m_firstIntersected[a] = 1;
         m_secondIntersected[b] = 1;
-        addSegment(firstCtx[a], s, t);
-        addSegment(secondCtx[b], s, t);
+        addSegment(firstCtx[a], s, t, h);
+        addSegment(secondCtx[b], s, t, h);
     }
+    m_hitPairs = ab;
+    m_hitSegments = seg;
     benchEnd_processPotentialIntersectedPairs = Clock::now();
 
     EdgeMap firstEdges, secondEdges;
@@ -349,12 +359,13 @@ bool SolidBoolean::combine()
 
     benchBegin_addUnintersectedTriangles = Clock::now();
     m_firstRemainingStart = m_newTriangles.size();
-    if (!addUnintersectedTriangles(m_firstMesh, m_firstIntersected, &firstHalfEdges))
+    if (!addUnintersectedTriangles(m_firstMesh, m_firstIntersected, &firstHalfEdges, 0))
         sbhost::diag("Add first mesh remaining triangles failed");
     const size_t firstRemainingCount = m_newTriangles.size() - m_firstRemainingStart;
     m_firstRemainingCount = firstRemainingCount;
     m_secondRemainingStart = m_newTriangles.size();
-    if (!addUnintersectedTriangles(m_secondMesh, m_secondIntersected, &secondHalfEdges))
+    if (!addUnintersectedTriangles(m_secondMesh, m_secondIntersected, &secondHalfEdges,
+                                   m_firstMesh->vertices()->size()))
         sbhost::diag("Add second mesh remaining triangles failed");
     const size_t secondRemainingCount = m_newTriangles.size() - m_secondRemainingStart;
     m_secondRemainingCount = secondRemainingCount;
@@ -433,6 +444,88 @@ void SolidBoolean::fetchIntersect(std::vector<std::vector<size_t>> &out)
 }
 
 const std::vector<Vector3> &SolidBoolean::resultVertices() { return m_newVertices; }
+
+namespace {
+
+struct D3 {
+    double x, y, z;
+};
+D3 dsub(const D3 &a, const D3 &b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+D3 dcross(const D3 &a, const D3 &b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+double ddot(const D3 &a, const D3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// One segment endpoint as CONSTRUCT_INTERSECTION makes it
+// (thirdparty/GuigueDevillers03/tri_tri_intersect.c:297-301 and siblings):
+// v1 = base - onplane, v2 = base - other, alpha = DOT(v1,N)/DOT(v2,N),
+// out = base - alpha * v2.
+D3 edge_plane(const D3 &base, const D3 &onplane, const D3 &other, const D3 &N)
+{
+    const D3 v1 = dsub(base, onplane), v2 = dsub(base, other);
+    const double alpha = ddot(v1, N) / ddot(v2, N);
+    return dsub(base, {alpha * v2.x, alpha * v2.y, alpha * v2.z});
+}
+
+bool same_bits(const D3 &a, const double *b)
+{
+    return std::memcmp(&a.x, b, 8) == 0 && std::memcmp(&a.y, b + 1, 8) == 0 && std::memcmp(&a.z, b + 2, 8) == 0;
+}
+
+}  // namespace
+
+bool SolidBoolean::vertexSources(std::vector<uint32_t> &src5) const
+{
+    const auto &va = *m_firstMesh->vertices();
+    const auto &ta = *m_firstMesh->triangles();
+    const auto &tb = *m_secondMesh->triangles();
+    const uint32_t offB = (uint32_t)va.size();
+    const uint32_t none = 0xffffffffu;
+    src5.assign(5 * m_vertexSource.size(), none);
+    for (size_t r = 0; r < m_vertexSource.size(); ++r) {
+        const int64_t o = m_vertexSource[r];
+        if (o >= 0) {
+            src5[5 * r] = (uint32_t)o;
+            continue;
+        }
+        const size_t h = (size_t)((-o - 1) / 2), e = (size_t)((-o - 1) % 2);
+        const size_t a = m_hitPairs[2 * h], b = m_hitPairs[2 * h + 1];
+        const double *pt = &m_hitSegments[6 * h + 3 * e];
+        // the two triangles, ids into the input vertices, as the GD test saw them
+        uint32_t id[2][3];
+        D3 v[2][3];
+        for (int k = 0; k < 3; ++k) {
+            id[0][k] = (uint32_t)ta[a][k];
+            id[1][k] = offB + (uint32_t)tb[b][k];
+            const Vector3 &pa = (*m_firstMesh->vertices())[ta[a][k]];
+            const Vector3 &pb = (*m_secondMesh->vertices())[tb[b][k]];
+            v[0][k] = {pa.x(), pa.y(), pa.z()};
+            v[1][k] = {pb.x(), pb.y(), pb.z()};
+        }
+        // N1 = (q1 - p1) x (r1 - p1), N2 = (p2 - r2) x (q2 - r2) (tri_tri_intersect.c:403-431)
+        const D3 N[2] = {dcross(dsub(v[0][1], v[0][0]), dsub(v[0][2], v[0][0])),
+                         dcross(dsub(v[1][0], v[1][2]), dsub(v[1][1], v[1][2]))};
+        bool found = false;
+        for (int owner = 0; owner < 2 && !found; ++owner)
+            for (int i = 0; i < 3 && !found; ++i)
+                for (int j = 0; j < 3 && !found; ++j) {
+                    if (i == j)
+                        continue;
+                    for (int k = 0; k < 3 && !found; ++k) {
+                        if (!same_bits(edge_plane(v[owner][i], v[1 - owner][k], v[owner][j], N[1 - owner]), pt))
+                            continue;
+                        found = true;
+                        uint32_t *d = &src5[5 * r];
+                        d[0] = id[owner][i];
+                        d[1] = id[owner][j];
+                        d[2] = id[1 - owner][0];
+                        d[3] = id[1 - owner][1];
+                        d[4] = id[1 - owner][2];
+                    }
+                }
+        if (!found)
+            return false;
+    }
+    return true;
+}
 
 std::vector<size_t> SolidBoolean::intersectedFaces(int which) const
 {

@@ -50,6 +50,12 @@ public:
     // mesh 0 / 1 the side of the group it landed in (0 outside, 1 inside) or
     // 2 when it was intersected.
     std::vector<size_t> intersectedFaces(int which) const;
+    // Where each result vertex comes from, 5 indices per vertex into the input
+    // vertices (first mesh's, then second's): an input vertex v is
+    // (v, ~0, ~0, ~0, ~0); a new vertex is (x, y, p0, p1, p2), the point where
+    // the line through x and y (an edge of one triangle) meets the plane through
+    // p0, p1, p2 (the other triangle).  False if some endpoint matches no edge.
+    bool vertexSources(std::vector<uint32_t> &src5) const;
     std::vector<uint8_t> triangleSides(int which) const;
 
     // open-addressed map from a half-edge key (first << 32 | second) to a triangle
@@ -70,6 +76,7 @@ public:
 private:
     struct IntersectedContext {
         std::vector<Vector3> points;
+        std::vector<int64_t> origin;  // per point: 2 * hit + endpoint of its first insertion
         std::map<PositionKey, size_t> positionMap;
         std::map<size_t, std::set<size_t>> neighborMap;
     };
@@ -86,10 +93,16 @@ private:
     std::vector<uint8_t> m_firstIntersected, m_secondIntersected;
     size_t m_firstRemainingStart = 0, m_secondRemainingStart = 0;
     size_t m_firstRemainingCount = 0, m_secondRemainingCount = 0;
+    // per result vertex: >= 0 an input vertex (first mesh's, then second's),
+    // < 0 a hit endpoint -(2 * hit + endpoint) - 1
+    std::vector<int64_t> m_vertexSource;
+    std::vector<uint32_t> m_hitPairs;  // the intersecting pairs (a, b)
+    std::vector<double> m_hitSegments;  // their segments (source, target)
 
     static uint64_t halfEdgeKey(size_t first, size_t second) { return ((uint64_t)first << 32) | (uint64_t)second; }
-    size_t addNewPoint(const Vector3 &position);
-    bool addUnintersectedTriangles(const SolidMesh *mesh, const std::vector<uint8_t> &used, HalfEdgeMap *halfEdges);
+    size_t addNewPoint(const Vector3 &position, int64_t origin);
+    bool addUnintersectedTriangles(const SolidMesh *mesh, const std::vector<uint8_t> &used, HalfEdgeMap *halfEdges,
+                                   size_t inputOffset);
     bool reTriangulate(const ContextMap &contexts, const SolidMesh *mesh, size_t startOldVertex, EdgeMap &edges,
                        HalfEdgeMap &halfEdges);
     bool buildPolygonsFromEdges(const EdgeMap &edges, std::vector<std::vector<size_t>> &polygons);

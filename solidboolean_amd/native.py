@@ -34,6 +34,7 @@ EXPORTS = [
     "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
     "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_context_set_timing",
     "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_calibrate_copy",
+    "sbg_adjoint",
     "sbg_batch_create", "sbg_batch_set_vertices", "sbg_batch_set_tuning", "sbg_batch_get_tuning", "sbg_batch_run",
     "sbg_batch_last_ms", "sbg_batch_counts", "sbg_batch_candidates", "sbg_batch_sides", "sbg_batch_autotune",
     "sbg_batch_destroy",
@@ -113,6 +114,7 @@ def load(path: str = LIB_PATH):
         "sbg_context_stage_ms": [_P, _D, C.c_int],
         "sbg_bench_tri_tri": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)],
         "sbg_calibrate_copy": [_P, _SZ, C.c_int, C.POINTER(C.c_double)],
+        "sbg_adjoint": [_P, _D, _SZ, _U32, _SZ, _U32, _SZ, C.c_double, _P, _P, _P, _P],
         "sbg_batch_create": [_P, _D, _U32, _U64, _U64, _SZ, _PP],
         "sbg_batch_set_vertices": [_P, _D],
         "sbg_batch_set_tuning": [_P, C.POINTER(BatchTuning)],
@@ -189,6 +191,23 @@ class Context:
         """pairs: sequence of (mesh A, mesh B), each mesh an object with
         `vertices` (n x 3) and `triangles` (m x 3) or a (vertices, triangles) tuple."""
         return Batch(self, pairs)
+
+    def adjoint(self, xyz_in, src, faces, vol_weight: float = 0.0, grad_derived=None, want_grad: bool = True):
+        """Topology-fixed adjoint (sbg_adjoint): returns (derived positions,
+        volume, grad_in or None)."""
+        xyz = np.ascontiguousarray(xyz_in, np.float64).reshape(-1, 3)
+        src = np.ascontiguousarray(src, np.uint32).reshape(-1, 5)
+        faces = np.ascontiguousarray(faces, np.uint32).reshape(-1, 3)
+        nin, nd, nf = len(xyz), len(src), len(faces)
+        derived = np.zeros((max(nd, 1), 3), np.float64)
+        vol = np.zeros(1, np.float64)
+        grad = np.zeros((max(nin, 1), 3), np.float64) if want_grad else None
+        gd = None if grad_derived is None else np.ascontiguousarray(grad_derived, np.float64).reshape(nd, 3)
+        _check(self.L.sbg_adjoint(self.h, xyz.reshape(-1) if nin else np.zeros(3), nin,
+                                  src.reshape(-1) if nd else np.zeros(5, np.uint32), nd,
+                                  faces.reshape(-1) if nf else np.zeros(3, np.uint32), nf, float(vol_weight),
+                                  _ptr(gd), _ptr(derived), _ptr(vol), _ptr(grad)))
+        return derived[:nd], float(vol[0]), (grad[:nin] if want_grad else None)
 
     def mesh(self, vertices, triangles, prepare: bool = True) -> "Mesh":
         m = Mesh(self, vertices, triangles)
