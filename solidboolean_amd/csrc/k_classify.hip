@@ -215,55 +215,136 @@ __device__ __forceinline__ GridSpan grid_span(const double *__restrict__ tri64, 
     return s;
 }
 
+// Grid build: a workgroup takes kGridBlockTris consecutive triangles of the
+// visiting order (Morton order, so they are neighbours) for one axis and first
+// counts its entries per cell in an LDS hash table; only then does it touch the
+// global cell counters, once per distinct cell.  Device-scope atomics are the
+// cost here (they are coherent across the chip's separate L2s), and
+// neighbouring triangles share most cells, so this removes most of them.  A
+// cell that finds no slot within kProbe probes falls back to per-entry global
+// atomics; both paths only ever advance a cell's counter, so they mix freely.
+constexpr int kGridBlockTris = 1024;  // 4 per lane
+constexpr int kGridSlotBits = 12;
+constexpr int kGridSlots = 1 << kGridSlotBits;
+constexpr int kProbe = 64;
+
+// slot holding `key` in the LDS table (inserting it), or -1.  Slots only fill,
+// so a key that once found no slot never finds one later in the same pass.
+__device__ __forceinline__ int table_slot(uint32_t *keys, uint32_t key)
+{
+    uint32_t h = (key * 2654435761u) >> (32 - kGridSlotBits);
+    for (int probe = 0; probe < kProbe; ++probe) {
+        const uint32_t cur = atomicCAS(&keys[h], 0u, key);
+        if (cur == 0u || cur == key)
+            return (int)h;
+        h = (h + 1) & (kGridSlots - 1);
+    }
+    return -1;
+}
+
+struct TriSpan {
+    GridSpan s;
+    uint32_t t, G;
+    size_t base;  // counter index of the mesh's first cell on this axis, + 1
+    bool valid;
+};
+
+__device__ __forceinline__ TriSpan tri_span(const double *__restrict__ tri64, uint32_t nt, const GridSet &gs,
+                                            int axis, uint32_t pos)
+{
+    TriSpan x;
+    x.valid = pos < nt;
+    if (!x.valid)
+        return x;
+    x.t = gs.order ? gs.order[pos] : pos;
+    const uint32_t m = gs.meshOf ? gs.meshOf[x.t] : 0u;
+    const uint2 d = grid_desc(gs, m);
+    x.G = d.x;
+    const GridFrame f = grid_frame(gs.boxes + 6 * (size_t)m, axis, x.G);
+    x.s = grid_span(tri64, x.t, axis, f, x.G);
+    x.base = (size_t)d.y + (size_t)axis * x.G * x.G + 1;
+    return x;
+}
+
 // cells: ncells + 1 u32, zero on entry; entries of cell c are counted in cells[c + 1]
 __global__ __launch_bounds__(256) void k_grid_count(const double *__restrict__ tri64, uint32_t nt, GridSet gs,
                                                     uint32_t *__restrict__ cells)
 {
-    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= 3u * nt)
-        return;
-    const int axis = (int)(g / nt);
-    // a stride coprime with nt spreads a wave's triangles over the mesh, so its
-    // atomics land on different cells
-    const uint32_t t = (uint32_t)(((uint64_t)(g - (uint32_t)axis * nt) * gs.mix) % nt);
-    const uint32_t m = gs.meshOf ? gs.meshOf[t] : 0u;
-    const uint2 d = grid_desc(gs, m);
-    const uint32_t G = d.x;
-    const GridFrame f = grid_frame(gs.boxes + 6 * (size_t)m, axis, G);
-    const GridSpan s = grid_span(tri64, t, axis, f, G);
-    const size_t base = (size_t)d.y + (size_t)axis * G * G + 1;
-    for (uint32_t cu = s.u0; cu <= s.u1; ++cu)
-        for (uint32_t cv = s.v0; cv <= s.v1; ++cv)
-            atomicAdd(&cells[base + (size_t)cu * G + cv], 1u);
+    __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
+    const int axis = blockIdx.y;
+    for (int k = threadIdx.x; k < kGridSlots; k += 256) {
+        s_key[k] = 0;
+        s_cnt[k] = 0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kGridBlockTris / 256; ++j) {
+        const TriSpan x = tri_span(tri64, nt, gs, axis, blockIdx.x * kGridBlockTris + j * 256 + threadIdx.x);
+        if (!x.valid)
+            continue;
+        for (uint32_t cu = x.s.u0; cu <= x.s.u1; ++cu)
+            for (uint32_t cv = x.s.v0; cv <= x.s.v1; ++cv) {
+                const uint32_t key = (uint32_t)(x.base + (size_t)cu * x.G + cv);
+                const int sl = table_slot(s_key, key);
+                if (sl >= 0)
+                    atomicAdd(&s_cnt[sl], 1u);
+                else
+                    atomicAdd(&cells[key], 1u);
+            }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kGridSlots; k += 256)
+        if (s_key[k])
+            atomicAdd(&cells[s_key[k]], s_cnt[k]);
 }
 
-// after the exclusive scan, cells[c + 1] is the start of cell c; each entry
-// advances it, so afterwards cell c spans [cells[c], cells[c + 1])
+// After the exclusive scan, cells[c + 1] is the start of cell c; the workgroup
+// reserves one range per distinct cell and hands out slots from LDS, so
+// afterwards cell c spans [cells[c], cells[c + 1]).
 __global__ __launch_bounds__(256) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt, GridSet gs,
                                                    uint32_t *__restrict__ cells, float4 *__restrict__ gbox,
                                                    uint2 *__restrict__ gref)
 {
-    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= 3u * nt)
-        return;
-    const int axis = (int)(g / nt);
-    // a stride coprime with nt spreads a wave's triangles over the mesh, so its
-    // atomics land on different cells
-    const uint32_t t = (uint32_t)(((uint64_t)(g - (uint32_t)axis * nt) * gs.mix) % nt);
-    const uint32_t m = gs.meshOf ? gs.meshOf[t] : 0u;
-    const uint2 d = grid_desc(gs, m);
-    const uint32_t G = d.x;
-    const GridFrame f = grid_frame(gs.boxes + 6 * (size_t)m, axis, G);
-    const GridSpan s = grid_span(tri64, t, axis, f, G);
-    const size_t base = (size_t)d.y + (size_t)axis * G * G + 1;
-    for (uint32_t cu = s.u0; cu <= s.u1; ++cu)
-        for (uint32_t cv = s.v0; cv <= s.v1; ++cv) {
-            const uint32_t slot = atomicAdd(&cells[base + (size_t)cu * G + cv], 1u);
-            if (slot < gs.cap) {
-                gbox[slot] = s.box;
-                gref[slot] = make_uint2(t, __float_as_uint(s.hiAxis));
+    __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
+    const int axis = blockIdx.y;
+    for (int k = threadIdx.x; k < kGridSlots; k += 256) {
+        s_key[k] = 0;
+        s_cnt[k] = 0;
+    }
+    __syncthreads();
+    TriSpan x[kGridBlockTris / 256];
+#pragma unroll
+    for (int j = 0; j < kGridBlockTris / 256; ++j) {
+        x[j] = tri_span(tri64, nt, gs, axis, blockIdx.x * kGridBlockTris + j * 256 + threadIdx.x);
+        if (!x[j].valid)
+            continue;
+        for (uint32_t cu = x[j].s.u0; cu <= x[j].s.u1; ++cu)
+            for (uint32_t cv = x[j].s.v0; cv <= x[j].s.v1; ++cv) {
+                const int sl = table_slot(s_key, (uint32_t)(x[j].base + (size_t)cu * x[j].G + cv));
+                if (sl >= 0)
+                    atomicAdd(&s_cnt[sl], 1u);
             }
-        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kGridSlots; k += 256)
+        if (s_key[k])
+            s_cnt[k] = atomicAdd(&cells[s_key[k]], s_cnt[k]);  // the range's first slot
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kGridBlockTris / 256; ++j) {
+        if (!x[j].valid)
+            continue;
+        for (uint32_t cu = x[j].s.u0; cu <= x[j].s.u1; ++cu)
+            for (uint32_t cv = x[j].s.v0; cv <= x[j].s.v1; ++cv) {
+                const uint32_t key = (uint32_t)(x[j].base + (size_t)cu * x[j].G + cv);
+                const int sl = table_slot(s_key, key);
+                const uint32_t slot = sl >= 0 ? atomicAdd(&s_cnt[sl], 1u) : atomicAdd(&cells[key], 1u);
+                if (slot < gs.cap) {
+                    gbox[slot] = x[j].s.box;
+                    gref[slot] = make_uint2(x[j].t, __float_as_uint(x[j].s.hiAxis));
+                }
+            }
+    }
 }
 
 // candidate range of the ray's cell in target mesh m
@@ -440,22 +521,6 @@ uint32_t grid_dim(uint32_t nt)
 
 size_t grid_cells(uint32_t G) { return 3 * (size_t)G * G + 1; }
 
-uint32_t grid_mix(uint32_t nt)
-{
-    if (nt < 2)
-        return 1;
-    uint32_t s = (uint32_t)(0.6180339887 * nt) | 1u;
-    for (;; s += 2) {
-        uint32_t a = s % nt, b = nt;
-        while (a) {
-            const uint32_t r = b % a;
-            b = a;
-            a = r;
-        }
-        if (b == 1)
-            return s % nt ? s % nt : 1;
-    }
-}
 
 void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
                        float4 *gbox, uint2 *gref, hipStream_t s)
@@ -464,10 +529,10 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
     launch_fill_u32(cells, 0u, m, s);
     if (nt == 0)
         return;
-    const unsigned blocks = (unsigned)((3ull * nt + 255) / 256);
-    hipLaunchKernelGGL(k_grid_count, dim3(blocks), dim3(256), 0, s, tri64, nt, gs, cells);
+    const dim3 grid((unsigned)((nt + kGridBlockTris - 1) / kGridBlockTris), 3);
+    hipLaunchKernelGGL(k_grid_count, grid, dim3(256), 0, s, tri64, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
-    hipLaunchKernelGGL(k_grid_fill, dim3(blocks), dim3(256), 0, s, tri64, nt, gs, cells, gbox, gref);
+    hipLaunchKernelGGL(k_grid_fill, grid, dim3(256), 0, s, tri64, nt, gs, cells, gbox, gref);
 }
 
 void launch_rays(const GridSet &gs, const double *tri64, const double *normals, const double *pts,

@@ -827,7 +827,7 @@ static GridSet grid_set(sbg_mesh m)
     gs.G0 = m->gridG;
     gs.cap = (uint32_t)m->gridCap;
     gs.ncells = (uint32_t)(grid_cells(m->gridG) - 1);
-    gs.mix = grid_mix((uint32_t)m->nt);
+    gs.order = m->sortedIds;
     return gs;
 }
 

@@ -65,7 +65,7 @@ struct sbg_batch_s {
 
     size_t frontCap = 0, candCap = 0, gridCap = 0;
     uint32_t spillPerWave = kBatchSpillInit;
-    uint32_t ncells = 0, mix = 1;
+    uint32_t ncells = 0;
     uint32_t *host = nullptr;  // pinned copy of the words
     bool keysInAlt = false, pairsInAlt = false;
 
@@ -204,7 +204,7 @@ int enqueue(sbg_batch b, int precision)
     gs.gref = b->gridRef.p;
     gs.cap = (uint32_t)b->gridCap;
     gs.ncells = b->ncells;
-    gs.mix = b->mix;
+    gs.order = b->ids.p;
     launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridBox.p, b->gridRef.p, s);
     launch_rays(gs, b->tri64.p, b->normals.p, b->pts.p, b->ids.p, b->flags.p, nt, b->bits.p, b->rayList.p,
                 w + kRayListW, w + kRayFlagW, s);
@@ -268,7 +268,6 @@ int sbg_batch_create(sbg_context ctx, const double *xyz, const uint32_t *tris, c
     b->meshBits = std::max(1, bits_for(nmeshes));
     b->mortonBits = std::min(30, 64 - b->localBits - b->meshBits);
     b->triBits = std::max(1, bits_for(nt));
-    b->mix = grid_mix((uint32_t)nt);
     int rc = SBG_OK;
     auto bail = [&](int r) {
         sbg_batch_destroy(b);

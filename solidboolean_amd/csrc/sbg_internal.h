@@ -124,10 +124,9 @@ struct GridSet {
     const uint32_t *cells;
     const float4 *gbox;
     const uint2 *gref;
+    const uint32_t *order;  // grid build's triangle visiting order, e.g. Morton (null: index order)
     uint32_t G0, cap, ncells;
-    uint32_t mix;  // triangle visiting stride of the grid build, coprime with the triangle count
 };
-uint32_t grid_mix(uint32_t nt);
 uint32_t grid_dim(uint32_t nt);
 size_t grid_cells(uint32_t G);  // 3 G^2 + 1
 void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,

@@ -31,9 +31,9 @@ def volume(xyz, src, faces):
 
 
 def volume_difference(xyz_plus, xyz_minus, src, faces):
-    """V(plus) - V(minus) summed face by face, so the large total cancels first."""
-    return float(np.sum(face_terms(all_vertices(xyz_plus, src), faces)
-                        - face_terms(all_vertices(xyz_minus, src), faces)))
+    """V(plus) - V(minus) summed face by face, so the large total cancels
+    first; evaluated in the inputs' precision (np.longdouble for checks)."""
+    return np.sum(face_terms(all_vertices(xyz_plus, src), faces) - face_terms(all_vertices(xyz_minus, src), faces))
 
 
 def vjp(xyz, src, faces, vol_weight=1.0, grad_derived=None):

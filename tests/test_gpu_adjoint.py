@@ -6,7 +6,10 @@ unit directions and against the FP64 CPU adjoint (tests/adjoint_ref.py).
 
 Tolerance: |g.u - fd| <= 1e-6 * max(|fd|, 1e-3 |g|) for the directional
 derivatives, i.e. 1e-6 relative unless the derivative along u is tiny next to
-the gradient's norm; CPU and GPU gradients agree to 1e-9 relative in norm."""
+the gradient's norm; CPU and GPU gradients agree to 1e-9 relative in norm.
+The differences are evaluated in x87 extended precision (np.longdouble): in
+FP64 the rounding of some 10^5 face terms about an origin a unit away,
+divided by 2h, is itself about 1e-5 of a directional derivative."""
 import numpy as np
 import pytest
 import torch
@@ -56,10 +59,11 @@ def test_union_volume_gradient(gpu, topo):
     src, faces = topo.src.astype(np.int64), topo.faces.astype(np.int64)
     gcpu = R.vjp(topo.input_xyz, src, faces, vol_weight=1.0)
     assert np.linalg.norm(g - gcpu) <= 1e-9 * np.linalg.norm(gcpu)
-    x0 = topo.input_xyz
+    x0 = topo.input_xyz.astype(np.longdouble)
+    h = np.longdouble(H)
     dirs = R.directions(len(x0), 64, seed=5)
-    _check_directional(g, lambda u: R.volume_difference(x0 + H * u, x0 - H * u, src, faces) / (2 * H), dirs,
-                       np.linalg.norm(g))
+    fd = lambda u: float(R.volume_difference(x0 + h * u, x0 - h * u, src, faces) / (2 * h))
+    _check_directional(g, fd, dirs, np.linalg.norm(g))
 
 
 def test_intersection_vertex_gradient(gpu, topo):
@@ -72,9 +76,10 @@ def test_intersection_vertex_gradient(gpu, topo):
     Wn = W.numpy()
     gcpu = R.vjp(topo.input_xyz, src, topo.faces.astype(np.int64), vol_weight=0.0, grad_derived=Wn)
     assert np.linalg.norm(g - gcpu) <= 1e-9 * np.linalg.norm(gcpu)
-    x0 = topo.input_xyz
+    x0 = topo.input_xyz.astype(np.longdouble)
+    h = np.longdouble(H)
     dirs = R.directions(len(x0), 64, seed=6)
-    fd = lambda u: float(np.sum(Wn * (R.derived(x0 + H * u, src) - R.derived(x0 - H * u, src)))) / (2 * H)
+    fd = lambda u: float(np.sum(Wn * (R.derived(x0 + h * u, src) - R.derived(x0 - h * u, src))) / (2 * h))
     _check_directional(g, fd, dirs, np.linalg.norm(g))
 
 
