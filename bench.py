@@ -56,6 +56,24 @@ def sha(*arrays) -> str:
     return h.hexdigest()
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_1310720_pair.md")
+
+
+def measured_traffic(kernels, grid: int):
+    """HBM bytes per launch of `kernels` at `grid` work-items from the committed
+    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE passes scaled by the copy
+    kernel's calibration), or None when the summary does not cover them."""
+    try:
+        rows = {}
+        for line in open(TRAFFIC_PROFILE):
+            cells = [c.strip().strip("`") for c in line.split("|")]
+            if len(cells) == 7 and cells[2].isdigit():
+                rows[(cells[1].replace("sbg::", ""), int(cells[2]))] = float(cells[5]) * 1e6
+        return sum(rows[(k, grid)] for k in kernels)
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def gd_bytes(n: int, hits: int, fallback: int, precision: int) -> float:
     """Algorithmic HBM bytes of one Guigue-Devillers launch over n pairs."""
     if precision == 0:
@@ -325,7 +343,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": sat_bytes / (sat_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": measured_traffic(("k_gd_filter", "k_gd_list"), 1048576),
+        "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/hbm_traffic_pmc_1310720_pair.md",
         "bytes_per_launch": sat_bytes,
         "avg_launch_ms": sat_ms,
     }

@@ -22,12 +22,16 @@ __device__ __forceinline__ int ord_of(float f)
 }
 __device__ __forceinline__ float float_of(int o) { return __int_as_float(o >= 0 ? o : o ^ 0x7fffffff); }
 
-// Per-mesh centroid bounds, as ord_of images (min xyz, max xyz); the array
-// holds (INT_MAX x 3, INT_MIN x 3) per mesh on entry.  A wave whose triangles
-// all belong to one mesh (the common case) reduces first and issues 6 atomics.
+// Per-mesh centroid bounds, as ord_of images (min xyz, max xyz); mb holds
+// (INT_MAX x 3, INT_MIN x 3) per mesh on entry.  Pass 1: a wave whose
+// triangles all belong to one mesh (all but the waves that straddle a mesh
+// boundary) stores its reduced bounds in its own slot of `waves`; a straddling
+// wave applies per-lane atomics.  Pass 2: one workgroup per mesh reduces the
+// slots of the waves inside the mesh and applies 6 atomics.  No address takes
+// more than a handful of atomics, whatever the mesh sizes.
 __global__ __launch_bounds__(256) void k_batch_bounds(const float *__restrict__ box32,
                                                       const uint32_t *__restrict__ meshOf, uint32_t nt,
-                                                      int *__restrict__ mb)
+                                                      int *__restrict__ mb, int *__restrict__ waves)
 {
     const uint32_t t = blockIdx.x * 256 + threadIdx.x;
     const bool valid = t < nt;
@@ -51,19 +55,48 @@ __global__ __launch_bounds__(256) void k_batch_bounds(const float *__restrict__ 
                 v[3 + k] = max(v[3 + k], __shfl_xor(v[3 + k], off, 64));
             }
         }
-        if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                atomicMin(mb + 6 * (size_t)m0 + k, v[k]);
-                atomicMax(mb + 6 * (size_t)m0 + 3 + k, v[3 + k]);
-            }
-        }
+        const int lane = threadIdx.x & 63;
+        if (lane < 6)
+            waves[6 * (size_t)(t / 64) + lane] = lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2]
+                                                 : lane == 3 ? v[3] : lane == 4 ? v[4] : v[5];
     } else if (valid) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             atomicMin(mb + 6 * (size_t)m + k, v[k]);
             atomicMax(mb + 6 * (size_t)m + 3 + k, v[3 + k]);
         }
+    }
+}
+
+// one workgroup per mesh over the waves lying wholly inside it
+__global__ __launch_bounds__(256) void k_batch_bounds_merge(const int *__restrict__ waves,
+                                                            const uint32_t *__restrict__ triBegin, uint32_t nt,
+                                                            int *__restrict__ mb)
+{
+    __shared__ int s[6][256];
+    const uint32_t m = blockIdx.x;
+    const uint32_t t0 = triBegin[m], t1 = triBegin[m + 1];
+    const uint32_t w0 = (t0 + 63) / 64;
+    const uint32_t w1 = t1 == nt ? (t1 + 63) / 64 : t1 / 64;  // a last, partial wave is whole
+    int v[6] = {0x7fffffff, 0x7fffffff, 0x7fffffff, (int)0x80000000, (int)0x80000000, (int)0x80000000};
+    for (uint32_t w = w0 + threadIdx.x; w < w1; w += 256)
+        for (int k = 0; k < 3; ++k) {
+            v[k] = min(v[k], waves[6 * (size_t)w + k]);
+            v[3 + k] = max(v[3 + k], waves[6 * (size_t)w + 3 + k]);
+        }
+    for (int k = 0; k < 6; ++k)
+        s[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int k = 0; k < 6; ++k)
+                s[k][threadIdx.x] = k < 3 ? min(s[k][threadIdx.x], s[k][threadIdx.x + h])
+                                          : max(s[k][threadIdx.x], s[k][threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 3 && w0 < w1) {
+        atomicMin(mb + 6 * (size_t)m + threadIdx.x, s[threadIdx.x][0]);
+        atomicMax(mb + 6 * (size_t)m + 3 + threadIdx.x, s[3 + threadIdx.x][0]);
     }
 }
 
@@ -246,10 +279,14 @@ void launch_batch_bounds_init(int *mb, uint32_t nmeshes, hipStream_t s)
         hipLaunchKernelGGL(k_batch_bounds_init, dim3(blocks_for(6ull * nmeshes)), dim3(256), 0, s, mb, nmeshes);
 }
 
-void launch_batch_bounds(const float *box32, const uint32_t *meshOf, uint32_t nt, int *mb, hipStream_t s)
+// waves: 6 * ceil(nt / 64) ints of scratch
+void launch_batch_bounds(const float *box32, const uint32_t *meshOf, const uint32_t *triBegin, uint32_t nmeshes,
+                         uint32_t nt, int *mb, int *waves, hipStream_t s)
 {
-    if (nt)
-        hipLaunchKernelGGL(k_batch_bounds, dim3(blocks_for(nt)), dim3(256), 0, s, box32, meshOf, nt, mb);
+    if (!nt)
+        return;
+    hipLaunchKernelGGL(k_batch_bounds, dim3(blocks_for(nt)), dim3(256), 0, s, box32, meshOf, nt, mb, waves);
+    hipLaunchKernelGGL(k_batch_bounds_merge, dim3(nmeshes), dim3(256), 0, s, waves, triBegin, nt, mb);
 }
 
 void launch_batch_keys(const float *box32, const uint32_t *meshOf, const uint32_t *triBegin, const int *mb,

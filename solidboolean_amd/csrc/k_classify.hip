@@ -272,6 +272,9 @@ __global__ __launch_bounds__(256) void k_grid_count(const double *__restrict__ t
 {
     __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
     const int axis = blockIdx.y;
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    if ((uint64_t)tile * kGridBlockTris >= nt)
+        return;
     for (int k = threadIdx.x; k < kGridSlots; k += 256) {
         s_key[k] = 0;
         s_cnt[k] = 0;
@@ -279,7 +282,7 @@ __global__ __launch_bounds__(256) void k_grid_count(const double *__restrict__ t
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kGridBlockTris / 256; ++j) {
-        const TriSpan x = tri_span(tri64, nt, gs, axis, blockIdx.x * kGridBlockTris + j * 256 + threadIdx.x);
+        const TriSpan x = tri_span(tri64, nt, gs, axis, tile * kGridBlockTris + j * 256 + threadIdx.x);
         if (!x.valid)
             continue;
         for (uint32_t cu = x.s.u0; cu <= x.s.u1; ++cu)
@@ -307,6 +310,9 @@ __global__ __launch_bounds__(256) void k_grid_fill(const double *__restrict__ tr
 {
     __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
     const int axis = blockIdx.y;
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    if ((uint64_t)tile * kGridBlockTris >= nt)
+        return;
     for (int k = threadIdx.x; k < kGridSlots; k += 256) {
         s_key[k] = 0;
         s_cnt[k] = 0;
@@ -315,7 +321,7 @@ __global__ __launch_bounds__(256) void k_grid_fill(const double *__restrict__ tr
     TriSpan x[kGridBlockTris / 256];
 #pragma unroll
     for (int j = 0; j < kGridBlockTris / 256; ++j) {
-        x[j] = tri_span(tri64, nt, gs, axis, blockIdx.x * kGridBlockTris + j * 256 + threadIdx.x);
+        x[j] = tri_span(tri64, nt, gs, axis, tile * kGridBlockTris + j * 256 + threadIdx.x);
         if (!x[j].valid)
             continue;
         for (uint32_t cu = x[j].s.u0; cu <= x[j].s.u1; ++cu)
@@ -529,7 +535,7 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
     launch_fill_u32(cells, 0u, m, s);
     if (nt == 0)
         return;
-    const dim3 grid((unsigned)((nt + kGridBlockTris - 1) / kGridBlockTris), 3);
+    const dim3 grid(xcd_grid((nt + kGridBlockTris - 1) / kGridBlockTris), 3);
     hipLaunchKernelGGL(k_grid_count, grid, dim3(256), 0, s, tri64, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
     hipLaunchKernelGGL(k_grid_fill, grid, dim3(256), 0, s, tri64, nt, gs, cells, gbox, gref);
@@ -544,7 +550,7 @@ void launch_rays(const GridSet &gs, const double *tri64, const double *normals, 
     const uint64_t lanes = 3ull * n;
     hipLaunchKernelGGL(k_rays_grid, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, gs, tri64, normals, pts,
                        order, skip, n, axisBits, ovfList, ovfCount, overflow);
-    hipLaunchKernelGGL(k_rays_wave, dim3(2048), dim3(64), 0, s, gs, tri64, normals, pts, order, n, axisBits, ovfList,
+    hipLaunchKernelGGL(k_rays_wave, dim3(8192), dim3(64), 0, s, gs, tri64, normals, pts, order, n, axisBits, ovfList,
                        ovfCount, overflow);
 }
 

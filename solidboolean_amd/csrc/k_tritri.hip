@@ -221,7 +221,7 @@ __global__ __launch_bounds__(256) void k_gd_fp64(const double *__restrict__ tri6
                                                  uint32_t *__restrict__ partial)
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t w = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     const uint64_t base = (uint64_t)w * 64u * items;
     if (base >= cap)
         return;
@@ -251,7 +251,7 @@ __global__ __launch_bounds__(256) void k_gd_filter(const float *__restrict__ tri
                                                    uint32_t *__restrict__ chunkCount, uint32_t *__restrict__ partial)
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t w = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     const uint64_t base = (uint64_t)w * 64u * items;
     if (base >= cap)
         return;
@@ -303,7 +303,7 @@ __global__ __launch_bounds__(256) void k_gd_list(const double *__restrict__ tri6
                                                  uint32_t *__restrict__ partial)
 {
     const int lane = threadIdx.x & 63;
-    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t w = xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
     if (w >= nChunks)
         return;
     const uint64_t base = (uint64_t)w * 64u * items;
@@ -465,7 +465,7 @@ void launch_tritri(const double *tri64A, const double *tri64B, const float *tri3
         return;
     const uint32_t items = gd_items(n);
     const uint32_t chunks = gd_chunks(n, items);
-    const dim3 grid((chunks + 3) / 4);
+    const dim3 grid(xcd_grid((chunks + 3) / 4));
     uint32_t *list = work;
     uint32_t *chunkCount = work + n;
     uint32_t *partial = stats ? chunkCount + chunks : nullptr;

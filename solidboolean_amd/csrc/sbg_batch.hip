@@ -39,13 +39,13 @@ struct sbg_batch_s {
     size_t nmeshes = 0, npairs = 0, nv = 0, nt = 0;
     std::vector<uint64_t> vBegin, tBegin;
     int localBits = 0, mortonBits = 30, meshBits = 0, triBits = 1;
-    sbg_batch_tuning tune = {8, 2048, 4, 1};
+    sbg_batch_tuning tune = {16, 1024, 4, 1};
 
     DevBuf<double> V;
     DevBuf<uint32_t> T, meshOf, target, triBegin;
     DevBuf<double> tri64, normals;
     DevBuf<float> tri32, box32, blockBounds, rootBox, meshBox;
-    DevBuf<int> meshBounds;
+    DevBuf<int> meshBounds, waveBounds;
     DevBuf<uint64_t> keys, keysAlt, codes;
     DevBuf<uint32_t> ids, leafParent, work, meshRoot, hist;
     DevBuf<BNode> nodes;
@@ -154,7 +154,8 @@ int enqueue(sbg_batch b, int precision)
     launch_records(b->V.p, b->T.p, nt, b->tri64.p, b->tri32.p, b->normals.p, b->box32.p, b->blockBounds.p, nb, s);
     // combined LBVH: every mesh a subtree
     launch_batch_bounds_init(b->meshBounds.p, (uint32_t)b->nmeshes, s);
-    launch_batch_bounds(b->box32.p, b->meshOf.p, nt, b->meshBounds.p, s);
+    launch_batch_bounds(b->box32.p, b->meshOf.p, b->triBegin.p, (uint32_t)b->nmeshes, nt, b->meshBounds.p,
+                        b->waveBounds.p, s);
     launch_batch_keys(b->box32.p, b->meshOf.p, b->triBegin.p, b->meshBounds.p, nt, b->localBits, b->mortonBits,
                       b->keys.p, s);
     const int keyRange[2] = {b->localBits, b->localBits + b->mortonBits + b->meshBits};
@@ -281,6 +282,7 @@ int sbg_batch_create(sbg_context ctx, const double *xyz, const uint32_t *tris, c
         (rc = b->normals.ensure(3 * nt)) || (rc = b->box32.ensure((size_t)kBox32Stride * nt)) ||
         (rc = b->blockBounds.ensure(6 * (size_t)records_blocks((uint32_t)nt))) || (rc = b->rootBox.ensure(6)) ||
         (rc = b->meshBox.ensure(6 * nmeshes)) || (rc = b->meshBounds.ensure(6 * nmeshes)) ||
+        (rc = b->waveBounds.ensure(6 * ((nt + 63) / 64))) ||
         (rc = b->keys.ensure(nt)) || (rc = b->keysAlt.ensure(nt)) || (rc = b->codes.ensure(nt)) ||
         (rc = b->ids.ensure(nt)) || (rc = b->leafParent.ensure(nt)) || (rc = b->work.ensure(4 * nt + 1)) ||
         (rc = b->meshRoot.ensure(nmeshes)) || (rc = b->nodes.ensure(nt > 1 ? nt - 1 : 1)) ||
@@ -417,6 +419,11 @@ int sbg_batch_run(sbg_batch b, int precision)
         if (!grow) {
             b->nCand = nCand;
             b->ran = true;
+            // the pair sort and the GD grid cover the whole capacity: keep it
+            // near the real count before a graph fixes it (buffers stay)
+            const size_t want = std::max<size_t>((size_t)1 << 14, (size_t)nCand + nCand / 4);
+            if (!fitted && b->candCap > 2 * want)
+                b->candCap = want;
             if (!fitted && b->tune.use_graph) {
                 fitted = true;  // capture now, so the next run is one graph launch
                 TRY(run_once(b, precision, true));
@@ -599,6 +606,7 @@ int sbg_batch_destroy(sbg_batch b)
     for (auto *p : u8)
         p->release();
     b->meshBounds.release();
+    b->waveBounds.release();
     b->nodes.release();
     b->front[0].release();
     b->front[1].release();

@@ -253,6 +253,19 @@ __device__ __forceinline__ bool box64_overlap(const Box64 &a, const Box64 &b)
 __device__ __forceinline__ float f_down(double x) { return __double2float_rd(x); }
 __device__ __forceinline__ float f_up(double x) { return __double2float_ru(x); }
 
+// ---- XCD-aware block order ---------------------------------------------------
+// The chip dispatches workgroups to its 8 XCDs, each with its own L2, in
+// round-robin order of the linear block index.  xcd_block maps the hardware
+// index so that each XCD works through one contiguous range of logical blocks;
+// neighbouring blocks, which read the same data, then share an L2.  The grid's
+// x dimension must be a multiple of 8 (xcd_grid); logical blocks past the real
+// count must return.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t hw, uint32_t gridBlocks)
+{
+    return (hw % kXcds) * (gridBlocks / kXcds) + hw / kXcds;
+}
+
 // ---- Morton codes ----------------------------------------------------------
 // spread the low 10 bits of x to every third bit
 __device__ __forceinline__ uint32_t spread3(uint32_t x)

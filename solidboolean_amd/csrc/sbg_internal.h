@@ -27,6 +27,9 @@ struct alignas(16) BNode {
 };
 constexpr uint32_t kLeafBit = 0x80000000u;
 
+// blocks rounded up to a multiple of the XCD count (see xcd_block)
+inline unsigned xcd_grid(uint64_t blocks) { return (unsigned)((blocks + 7) / 8 * 8); }
+
 // ---- launchers (each .hip file owns its kernels) --------------------------
 // k_prepare.hip
 void launch_records(const double *V, const uint32_t *T, uint32_t nt, double *tri64, float *tri32,
@@ -146,7 +149,9 @@ void launch_adjoint(const double *xyz, uint32_t nin, const uint32_t *src, uint32
 
 // k_batch.hip
 void launch_batch_bounds_init(int *mb, uint32_t nmeshes, hipStream_t s);
-void launch_batch_bounds(const float *box32, const uint32_t *meshOf, uint32_t nt, int *mb, hipStream_t s);
+// waves: 6 * ceil(nt / 64) ints of scratch
+void launch_batch_bounds(const float *box32, const uint32_t *meshOf, const uint32_t *triBegin, uint32_t nmeshes,
+                         uint32_t nt, int *mb, int *waves, hipStream_t s);
 void launch_batch_keys(const float *box32, const uint32_t *meshOf, const uint32_t *triBegin, const int *mb,
                        uint32_t nt, int localBits, int mortonBits, uint64_t *keys, hipStream_t s);
 void launch_batch_codes(const uint64_t *keys, uint32_t n, int localBits, int mortonBits, const uint32_t *triBegin,
