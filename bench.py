@@ -160,7 +160,7 @@ def cpu_baseline(a, b, pairs_ref, budget_s: float = 12.0):
     }
 
 
-def fp64_vs_batched(ctx, a, b, step, N, steps: int) -> dict:
+def fp64_vs_batched(ctx, a, b, step, N, steps: int, graph: bool = True) -> dict:
     """The 1.31M-triangle pair end to end two ways: the plain FP64 path (single
     pair API, FP64 Guigue-Devillers) and the FP32-filtered batched path (the
     batch API with this one pair, replaying its captured graph)."""
@@ -173,6 +173,8 @@ def fp64_vs_batched(ctx, a, b, step, N, steps: int) -> dict:
     ctx.synchronize()
     plain = (time.perf_counter() - t0) / steps * 1e3
     batch = ctx.batch([(a, b)])
+    if not graph:
+        batch.set_tuning(use_graph=0)
     batch.run(N.FP32_FILTERED)
     batch.run(N.FP32_FILTERED)
     dev, t0 = [], time.perf_counter()
@@ -186,7 +188,7 @@ def fp64_vs_batched(ctx, a, b, step, N, steps: int) -> dict:
     return out
 
 
-def batch_section(ctx, mg, N, steps: int) -> dict:
+def batch_section(ctx, mg, N, steps: int, graph: bool = True) -> dict:
     """4,096 independent icosphere pairs (levels 1-6, P ~ 4^-level, radii
     U[0.5, 2], offsets U[-0.8, 0.8]^3 x radius, seeded rotations) as one batch:
     autotuned, then timed as graph replays."""
@@ -194,6 +196,8 @@ def batch_section(ctx, mg, N, steps: int) -> dict:
     pairs = mg.batch_pairs(4096, seed=1)
     batch = ctx.batch(pairs)
     log(f"batch: {len(pairs)} pairs, {batch.nt} triangles, built in {time.perf_counter() - t0:.1f}s")
+    if not graph:
+        batch.set_tuning(use_graph=0)
     batch.run(N.FP32_FILTERED)
     tuned, tuned_ms = batch.autotune(N.FP32_FILTERED, reps=2)
     log(f"batch autotune: {tuned} ({tuned_ms:.3f} ms)")
@@ -210,7 +214,8 @@ def batch_section(ctx, mg, N, steps: int) -> dict:
     wall, dev = res["filtered"]
     out = {
         "workload": "4,096 icosphere pairs, levels 1-6 with P ~ 4^-level, radii U[0.5, 2], offsets U[-0.8, 0.8]^3 "
-                    "x radius, seeded rotations (seed 1), one captured graph per run",
+                    "x radius, seeded rotations (seed 1), " + ("one captured graph per run" if graph else
+                                                              "plain launches (--no-graph)"),
         "pairs": len(pairs),
         "triangles": batch.nt,
         "candidate_pairs": ncand,
@@ -248,6 +253,7 @@ def main():
     ap.add_argument("--level", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4,096-pair batch and the FP64 comparison")
+    ap.add_argument("--no-graph", action="store_true", help="run the batch as plain launches (for kernel tracing)")
     ap.add_argument("--gd-only", type=int, default=0, help="profiling aid: only N back-to-back GD launches")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -291,6 +297,8 @@ def main():
     for i in range(args.warmup):
         step()
         log(f"warmup step {i + 1}/{args.warmup} done")
+    tuning, tuned_ms = ctx.autotune_pair(A, B, reps=2)
+    log(f"autotuned single-pair knobs: {tuning} ({tuned_ms:.2f} ms search + GD + classify)")
     ctx.synchronize()
     ctx.stage_ms(reset=True)
     ctx.set_timing(True)
@@ -371,6 +379,7 @@ def main():
             "candidate_pairs": n_pairs,
             "intersecting_pairs": hits,
             "parallelism": "single GPU",
+            "tuning": tuning,
         },
         "end_to_end_ms": elapsed / args.steps * 1e3,
         "stage_ms": stages,
@@ -394,8 +403,8 @@ def main():
         "parity": parity,
     }
     if not args.no_batch:
-        line["fp64_vs_filtered_batched"] = fp64_vs_batched(ctx, a, b, step, N, args.steps)
-        line["batch"] = batch_section(ctx, mg, N, args.steps)
+        line["fp64_vs_filtered_batched"] = fp64_vs_batched(ctx, a, b, step, N, args.steps, not args.no_graph)
+        line["batch"] = batch_section(ctx, mg, N, args.steps, not args.no_graph)
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline(a, b, None)

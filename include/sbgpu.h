@@ -151,6 +151,20 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, int precision, in
  * traffic counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */
 int sbg_calibrate_copy(sbg_context ctx, size_t bytes, int iters, double *avg_ms);
 
+/* ---- tuning of the single-pair path -----------------------------------------
+ * Knobs that change speed, never results.  sbg_autotune_pair times the chain
+ * find_pairs -> tri_tri (FP32-filtered) -> classify_triangles of both meshes
+ * over a coordinate search of the knobs and leaves the fastest set. */
+typedef struct sbg_tuning {
+    int bfs_levels;   /* breadth-first levels before the persistent depth-first search (0..64) */
+    int dfs_waves;    /* one-wave workgroups of the depth-first search (64..16384) */
+    int grid_density; /* ray-grid cells per axis plane = triangles * grid_density / 16 (1..64) */
+    int depth_first;  /* 0: breadth-first levels only, with a host check every 8 levels */
+} sbg_tuning;
+int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t);
+int sbg_context_get_tuning(sbg_context ctx, sbg_tuning *t);
+int sbg_autotune_pair(sbg_mesh a, sbg_mesh b, int reps, sbg_tuning *best, double *best_ms);
+
 /* ---- topology-fixed adjoint ------------------------------------------------
  * No reference counterpart (the reference computes no derivatives).  nin
  * input vertices xyz_in; nd derived vertices, derived d being the point where

@@ -118,15 +118,14 @@ struct KeySet {
 
 // candidate triangle t: FP64 box test, plane hit, three-normal inside test;
 // on a hit, `key` is the hit point's PositionKey
-__device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ tri64,
-                                        const double *__restrict__ normals, const Ray &r, Key3 &key)
+__device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ tri64, const Ray &r, Key3 &key)
 {
     const double2 *x = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * t);
     const double2 r0 = x[0], r1 = x[1], r2 = x[2], r3 = x[3], r4 = x[4], r5 = x[5];
     const V3 v0 = {r0.x, r0.y, r1.x}, v1 = {r2.x, r2.y, r3.x}, v2 = {r4.x, r4.y, r5.x};
     if (!box64_overlap(tri_box64(v0, v1, v2), r.rb))
         return false;
-    const V3 n = {normals[3 * (size_t)t], normals[3 * (size_t)t + 1], normals[3 * (size_t)t + 2]};
+    const V3 n = {r1.y, r3.y, r5.y};  // the record's pad slots hold the FP64 normal
     const V3 w = vsub(r.p, v0);
     const double d = n.x * r.u.x + n.y * r.u.y + n.z * r.u.z;
     const double num = (-n.x) * w.x + (-n.y) * w.y + (-n.z) * w.z;
@@ -387,7 +386,7 @@ __device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, con
 // k_rays_wave.  Query i's rays go to mesh target[i] (mesh 0 without a target
 // list); a query without a target mesh (~0u) counts as outside.
 __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__restrict__ tri64,
-                                                   const double *__restrict__ normals, const double *__restrict__ pts,
+                                                   const double *__restrict__ pts,
                                                    const uint32_t *__restrict__ order, const uint8_t *__restrict__ skip,
                                                    uint32_t n, uint8_t *__restrict__ bits, uint32_t *__restrict__ ovfList,
                                                    uint32_t *ovfCount, uint32_t *overflow)
@@ -417,7 +416,7 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
     for (uint32_t k = range.x; !defer && k < range.y; ++k) {
         uint32_t t;
         Key3 key;
-        if (entry_may_hit(gs, k, r, (int)axis, t) && ray_hit(t, tri64, normals, r, key)) {
+        if (entry_may_hit(gs, k, r, (int)axis, t) && ray_hit(t, tri64, r, key)) {
             keys.add(key);
             defer = keys.full;
         }
@@ -434,7 +433,7 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
 // the cell's entries, hit keys are gathered in LDS (up to kWaveHits) and the
 // distinct ones counted; more hits than that raise `overflow`.
 __global__ __launch_bounds__(64) void k_rays_wave(GridSet gs, const double *__restrict__ tri64,
-                                                  const double *__restrict__ normals, const double *__restrict__ pts,
+                                                  const double *__restrict__ pts,
                                                   const uint32_t *__restrict__ order, uint32_t n,
                                                   uint8_t *__restrict__ bits, const uint32_t *__restrict__ list,
                                                   const uint32_t *__restrict__ count, uint32_t *overflow)
@@ -456,7 +455,7 @@ __global__ __launch_bounds__(64) void k_rays_wave(GridSet gs, const double *__re
             const uint32_t e = k0 + lane;
             uint32_t t;
             Key3 key;
-            const bool hit = e < range.y && entry_may_hit(gs, e, r, (int)axis, t) && ray_hit(t, tri64, normals, r, key);
+            const bool hit = e < range.y && entry_may_hit(gs, e, r, (int)axis, t) && ray_hit(t, tri64, r, key);
             const unsigned long long mask = __ballot(hit);
             const uint32_t pos = nh + (uint32_t)__popcll(mask & below);
             if (hit && pos < (uint32_t)kWaveHits) {
@@ -519,9 +518,10 @@ void launch_centroids(const double *tri64, const uint8_t *flags, uint32_t nt, do
     hipLaunchKernelGGL(k_centroids, dim3((nt + 255) / 256), dim3(256), 0, s, tri64, flags, nt, pts);
 }
 
-uint32_t grid_dim(uint32_t nt)
+// cells per axis side: G^2 = nt * density / 16, G in [1, 1024]
+uint32_t grid_dim(uint32_t nt, int density)
 {
-    uint32_t G = (uint32_t)std::ceil(std::sqrt((double)nt / 4.0));
+    uint32_t G = (uint32_t)std::ceil(std::sqrt((double)nt * density / 16.0));
     return G < 1u ? 1u : (G > 1024u ? 1024u : G);
 }
 
@@ -541,16 +541,16 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
     hipLaunchKernelGGL(k_grid_fill, grid, dim3(256), 0, s, tri64, nt, gs, cells, gbox, gref);
 }
 
-void launch_rays(const GridSet &gs, const double *tri64, const double *normals, const double *pts,
+void launch_rays(const GridSet &gs, const double *tri64, const double *pts,
                  const uint32_t *order, const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList,
                  uint32_t *ovfCount, uint32_t *overflow, hipStream_t s)
 {
     if (n == 0)
         return;
     const uint64_t lanes = 3ull * n;
-    hipLaunchKernelGGL(k_rays_grid, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, gs, tri64, normals, pts,
+    hipLaunchKernelGGL(k_rays_grid, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, gs, tri64, pts,
                        order, skip, n, axisBits, ovfList, ovfCount, overflow);
-    hipLaunchKernelGGL(k_rays_wave, dim3(8192), dim3(64), 0, s, gs, tri64, normals, pts, order, n, axisBits, ovfList,
+    hipLaunchKernelGGL(k_rays_wave, dim3(8192), dim3(64), 0, s, gs, tri64, pts, order, n, axisBits, ovfList,
                        ovfCount, overflow);
 }
 

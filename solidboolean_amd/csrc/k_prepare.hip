@@ -29,14 +29,16 @@ __global__ __launch_bounds__(kRecBlock) void k_records(const double *__restrict_
         V3 a = {V[3 * ia], V[3 * ia + 1], V[3 * ia + 2]};
         V3 b = {V[3 * ib], V[3 * ib + 1], V[3 * ib + 2]};
         V3 c = {V[3 * ic], V[3 * ic + 1], V[3 * ic + 2]};
+        // FP64 record: each vertex's pad slot carries one normal component, so
+        // a ray's triangle test reads one 96-B record and nothing else
+        const V3 n = tri_normal(a, b, c);
         double2 *r64 = reinterpret_cast<double2 *>(tri64 + (size_t)kTri64Stride * i);
         r64[0] = make_double2(a.x, a.y);
-        r64[1] = make_double2(a.z, 0.0);
+        r64[1] = make_double2(a.z, n.x);
         r64[2] = make_double2(b.x, b.y);
-        r64[3] = make_double2(b.z, 0.0);
+        r64[3] = make_double2(b.z, n.y);
         r64[4] = make_double2(c.x, c.y);
-        r64[5] = make_double2(c.z, 0.0);
-        V3 n = tri_normal(a, b, c);
+        r64[5] = make_double2(c.z, n.z);
         normals[3 * (size_t)i] = n.x;
         normals[3 * (size_t)i + 1] = n.y;
         normals[3 * (size_t)i + 2] = n.z;

@@ -294,6 +294,50 @@ __global__ __launch_bounds__(256) void k_gd_filter(const float *__restrict__ tri
     }
 }
 
+// Small launches (one wave chunk of 64 pairs per wave): the FP32 certificates
+// and, in the same lane, the FP64 sequence for a pair they leave undecided or
+// certify as a hit.  At a few tens of thousands of pairs the two-kernel split
+// costs a second launch's latency for nothing; results are the same.
+__global__ __launch_bounds__(256) void k_gd_fused(const float *__restrict__ tri32A, const float *__restrict__ tri32B,
+                                                  const double *__restrict__ tri64A,
+                                                  const double *__restrict__ tri64B,
+                                                  const uint64_t *__restrict__ pairs, uint32_t n,
+                                                  uint8_t *__restrict__ status, double *__restrict__ seg,
+                                                  uint32_t *__restrict__ partial)
+{
+    const int lane = threadIdx.x & 63;
+    const uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint64_t i = (uint64_t)w * 64u + lane;
+    if ((uint64_t)w * 64u >= n)
+        return;
+    const bool valid = i < n;
+    int f = -3, exitKind = -1;
+    if (valid) {
+        const uint64_t pr = pairs[i];
+        F3 p1, q1, r1, p2, q2, r2;
+        float extA, magA, extB, magB;
+        load_tri32(tri32A, (uint32_t)(pr >> 32), p1, q1, r1, extA, magA);
+        load_tri32(tri32B, (uint32_t)pr, p2, q2, r2, extB, magB);
+        f = gd_filter(p1, q1, r1, p2, q2, r2, pair_bound(p1, p2, extA, extB, magA, magB));
+        if (f >= 0 && f <= GD_REJECT_INTERVAL) {
+            status[i] = 0;
+            exitKind = f;
+        } else {
+            gd_pair_fp64(tri64A, tri64B, pr, (uint32_t)i, status, seg, exitKind);
+        }
+    }
+    if (partial) {
+        uint32_t ex[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k)
+            ex[k] = wave_popc(valid && exitKind == k);
+        const uint32_t fb = wave_popc(valid && f < 0);
+        const uint32_t ft = wave_popc(valid && f >= 0 && f <= GD_REJECT_INTERVAL);
+        if (lane < kStatSlots)
+            partial[(size_t)w * kStatSlots + lane] = lane < 5 ? pick5(ex, lane) : lane == 5 ? fb : lane == 6 ? ft : 0u;
+    }
+}
+
 // FP64 on each chunk's undecided pairs; adds its exits to the chunk's counts
 __global__ __launch_bounds__(256) void k_gd_list(const double *__restrict__ tri64A, const double *__restrict__ tri64B,
                                                  const uint64_t *__restrict__ pairs, uint32_t nChunks, uint32_t items,
@@ -439,6 +483,9 @@ int compact_blocks(uint32_t n) { return (int)((n + kCompactBlock - 1) / kCompact
 
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 
+// up to this many pairs the filtered precision runs as one fused launch
+constexpr uint32_t kFusedMaxPairs = 1u << 18;
+
 // pairs per wave chunk / 64: one chunk per wave until about 8192 waves are
 // available, then up to 16 pairs per lane
 static uint32_t gd_items(uint32_t n)
@@ -469,7 +516,10 @@ void launch_tritri(const double *tri64A, const double *tri64B, const float *tri3
     uint32_t *list = work;
     uint32_t *chunkCount = work + n;
     uint32_t *partial = stats ? chunkCount + chunks : nullptr;
-    if (precision == 1) {
+    if (precision == 1 && !count && items == 1 && n <= kFusedMaxPairs) {
+        hipLaunchKernelGGL(k_gd_fused, grid, dim3(256), 0, s, tri32A, tri32B, tri64A, tri64B, pairs, n, status, seg,
+                           partial);
+    } else if (precision == 1) {
         hipLaunchKernelGGL(k_gd_filter, grid, dim3(256), 0, s, tri32A, tri32B, pairs, n, count, items, status, list,
                            chunkCount, partial);
         hipLaunchKernelGGL(k_gd_list, grid, dim3(256), 0, s, tri64A, tri64B, pairs, chunks, items, list, chunkCount,

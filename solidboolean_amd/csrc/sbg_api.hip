@@ -1,6 +1,7 @@
 // C-ABI implementation (include/sbgpu.h): contexts, meshes, pair lists and the
 // host-side sequencing of the kernels in k_*.hip.
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -79,6 +80,7 @@ struct sbg_mesh_s {
     DevBuf<uint2> gridRef;
     size_t gridCap = 0;
     uint32_t gridG = 0;
+    int gridDensity = 0;
     bool gridReady = false;
 };
 
@@ -497,8 +499,8 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
         uint32_t *head = cnt + kMaxLevels + 3;         // depth-first seed claims
         const uint32_t fcap = (uint32_t)std::min<size_t>(c->frontCap, 0xffffffffu);
         const uint32_t ccap = (uint32_t)std::min<size_t>(c->candCap, 0xffffffffu);
-        const bool dfs = c->depthFirst;
-        if (dfs && (rc = c->spill.ensure((size_t)kDfsWaves * c->spillPerWave)) != SBG_OK) {
+        const bool dfs = c->depthFirst && c->tune.depth_first;
+        if (dfs && (rc = c->spill.ensure((size_t)c->tune.dfs_waves * c->spillPerWave)) != SBG_OK) {
             sbg_pairs_destroy(p);
             return rc;
         }
@@ -512,11 +514,11 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
             if (dfs) {
                 // a fixed number of levels with no host round trip, then one
                 // persistent launch over the frontier, then one sync
-                for (; level < kBfsLevels; ++level)
+                for (; level < c->tune.bfs_levels; ++level)
                     launch_expand(t, c->front[level & 1].p, cnt + level, c->front[(level + 1) & 1].p, cnt + level + 1,
                                   fcap, p->pairs.p, candCount, ccap, overflow, kExpandGrid, c->stream);
                 launch_traverse_dfs(t, c->front[level & 1].p, cnt + level, fcap, head, c->spill.p, c->spillPerWave,
-                                    kDfsWaves, p->pairs.p, candCount, ccap, overflow, c->stream);
+                                    c->tune.dfs_waves, p->pairs.p, candCount, ccap, overflow, c->stream);
                 rc = check_launch("depth-first search");
                 done = true;
             }
@@ -833,10 +835,11 @@ static GridSet grid_set(sbg_mesh m)
 
 static int ensure_grid(sbg_mesh m)
 {
-    if (m->gridReady)
+    if (m->gridReady && m->gridDensity == m->ctx->tune.grid_density)
         return SBG_OK;
     sbg_context c = m->ctx;
-    const uint32_t G = grid_dim((uint32_t)m->nt);
+    const uint32_t G = grid_dim((uint32_t)m->nt, c->tune.grid_density);
+    m->gridDensity = c->tune.grid_density;
     if (!m->gridCap)
         m->gridCap = 6 * m->nt + 1024;
     if (m->gridCap > 0xffffffffu)
@@ -876,7 +879,7 @@ static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder,
         {
             Stage st(c, SBG_STAGE_CLASSIFY);
             launch_fill_u32(c->ovf.p, 0u, 2, c->stream);
-            launch_rays(grid_set(target), target->tri64.p, target->normals.p, dpts, dorder, dskip, (uint32_t)n, c->bits.p,
+            launch_rays(grid_set(target), target->tri64.p, dpts, dorder, dskip, (uint32_t)n, c->bits.p,
                         c->rayList.p, c->ovf.p + 1, c->ovf.p, c->stream);
             launch_vote(c->bits.p, dskip, (uint32_t)n, c->votes.p, c->stream);
             TRY(check_launch("rays"));
@@ -964,6 +967,107 @@ int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side,
     TRY(run_rays(other, c->pts.p, self->sortedIds, c->flags.p, nt));
     HIP_TRY(hipMemcpyAsync(out, c->votes.p, nt, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return SBG_OK;
+}
+
+int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t)
+{
+    TRY(use(ctx));
+    if (!t)
+        return fail(SBG_ERR_INVALID, "null tuning");
+    if (t->bfs_levels < 0 || t->bfs_levels > 64 || t->dfs_waves < 64 || t->dfs_waves > 16384 ||
+        t->grid_density < 1 || t->grid_density > 64)
+        return fail(SBG_ERR_INVALID, "tuning out of range");
+    ctx->tune = *t;
+    ctx->tune.depth_first = t->depth_first ? 1 : 0;
+    return SBG_OK;
+}
+
+int sbg_context_get_tuning(sbg_context ctx, sbg_tuning *t)
+{
+    TRY(use(ctx));
+    if (!t)
+        return fail(SBG_ERR_INVALID, "null tuning");
+    *t = ctx->tune;
+    return SBG_OK;
+}
+
+// wall time of find_pairs -> tri_tri -> classify both meshes, `reps` times
+static int time_pair_chain(sbg_mesh a, sbg_mesh b, int reps, double &ms)
+{
+    std::vector<uint8_t> ca(a->nt + 1), cb(b->nt + 1);
+    auto once = [&]() -> int {
+        sbg_pairs p = nullptr;
+        TRY(sbg_find_pairs(a, b, &p));
+        int rc = sbg_tri_tri(a, b, p, SBG_PRECISION_FP32_FILTERED, nullptr);
+        if (rc == SBG_OK)
+            rc = sbg_classify_triangles(a, b, p, 0, ca.data());
+        if (rc == SBG_OK)
+            rc = sbg_classify_triangles(b, a, p, 1, cb.data());
+        sbg_pairs_destroy(p);
+        return rc;
+    };
+    TRY(once());  // builds grids and sizes queues for this setting
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int r = 0; r < reps; ++r) {
+        a->gridReady = b->gridReady = false;  // a step rebuilds the grids, as a new mesh would
+        TRY(once());
+    }
+    ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+    return SBG_OK;
+}
+
+int sbg_autotune_pair(sbg_mesh a, sbg_mesh b, int reps, sbg_tuning *best, double *best_ms)
+{
+    if (!a || !b || reps <= 0)
+        return fail(SBG_ERR_INVALID, "bad argument");
+    if (a->ctx != b->ctx)
+        return fail(SBG_ERR_INVALID, "meshes belong to different contexts");
+    if (!a->prepared || !b->prepared)
+        return fail(SBG_ERR_STATE, "mesh not prepared");
+    sbg_context c = a->ctx;
+    TRY(use(c));
+    sbg_tuning cur = c->tune;
+    double bestMs = 0.0;
+    TRY(time_pair_chain(a, b, reps, bestMs));
+    const int levels[] = {8, 12, 16, 20, 24};
+    const int waves[] = {256, 512, 1024, 2048, 4096};
+    const int density[] = {2, 4, 8, 16};
+    auto trial = [&](const sbg_tuning &t) -> int {
+        c->tune = t;
+        double ms = 0.0;
+        TRY(time_pair_chain(a, b, reps, ms));
+        if (ms < bestMs) {
+            bestMs = ms;
+            cur = t;
+        }
+        return SBG_OK;
+    };
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int v : levels)
+            if (v != cur.bfs_levels) {
+                sbg_tuning t = cur;
+                t.bfs_levels = v;
+                TRY(trial(t));
+            }
+        for (int v : waves)
+            if (v != cur.dfs_waves) {
+                sbg_tuning t = cur;
+                t.dfs_waves = v;
+                TRY(trial(t));
+            }
+        for (int v : density)
+            if (v != cur.grid_density) {
+                sbg_tuning t = cur;
+                t.grid_density = v;
+                TRY(trial(t));
+            }
+    }
+    c->tune = cur;
+    if (best)
+        *best = cur;
+    if (best_ms)
+        *best_ms = bestMs;
     return SBG_OK;
 }
 

@@ -98,8 +98,7 @@ uint32_t grid_layout(sbg_batch b, std::vector<uint2> &desc)
     uint64_t base = 0;
     for (size_t m = 0; m < b->nmeshes; ++m) {
         const double nt = (double)(b->tBegin[m + 1] - b->tBegin[m]);
-        uint32_t G = (uint32_t)std::ceil(std::sqrt(nt * b->tune.grid_density / 16.0));
-        G = std::max(1u, std::min(G, 1024u));
+        const uint32_t G = grid_dim((uint32_t)nt, b->tune.grid_density);
         desc[m] = make_uint2(G, (uint32_t)base);
         base += 3ull * G * G;
     }
@@ -207,7 +206,7 @@ int enqueue(sbg_batch b, int precision)
     gs.ncells = b->ncells;
     gs.order = b->ids.p;
     launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridBox.p, b->gridRef.p, s);
-    launch_rays(gs, b->tri64.p, b->normals.p, b->pts.p, b->ids.p, b->flags.p, nt, b->bits.p, b->rayList.p,
+    launch_rays(gs, b->tri64.p, b->pts.p, b->ids.p, b->flags.p, nt, b->bits.p, b->rayList.p,
                 w + kRayListW, w + kRayFlagW, s);
     launch_vote(b->bits.p, b->flags.p, nt, b->votes.p, s);
     launch_copy_u32(b->gridCells.p + b->ncells, w + kGridW, 1, s);

@@ -130,14 +130,15 @@ struct GridSet {
     const uint32_t *order;  // grid build's triangle visiting order, e.g. Morton (null: index order)
     uint32_t G0, cap, ncells;
 };
-uint32_t grid_dim(uint32_t nt);
+uint32_t grid_dim(uint32_t nt, int density);
 size_t grid_cells(uint32_t G);  // 3 G^2 + 1
 void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
                        float4 *gbox, uint2 *gref, hipStream_t s);
 // order (may be null): query q uses point order[q]; ovfList holds 3n entries,
 // *ovfCount must be zero on entry; overflow bit 0: a ray has too many distinct
 // hits, bit 1: the grid was truncated (cells[ncells] > cap)
-void launch_rays(const GridSet &gs, const double *tri64, const double *normals, const double *pts,
+// tri64 records carry the FP64 normal in their pad slots
+void launch_rays(const GridSet &gs, const double *tri64, const double *pts,
                  const uint32_t *order, const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList,
                  uint32_t *ovfCount, uint32_t *overflow, hipStream_t s);
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);

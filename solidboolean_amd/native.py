@@ -34,6 +34,7 @@ EXPORTS = [
     "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
     "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_context_set_timing",
     "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_calibrate_copy",
+    "sbg_context_set_tuning", "sbg_context_get_tuning", "sbg_autotune_pair",
     "sbg_adjoint",
     "sbg_batch_create", "sbg_batch_set_vertices", "sbg_batch_set_tuning", "sbg_batch_get_tuning", "sbg_batch_run",
     "sbg_batch_last_ms", "sbg_batch_counts", "sbg_batch_candidates", "sbg_batch_sides", "sbg_batch_autotune",
@@ -49,6 +50,13 @@ class SbgError(RuntimeError):
 
 class BatchTuning(C.Structure):
     _fields_ = [("bfs_levels", C.c_int), ("dfs_waves", C.c_int), ("grid_density", C.c_int), ("use_graph", C.c_int)]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+class Tuning(C.Structure):
+    _fields_ = [("bfs_levels", C.c_int), ("dfs_waves", C.c_int), ("grid_density", C.c_int), ("depth_first", C.c_int)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -114,6 +122,9 @@ def load(path: str = LIB_PATH):
         "sbg_context_stage_ms": [_P, _D, C.c_int],
         "sbg_bench_tri_tri": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)],
         "sbg_calibrate_copy": [_P, _SZ, C.c_int, C.POINTER(C.c_double)],
+        "sbg_context_set_tuning": [_P, C.POINTER(Tuning)],
+        "sbg_context_get_tuning": [_P, C.POINTER(Tuning)],
+        "sbg_autotune_pair": [_P, _P, C.c_int, C.POINTER(Tuning), C.POINTER(C.c_double)],
         "sbg_adjoint": [_P, _D, _SZ, _U32, _SZ, _U32, _SZ, C.c_double, _P, _P, _P, _P],
         "sbg_batch_create": [_P, _D, _U32, _U64, _U64, _SZ, _PP],
         "sbg_batch_set_vertices": [_P, _D],
@@ -186,6 +197,26 @@ class Context:
         ms = C.c_double(0)
         _check(self.L.sbg_calibrate_copy(self.h, nbytes, iters, C.byref(ms)))
         return ms.value
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        _check(self.L.sbg_context_get_tuning(self.h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = Tuning()
+        _check(self.L.sbg_context_get_tuning(self.h, C.byref(t)))
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        _check(self.L.sbg_context_set_tuning(self.h, C.byref(t)))
+
+    def autotune_pair(self, a: "Mesh", b: "Mesh", reps: int = 3):
+        """Coordinate search of the single-pair knobs (sbg_autotune_pair);
+        leaves the fastest set and returns it with its chain time in ms."""
+        t = Tuning()
+        ms = C.c_double(0)
+        _check(self.L.sbg_autotune_pair(a.h, b.h, reps, C.byref(t), C.byref(ms)))
+        return t.as_dict(), ms.value
 
     def batch(self, pairs) -> "Batch":
         """pairs: sequence of (mesh A, mesh B), each mesh an object with

@@ -278,3 +278,28 @@ def test_determinism(gpu):
     s1, g1 = p1.results()
     s2, g2 = p2.results()
     assert np.array_equal(s1, s2) and np.array_equal(bits(g1[s1 == 1]), bits(g2[s2 == 1]))
+
+
+def test_tuning_changes_speed_not_results(gpu):
+    """Every setting of the single-pair knobs (breadth-first levels, depth-first
+    waves, breadth-first only, ray-grid density) gives the same pairs,
+    segments and classifications."""
+    a, b = mg.icosphere_pair(5, 4, (0.3, -0.2, 0.1), noise=1e-3)
+    A, B = gpu.mesh(a.vertices, a.triangles), gpu.mesh(b.vertices, b.triangles)
+    saved = gpu.tuning()
+
+    def run():
+        P = A.find_pairs(B)
+        P.tri_tri(A, B, N.FP32_FILTERED, stats=False)
+        st, seg = P.results()
+        return P.array(), st, bits(seg[st == 1]), A.classify_triangles(B, P, 0), B.classify_triangles(A, P, 1)
+
+    try:
+        ref = run()
+        for kw in ({"bfs_levels": 0}, {"bfs_levels": 64, "dfs_waves": 64}, {"depth_first": 0},
+                   {"grid_density": 1}, {"grid_density": 64}, {"dfs_waves": 16384, "bfs_levels": 4}):
+            gpu.set_tuning(**{**saved, **kw})
+            got = run()
+            assert all(np.array_equal(x, y) for x, y in zip(ref, got)), kw
+    finally:
+        gpu.set_tuning(**saved)
