@@ -188,6 +188,47 @@ def fp64_vs_batched(ctx, a, b, step, N, steps: int, graph: bool = True) -> dict:
     return out
 
 
+def stress_section(ctx, mg, N, steps: int) -> dict:
+    """The filter stress cases: per case the candidate count, hits, coplanar
+    pairs, the FP32 filter's fallback rate, both GD precisions per launch, and
+    the end-to-end step (prepare both meshes, search, filtered GD, classify
+    both)."""
+    out = {}
+    cases = (("torus_4194304_vs_sphere_327680", lambda: mg.torus_sphere_pair(1)),
+             ("cube_grids_3145728_x2_near_coplanar", lambda: mg.cube_pair(512)))
+    for name, make in cases:
+        a, b = make()
+        A = ctx.mesh(a.vertices, a.triangles, prepare=False)
+        B = ctx.mesh(b.vertices, b.triangles, prepare=False)
+
+        def step():
+            A.prepare()
+            B.prepare()
+            P = A.find_pairs(B)
+            P.tri_tri(A, B, N.FP32_FILTERED, stats=False)
+            A.classify_triangles(B, P, 0)
+            B.classify_triangles(A, P, 1)
+            return P
+
+        step()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            P = step()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        st = P.tri_tri(A, B, N.FP32_FILTERED, stats=True)
+        f_us = P.bench(A, B, N.FP32_FILTERED, 50) * 1e3
+        d_us = P.bench(A, B, N.FP64, 50) * 1e3
+        out[name] = {"triangles": [a.nt, b.nt], "candidate_pairs": st["tested"], "hits": st["hits"],
+                     "coplanar": st["coplanar"], "fallback_pairs": st["fallback_pairs"],
+                     "fallback_rate": st["fallback_pairs"] / max(1, st["tested"]),
+                     "filtered_us_per_launch": f_us, "fp64_us_per_launch": d_us, "end_to_end_ms": ms}
+        log(f"stress {name}: {st['tested']} pairs, fallback {st['fallback_pairs']}, {ms:.2f} ms/step")
+        del P, A, B
+    return out
+
+
 def batch_section(ctx, mg, N, steps: int, graph: bool = True) -> dict:
     """4,096 independent icosphere pairs (levels 1-6, P ~ 4^-level, radii
     U[0.5, 2], offsets U[-0.8, 0.8]^3 x radius, seeded rotations) as one batch:
@@ -254,6 +295,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 4,096-pair batch and the FP64 comparison")
     ap.add_argument("--no-graph", action="store_true", help="run the batch as plain launches (for kernel tracing)")
+    ap.add_argument("--no-stress", action="store_true", help="skip the torus / cube filter stress cases")
     ap.add_argument("--gd-only", type=int, default=0, help="profiling aid: only N back-to-back GD launches")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -402,6 +444,8 @@ def main():
         "copy_kernel_gbs": copy_gbs,
         "parity": parity,
     }
+    if not args.no_stress:
+        line["stress"] = stress_section(ctx, mg, N, args.steps)
     if not args.no_batch:
         line["fp64_vs_filtered_batched"] = fp64_vs_batched(ctx, a, b, step, N, args.steps, not args.no_graph)
         line["batch"] = batch_section(ctx, mg, N, args.steps, not args.no_graph)

@@ -196,12 +196,8 @@ struct GridSpan {
     float hiAxis; // FP32 upper bound of the box along the ray axis
 };
 
-__device__ __forceinline__ GridSpan grid_span(const double *__restrict__ tri64, uint32_t t, int axis,
-                                              const GridFrame &f, uint32_t G)
+__device__ __forceinline__ GridSpan grid_span(const Box64 &b, int axis, const GridFrame &f, uint32_t G)
 {
-    const double2 *x = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * t);
-    const double2 r0 = x[0], r1 = x[1], r2 = x[2], r3 = x[3], r4 = x[4], r5 = x[5];
-    const Box64 b = tri_box64({r0.x, r0.y, r1.x}, {r2.x, r2.y, r3.x}, {r4.x, r4.y, r5.x});
     const int u = axis_u(axis), v = axis_v(axis);
     const double lu = comp(b.mn, u), hu = comp(b.mx, u), lv = comp(b.mn, v), hv = comp(b.mx, v);
     GridSpan s;
@@ -215,15 +211,16 @@ __device__ __forceinline__ GridSpan grid_span(const double *__restrict__ tri64, 
 }
 
 // Grid build: a workgroup takes kGridBlockTris consecutive triangles of the
-// visiting order (Morton order, so they are neighbours) for one axis and first
-// counts its entries per cell in an LDS hash table; only then does it touch the
-// global cell counters, once per distinct cell.  Device-scope atomics are the
-// cost here (they are coherent across the chip's separate L2s), and
-// neighbouring triangles share most cells, so this removes most of them.  A
-// cell that finds no slot within kProbe probes falls back to per-entry global
-// atomics; both paths only ever advance a cell's counter, so they mix freely.
-constexpr int kGridBlockTris = 1024;  // 4 per lane
-constexpr int kGridSlotBits = 12;
+// visiting order (Morton order, so they are neighbours), files each under its
+// cells on all three axes (one record load per triangle), and first counts its
+// entries per cell in an LDS hash table; only then does it touch the global
+// cell counters, once per distinct cell.  Device-scope atomics are the cost
+// here (they are coherent across the chip's separate L2s), and neighbouring
+// triangles share most cells, so this removes most of them.  A cell that finds
+// no slot within kProbe probes falls back to per-entry global atomics; both
+// paths only ever advance a cell's counter, so they mix freely.
+constexpr int kGridBlockTris = 512;  // one per lane
+constexpr int kGridSlotBits = 13;
 constexpr int kGridSlots = 1 << kGridSlotBits;
 constexpr int kProbe = 64;
 
@@ -241,17 +238,18 @@ __device__ __forceinline__ int table_slot(uint32_t *keys, uint32_t key)
     return -1;
 }
 
-struct TriSpan {
-    GridSpan s;
+// one triangle's spans on the three axes
+struct TriSpans {
+    GridSpan s[3];
     uint32_t t, G;
-    size_t base;  // counter index of the mesh's first cell on this axis, + 1
+    size_t base;  // counter index of the mesh's first cell, + 1
     bool valid;
 };
 
-__device__ __forceinline__ TriSpan tri_span(const double *__restrict__ tri64, uint32_t nt, const GridSet &gs,
-                                            int axis, uint32_t pos)
+__device__ __forceinline__ TriSpans tri_spans(const double *__restrict__ tri64, uint32_t nt, const GridSet &gs,
+                                              uint32_t pos)
 {
-    TriSpan x;
+    TriSpans x;
     x.valid = pos < nt;
     if (!x.valid)
         return x;
@@ -259,43 +257,51 @@ __device__ __forceinline__ TriSpan tri_span(const double *__restrict__ tri64, ui
     const uint32_t m = gs.meshOf ? gs.meshOf[x.t] : 0u;
     const uint2 d = grid_desc(gs, m);
     x.G = d.x;
-    const GridFrame f = grid_frame(gs.boxes + 6 * (size_t)m, axis, x.G);
-    x.s = grid_span(tri64, x.t, axis, f, x.G);
-    x.base = (size_t)d.y + (size_t)axis * x.G * x.G + 1;
+    x.base = (size_t)d.y + 1;
+    const double2 *r = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * x.t);
+    const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
+    const Box64 b = tri_box64({r0.x, r0.y, r1.x}, {r2.x, r2.y, r3.x}, {r4.x, r4.y, r5.x});
+    const float *bx = gs.boxes + 6 * (size_t)m;
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis)
+        x.s[axis] = grid_span(b, axis, grid_frame(bx, axis, x.G), x.G);
     return x;
 }
 
+__device__ __forceinline__ uint32_t cell_key(const TriSpans &x, int axis, uint32_t cu, uint32_t cv)
+{
+    return (uint32_t)(x.base + (size_t)axis * x.G * x.G + (size_t)cu * x.G + cv);
+}
+
 // cells: ncells + 1 u32, zero on entry; entries of cell c are counted in cells[c + 1]
-__global__ __launch_bounds__(256) void k_grid_count(const double *__restrict__ tri64, uint32_t nt, GridSet gs,
-                                                    uint32_t *__restrict__ cells)
+__global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const double *__restrict__ tri64, uint32_t nt,
+                                                               GridSet gs, uint32_t *__restrict__ cells)
 {
     __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
-    const int axis = blockIdx.y;
     const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
     if ((uint64_t)tile * kGridBlockTris >= nt)
         return;
-    for (int k = threadIdx.x; k < kGridSlots; k += 256) {
+    for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris) {
         s_key[k] = 0;
         s_cnt[k] = 0;
     }
     __syncthreads();
+    const TriSpans x = tri_spans(tri64, nt, gs, tile * kGridBlockTris + threadIdx.x);
+    if (x.valid) {
 #pragma unroll
-    for (int j = 0; j < kGridBlockTris / 256; ++j) {
-        const TriSpan x = tri_span(tri64, nt, gs, axis, tile * kGridBlockTris + j * 256 + threadIdx.x);
-        if (!x.valid)
-            continue;
-        for (uint32_t cu = x.s.u0; cu <= x.s.u1; ++cu)
-            for (uint32_t cv = x.s.v0; cv <= x.s.v1; ++cv) {
-                const uint32_t key = (uint32_t)(x.base + (size_t)cu * x.G + cv);
-                const int sl = table_slot(s_key, key);
-                if (sl >= 0)
-                    atomicAdd(&s_cnt[sl], 1u);
-                else
-                    atomicAdd(&cells[key], 1u);
-            }
+        for (int axis = 0; axis < 3; ++axis)
+            for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
+                for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
+                    const uint32_t key = cell_key(x, axis, cu, cv);
+                    const int sl = table_slot(s_key, key);
+                    if (sl >= 0)
+                        atomicAdd(&s_cnt[sl], 1u);
+                    else
+                        atomicAdd(&cells[key], 1u);
+                }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < kGridSlots; k += 256)
+    for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris)
         if (s_key[k])
             atomicAdd(&cells[s_key[k]], s_cnt[k]);
 }
@@ -303,61 +309,64 @@ __global__ __launch_bounds__(256) void k_grid_count(const double *__restrict__ t
 // After the exclusive scan, cells[c + 1] is the start of cell c; the workgroup
 // reserves one range per distinct cell and hands out slots from LDS, so
 // afterwards cell c spans [cells[c], cells[c + 1]).
-__global__ __launch_bounds__(256) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt, GridSet gs,
-                                                   uint32_t *__restrict__ cells, float4 *__restrict__ gbox,
-                                                   uint2 *__restrict__ gref)
+__global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt,
+                                                              GridSet gs, uint32_t *__restrict__ cells,
+                                                              float4 *__restrict__ gbox, uint2 *__restrict__ gref)
 {
     __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
-    const int axis = blockIdx.y;
     const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
     if ((uint64_t)tile * kGridBlockTris >= nt)
         return;
-    for (int k = threadIdx.x; k < kGridSlots; k += 256) {
+    for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris) {
         s_key[k] = 0;
         s_cnt[k] = 0;
     }
     __syncthreads();
-    TriSpan x[kGridBlockTris / 256];
+    const TriSpans x = tri_spans(tri64, nt, gs, tile * kGridBlockTris + threadIdx.x);
+    if (x.valid) {
 #pragma unroll
-    for (int j = 0; j < kGridBlockTris / 256; ++j) {
-        x[j] = tri_span(tri64, nt, gs, axis, tile * kGridBlockTris + j * 256 + threadIdx.x);
-        if (!x[j].valid)
-            continue;
-        for (uint32_t cu = x[j].s.u0; cu <= x[j].s.u1; ++cu)
-            for (uint32_t cv = x[j].s.v0; cv <= x[j].s.v1; ++cv) {
-                const int sl = table_slot(s_key, (uint32_t)(x[j].base + (size_t)cu * x[j].G + cv));
-                if (sl >= 0)
-                    atomicAdd(&s_cnt[sl], 1u);
-            }
+        for (int axis = 0; axis < 3; ++axis)
+            for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
+                for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
+                    const int sl = table_slot(s_key, cell_key(x, axis, cu, cv));
+                    if (sl >= 0)
+                        atomicAdd(&s_cnt[sl], 1u);
+                }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < kGridSlots; k += 256)
+    for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris)
         if (s_key[k])
             s_cnt[k] = atomicAdd(&cells[s_key[k]], s_cnt[k]);  // the range's first slot
     __syncthreads();
+    if (!x.valid)
+        return;
 #pragma unroll
-    for (int j = 0; j < kGridBlockTris / 256; ++j) {
-        if (!x[j].valid)
-            continue;
-        for (uint32_t cu = x[j].s.u0; cu <= x[j].s.u1; ++cu)
-            for (uint32_t cv = x[j].s.v0; cv <= x[j].s.v1; ++cv) {
-                const uint32_t key = (uint32_t)(x[j].base + (size_t)cu * x[j].G + cv);
+    for (int axis = 0; axis < 3; ++axis)
+        for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
+            for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
+                const uint32_t key = cell_key(x, axis, cu, cv);
                 const int sl = table_slot(s_key, key);
                 const uint32_t slot = sl >= 0 ? atomicAdd(&s_cnt[sl], 1u) : atomicAdd(&cells[key], 1u);
                 if (slot < gs.cap) {
-                    gbox[slot] = x[j].s.box;
-                    gref[slot] = make_uint2(x[j].t, __float_as_uint(x[j].s.hiAxis));
+                    gbox[slot] = x.s[axis].box;
+                    gref[slot] = make_uint2(x.t, __float_as_uint(x.s[axis].hiAxis));
                 }
             }
-    }
 }
 
-// candidate range of the ray's cell in target mesh m
+// candidate range of the ray's cell in target mesh m.  A ray whose FP32
+// outward box misses the mesh's FP32 outward root box (which contains every
+// triangle's FP64 box) has no candidate at all: without this, rays from
+// outside the grid's square would be clamped into border cells, which hold
+// every face seen edge-on.
 __device__ __forceinline__ uint2 ray_cell(const GridSet &gs, uint32_t m, const Ray &r, int axis)
 {
+    const float *bx = gs.boxes + 6 * (size_t)m;
+    if (!rbox_overlap(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], r.rf))
+        return make_uint2(0u, 0u);
     const uint2 d = grid_desc(gs, m);
     const uint32_t G = d.x;
-    const GridFrame f = grid_frame(gs.boxes + 6 * (size_t)m, axis, G);
+    const GridFrame f = grid_frame(bx, axis, G);
     const uint32_t cu = grid_cell(comp(r.p, axis_u(axis)), f.lu, f.iu, G);
     const uint32_t cv = grid_cell(comp(r.p, axis_v(axis)), f.lv, f.iv, G);
     const size_t c = (size_t)d.y + (size_t)axis * G * G + (size_t)cu * G + cv;
@@ -535,10 +544,10 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
     launch_fill_u32(cells, 0u, m, s);
     if (nt == 0)
         return;
-    const dim3 grid(xcd_grid((nt + kGridBlockTris - 1) / kGridBlockTris), 3);
-    hipLaunchKernelGGL(k_grid_count, grid, dim3(256), 0, s, tri64, nt, gs, cells);
+    const dim3 grid(xcd_grid((nt + kGridBlockTris - 1) / kGridBlockTris));
+    hipLaunchKernelGGL(k_grid_count, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
-    hipLaunchKernelGGL(k_grid_fill, grid, dim3(256), 0, s, tri64, nt, gs, cells, gbox, gref);
+    hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gbox, gref);
 }
 
 void launch_rays(const GridSet &gs, const double *tri64, const double *pts,

@@ -37,8 +37,7 @@ __device__ __forceinline__ int delta(const uint64_t *__restrict__ codes, int n, 
 
 // Leaves are refit in tiles of kRefitTile sorted positions; a node whose leaf
 // range crosses a tile boundary is a "top" node, finished by k_refit_top.
-constexpr int kRefitTile = 1024;
-constexpr int kRefitBlock = 256;
+constexpr int kRefitTile = 1024;  // leaves per workgroup, one per lane
 
 template <typename K>
 __global__ __launch_bounds__(256) void k_hierarchy(const K *__restrict__ codes,
@@ -117,30 +116,48 @@ __device__ __forceinline__ void write_root(float *rootBox, float4 lo, float4 hi)
     rootBox[5] = hi.z;
 }
 
-// Refit, stage 1: one workgroup per tile of kRefitTile sorted leaves.  Every
-// leaf walks up; at a node whose leaf range lies inside the tile the first of
-// its two children to arrive leaves its box in LDS and stops, the second
-// merges and continues.  All hand-offs stay inside the workgroup (LDS +
-// workgroup-scope fences), so no lane ever waits and no device-wide fence is
-// needed.  At a node whose range leaves the tile the lane stores its box in
-// the node, counts the arrival in topFlags and stops.
-__global__ __launch_bounds__(kRefitBlock) void k_refit_tiles(const uint32_t *__restrict__ ids,
-                                                             const float *__restrict__ box32, int n,
-                                                             BNode *__restrict__ nodes,
-                                                             const uint32_t *__restrict__ leafParent,
-                                                             uint32_t *__restrict__ topFlags, float *rootBox)
+// Refit, stage 1: one workgroup per tile of kRefitTile sorted leaves, one
+// leaf per lane.  Every leaf walks up; at a node whose leaf range lies inside
+// the tile the first of its two children to arrive leaves its box in LDS and
+// stops, the second merges and continues.  All hand-offs stay inside the
+// workgroup (LDS + workgroup-scope fences), so no lane ever waits and no
+// device-wide fence is needed.  At a node whose range leaves the tile (a
+// "top" node) the lane stores its box in the node and stops.
+//
+// The workgroup also writes, for every leaf position, the union of the tile's
+// leaf boxes up to it (prefix) and from it (suffix), and the tile's box.  Box
+// unions are exact (min / max), so stage 2 can assemble any top node's box
+// from these pieces in any order.
+__device__ __forceinline__ void put6(float *dst, const float v[6])
+{
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+        dst[k] = v[k];
+}
+
+__global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__restrict__ ids,
+                                                            const float *__restrict__ box32, int n,
+                                                            BNode *__restrict__ nodes,
+                                                            const uint32_t *__restrict__ leafParent,
+                                                            float *__restrict__ prefix, float *__restrict__ suffix,
+                                                            float *__restrict__ tileBox, float *rootBox)
 {
     __shared__ float s_box[kRefitTile][2][6];
     __shared__ uint32_t s_flag[kRefitTile];
+    __shared__ float s_wave[kRefitTile / 64][6];
     const int tile0 = blockIdx.x * kRefitTile;
     const int tile1 = min(tile0 + kRefitTile, n);
-    for (int k = threadIdx.x; k < kRefitTile; k += kRefitBlock)
-        s_flag[k] = 0;
+    const int pos = tile0 + threadIdx.x;
+    const bool valid = pos < tile1;
+    s_flag[threadIdx.x] = 0;
     __syncthreads();
-    for (int pos = tile0 + threadIdx.x; pos < tile1; pos += kRefitBlock) {
+    float4 lo = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
+    float4 hi = make_float4(-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), 0.0f);
+    if (valid) {
         const uint32_t tri = ids[pos];
-        float4 lo = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * tri)[0];
-        float4 hi = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * tri)[1];
+        lo = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * tri)[0];
+        hi = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * tri)[1];
+        float4 wl = lo, wh = hi;
         int node = (int)leafParent[pos];
         int child = (int)(kLeafBit | tri);
         // each step moves to a strictly larger leaf range, so a walk ends
@@ -149,91 +166,124 @@ __global__ __launch_bounds__(kRefitBlock) void k_refit_tiles(const uint32_t *__r
             BNode *nd = nodes + node;
             const int4 d = nd->d;
             const int slot = (d.x == child) ? 0 : 1;
-            store_child_box(nd, slot, lo, hi);
+            store_child_box(nd, slot, wl, wh);
             const int lo_leaf = min(node, d.w), hi_leaf = max(node, d.w);
-            if (lo_leaf < tile0 || hi_leaf >= tile1 || (lo_leaf / kRefitTile) != (hi_leaf / kRefitTile)) {
-                atomicAdd(topFlags + node, 1u);
+            if (lo_leaf < tile0 || hi_leaf >= tile1 || (lo_leaf / kRefitTile) != (hi_leaf / kRefitTile))
                 break;
-            }
             const int local = node - tile0;
             float *mine = s_box[local][slot];
-            mine[0] = lo.x;
-            mine[1] = lo.y;
-            mine[2] = lo.z;
-            mine[3] = hi.x;
-            mine[4] = hi.y;
-            mine[5] = hi.z;
+            mine[0] = wl.x;
+            mine[1] = wl.y;
+            mine[2] = wl.z;
+            mine[3] = wh.x;
+            mine[4] = wh.y;
+            mine[5] = wh.z;
             __threadfence_block();
             if (atomicAdd(&s_flag[local], 1u) == 0)
                 break;
             __threadfence_block();
             const volatile float *o = s_box[local][1 - slot];
-            lo = make_float4(fminf(lo.x, o[0]), fminf(lo.y, o[1]), fminf(lo.z, o[2]), 0.0f);
-            hi = make_float4(fmaxf(hi.x, o[3]), fmaxf(hi.y, o[4]), fmaxf(hi.z, o[5]), 0.0f);
+            wl = make_float4(fminf(wl.x, o[0]), fminf(wl.y, o[1]), fminf(wl.z, o[2]), 0.0f);
+            wh = make_float4(fmaxf(wh.x, o[3]), fmaxf(wh.y, o[4]), fmaxf(wh.z, o[5]), 0.0f);
             child = node;
             node = d.z;
             if (node < 0) {  // the whole tree fits in this tile
-                write_root(rootBox, lo, hi);
+                write_root(rootBox, wl, wh);
                 break;
             }
         }
     }
+    // prefix / suffix unions over the tile's positions: wave scans, then the
+    // waves' totals
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    float pv[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z}, sv[6] = {lo.x, lo.y, lo.z, hi.x, hi.y, hi.z};
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const float up = __shfl_up(pv[k], off, 64), dn = __shfl_down(sv[k], off, 64);
+            if (lane >= off)
+                pv[k] = k < 3 ? fminf(pv[k], up) : fmaxf(pv[k], up);
+            if (lane + off < 64)
+                sv[k] = k < 3 ? fminf(sv[k], dn) : fmaxf(sv[k], dn);
+        }
+    }
+    if (lane == 63)
+        put6(s_wave[wave], pv);
+    __syncthreads();
+    for (int w = 0; w < kRefitTile / 64; ++w) {
+        if (w < wave)
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                pv[k] = k < 3 ? fminf(pv[k], s_wave[w][k]) : fmaxf(pv[k], s_wave[w][k]);
+        if (w > wave)
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+                sv[k] = k < 3 ? fminf(sv[k], s_wave[w][k]) : fmaxf(sv[k], s_wave[w][k]);
+    }
+    if (valid) {
+        put6(prefix + 6 * (size_t)pos, pv);
+        put6(suffix + 6 * (size_t)pos, sv);
+    }
+    if (threadIdx.x == kRefitTile - 1)
+        put6(tileBox + 6 * (size_t)blockIdx.x, pv);
 }
 
-// Refit, stage 2: one workgroup finishes the top nodes (those spanning tiles)
-// from a ready list.  A top node is ready once both children are complete
-// (topFlags == 2).  Each round merges every ready node's two child boxes into
-// its parent's slot and counts the arrival in the parent's flag; the arrival
-// that brings a parent to 2 appends it to the next round's list.  Every node
-// is merged once, so the work is O(ntop) and the number of rounds is the
-// height of the top tree.  All hand-offs stay inside the workgroup (barriers
-// and workgroup-scope fences); the loop ends when a round's list is empty,
-// which the root's merge guarantees within ntop + 1 rounds.
-__global__ __launch_bounds__(1024) void k_refit_top(BNode *__restrict__ nodes, const uint32_t *__restrict__ topList,
-                                                    const uint32_t *__restrict__ topCount, uint32_t *topFlags,
-                                                    uint32_t *__restrict__ readyA, uint32_t *__restrict__ readyB,
-                                                    float *rootBox)
+// Sparse table over tile boxes: level j holds the union of tiles [t, t + 2^j).
+// One workgroup, a barrier between levels.
+__global__ __launch_bounds__(1024) void k_tile_table(float *__restrict__ table, uint32_t ntiles, int levels)
 {
-    __shared__ uint32_t s_n[2];
-    const uint32_t ntop = *topCount;
-    if (threadIdx.x == 0) {
-        s_n[0] = 0;
-        s_n[1] = 0;
-    }
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < ntop; k += 1024) {
-        const uint32_t p = topList[k];
-        if (topFlags[p] == 2)
-            readyA[atomicAdd(&s_n[0], 1u)] = p;
-    }
-    int cur = 0;
-    for (uint32_t round = 0; round <= ntop; ++round) {
-        __syncthreads();
-        const uint32_t m = s_n[cur];
-        if (m == 0)
-            break;
-        uint32_t *in = cur ? readyB : readyA;
-        uint32_t *out = cur ? readyA : readyB;
-        if (threadIdx.x == 0)
-            s_n[cur ^ 1] = 0;
-        __syncthreads();
-        for (uint32_t k = threadIdx.x; k < m; k += 1024) {
-            const uint32_t p = in[k];
-            const volatile float *b = reinterpret_cast<const volatile float *>(nodes + p);
-            const float4 lo = make_float4(fminf(b[0], b[6]), fminf(b[1], b[7]), fminf(b[2], b[8]), 0.0f);
-            const float4 hi = make_float4(fmaxf(b[3], b[9]), fmaxf(b[4], b[10]), fmaxf(b[5], b[11]), 0.0f);
-            const int parent = nodes[p].d.z;
-            if (parent < 0) {
-                write_root(rootBox, lo, hi);
-                continue;
+    for (int j = 1; j < levels; ++j) {
+        const uint32_t half = 1u << (j - 1);
+        const float *src = table + 6 * (size_t)(j - 1) * ntiles;
+        float *dst = table + 6 * (size_t)j * ntiles;
+        for (uint32_t t = threadIdx.x; t + 2 * half <= ntiles; t += 1024)
+            for (int k = 0; k < 6; ++k) {
+                const float a = src[6 * (size_t)t + k], b = src[6 * (size_t)(t + half) + k];
+                dst[6 * (size_t)t + k] = k < 3 ? fminf(a, b) : fmaxf(a, b);
             }
-            BNode *pn = nodes + parent;
-            store_child_box(pn, pn->d.x == (int)p ? 0 : 1, lo, hi);
-            __threadfence_block();
-            if (atomicAdd(topFlags + parent, 1u) == 1u)
-                out[atomicAdd(&s_n[cur ^ 1], 1u)] = (uint32_t)parent;
+        __syncthreads();
+    }
+}
+
+// Refit, stage 2: every top node at once.  A top node's leaf range [lo, hi]
+// runs from inside tile tl to inside tile th > tl; its box is the suffix of tl
+// at lo, the prefix of th at hi, and (when th > tl + 1) two sparse-table
+// entries covering the whole tiles between.  It goes into the parent's child
+// slot (or the root box), so nothing waits on anything.
+__global__ __launch_bounds__(256) void k_refit_top(BNode *__restrict__ nodes, const uint32_t *__restrict__ topList,
+                                                   const uint32_t *__restrict__ topCount,
+                                                   const float *__restrict__ prefix, const float *__restrict__ suffix,
+                                                   const float *__restrict__ table, uint32_t ntiles, float *rootBox)
+{
+    const uint32_t ntop = *topCount;
+    for (uint32_t k = blockIdx.x * 256 + threadIdx.x; k < ntop; k += gridDim.x * 256) {
+        const uint32_t p = topList[k];
+        const int4 d = nodes[p].d;
+        const uint32_t lo = p < (uint32_t)d.w ? p : (uint32_t)d.w;
+        const uint32_t hi = p < (uint32_t)d.w ? (uint32_t)d.w : p;
+        const uint32_t tl = lo / kRefitTile, th = hi / kRefitTile;
+        float v[6];
+        const float *a = suffix + 6 * (size_t)lo, *b = prefix + 6 * (size_t)hi;
+#pragma unroll
+        for (int c = 0; c < 6; ++c)
+            v[c] = c < 3 ? fminf(a[c], b[c]) : fmaxf(a[c], b[c]);
+        if (th > tl + 1) {
+            const uint32_t first = tl + 1, len = th - 1 - first + 1;
+            const int j = 31 - __clz(len);
+            const float *x = table + 6 * ((size_t)j * ntiles + first);
+            const float *y = table + 6 * ((size_t)j * ntiles + (th - 1) - (1u << j) + 1);
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                v[c] = c < 3 ? fminf(v[c], fminf(x[c], y[c])) : fmaxf(v[c], fmaxf(x[c], y[c]));
         }
-        cur ^= 1;
+        const float4 blo = make_float4(v[0], v[1], v[2], 0.0f), bhi = make_float4(v[3], v[4], v[5], 0.0f);
+        if (d.z < 0) {
+            write_root(rootBox, blo, bhi);
+        } else {
+            BNode *pn = nodes + d.z;
+            store_child_box(pn, pn->d.x == (int)p ? 0 : 1, blo, bhi);
+        }
     }
 }
 
@@ -279,8 +329,7 @@ void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, value, n);
 }
 
-// work: 4n + 1 u32 = top flags [0, n-1), top list [n, 2n-1), top count [2n],
-// two ready lists of the top refit [2n + 1, 3n + 1) and [3n + 1, 4n + 1)
+// work: 2n + 1 u32 = unused [0, n), top list [n, 2n-1), top count [2n]
 void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
                       uint32_t *leafParent, uint32_t *work, hipStream_t s)
 {
@@ -301,15 +350,35 @@ void launch_hierarchy64(const uint64_t *codes, const uint32_t *ids, uint32_t n, 
                        leafParent, work + n, work + 2 * (size_t)n);
 }
 
+static int table_levels(uint32_t ntiles)
+{
+    int L = 1;
+    while ((1u << L) <= ntiles)
+        ++L;
+    return L;
+}
+
+size_t refit_scratch_floats(uint32_t n)
+{
+    const uint32_t ntiles = (n + kRefitTile - 1) / kRefitTile;
+    return 12 * (size_t)n + 6 * (size_t)ntiles * table_levels(ntiles) + 6;
+}
+
+// scratch: refit_scratch_floats(n) floats (prefix, suffix, tile table)
 void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *nodes,
-                  const uint32_t *leafParent, uint32_t *work, float *rootBox, hipStream_t s)
+                  const uint32_t *leafParent, uint32_t *work, float *scratch, float *rootBox, hipStream_t s)
 {
     if (n < 2)
         return;
-    hipLaunchKernelGGL(k_refit_tiles, dim3((n + kRefitTile - 1) / kRefitTile), dim3(kRefitBlock), 0, s, ids, box32,
-                       (int)n, nodes, leafParent, work, rootBox);
-    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(1024), 0, s, nodes, work + n, work + 2 * (size_t)n, work,
-                       work + 2 * (size_t)n + 1, work + 3 * (size_t)n + 1, rootBox);
+    const uint32_t ntiles = (n + kRefitTile - 1) / kRefitTile;
+    float *prefix = scratch, *suffix = scratch + 6 * (size_t)n, *table = scratch + 12 * (size_t)n;
+    hipLaunchKernelGGL(k_refit_tiles, dim3(ntiles), dim3(kRefitTile), 0, s, ids, box32, (int)n, nodes, leafParent,
+                       prefix, suffix, table, rootBox);
+    if (ntiles < 2)
+        return;
+    hipLaunchKernelGGL(k_tile_table, dim3(1), dim3(1024), 0, s, table, ntiles, table_levels(ntiles));
+    hipLaunchKernelGGL(k_refit_top, dim3(256), dim3(256), 0, s, nodes, work + n, work + 2 * (size_t)n, prefix,
+                       suffix, table, ntiles, rootBox);
 }
 
 }  // namespace sbg

@@ -68,7 +68,7 @@ struct sbg_mesh_s {
     DevBuf<double> V;
     DevBuf<uint32_t> T;
     DevBuf<double> tri64, normals;
-    DevBuf<float> tri32, box32, blockBounds, bounds, rootBox;
+    DevBuf<float> tri32, box32, blockBounds, bounds, rootBox, refitScratch;
     DevBuf<uint32_t> keys, vals, keysAlt, valsAlt, leafParent, flags;
     DevBuf<BNode> nodes;
     uint32_t root = 0;
@@ -292,6 +292,7 @@ int sbg_mesh_destroy(sbg_mesh m)
     m->blockBounds.release();
     m->bounds.release();
     m->rootBox.release();
+    m->refitScratch.release();
     m->keys.release();
     m->vals.release();
     m->keysAlt.release();
@@ -341,7 +342,8 @@ int sbg_mesh_prepare(sbg_mesh m)
     TRY(m->valsAlt.ensure(nt));
     TRY(m->leafParent.ensure(nt));
     TRY(m->nodes.ensure(nt > 1 ? nt - 1 : 1));
-    TRY(m->flags.ensure(4 * nt + 1));
+    TRY(m->flags.ensure(2 * nt + 1));
+    TRY(m->refitScratch.ensure(refit_scratch_floats((uint32_t)nt)));
     TRY(c->hist.ensure(sort_hist_entries(nt)));
     m->prepared = false;
     m->gridReady = false;
@@ -377,8 +379,8 @@ int sbg_mesh_prepare(sbg_mesh m)
         } else {
             m->root = 0;
             launch_hierarchy(sk, sv, (uint32_t)nt, m->nodes.p, m->leafParent.p, m->flags.p, c->stream);
-            launch_refit(sv, m->box32.p, (uint32_t)nt, m->nodes.p, m->leafParent.p, m->flags.p, m->rootBox.p,
-                         c->stream);
+            launch_refit(sv, m->box32.p, (uint32_t)nt, m->nodes.p, m->leafParent.p, m->flags.p, m->refitScratch.p,
+                         m->rootBox.p, c->stream);
         }
         TRY(check_launch("bvh"));
     }

@@ -44,7 +44,7 @@ struct sbg_batch_s {
     DevBuf<double> V;
     DevBuf<uint32_t> T, meshOf, target, triBegin;
     DevBuf<double> tri64, normals;
-    DevBuf<float> tri32, box32, blockBounds, rootBox, meshBox;
+    DevBuf<float> tri32, box32, blockBounds, rootBox, meshBox, refitScratch;
     DevBuf<int> meshBounds, waveBounds;
     DevBuf<uint64_t> keys, keysAlt, codes;
     DevBuf<uint32_t> ids, leafParent, work, meshRoot, hist;
@@ -162,7 +162,8 @@ int enqueue(sbg_batch b, int precision)
     const uint64_t *sortedKeys = b->keysInAlt ? b->keysAlt.p : b->keys.p;
     launch_batch_codes(sortedKeys, nt, b->localBits, b->mortonBits, b->triBegin.p, b->codes.p, b->ids.p, s);
     launch_hierarchy64(b->codes.p, b->ids.p, nt, b->nodes.p, b->leafParent.p, b->work.p, s);
-    launch_refit(b->ids.p, b->box32.p, nt, b->nodes.p, b->leafParent.p, b->work.p, b->rootBox.p, s);
+    launch_refit(b->ids.p, b->box32.p, nt, b->nodes.p, b->leafParent.p, b->work.p, b->refitScratch.p, b->rootBox.p,
+                 s);
     launch_fill_u32(b->meshRoot.p, ~0u, b->nmeshes, s);
     launch_fill_u32(reinterpret_cast<uint32_t *>(b->meshBox.p), 0u, 6 * b->nmeshes, s);  // empty meshes: zero box
     launch_mesh_roots(b->nodes.p, b->codes.p, b->ids.p, nt, b->mortonBits, b->box32.p, b->rootBox.p, b->meshRoot.p,
@@ -283,7 +284,8 @@ int sbg_batch_create(sbg_context ctx, const double *xyz, const uint32_t *tris, c
         (rc = b->meshBox.ensure(6 * nmeshes)) || (rc = b->meshBounds.ensure(6 * nmeshes)) ||
         (rc = b->waveBounds.ensure(6 * ((nt + 63) / 64))) ||
         (rc = b->keys.ensure(nt)) || (rc = b->keysAlt.ensure(nt)) || (rc = b->codes.ensure(nt)) ||
-        (rc = b->ids.ensure(nt)) || (rc = b->leafParent.ensure(nt)) || (rc = b->work.ensure(4 * nt + 1)) ||
+        (rc = b->ids.ensure(nt)) || (rc = b->leafParent.ensure(nt)) || (rc = b->work.ensure(2 * nt + 1)) ||
+        (rc = b->refitScratch.ensure(refit_scratch_floats((uint32_t)nt))) ||
         (rc = b->meshRoot.ensure(nmeshes)) || (rc = b->nodes.ensure(nt > 1 ? nt - 1 : 1)) ||
         (rc = b->words.ensure(kWords)) || (rc = b->flags.ensure(nt + 4)) || (rc = b->bits.ensure(3 * nt + 4)) ||
         (rc = b->votes.ensure(nt)) || (rc = b->pts.ensure(3 * nt)) || (rc = b->rayList.ensure(3 * nt)))
@@ -590,7 +592,7 @@ int sbg_batch_destroy(sbg_batch b)
     DevBuf<double> *d64[] = {&b->V, &b->tri64, &b->normals, &b->seg, &b->pts};
     for (auto *p : d64)
         p->release();
-    DevBuf<float> *f32[] = {&b->tri32, &b->box32, &b->blockBounds, &b->rootBox, &b->meshBox};
+    DevBuf<float> *f32[] = {&b->tri32, &b->box32, &b->blockBounds, &b->rootBox, &b->meshBox, &b->refitScratch};
     for (auto *p : f32)
         p->release();
     DevBuf<uint32_t> *u32[] = {&b->T,    &b->meshOf,     &b->target,   &b->triBegin, &b->ids,

@@ -58,14 +58,16 @@ int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRa
                    uint32_t *hist, hipStream_t s);
 
 // k_lbvh.hip
-// work: 4n + 1 u32 of scratch shared by the two (top-node flags, list, count,
-// the top refit's two ready lists)
+// work: 2n + 1 u32 of scratch shared by the two (top-node list and count);
+// refit scratch: refit_scratch_floats(n) floats (per-tile prefix / suffix box
+// unions and the tile sparse table)
+size_t refit_scratch_floats(uint32_t n);
 void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
                       uint32_t *leafParent, uint32_t *work, hipStream_t s);
 void launch_hierarchy64(const uint64_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
                         uint32_t *leafParent, uint32_t *work, hipStream_t s);
 void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *nodes,
-                  const uint32_t *leafParent, uint32_t *work, float *rootBox, hipStream_t s);
+                  const uint32_t *leafParent, uint32_t *work, float *scratch, float *rootBox, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s);
 void launch_copy16(const void *src, void *dst, size_t n16, hipStream_t s);
 void launch_copy_u32(const uint32_t *src, uint32_t *dst, size_t n, hipStream_t s);

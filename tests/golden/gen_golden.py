@@ -237,10 +237,70 @@ def earcut_vectors(L: RefLib, n: int = 1500) -> dict:
                 index_counts=np.array([len(i) for i in idx]), indices=np.concatenate(idx).astype(np.int64))
 
 
+_FORK = {}
+
+
+def _fork_votes(args):
+    which, lo, hi = args
+    target, pts = _FORK[which]
+    return votes(target, pts[lo:hi])[1]
+
+
+def parallel_votes(target, pts: np.ndarray, procs: int = 8) -> np.ndarray:
+    """votes() split over forked workers that share `target` (the reference's
+    prepared mesh) copy-on-write."""
+    import multiprocessing as mp
+
+    key = id(target)
+    _FORK[key] = (target, pts)
+    chunks = [(key, lo, min(lo + 65536, len(pts))) for lo in range(0, len(pts), 65536)]
+    with mp.get_context("fork").Pool(procs) as pool:
+        parts = pool.map(_fork_votes, chunks)
+    del _FORK[key]
+    return np.concatenate(parts) if parts else np.zeros(0, np.uint8)
+
+
+def stress_record(L: RefLib, a: mg.Mesh, b: mg.Mesh, label: str):
+    """Digests of the candidate set, the intersecting pairs with their
+    segments, and the per-triangle votes of both meshes; no combine()."""
+    t0 = time.time()
+    ra = L.mesh(a.vertices, a.triangles)
+    rb = L.mesh(b.vertices, b.triangles)
+    pairs, _ = ra.find_pairs(rb)
+    status, _, seg, _ = ra.tri_tri_pairs(rb, pairs)
+    hit = np.nonzero(status == 1)[0]
+    rec = dict(label=np.array(label), digest_a=np.array(mg.mesh_digest(a)), digest_b=np.array(mg.mesh_digest(b)),
+               n_pairs=np.array(len(pairs)), pair_digest=np.array(pair_digest(pairs)),
+               hits_digest=np.array(hits_digest(pairs[hit], seg[hit])), n_hits=np.array(len(hit)),
+               n_coplanar=np.array(int((status == 2).sum())),
+               status_digest=np.array(sha(status[np.lexsort((pairs[:, 1], pairs[:, 0]))].astype(np.uint8))))
+    for name, m, other, side in (("a", a, rb, 0), ("b", b, ra, 1)):
+        inter = np.zeros(m.nt, bool)
+        inter[pairs[hit, side]] = True
+        c = np.full(m.nt, 2, np.uint8)
+        keep = np.nonzero(~inter)[0]
+        c[keep] = parallel_votes(other, centroids(m.vertices, m.triangles)[keep])
+        rec[f"cls_digest_{name}"] = np.array(sha(c))
+        rec[f"cls_count_{name}"] = np.bincount(c, minlength=3)
+    print(f"  {label}: {len(pairs)} pairs, {len(hit)} hits, {int(rec['n_coplanar'])} coplanar, "
+          f"{time.time() - t0:.1f}s", flush=True)
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--small", action="store_true", help="skip the million-triangle cases")
+    ap.add_argument("--stress", action="store_true", help="only the torus / cube stress digests")
     args = ap.parse_args()
+    if args.stress:
+        L = RefLib()
+        print("torus (4,194,304 triangles) vs subdivision-7 sphere r=0.8 at (1, 0, 0), noise 1e-4")
+        a, b = mg.torus_sphere_pair(1)
+        np.savez_compressed(os.path.join(HERE, "stress_torus.npz"), **stress_record(L, a, b, "stress_torus"))
+        print("cube grids 512x512 per face (3,145,728 triangles) vs the same shifted by (1e-6, 0.5, 0.5)")
+        a, b = mg.cube_pair(512)
+        np.savez_compressed(os.path.join(HERE, "stress_cube.npz"), **stress_record(L, a, b, "stress_cube"))
+        return
     L = RefLib()
     out = HERE
     print("GD vectors")

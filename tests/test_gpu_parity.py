@@ -176,7 +176,7 @@ def _check_digests(gpu, g, a, b):
     assert np.array_equal(np.bincount(ca, minlength=3), g["cls_count_a"])
     assert np.array_equal(np.bincount(cb, minlength=3), g["cls_count_b"])
     assert sha(ca) == str(g["cls_digest_a"]) and sha(cb) == str(g["cls_digest_b"])
-    return P
+    return P, st  # st: statistics of the FP32-filtered run
 
 
 @pytest.mark.skipif(not golden_exists("large_s1"), reason="large fixture not generated")
@@ -186,6 +186,21 @@ def test_large_pair_digests(gpu):
     g = load_golden("large_s1")
     a, b = mg.large_pair(1)
     _check_digests(gpu, g, a, b)
+
+
+@pytest.mark.parametrize("name", ["stress_torus", "stress_cube"])
+def test_stress_digests(gpu, name):
+    """Filter stress cases: a 4.19M-triangle torus against a noisy
+    subdivision-7 sphere, and two 512x512-per-face cube grids offset by
+    (1e-6, 0.5, 0.5) with near-coplanar faces and exact-zero orientations:
+    pair set, statuses and segments in both precisions, and every triangle's
+    classification against the reference's digests."""
+    g = load_golden(name)
+    a, b = mg.torus_sphere_pair(1) if name == "stress_torus" else mg.cube_pair(512)
+    assert mg.mesh_digest(a) == str(g["digest_a"]) and mg.mesh_digest(b) == str(g["digest_b"])
+    _, st = _check_digests(gpu, g, a, b)
+    assert st["hits"] == int(g["n_hits"])
+    assert st["fallback_pairs"] + st["filtered_pairs"] <= st["tested"]
 
 
 # ---------------------------------------------------------------------------
