@@ -188,6 +188,57 @@ def fp64_vs_batched(ctx, a, b, step, N, steps: int, graph: bool = True) -> dict:
     return out
 
 
+def dropin_section(mg, steps: int) -> dict:
+    """The whole boolean where the reference's combine() succeeds: the
+    subdivision-6 pair (81,920 triangles each, seed 1, offset (0.5, 0.2, 0.1))
+    through the drop-in SolidMesh / SolidBoolean (GPU hot path + host weld,
+    retriangulation, face groups), next to the compiled reference's own
+    combine() on this host (one core) when it is available."""
+    from solidboolean_amd import dropin
+
+    a, b = mg.icosphere_pair(6, 1, (0.5, 0.2, 0.1))
+
+    def run():
+        A, B = dropin.SolidMesh(a.vertices, a.triangles), dropin.SolidMesh(b.vertices, b.triangles)
+        A.prepare()
+        B.prepare()
+        sb = dropin.SolidBoolean(A, B)
+        ok = sb.combine()
+        return ok, sb, (A, B)
+
+    run()
+    times = []
+    for _ in range(max(3, steps // 2)):
+        t0 = time.perf_counter()
+        ok, sb, keep = run()
+        times.append(time.perf_counter() - t0)
+    counts = [len(sb.fetch_union()), len(sb.fetch_diff()), len(sb.fetch_intersect())]
+    out = {"workload": "subdivision-6 icosphere pair (81,920 triangles each), seed 1, offset (0.5, 0.2, 0.1)",
+           "combine_ok": ok, "face_counts": counts, "ms": float(np.median(times)) * 1e3,
+           "stage_seconds": sb.stage_seconds()}
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    try:
+        import reflib
+
+        if reflib.available():
+            L = reflib.RefLib()
+            rt = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                ra, rb = L.mesh(a.vertices, a.triangles), L.mesh(b.vertices, b.triangles)
+                bo = reflib.RefBoolean(ra, rb)
+                rt.append(time.perf_counter() - t0)
+            out["reference_cpu_ms"] = float(np.median(rt)) * 1e3
+            out["reference_face_counts"] = [len(bo.fetch(i)) for i in range(3)]
+            out["reference_combine_ok"] = bo.ok
+            out["speedup_vs_reference"] = out["reference_cpu_ms"] / out["ms"]
+    except ImportError:
+        pass
+    log(f"drop-in combine (sub-6 pair): {out['ms']:.1f} ms, faces {counts}"
+        + (f"; reference {out['reference_cpu_ms']:.1f} ms" if "reference_cpu_ms" in out else ""))
+    return out
+
+
 def stress_section(ctx, mg, N, steps: int) -> dict:
     """The filter stress cases: per case the candidate count, hits, coplanar
     pairs, the FP32 filter's fallback rate, both GD precisions per launch, and
@@ -446,6 +497,7 @@ def main():
     }
     if not args.no_stress:
         line["stress"] = stress_section(ctx, mg, N, args.steps)
+        line["dropin_combine"] = dropin_section(mg, args.steps)
     if not args.no_batch:
         line["fp64_vs_filtered_batched"] = fp64_vs_batched(ctx, a, b, step, N, args.steps, not args.no_graph)
         line["batch"] = batch_section(ctx, mg, N, args.steps, not args.no_graph)

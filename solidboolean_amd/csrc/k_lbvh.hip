@@ -230,9 +230,43 @@ __global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__re
 }
 
 // Sparse table over tile boxes: level j holds the union of tiles [t, t + 2^j).
-// One workgroup, a barrier between levels.
+// One workgroup.  Up to kLdsTiles tiles the levels are built in place in LDS
+// (read, barrier, write, barrier) and copied out as they complete; beyond that
+// each level is a pass over global memory.
+constexpr uint32_t kLdsTiles = 2048;
+
 __global__ __launch_bounds__(1024) void k_tile_table(float *__restrict__ table, uint32_t ntiles, int levels)
 {
+    __shared__ float s_t[6][kLdsTiles];
+    if (ntiles <= kLdsTiles) {
+        for (uint32_t t = threadIdx.x; t < ntiles; t += 1024)
+            for (int k = 0; k < 6; ++k)
+                s_t[k][t] = table[6 * (size_t)t + k];
+        __syncthreads();
+        for (int j = 1; j < levels; ++j) {
+            const uint32_t half = 1u << (j - 1);
+            float v[2][6];
+            for (int r = 0; r < 2; ++r) {
+                const uint32_t t = threadIdx.x + 1024u * r;
+                if (t + 2 * half <= ntiles)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k)
+                        v[r][k] = k < 3 ? fminf(s_t[k][t], s_t[k][t + half]) : fmaxf(s_t[k][t], s_t[k][t + half]);
+            }
+            __syncthreads();
+            for (int r = 0; r < 2; ++r) {
+                const uint32_t t = threadIdx.x + 1024u * r;
+                if (t + 2 * half <= ntiles)
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        s_t[k][t] = v[r][k];
+                        table[6 * ((size_t)j * ntiles + t) + k] = v[r][k];
+                    }
+            }
+            __syncthreads();
+        }
+        return;
+    }
     for (int j = 1; j < levels; ++j) {
         const uint32_t half = 1u << (j - 1);
         const float *src = table + 6 * (size_t)(j - 1) * ntiles;
