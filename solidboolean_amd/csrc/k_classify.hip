@@ -34,16 +34,6 @@ constexpr int kFastKeys = 4;     // distinct keys a fast lane keeps
 constexpr int kFastCands = 48;   // cell entries a fast lane scans alone
 constexpr int kWaveHits = 256;   // hits a wave gathers for one deferred ray
 
-struct RBox {
-    float lx, ly, lz, hx, hy, hz;
-};
-
-__device__ __forceinline__ bool rbox_overlap(float alx, float aly, float alz, float ahx, float ahy, float ahz,
-                                             const RBox &b)
-{
-    return alx <= b.hx && ahx >= b.lx && aly <= b.hy && ahy >= b.ly && alz <= b.hz && ahz >= b.lz;
-}
-
 __global__ __launch_bounds__(256) void k_mark_intersected(const uint64_t *__restrict__ pairs,
                                                           const uint8_t *__restrict__ status, uint32_t n, int side,
                                                           uint8_t *__restrict__ flags)
@@ -68,10 +58,11 @@ __global__ __launch_bounds__(256) void k_centroids(const double *__restrict__ tr
     pts[3 * (size_t)t + 2] = (r[2] + r[6] + r[10]) / 3.0;
 }
 
+// A ray is the segment from p to e = p + dir (src/solidboolean.cpp:60-66):
+// dir is DBL_MAX along the axis and DBL_EPSILON along the other two, so
+// e >= p in every component and the segment's box (:67-69) is exactly [p, e].
 struct Ray {
-    V3 p, u;
-    Box64 rb;
-    RBox rf;
+    V3 p, e;
 };
 
 __device__ __forceinline__ Ray ray_setup(const double *__restrict__ pts, uint32_t i, uint32_t axis)
@@ -80,29 +71,38 @@ __device__ __forceinline__ Ray ray_setup(const double *__restrict__ pts, uint32_
     Ray r;
     r.p = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
     const V3 dir = {axis == 0 ? big : eps, axis == 1 ? big : eps, axis == 2 ? big : eps};
-    const V3 e = {r.p.x + dir.x, r.p.y + dir.y, r.p.z + dir.z};
-    r.rb = {{big, big, big}, {-big, -big, -big}};
-    box_update(r.rb, r.p);
-    box_update(r.rb, e);
-    r.rf = {f_down(r.rb.mn.x), f_down(r.rb.mn.y), f_down(r.rb.mn.z),
-            f_up(r.rb.mx.x),   f_up(r.rb.mx.y),   f_up(r.rb.mx.z)};
-    r.u = vsub(e, r.p);
+    r.e = {r.p.x + dir.x, r.p.y + dir.y, r.p.z + dir.z};
     return r;
 }
 
-// Distinct PositionKeys of one ray, CAP of them in registers.  New keys enter
-// at slot 0 and shift the others up, so every slot index is a compile-time
-// constant and the set never lands in scratch memory.
+// Distinct PositionKeys of one ray, CAP of them in registers.  Along an axis
+// ray the hit points share (nearly always) the key's two off-axis components,
+// so only the on-axis component is kept per key; a key whose off-axis
+// components differ from the first key's, or one more than CAP, sets `full`
+// and the ray is finished by the wave path, which compares whole keys.  New
+// keys enter at slot 0 and shift the others up, so every slot index is a
+// compile-time constant and the set never lands in scratch memory.
 template <int CAP>
-struct KeySet {
-    Key3 k[CAP];
+struct AxisKeys {
+    long long a[CAP];
+    long long bu, bv;
     int n = 0;
     bool full = false;
-    __device__ __forceinline__ void add(const Key3 &key)
+    __device__ __forceinline__ void add(const Key3 &key, int axis)
     {
+        const long long ka = axis == 0 ? key.x : (axis == 1 ? key.y : key.z);
+        const long long ku = axis == 0 ? key.y : key.x;
+        const long long kv = axis == 2 ? key.y : key.z;
+        if (n == 0) {
+            bu = ku;
+            bv = kv;
+        } else if (ku != bu || kv != bv) {
+            full = true;
+            return;
+        }
 #pragma unroll
         for (int j = 0; j < CAP; ++j)
-            if (j < n && k[j].x == key.x && k[j].y == key.y && k[j].z == key.z)
+            if (j < n && a[j] == ka)
                 return;
         if (n == CAP) {
             full = true;
@@ -110,8 +110,8 @@ struct KeySet {
         }
 #pragma unroll
         for (int j = CAP - 1; j > 0; --j)
-            k[j] = k[j - 1];
-        k[0] = key;
+            a[j] = a[j - 1];
+        a[0] = ka;
         ++n;
     }
 };
@@ -123,18 +123,19 @@ __device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ t
     const double2 *x = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * t);
     const double2 r0 = x[0], r1 = x[1], r2 = x[2], r3 = x[3], r4 = x[4], r5 = x[5];
     const V3 v0 = {r0.x, r0.y, r1.x}, v1 = {r2.x, r2.y, r3.x}, v2 = {r4.x, r4.y, r5.x};
-    if (!box64_overlap(tri_box64(v0, v1, v2), r.rb))
+    if (!box64_overlap(tri_box64(v0, v1, v2), Box64{r.p, r.e}))
         return false;
+    const V3 u = vsub(r.e, r.p);
     const V3 n = {r1.y, r3.y, r5.y};  // the record's pad slots hold the FP64 normal
     const V3 w = vsub(r.p, v0);
-    const double d = n.x * r.u.x + n.y * r.u.y + n.z * r.u.z;
+    const double d = n.x * u.x + n.y * u.y + n.z * u.z;
     const double num = (-n.x) * w.x + (-n.y) * w.y + (-n.z) * w.z;
     if (__builtin_fabs(d) <= 2.220446049250313080847e-16)
         return false;
     const double s = num / d;
     if (s < 0 || s > 1 || __builtin_isnan(s) || __builtin_isinf(s))
         return false;
-    const V3 hit = {r.p.x + s * r.u.x, r.p.y + s * r.u.y, r.p.z + s * r.u.z};
+    const V3 hit = {r.p.x + s * u.x, r.p.y + s * u.y, r.p.z + s * u.z};
     const V3 n0 = tri_normal(hit, v0, v1);
     const V3 n1 = tri_normal(hit, v1, v2);
     const V3 n2 = tri_normal(hit, v2, v0);
@@ -150,17 +151,27 @@ __device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ t
 // x, so a triangle filed under every cell its slack-widened (u, v) box spans
 // is filed under the cell of every ray origin whose segment box it can
 // overlap: the segment covers [p.u, p.u + eps] and the slack
-// (1e-15 (1 + |lo|)) exceeds eps plus the rounding of lo - slack.  Each entry
-// holds the triangle's FP32 outward-rounded (u, v) box and its upper bound
-// along `axis`, so most of a cell's entries are rejected from 24 B without
-// touching the FP64 record.  Several meshes (a batch) share one cell array:
-// mesh m owns 3 G_m^2 cells from its first cell on.
+// (1e-15 (1 + |lo|)) exceeds eps plus the rounding of lo - slack.  Several
+// meshes (a batch) share one cell array: mesh m owns 3 G_m^2 cells from its
+// first cell on.
+//
+// An entry is 8 bytes of 12-bit codes plus the triangle index.  The codes are
+// the triangle's FP64 box bounds lo_u, hi_u, lo_v, hi_v relative to the
+// entry's cell (in 1/4096 of a cell, clamped to the cell) and hi_a relative to
+// the mesh box along the axis (in 1/4096 of its extent).  Every code is one
+// monotone function of its coordinate, and a ray's own codes come from the
+// same function in the same cell, so lo_u <= e.u implies code(lo_u) <=
+// code(e.u) and so on: the code test passes every entry whose FP64 box
+// overlaps the ray's segment box, and most others fail it from 8 bytes.
 struct GridFrame {
-    double lu, lv, iu, iv;
+    double lu, lv, la, iu, iv, ia;
 };
 
 __device__ __forceinline__ int axis_u(int axis) { return axis == 0 ? 1 : 0; }
 __device__ __forceinline__ int axis_v(int axis) { return axis == 2 ? 1 : 2; }
+
+constexpr double kCodeSteps = 4096.0;
+constexpr uint32_t kCodeMax = 4095u;
 
 __device__ __forceinline__ GridFrame grid_frame(const float *__restrict__ box, int axis, uint32_t G)
 {
@@ -168,18 +179,46 @@ __device__ __forceinline__ GridFrame grid_frame(const float *__restrict__ box, i
     GridFrame f;
     f.lu = (double)box[u];
     f.lv = (double)box[v];
+    f.la = (double)box[axis];
     const double su = (double)box[3 + u] - f.lu, sv = (double)box[3 + v] - f.lv;
+    const double sa = (double)box[3 + axis] - f.la;
     f.iu = su > 0.0 ? (double)G / su : 0.0;
     f.iv = sv > 0.0 ? (double)G / sv : 0.0;
+    f.ia = sa > 0.0 ? kCodeSteps / sa : 0.0;
     return f;
+}
+
+// x in cell units from lo (inv >= 0, so monotone in x)
+__device__ __forceinline__ double cell_coord(double x, double lo, double inv) { return (x - lo) * inv; }
+
+__device__ __forceinline__ uint32_t cell_of(double t, uint32_t G)
+{
+    if (!(t >= 0.0))
+        return 0;
+    return t < (double)G ? (uint32_t)t : G - 1u;
 }
 
 __device__ __forceinline__ uint32_t grid_cell(double x, double lo, double inv, uint32_t G)
 {
-    const double t = (x - lo) * inv;
-    if (!(t >= 0.0))
+    return cell_of(cell_coord(x, lo, inv), G);
+}
+
+// monotone 12-bit code of y (NaN and negatives give 0)
+__device__ __forceinline__ uint32_t code12(double y)
+{
+    if (!(y >= 0.0))
         return 0;
-    return t < (double)G ? (uint32_t)t : G - 1u;
+    return y < (double)kCodeMax ? (uint32_t)y : kCodeMax;
+}
+
+// code of cell coordinate t inside cell c
+__device__ __forceinline__ uint32_t cell_code(double t, uint32_t c) { return code12((t - (double)c) * kCodeSteps); }
+
+// entry layout: x = lo_u | hi_u << 12 | (hi_a & 255) << 24,
+//               y = lo_v | hi_v << 12 | (hi_a >> 8) << 24
+__device__ __forceinline__ uint2 pack_code(uint32_t lu, uint32_t hu, uint32_t lv, uint32_t hv, uint32_t ha)
+{
+    return make_uint2(lu | hu << 12 | (ha & 255u) << 24, lv | hv << 12 | (ha >> 8) << 24);
 }
 
 __device__ __forceinline__ double comp(const V3 &a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
@@ -191,9 +230,9 @@ __device__ __forceinline__ uint2 grid_desc(const GridSet &gs, uint32_t m)
 }
 
 struct GridSpan {
-    uint32_t u0, u1, v0, v1;
-    float4 box;   // FP32 (lo_u, hi_u, lo_v, hi_v), outward
-    float hiAxis; // FP32 upper bound of the box along the ray axis
+    uint32_t u0, u1, v0, v1;  // cells spanned (slack-widened low ends)
+    double lu, hu, lv, hv;    // the FP64 box bounds in cell coordinates
+    uint32_t ha;              // code of the box's upper bound along the axis
 };
 
 __device__ __forceinline__ GridSpan grid_span(const Box64 &b, int axis, const GridFrame &f, uint32_t G)
@@ -202,12 +241,21 @@ __device__ __forceinline__ GridSpan grid_span(const Box64 &b, int axis, const Gr
     const double lu = comp(b.mn, u), hu = comp(b.mx, u), lv = comp(b.mn, v), hv = comp(b.mx, v);
     GridSpan s;
     s.u0 = grid_cell(lu - 1e-15 * (1.0 + __builtin_fabs(lu)), f.lu, f.iu, G);
-    s.u1 = grid_cell(hu, f.lu, f.iu, G);
     s.v0 = grid_cell(lv - 1e-15 * (1.0 + __builtin_fabs(lv)), f.lv, f.iv, G);
-    s.v1 = grid_cell(hv, f.lv, f.iv, G);
-    s.box = make_float4(f_down(lu), f_up(hu), f_down(lv), f_up(hv));
-    s.hiAxis = f_up(comp(b.mx, axis));
+    s.lu = cell_coord(lu, f.lu, f.iu);
+    s.hu = cell_coord(hu, f.lu, f.iu);
+    s.lv = cell_coord(lv, f.lv, f.iv);
+    s.hv = cell_coord(hv, f.lv, f.iv);
+    s.u1 = cell_of(s.hu, G);
+    s.v1 = cell_of(s.hv, G);
+    s.ha = code12(cell_coord(comp(b.mx, axis), f.la, f.ia));
     return s;
+}
+
+// the entry of span s in cell (cu, cv)
+__device__ __forceinline__ uint2 span_code(const GridSpan &s, uint32_t cu, uint32_t cv)
+{
+    return pack_code(cell_code(s.lu, cu), cell_code(s.hu, cu), cell_code(s.lv, cv), cell_code(s.hv, cv), s.ha);
 }
 
 // Grid build: a workgroup takes kGridBlockTris consecutive triangles of the
@@ -311,7 +359,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const double *__r
 // afterwards cell c spans [cells[c], cells[c + 1]).
 __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt,
                                                               GridSet gs, uint32_t *__restrict__ cells,
-                                                              float4 *__restrict__ gbox, uint2 *__restrict__ gref)
+                                                              uint2 *__restrict__ gcode, uint32_t *__restrict__ gtri)
 {
     __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
     const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
@@ -348,45 +396,58 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__re
                 const int sl = table_slot(s_key, key);
                 const uint32_t slot = sl >= 0 ? atomicAdd(&s_cnt[sl], 1u) : atomicAdd(&cells[key], 1u);
                 if (slot < gs.cap) {
-                    gbox[slot] = x.s[axis].box;
-                    gref[slot] = make_uint2(x.t, __float_as_uint(x.s[axis].hiAxis));
+                    gcode[slot] = span_code(x.s[axis], cu, cv);
+                    gtri[slot] = x.t;
                 }
             }
 }
 
-// candidate range of the ray's cell in target mesh m.  A ray whose FP32
-// outward box misses the mesh's FP32 outward root box (which contains every
-// triangle's FP64 box) has no candidate at all: without this, rays from
-// outside the grid's square would be clamped into border cells, which hold
-// every face seen edge-on.
-__device__ __forceinline__ uint2 ray_cell(const GridSet &gs, uint32_t m, const Ray &r, int axis)
+// A ray's cell in its target mesh: the candidate range and the ray's codes
+// (lo ends from p, hi ends from e) in that cell.  A ray whose FP32 outward box
+// misses the mesh's FP32 outward root box (which contains every triangle's
+// FP64 box) has no candidate at all: without this, rays from outside the
+// grid's square would be clamped into border cells, which hold every face
+// seen edge-on.
+struct RayCell {
+    uint2 range;
+    uint32_t pu, eu, pv, ev, pa;
+};
+
+__device__ __forceinline__ RayCell ray_cell(const GridSet &gs, uint32_t m, const Ray &r, int axis)
 {
+    RayCell c;
+    c.range = make_uint2(0u, 0u);
     const float *bx = gs.boxes + 6 * (size_t)m;
-    if (!rbox_overlap(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5], r.rf))
-        return make_uint2(0u, 0u);
+    if (!(bx[0] <= f_up(r.e.x) && bx[3] >= f_down(r.p.x) && bx[1] <= f_up(r.e.y) && bx[4] >= f_down(r.p.y) &&
+          bx[2] <= f_up(r.e.z) && bx[5] >= f_down(r.p.z)))
+        return c;
     const uint2 d = grid_desc(gs, m);
     const uint32_t G = d.x;
     const GridFrame f = grid_frame(bx, axis, G);
-    const uint32_t cu = grid_cell(comp(r.p, axis_u(axis)), f.lu, f.iu, G);
-    const uint32_t cv = grid_cell(comp(r.p, axis_v(axis)), f.lv, f.iv, G);
-    const size_t c = (size_t)d.y + (size_t)axis * G * G + (size_t)cu * G + cv;
-    return make_uint2(gs.cells[c], gs.cells[c + 1]);
+    const int u = axis_u(axis), v = axis_v(axis);
+    const double tu = cell_coord(comp(r.p, u), f.lu, f.iu), tv = cell_coord(comp(r.p, v), f.lv, f.iv);
+    const uint32_t cu = cell_of(tu, G), cv = cell_of(tv, G);
+    c.pu = cell_code(tu, cu);
+    c.eu = cell_code(cell_coord(comp(r.e, u), f.lu, f.iu), cu);
+    c.pv = cell_code(tv, cv);
+    c.ev = cell_code(cell_coord(comp(r.e, v), f.lv, f.iv), cv);
+    c.pa = code12(cell_coord(comp(r.p, axis), f.la, f.ia));
+    const size_t k = (size_t)d.y + (size_t)axis * G * G + (size_t)cu * G + cv;
+    c.range = make_uint2(gs.cells[k], gs.cells[k + 1]);
+    return c;
 }
 
-// FP32 pre-test of grid entry k against the ray (a superset of the FP64 box
+// code test of grid entry k against the ray (a superset of the FP64 box
 // test); on success `t` is the triangle
-__device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, const Ray &r, int axis, uint32_t &t)
+__device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, const RayCell &c, uint32_t &t)
 {
-    const int u = axis_u(axis), v = axis_v(axis);
-    const float rlu = u == 0 ? r.rf.lx : r.rf.ly, rhu = u == 0 ? r.rf.hx : r.rf.hy;
-    const float rlv = v == 1 ? r.rf.ly : r.rf.lz, rhv = v == 1 ? r.rf.hy : r.rf.hz;
-    const float rla = axis == 0 ? r.rf.lx : (axis == 1 ? r.rf.ly : r.rf.lz);
-    const float4 e = gs.gbox[k];
-    if (!(e.x <= rhu && e.y >= rlu && e.z <= rhv && e.w >= rlv))
+    const uint2 e = gs.gcode[k];
+    const uint32_t ha = e.x >> 24 | (e.y >> 24) << 8;
+    if (!((e.x & kCodeMax) <= c.eu && ((e.x >> 12) & kCodeMax) >= c.pu && (e.y & kCodeMax) <= c.ev &&
+          ((e.y >> 12) & kCodeMax) >= c.pv && ha >= c.pa))
         return false;
-    const uint2 ref = gs.gref[k];
-    t = ref.x;
-    return __uint_as_float(ref.y) >= rla;
+    t = gs.gtri[k];
+    return true;
 }
 
 // One lane per (axis, query).  A lane whose cell holds at most kFastCands
@@ -419,14 +480,14 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
         return;
     }
     const Ray r = ray_setup(pts, i, axis);
-    const uint2 range = ray_cell(gs, m, r, (int)axis);
-    bool defer = range.y - range.x > (uint32_t)kFastCands;
-    KeySet<kFastKeys> keys;
-    for (uint32_t k = range.x; !defer && k < range.y; ++k) {
+    const RayCell c = ray_cell(gs, m, r, (int)axis);
+    bool defer = c.range.y - c.range.x > (uint32_t)kFastCands;
+    AxisKeys<kFastKeys> keys;
+    for (uint32_t k = c.range.x; !defer && k < c.range.y; ++k) {
         uint32_t t;
         Key3 key;
-        if (entry_may_hit(gs, k, r, (int)axis, t) && ray_hit(t, tri64, r, key)) {
-            keys.add(key);
+        if (entry_may_hit(gs, k, c, t) && ray_hit(t, tri64, r, key)) {
+            keys.add(key, (int)axis);
             defer = keys.full;
         }
     }
@@ -458,13 +519,13 @@ __global__ __launch_bounds__(64) void k_rays_wave(GridSet gs, const double *__re
         const uint32_t i = order ? order[q] : q;
         const uint32_t m = gs.target ? gs.target[i] : 0u;
         const Ray r = ray_setup(pts, i, axis);
-        const uint2 range = ray_cell(gs, m, r, (int)axis);
+        const RayCell c = ray_cell(gs, m, r, (int)axis);
         uint32_t nh = 0;
-        for (uint32_t k0 = range.x; k0 < range.y; k0 += 64) {
+        for (uint32_t k0 = c.range.x; k0 < c.range.y; k0 += 64) {
             const uint32_t e = k0 + lane;
             uint32_t t;
             Key3 key;
-            const bool hit = e < range.y && entry_may_hit(gs, e, r, (int)axis, t) && ray_hit(t, tri64, r, key);
+            const bool hit = e < c.range.y && entry_may_hit(gs, e, c, t) && ray_hit(t, tri64, r, key);
             const unsigned long long mask = __ballot(hit);
             const uint32_t pos = nh + (uint32_t)__popcll(mask & below);
             if (hit && pos < (uint32_t)kWaveHits) {
@@ -538,7 +599,7 @@ size_t grid_cells(uint32_t G) { return 3 * (size_t)G * G + 1; }
 
 
 void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
-                       float4 *gbox, uint2 *gref, hipStream_t s)
+                       uint2 *gcode, uint32_t *gtri, hipStream_t s)
 {
     const size_t m = (size_t)gs.ncells + 1;
     launch_fill_u32(cells, 0u, m, s);
@@ -547,7 +608,7 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
     const dim3 grid(xcd_grid((nt + kGridBlockTris - 1) / kGridBlockTris));
     hipLaunchKernelGGL(k_grid_count, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
-    hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gbox, gref);
+    hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gcode, gtri);
 }
 
 void launch_rays(const GridSet &gs, const double *tri64, const double *pts,

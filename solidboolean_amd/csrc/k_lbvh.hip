@@ -233,7 +233,8 @@ __global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__re
 // One workgroup.  Up to kLdsTiles tiles the levels are built in place in LDS
 // (read, barrier, write, barrier) and copied out as they complete; beyond that
 // each level is a pass over global memory.
-constexpr uint32_t kLdsTiles = 2048;
+constexpr uint32_t kLdsTiles = 6144;  // 144 KB of LDS (gfx950 gives a workgroup 160 KB)
+constexpr int kLdsTileRounds = kLdsTiles / 1024;
 
 __global__ __launch_bounds__(1024) void k_tile_table(float *__restrict__ table, uint32_t ntiles, int levels)
 {
@@ -245,8 +246,9 @@ __global__ __launch_bounds__(1024) void k_tile_table(float *__restrict__ table, 
         __syncthreads();
         for (int j = 1; j < levels; ++j) {
             const uint32_t half = 1u << (j - 1);
-            float v[2][6];
-            for (int r = 0; r < 2; ++r) {
+            float v[kLdsTileRounds][6];
+#pragma unroll
+            for (int r = 0; r < kLdsTileRounds; ++r) {
                 const uint32_t t = threadIdx.x + 1024u * r;
                 if (t + 2 * half <= ntiles)
 #pragma unroll
@@ -254,7 +256,8 @@ __global__ __launch_bounds__(1024) void k_tile_table(float *__restrict__ table, 
                         v[r][k] = k < 3 ? fminf(s_t[k][t], s_t[k][t + half]) : fmaxf(s_t[k][t], s_t[k][t + half]);
             }
             __syncthreads();
-            for (int r = 0; r < 2; ++r) {
+#pragma unroll
+            for (int r = 0; r < kLdsTileRounds; ++r) {
                 const uint32_t t = threadIdx.x + 1024u * r;
                 if (t + 2 * half <= ntiles)
 #pragma unroll
