@@ -348,6 +348,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="run the batch as plain launches (for kernel tracing)")
     ap.add_argument("--no-stress", action="store_true", help="skip the torus / cube filter stress cases")
     ap.add_argument("--gd-only", type=int, default=0, help="profiling aid: only N back-to-back GD launches")
+    ap.add_argument("--step-only", type=int, default=0, help="profiling aid: only N steps with default knobs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the hot path of one mesh pair "
@@ -385,6 +386,13 @@ def main():
         log(f"gd-only: {len(P)} pairs, filtered {ms * 1e3:.2f} us, fp64 {ms64 * 1e3:.2f} us per launch; "
             f"saturated {len(Psat)} pairs: filtered {sat:.3f} ms, fp64 {sat64:.3f} ms; copy 1 GiB {cms:.3f} ms")
         del Psat, P
+        return
+
+    if args.step_only:
+        for i in range(args.step_only):
+            step()
+            ctx.synchronize()
+            log(f"step {i + 1}/{args.step_only} done")
         return
 
     for i in range(args.warmup):

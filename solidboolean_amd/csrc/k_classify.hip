@@ -32,6 +32,7 @@ namespace sbg {
 
 constexpr int kFastKeys = 4;     // distinct keys a fast lane keeps
 constexpr int kFastCands = 48;   // cell entries a fast lane scans alone
+constexpr int kCodeGroup = 4;    // entry codes a fast lane loads at once
 constexpr int kWaveHits = 256;   // hits a wave gathers for one deferred ray
 
 __global__ __launch_bounds__(256) void k_mark_intersected(const uint64_t *__restrict__ pairs,
@@ -268,7 +269,7 @@ __device__ __forceinline__ uint2 span_code(const GridSpan &s, uint32_t cu, uint3
 // no slot within kProbe probes falls back to per-entry global atomics; both
 // paths only ever advance a cell's counter, so they mix freely.
 constexpr int kGridBlockTris = 512;  // one per lane
-constexpr int kGridSlotBits = 13;
+constexpr int kGridSlotBits = 12;
 constexpr int kGridSlots = 1 << kGridSlotBits;
 constexpr int kProbe = 64;
 
@@ -437,14 +438,18 @@ __device__ __forceinline__ RayCell ray_cell(const GridSet &gs, uint32_t m, const
     return c;
 }
 
-// code test of grid entry k against the ray (a superset of the FP64 box
-// test); on success `t` is the triangle
+// code test of an entry against the ray (a superset of the FP64 box test)
+__device__ __forceinline__ bool code_may_hit(const uint2 &e, const RayCell &c)
+{
+    const uint32_t ha = e.x >> 24 | (e.y >> 24) << 8;
+    return (e.x & kCodeMax) <= c.eu && ((e.x >> 12) & kCodeMax) >= c.pu && (e.y & kCodeMax) <= c.ev &&
+           ((e.y >> 12) & kCodeMax) >= c.pv && ha >= c.pa;
+}
+
+// code test of grid entry k; on success `t` is the triangle
 __device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, const RayCell &c, uint32_t &t)
 {
-    const uint2 e = gs.gcode[k];
-    const uint32_t ha = e.x >> 24 | (e.y >> 24) << 8;
-    if (!((e.x & kCodeMax) <= c.eu && ((e.x >> 12) & kCodeMax) >= c.pu && (e.y & kCodeMax) <= c.ev &&
-          ((e.y >> 12) & kCodeMax) >= c.pv && ha >= c.pa))
+    if (!code_may_hit(gs.gcode[k], c))
         return false;
     t = gs.gtri[k];
     return true;
@@ -483,12 +488,24 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
     const RayCell c = ray_cell(gs, m, r, (int)axis);
     bool defer = c.range.y - c.range.x > (uint32_t)kFastCands;
     AxisKeys<kFastKeys> keys;
-    for (uint32_t k = c.range.x; !defer && k < c.range.y; ++k) {
-        uint32_t t;
-        Key3 key;
-        if (entry_may_hit(gs, k, c, t) && ray_hit(t, tri64, r, key)) {
-            keys.add(key, (int)axis);
-            defer = keys.full;
+    // codes in groups of kCodeGroup independent loads (indices clamped into the
+    // range, the excess masked off), then the FP64 test of the entries passing
+    for (uint32_t k0 = c.range.x; !defer && k0 < c.range.y; k0 += kCodeGroup) {
+        uint32_t pass = 0;
+#pragma unroll
+        for (int j = 0; j < kCodeGroup; ++j) {
+            const uint32_t k = k0 + j < c.range.y ? k0 + j : c.range.y - 1;
+            if (code_may_hit(gs.gcode[k], c) && k0 + j < c.range.y)
+                pass |= 1u << j;
+        }
+        while (pass && !defer) {
+            const int j = __builtin_ctz(pass);
+            pass &= pass - 1;
+            Key3 key;
+            if (ray_hit(gs.gtri[k0 + j], tri64, r, key)) {
+                keys.add(key, (int)axis);
+                defer = keys.full;
+            }
         }
     }
     if (defer) {

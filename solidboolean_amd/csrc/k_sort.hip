@@ -1,20 +1,23 @@
 // Stable LSD radix sort, 8-bit digits, for u32 keys (+ u32 values) and u64 keys.
 //
-// Per pass: (1) every block counts the digits of its 4096-key tile into LDS and
-// writes a 256 x nblocks histogram column; (2) one block turns the histogram
-// (digit-major) into exclusive global offsets; (3) every block re-reads its tile
-// in order, 256 keys per round, ranks each key among equal digits with a 64-lane
-// ballot match (stable within the wave), adds the counts of earlier waves and
-// earlier rounds kept in LDS, and scatters.  Per key and pass: read 4 (8) B key
-// (+4 B value) twice, write once.
+// Per pass: (1) every block counts the digits of its tile (256 x ITEMS keys)
+// into LDS and writes a 256 x nblocks histogram column; (2) one block per digit
+// turns the histogram (digit-major) into exclusive offsets; (3) every block
+// loads its whole tile into registers, then ranks it in order, 256 keys per
+// round, each key among equal digits with a 64-lane ballot match (stable within
+// the wave), adds the counts of earlier waves and earlier rounds kept in LDS,
+// and scatters.  ITEMS shrinks with n so that short sorts still spread over
+// many workgroups.  Per key and pass: read 4 (8) B key (+4 B value) twice,
+// write once.
 #include "sbg_internal.h"
 
 namespace sbg {
 
 constexpr int kSortBlock = 256;
-constexpr int kSortItems = 16;
-constexpr int kSortTile = kSortBlock * kSortItems;
 constexpr int kWaves = kSortBlock / 64;
+
+// keys per thread for a sort of n keys
+static int sort_items(size_t n) { return n >= ((size_t)1 << 20) ? 16 : (n >= ((size_t)1 << 17) ? 8 : 4); }
 
 template <typename K>
 __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask)
@@ -22,16 +25,16 @@ __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask)
     return (uint32_t)(key >> shift) & mask;
 }
 
-template <typename K>
+template <typename K, int ITEMS>
 __global__ __launch_bounds__(kSortBlock) void k_hist(const K *__restrict__ keys, size_t n, int shift,
                                                      uint32_t mask, uint32_t *__restrict__ hist, int nblocks)
 {
     __shared__ uint32_t s_cnt[256];
     s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kSortTile;
-#pragma unroll 4
-    for (int r = 0; r < kSortItems; ++r) {
+    const size_t base = (size_t)blockIdx.x * (kSortBlock * ITEMS);
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
         size_t idx = base + (size_t)r * kSortBlock + threadIdx.x;
         if (idx < n)
             atomicAdd(&s_cnt[digit_of(keys[idx], shift, mask)], 1u);
@@ -167,7 +170,7 @@ __global__ __launch_bounds__(256) void k_scan_rows(uint32_t *__restrict__ hist, 
         rowTotal[blockIdx.x] = s[255];
 }
 
-template <typename K, bool HAS_VAL>
+template <typename K, bool HAS_VAL, int ITEMS>
 __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ keys, const uint32_t *__restrict__ vals,
                                                         K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
                                                         size_t n, int shift, uint32_t mask,
@@ -176,6 +179,16 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ke
 {
     __shared__ uint32_t s_base[256];
     __shared__ uint32_t s_wcnt[kWaves][256];
+    // the tile first, so that its loads overlap the offset set-up
+    const size_t base = (size_t)blockIdx.x * (kSortBlock * ITEMS);
+    K kreg[ITEMS];
+    uint32_t vreg[ITEMS];
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
+        const size_t idx = base + (size_t)r * kSortBlock + threadIdx.x;
+        kreg[r] = idx < n ? keys[idx] : (K)0;
+        vreg[r] = (HAS_VAL && idx < n) ? vals[idx] : 0u;
+    }
     // digit bases: exclusive scan of the 256 row totals
     const uint32_t total = rowTotal[threadIdx.x];
     s_base[threadIdx.x] = total;
@@ -195,12 +208,12 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ke
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const unsigned long long ltmask = (1ull << lane) - 1ull;
-    const size_t base = (size_t)blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortItems; ++r) {
+#pragma unroll
+    for (int r = 0; r < ITEMS; ++r) {
         const size_t idx = base + (size_t)r * kSortBlock + threadIdx.x;
         const bool valid = idx < n;
-        K key = valid ? keys[idx] : (K)0;
-        uint32_t val = (HAS_VAL && valid) ? vals[idx] : 0u;
+        const K key = kreg[r];
+        const uint32_t val = vreg[r];
         const uint32_t d = digit_of(key, shift, mask);
         // lanes of this wave holding the same digit
         unsigned long long peers = __ballot(valid);
@@ -244,7 +257,11 @@ void launch_exclusive_scan(uint32_t *data, size_t m, hipStream_t s)
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, data, m);
 }
 
-int sort_blocks(size_t n) { return (int)((n + kSortTile - 1) / kSortTile); }
+int sort_blocks(size_t n)
+{
+    const size_t tile = (size_t)kSortBlock * sort_items(n);
+    return (int)((n + tile - 1) / tile);
+}
 // 256 x nblocks histogram + 256 row totals
 size_t sort_hist_entries(size_t n) { return (size_t)256 * (size_t)(sort_blocks(n) > 0 ? sort_blocks(n) : 1) + 256; }
 
@@ -255,10 +272,23 @@ static void one_pass(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout,
     const int nb = sort_blocks(n);
     const uint32_t mask = (1u << bits) - 1u;
     uint32_t *rowTotal = hist + (size_t)256 * nb;
-    hipLaunchKernelGGL(k_hist<K>, dim3(nb), dim3(kSortBlock), 0, s, kin, n, shift, mask, hist, nb);
+    const int items = sort_items(n);
+    if (items == 16)
+        hipLaunchKernelGGL((k_hist<K, 16>), dim3(nb), dim3(kSortBlock), 0, s, kin, n, shift, mask, hist, nb);
+    else if (items == 8)
+        hipLaunchKernelGGL((k_hist<K, 8>), dim3(nb), dim3(kSortBlock), 0, s, kin, n, shift, mask, hist, nb);
+    else
+        hipLaunchKernelGGL((k_hist<K, 4>), dim3(nb), dim3(kSortBlock), 0, s, kin, n, shift, mask, hist, nb);
     hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(256), 0, s, hist, nb, rowTotal);
-    hipLaunchKernelGGL((k_scatter<K, HAS_VAL>), dim3(nb), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
-                       shift, mask, hist, nb, rowTotal);
+    if (items == 16)
+        hipLaunchKernelGGL((k_scatter<K, HAS_VAL, 16>), dim3(nb), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
+                           shift, mask, hist, nb, rowTotal);
+    else if (items == 8)
+        hipLaunchKernelGGL((k_scatter<K, HAS_VAL, 8>), dim3(nb), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
+                           shift, mask, hist, nb, rowTotal);
+    else
+        hipLaunchKernelGGL((k_scatter<K, HAS_VAL, 4>), dim3(nb), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
+                           shift, mask, hist, nb, rowTotal);
 }
 
 int radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *valsAlt, size_t n,
