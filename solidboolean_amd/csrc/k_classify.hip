@@ -59,6 +59,34 @@ __global__ __launch_bounds__(256) void k_centroids(const double *__restrict__ tr
     pts[3 * (size_t)t + 2] = (r[2] + r[6] + r[10]) / 3.0;
 }
 
+// The same centroids in visiting order: query q (triangle order[q]) gets
+// (x, y, z, tag) at q4 + 4q, with tag = the target mesh (target[t], 0 without
+// a target list), kTagSkip for a triangle flagged as intersected, kTagNone for
+// one without a target mesh.  A ray lane then reads its point without first
+// waiting for order[q].
+constexpr double kTagSkip = -1.0, kTagNone = -2.0;
+
+__global__ __launch_bounds__(256) void k_centroids_ordered(const double *__restrict__ tri64,
+                                                           const uint8_t *__restrict__ flags,
+                                                           const uint32_t *__restrict__ order,
+                                                           const uint32_t *__restrict__ target, uint32_t nt,
+                                                           double *__restrict__ q4)
+{
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= nt)
+        return;
+    const uint32_t t = order[q];
+    double2 *out = reinterpret_cast<double2 *>(q4) + 2 * (size_t)q;
+    if (flags && flags[t]) {
+        out[1] = make_double2(0.0, kTagSkip);
+        return;
+    }
+    const uint32_t m = target ? target[t] : 0u;
+    const double *r = tri64 + (size_t)kTri64Stride * t;
+    out[0] = make_double2((r[0] + r[4] + r[8]) / 3.0, (r[1] + r[5] + r[9]) / 3.0);
+    out[1] = make_double2((r[2] + r[6] + r[10]) / 3.0, m == ~0u ? kTagNone : (double)m);
+}
+
 // A ray is the segment from p to e = p + dir (src/solidboolean.cpp:60-66):
 // dir is DBL_MAX along the axis and DBL_EPSILON along the other two, so
 // e >= p in every component and the segment's box (:67-69) is exactly [p, e].
@@ -66,14 +94,45 @@ struct Ray {
     V3 p, e;
 };
 
-__device__ __forceinline__ Ray ray_setup(const double *__restrict__ pts, uint32_t i, uint32_t axis)
+__device__ __forceinline__ Ray ray_from(const V3 &p, uint32_t axis)
 {
     const double eps = 2.220446049250313080847e-16, big = 1.7976931348623157e308;
     Ray r;
-    r.p = {pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]};
+    r.p = p;
     const V3 dir = {axis == 0 ? big : eps, axis == 1 ? big : eps, axis == 2 ? big : eps};
     r.e = {r.p.x + dir.x, r.p.y + dir.y, r.p.z + dir.z};
     return r;
+}
+
+// Lane g = axis * n + q: its ray, output index i and target mesh m.  Q4: the
+// points are (x, y, z, tag) records in visiting order; otherwise xyz triples
+// by output index, with optional order / skip / gs.target.  Returns 0 for a
+// skipped query (no output), 1 for one without a target (outside), 2 to cast.
+template <bool Q4>
+__device__ __forceinline__ int load_query(const GridSet &gs, const double *__restrict__ pts,
+                                          const uint32_t *__restrict__ order, const uint8_t *__restrict__ skip,
+                                          uint32_t q, uint32_t axis, uint32_t &i, uint32_t &m, Ray &r)
+{
+    if (Q4) {
+        const double2 *x = reinterpret_cast<const double2 *>(pts) + 2 * (size_t)q;
+        const double2 a = x[0], b = x[1];
+        i = order[q];
+        if (b.y == kTagSkip)
+            return 0;
+        if (b.y == kTagNone)
+            return 1;
+        m = (uint32_t)b.y;
+        r = ray_from({a.x, a.y, b.x}, axis);
+        return 2;
+    }
+    i = order ? order[q] : q;
+    if (skip && skip[i])
+        return 0;
+    m = gs.target ? gs.target[i] : 0u;
+    if (m == ~0u)
+        return 1;
+    r = ray_from({pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]}, axis);
+    return 2;
 }
 
 // Distinct PositionKeys of one ray, CAP of them in registers.  Along an axis
@@ -156,14 +215,17 @@ __device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ t
 // meshes (a batch) share one cell array: mesh m owns 3 G_m^2 cells from its
 // first cell on.
 //
-// An entry is 8 bytes of 12-bit codes plus the triangle index.  The codes are
+// An entry is 12 bytes: two code words and the triangle index.  The codes are
 // the triangle's FP64 box bounds lo_u, hi_u, lo_v, hi_v relative to the
-// entry's cell (in 1/4096 of a cell, clamped to the cell) and hi_a relative to
-// the mesh box along the axis (in 1/4096 of its extent).  Every code is one
-// monotone function of its coordinate, and a ray's own codes come from the
-// same function in the same cell, so lo_u <= e.u implies code(lo_u) <=
-// code(e.u) and so on: the code test passes every entry whose FP64 box
-// overlaps the ray's segment box, and most others fail it from 8 bytes.
+// entry's cell (12 bits each, in 1/4096 of a cell, clamped to the cell) and
+// hi_a relative to the mesh box along the axis (16 bits, in 1/65536 of its
+// extent).  Every code is one monotone function of its coordinate, and a ray's
+// own codes come from the same function in the same cell, so lo_u <= e.u
+// implies code(lo_u) <= code(e.u) and so on: the code test passes every entry
+// whose FP64 box overlaps the ray's segment box, and most others fail it
+// without touching the FP64 record.  The resolution matters where faces lie
+// nearly along the rays (edge-on, or close to a query face's plane): coarser
+// codes pass whole layers of such faces to the FP64 test.
 struct GridFrame {
     double lu, lv, la, iu, iv, ia;
 };
@@ -171,8 +233,10 @@ struct GridFrame {
 __device__ __forceinline__ int axis_u(int axis) { return axis == 0 ? 1 : 0; }
 __device__ __forceinline__ int axis_v(int axis) { return axis == 2 ? 1 : 2; }
 
-constexpr double kCodeSteps = 4096.0;
-constexpr uint32_t kCodeMax = 4095u;
+constexpr double kCellSteps = 4096.0;  // u, v codes per cell
+constexpr uint32_t kCellMax = 4095u;
+constexpr double kAxisSteps = 65536.0;  // axis codes over the mesh extent
+constexpr uint32_t kAxisMax = 65535u;
 
 __device__ __forceinline__ GridFrame grid_frame(const float *__restrict__ box, int axis, uint32_t G)
 {
@@ -185,7 +249,7 @@ __device__ __forceinline__ GridFrame grid_frame(const float *__restrict__ box, i
     const double sa = (double)box[3 + axis] - f.la;
     f.iu = su > 0.0 ? (double)G / su : 0.0;
     f.iv = sv > 0.0 ? (double)G / sv : 0.0;
-    f.ia = sa > 0.0 ? kCodeSteps / sa : 0.0;
+    f.ia = sa > 0.0 ? kAxisSteps / sa : 0.0;
     return f;
 }
 
@@ -204,22 +268,26 @@ __device__ __forceinline__ uint32_t grid_cell(double x, double lo, double inv, u
     return cell_of(cell_coord(x, lo, inv), G);
 }
 
-// monotone 12-bit code of y (NaN and negatives give 0)
-__device__ __forceinline__ uint32_t code12(double y)
+// monotone code of y in [0, max] (NaN and negatives give 0)
+__device__ __forceinline__ uint32_t code_of(double y, uint32_t max)
 {
     if (!(y >= 0.0))
         return 0;
-    return y < (double)kCodeMax ? (uint32_t)y : kCodeMax;
+    return y < (double)max ? (uint32_t)y : max;
 }
 
 // code of cell coordinate t inside cell c
-__device__ __forceinline__ uint32_t cell_code(double t, uint32_t c) { return code12((t - (double)c) * kCodeSteps); }
-
-// entry layout: x = lo_u | hi_u << 12 | (hi_a & 255) << 24,
-//               y = lo_v | hi_v << 12 | (hi_a >> 8) << 24
-__device__ __forceinline__ uint2 pack_code(uint32_t lu, uint32_t hu, uint32_t lv, uint32_t hv, uint32_t ha)
+__device__ __forceinline__ uint32_t cell_code(double t, uint32_t c)
 {
-    return make_uint2(lu | hu << 12 | (ha & 255u) << 24, lv | hv << 12 | (ha >> 8) << 24);
+    return code_of((t - (double)c) * kCellSteps, kCellMax);
+}
+
+// entry: x = lo_u | hi_u << 12 | (hi_a & 255) << 24,
+//        y = lo_v | hi_v << 12 | (hi_a >> 8) << 24, z = triangle
+__device__ __forceinline__ uint3 pack_entry(uint32_t lu, uint32_t hu, uint32_t lv, uint32_t hv, uint32_t ha,
+                                            uint32_t t)
+{
+    return make_uint3(lu | hu << 12 | (ha & 255u) << 24, lv | hv << 12 | (ha >> 8) << 24, t);
 }
 
 __device__ __forceinline__ double comp(const V3 &a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
@@ -249,14 +317,14 @@ __device__ __forceinline__ GridSpan grid_span(const Box64 &b, int axis, const Gr
     s.hv = cell_coord(hv, f.lv, f.iv);
     s.u1 = cell_of(s.hu, G);
     s.v1 = cell_of(s.hv, G);
-    s.ha = code12(cell_coord(comp(b.mx, axis), f.la, f.ia));
+    s.ha = code_of(cell_coord(comp(b.mx, axis), f.la, f.ia), kAxisMax);
     return s;
 }
 
 // the entry of span s in cell (cu, cv)
-__device__ __forceinline__ uint2 span_code(const GridSpan &s, uint32_t cu, uint32_t cv)
+__device__ __forceinline__ uint3 span_entry(const GridSpan &s, uint32_t cu, uint32_t cv, uint32_t t)
 {
-    return pack_code(cell_code(s.lu, cu), cell_code(s.hu, cu), cell_code(s.lv, cv), cell_code(s.hv, cv), s.ha);
+    return pack_entry(cell_code(s.lu, cu), cell_code(s.hu, cu), cell_code(s.lv, cv), cell_code(s.hv, cv), s.ha, t);
 }
 
 // Grid build: a workgroup takes kGridBlockTris consecutive triangles of the
@@ -360,7 +428,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const double *__r
 // afterwards cell c spans [cells[c], cells[c + 1]).
 __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt,
                                                               GridSet gs, uint32_t *__restrict__ cells,
-                                                              uint2 *__restrict__ gcode, uint32_t *__restrict__ gtri)
+                                                              uint3 *__restrict__ gent)
 {
     __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
     const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
@@ -397,8 +465,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__re
                 const int sl = table_slot(s_key, key);
                 const uint32_t slot = sl >= 0 ? atomicAdd(&s_cnt[sl], 1u) : atomicAdd(&cells[key], 1u);
                 if (slot < gs.cap) {
-                    gcode[slot] = span_code(x.s[axis], cu, cv);
-                    gtri[slot] = x.t;
+                    gent[slot] = span_entry(x.s[axis], cu, cv, x.t);
                 }
             }
 }
@@ -432,27 +499,26 @@ __device__ __forceinline__ RayCell ray_cell(const GridSet &gs, uint32_t m, const
     c.eu = cell_code(cell_coord(comp(r.e, u), f.lu, f.iu), cu);
     c.pv = cell_code(tv, cv);
     c.ev = cell_code(cell_coord(comp(r.e, v), f.lv, f.iv), cv);
-    c.pa = code12(cell_coord(comp(r.p, axis), f.la, f.ia));
+    c.pa = code_of(cell_coord(comp(r.p, axis), f.la, f.ia), kAxisMax);
     const size_t k = (size_t)d.y + (size_t)axis * G * G + (size_t)cu * G + cv;
     c.range = make_uint2(gs.cells[k], gs.cells[k + 1]);
     return c;
 }
 
 // code test of an entry against the ray (a superset of the FP64 box test)
-__device__ __forceinline__ bool code_may_hit(const uint2 &e, const RayCell &c)
+__device__ __forceinline__ bool code_may_hit(const uint3 &e, const RayCell &c)
 {
     const uint32_t ha = e.x >> 24 | (e.y >> 24) << 8;
-    return (e.x & kCodeMax) <= c.eu && ((e.x >> 12) & kCodeMax) >= c.pu && (e.y & kCodeMax) <= c.ev &&
-           ((e.y >> 12) & kCodeMax) >= c.pv && ha >= c.pa;
+    return (e.x & kCellMax) <= c.eu && ((e.x >> 12) & kCellMax) >= c.pu && (e.y & kCellMax) <= c.ev &&
+           ((e.y >> 12) & kCellMax) >= c.pv && ha >= c.pa;
 }
 
 // code test of grid entry k; on success `t` is the triangle
 __device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, const RayCell &c, uint32_t &t)
 {
-    if (!code_may_hit(gs.gcode[k], c))
-        return false;
-    t = gs.gtri[k];
-    return true;
+    const uint3 e = gs.gent[k];
+    t = e.z;
+    return code_may_hit(e, c);
 }
 
 // One lane per (axis, query).  A lane whose cell holds at most kFastCands
@@ -460,6 +526,7 @@ __device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, con
 // lane (a crowded cell, or more distinct hits) hands (axis, query) to
 // k_rays_wave.  Query i's rays go to mesh target[i] (mesh 0 without a target
 // list); a query without a target mesh (~0u) counts as outside.
+template <bool Q4>
 __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__restrict__ tri64,
                                                    const double *__restrict__ pts,
                                                    const uint32_t *__restrict__ order, const uint8_t *__restrict__ skip,
@@ -476,15 +543,14 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
     }
     const uint32_t axis = g / n;
     const uint32_t q = g - axis * n;
-    const uint32_t i = order ? order[q] : q;
-    if (skip && skip[i])
-        return;
-    const uint32_t m = gs.target ? gs.target[i] : 0u;
-    if (m == ~0u) {
-        bits[(size_t)axis * n + i] = 0;
+    uint32_t i, m = 0;
+    Ray r;
+    const int kind = load_query<Q4>(gs, pts, order, skip, q, axis, i, m, r);
+    if (kind != 2) {
+        if (kind == 1)
+            bits[(size_t)axis * n + i] = 0;
         return;
     }
-    const Ray r = ray_setup(pts, i, axis);
     const RayCell c = ray_cell(gs, m, r, (int)axis);
     bool defer = c.range.y - c.range.x > (uint32_t)kFastCands;
     AxisKeys<kFastKeys> keys;
@@ -492,17 +558,21 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
     // range, the excess masked off), then the FP64 test of the entries passing
     for (uint32_t k0 = c.range.x; !defer && k0 < c.range.y; k0 += kCodeGroup) {
         uint32_t pass = 0;
+        uint32_t tri[kCodeGroup];
 #pragma unroll
         for (int j = 0; j < kCodeGroup; ++j) {
             const uint32_t k = k0 + j < c.range.y ? k0 + j : c.range.y - 1;
-            if (code_may_hit(gs.gcode[k], c) && k0 + j < c.range.y)
+            const uint3 e = gs.gent[k];
+            tri[j] = e.z;
+            if (code_may_hit(e, c) && k0 + j < c.range.y)
                 pass |= 1u << j;
         }
-        while (pass && !defer) {
-            const int j = __builtin_ctz(pass);
-            pass &= pass - 1;
+#pragma unroll
+        for (int j = 0; j < kCodeGroup; ++j) {
+            if (!(pass >> j & 1u) || defer)
+                continue;
             Key3 key;
-            if (ray_hit(gs.gtri[k0 + j], tri64, r, key)) {
+            if (ray_hit(tri[j], tri64, r, key)) {
                 keys.add(key, (int)axis);
                 defer = keys.full;
             }
@@ -519,6 +589,7 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
 // One wave per deferred (axis, query), g = axis * n + q: the 64 lanes split
 // the cell's entries, hit keys are gathered in LDS (up to kWaveHits) and the
 // distinct ones counted; more hits than that raise `overflow`.
+template <bool Q4>
 __global__ __launch_bounds__(64) void k_rays_wave(GridSet gs, const double *__restrict__ tri64,
                                                   const double *__restrict__ pts,
                                                   const uint32_t *__restrict__ order, uint32_t n,
@@ -533,9 +604,9 @@ __global__ __launch_bounds__(64) void k_rays_wave(GridSet gs, const double *__re
         const uint32_t g = list[k];
         const uint32_t axis = g / n;
         const uint32_t q = g - axis * n;
-        const uint32_t i = order ? order[q] : q;
-        const uint32_t m = gs.target ? gs.target[i] : 0u;
-        const Ray r = ray_setup(pts, i, axis);
+        uint32_t i, m = 0;
+        Ray r;
+        load_query<Q4>(gs, pts, order, nullptr, q, axis, i, m, r);  // deferred lanes always cast
         const RayCell c = ray_cell(gs, m, r, (int)axis);
         uint32_t nh = 0;
         for (uint32_t k0 = c.range.x; k0 < c.range.y; k0 += 64) {
@@ -616,7 +687,7 @@ size_t grid_cells(uint32_t G) { return 3 * (size_t)G * G + 1; }
 
 
 void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
-                       uint2 *gcode, uint32_t *gtri, hipStream_t s)
+                       uint3 *gent, hipStream_t s)
 {
     const size_t m = (size_t)gs.ncells + 1;
     launch_fill_u32(cells, 0u, m, s);
@@ -625,7 +696,7 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
     const dim3 grid(xcd_grid((nt + kGridBlockTris - 1) / kGridBlockTris));
     hipLaunchKernelGGL(k_grid_count, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
-    hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gcode, gtri);
+    hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gent);
 }
 
 void launch_rays(const GridSet &gs, const double *tri64, const double *pts,
@@ -635,9 +706,31 @@ void launch_rays(const GridSet &gs, const double *tri64, const double *pts,
     if (n == 0)
         return;
     const uint64_t lanes = 3ull * n;
-    hipLaunchKernelGGL(k_rays_grid, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, gs, tri64, pts,
+    hipLaunchKernelGGL(k_rays_grid<false>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, gs, tri64, pts,
                        order, skip, n, axisBits, ovfList, ovfCount, overflow);
-    hipLaunchKernelGGL(k_rays_wave, dim3(8192), dim3(64), 0, s, gs, tri64, pts, order, n, axisBits, ovfList,
+    hipLaunchKernelGGL(k_rays_wave<false>, dim3(8192), dim3(64), 0, s, gs, tri64, pts, order, n, axisBits, ovfList,
+                       ovfCount, overflow);
+}
+
+void launch_centroids_ordered(const double *tri64, const uint8_t *flags, const uint32_t *order,
+                              const uint32_t *target, uint32_t nt, double *q4, hipStream_t s)
+{
+    if (nt == 0)
+        return;
+    hipLaunchKernelGGL(k_centroids_ordered, dim3((nt + 255) / 256), dim3(256), 0, s, tri64, flags, order, target,
+                       nt, q4);
+}
+
+void launch_rays_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
+                         uint32_t n, uint8_t *axisBits, uint32_t *ovfList, uint32_t *ovfCount, uint32_t *overflow,
+                         hipStream_t s)
+{
+    if (n == 0)
+        return;
+    const uint64_t lanes = 3ull * n;
+    hipLaunchKernelGGL(k_rays_grid<true>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, gs, tri64, q4,
+                       order, nullptr, n, axisBits, ovfList, ovfCount, overflow);
+    hipLaunchKernelGGL(k_rays_wave<true>, dim3(8192), dim3(64), 0, s, gs, tri64, q4, order, n, axisBits, ovfList,
                        ovfCount, overflow);
 }
 

@@ -76,8 +76,7 @@ struct sbg_mesh_s {
     bool prepared = false;
     // per-axis ray grid, built on first use as a classification target
     DevBuf<uint32_t> gridCells, gridTemp;
-    DevBuf<uint2> gridCode;
-    DevBuf<uint32_t> gridTri;
+    DevBuf<uint3> gridEnt;
     size_t gridCap = 0;
     uint32_t gridG = 0;
     int gridDensity = 0;
@@ -145,6 +144,36 @@ int sbg_context_create(int device, sbg_context *out)
 }  // extern "C"
 
 namespace sbg {
+static void pairs_free_buffers(sbg_pairs p)
+{
+    p->pairs.release();
+    p->alt.release();
+    p->status.release();
+    p->seg.release();
+    p->hitSeg.release();
+    p->hitPairs.release();
+    p->blockCounts.release();
+    p->nHits.release();
+    p->work.release();
+}
+
+// a new pair list, on the buffers of the last destroyed one when there is one
+// (stream order makes the reuse safe, and a step that makes and drops one
+// list per call allocates nothing)
+static sbg_pairs pairs_new(sbg_context c)
+{
+    sbg_pairs p = c->sparePairs;
+    c->sparePairs = nullptr;
+    if (!p)
+        p = new sbg_pairs_s;
+    p->ctx = c;
+    p->n = 0;
+    p->tested = false;
+    p->maxA = p->maxB = 0;
+    ++c->refs;
+    return p;
+}
+
 void context_free(sbg_context ctx)
 {
     (void)hipSetDevice(ctx->device);
@@ -176,6 +205,11 @@ void context_free(sbg_context ctx)
     ctx->adjVol.release();
     ctx->adjSrc.release();
     ctx->adjFaces.release();
+    if (ctx->sparePairs) {
+        pairs_free_buffers(ctx->sparePairs);
+        delete ctx->sparePairs;
+        ctx->sparePairs = nullptr;
+    }
     (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -302,8 +336,7 @@ int sbg_mesh_destroy(sbg_mesh m)
     m->nodes.release();
     m->gridCells.release();
     m->gridTemp.release();
-    m->gridCode.release();
-    m->gridTri.release();
+    m->gridEnt.release();
     sbg_context c = m->ctx;
     delete m;
     context_release(c);
@@ -445,19 +478,15 @@ int sbg_pairs_destroy(sbg_pairs p)
 {
     if (!p)
         return SBG_OK;
-    (void)hipSetDevice(p->ctx->device);
-    (void)hipStreamSynchronize(p->ctx->stream);
-    p->pairs.release();
-    p->alt.release();
-    p->status.release();
-    p->seg.release();
-    p->hitSeg.release();
-    p->hitPairs.release();
-    p->blockCounts.release();
-    p->nHits.release();
-    p->work.release();
     sbg_context c = p->ctx;
-    delete p;
+    (void)hipSetDevice(c->device);
+    if (!c->sparePairs && !c->destroyed) {
+        c->sparePairs = p;  // keeps its buffers for the next list
+    } else {
+        (void)hipStreamSynchronize(c->stream);
+        pairs_free_buffers(p);
+        delete p;
+    }
     context_release(c);
     return SBG_OK;
 }
@@ -473,9 +502,7 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
         return fail(SBG_ERR_STATE, "mesh not prepared");
     sbg_context c = a->ctx;
     TRY(use(c));
-    sbg_pairs p = new sbg_pairs_s;
-    p->ctx = c;
-    ++c->refs;
+    sbg_pairs p = pairs_new(c);
     if (a->nt == 0 || b->nt == 0) {
         *out = p;
         return SBG_OK;
@@ -624,9 +651,7 @@ int sbg_pairs_create(sbg_context ctx, const uint32_t *ab, size_t n, sbg_pairs *o
     TRY(use(ctx));
     if (n && !ab)
         return fail(SBG_ERR_INVALID, "null pair array");
-    sbg_pairs p = new sbg_pairs_s;
-    p->ctx = ctx;
-    ++ctx->refs;
+    sbg_pairs p = pairs_new(ctx);
     p->n = n;
     int rc = p->pairs.ensure(n);
     if (rc != SBG_OK) {
@@ -826,8 +851,7 @@ static GridSet grid_set(sbg_mesh m)
     GridSet gs = {};
     gs.boxes = m->rootBox.p;
     gs.cells = m->gridCells.p;
-    gs.gcode = m->gridCode.p;
-    gs.gtri = m->gridTri.p;
+    gs.gent = m->gridEnt.p;
     gs.G0 = m->gridG;
     gs.cap = (uint32_t)m->gridCap;
     gs.ncells = (uint32_t)(grid_cells(m->gridG) - 1);
@@ -848,13 +872,12 @@ static int ensure_grid(sbg_mesh m)
         return fail(SBG_ERR_OVERFLOW, "ray grid needs %zu entries", m->gridCap);
     TRY(m->gridCells.ensure(grid_cells(G)));
     TRY(m->gridTemp.ensure(scan_large_temp(grid_cells(G))));
-    TRY(m->gridCode.ensure(m->gridCap));
-    TRY(m->gridTri.ensure(m->gridCap));
+    TRY(m->gridEnt.ensure(m->gridCap));
     m->gridG = G;
     {
         Stage st(c, SBG_STAGE_GRID);
-        launch_grid_build(m->tri64.p, (uint32_t)m->nt, grid_set(m), m->gridCells.p, m->gridTemp.p, m->gridCode.p,
-                          m->gridTri.p, c->stream);
+        launch_grid_build(m->tri64.p, (uint32_t)m->nt, grid_set(m), m->gridCells.p, m->gridTemp.p, m->gridEnt.p,
+                          c->stream);
         TRY(check_launch("ray grid"));
     }
     m->gridReady = true;
@@ -865,24 +888,41 @@ static int ensure_grid(sbg_mesh m)
 // ovf[0]: overflow flags, ovf[1]: number of rays handed to the list kernel.
 // A grid that turned out too small is rebuilt at its exact size and the rays
 // run again.
-static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder, const uint8_t *dskip, size_t n)
+// Casts the 3 n rays of the queries into target's grid and votes.  ordered:
+// dpts is the visiting-order (x, y, z, tag) layout over dorder; otherwise xyz
+// triples by index with optional dorder / dskip.  With hostVotes the votes are
+// copied out under the same stream synchronisation that reads the overflow
+// flags.
+static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder, const uint8_t *dskip, size_t n,
+                    bool ordered, uint8_t *hostVotes)
 {
     sbg_context c = target->ctx;
     TRY(c->ovf.ensure(2));
     TRY(c->rayList.ensure(3 * n));
     if (target->nt == 0) {
-        Stage st(c, SBG_STAGE_CLASSIFY);
-        launch_fill_u32((uint32_t *)c->bits.p, 0u, (3 * n + 3) / 4, c->stream);
-        launch_vote(c->bits.p, dskip, (uint32_t)n, c->votes.p, c->stream);
-        return check_launch("rays");
+        {
+            Stage st(c, SBG_STAGE_CLASSIFY);
+            launch_fill_u32((uint32_t *)c->bits.p, 0u, (3 * n + 3) / 4, c->stream);
+            launch_vote(c->bits.p, dskip, (uint32_t)n, c->votes.p, c->stream);
+            TRY(check_launch("rays"));
+        }
+        if (hostVotes) {
+            HIP_TRY(hipMemcpyAsync(hostVotes, c->votes.p, n, hipMemcpyDeviceToHost, c->stream));
+            HIP_TRY(hipStreamSynchronize(c->stream));
+        }
+        return SBG_OK;
     }
     for (int attempt = 0; attempt < 2; ++attempt) {
         TRY(ensure_grid(target));
         {
             Stage st(c, SBG_STAGE_CLASSIFY);
             launch_fill_u32(c->ovf.p, 0u, 2, c->stream);
-            launch_rays(grid_set(target), target->tri64.p, dpts, dorder, dskip, (uint32_t)n, c->bits.p,
-                        c->rayList.p, c->ovf.p + 1, c->ovf.p, c->stream);
+            if (ordered)
+                launch_rays_ordered(grid_set(target), target->tri64.p, dpts, dorder, (uint32_t)n, c->bits.p,
+                                    c->rayList.p, c->ovf.p + 1, c->ovf.p, c->stream);
+            else
+                launch_rays(grid_set(target), target->tri64.p, dpts, dorder, dskip, (uint32_t)n, c->bits.p,
+                            c->rayList.p, c->ovf.p + 1, c->ovf.p, c->stream);
             launch_vote(c->bits.p, dskip, (uint32_t)n, c->votes.p, c->stream);
             TRY(check_launch("rays"));
         }
@@ -890,6 +930,8 @@ static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder,
         HIP_TRY(hipMemcpyAsync(h, c->ovf.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipMemcpyAsync(h + 1, target->gridCells.p + (grid_cells(target->gridG) - 1),
                                sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        if (hostVotes)
+            HIP_TRY(hipMemcpyAsync(hostVotes, c->votes.p, n, hipMemcpyDeviceToHost, c->stream));
         HIP_TRY(hipStreamSynchronize(c->stream));
         if (h[0] & 2u) {
             target->gridCap = (size_t)h[1] + h[1] / 16 + 1024;
@@ -919,9 +961,7 @@ int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *a
     TRY(c->bits.ensure(3 * n + 4));
     TRY(c->votes.ensure(n));
     HIP_TRY(hipMemcpyAsync(c->pts.p, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-    TRY(run_rays(target, c->pts.p, nullptr, nullptr, n));
-    if (inside)
-        HIP_TRY(hipMemcpyAsync(inside, c->votes.p, n, hipMemcpyDeviceToHost, c->stream));
+    TRY(run_rays(target, c->pts.p, nullptr, nullptr, n, false, inside));
     std::vector<uint8_t> bits;
     if (axis_inside) {
         bits.resize(3 * n);
@@ -956,20 +996,18 @@ int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side,
     if (!nt)
         return SBG_OK;
     TRY(c->flags.ensure(nt + 4));
-    TRY(c->pts.ensure(3 * nt));
+    TRY(c->pts.ensure(4 * nt));
     TRY(c->bits.ensure(3 * nt + 4));
     TRY(c->votes.ensure(nt));
     {
         Stage st(c, SBG_STAGE_CLASSIFY);
         launch_fill_u32((uint32_t *)c->flags.p, 0u, (nt + 3) / 4, c->stream);
         launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, c->flags.p, c->stream);
-        launch_centroids(self->tri64.p, c->flags.p, (uint32_t)nt, c->pts.p, c->stream);
+        launch_centroids_ordered(self->tri64.p, c->flags.p, self->sortedIds, nullptr, (uint32_t)nt, c->pts.p,
+                                 c->stream);
         TRY(check_launch("classify"));
     }
-    TRY(run_rays(other, c->pts.p, self->sortedIds, c->flags.p, nt));
-    HIP_TRY(hipMemcpyAsync(out, c->votes.p, nt, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return SBG_OK;
+    return run_rays(other, c->pts.p, self->sortedIds, c->flags.p, nt, true, out);
 }
 
 int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t)

@@ -60,8 +60,7 @@ struct sbg_batch_s {
     DevBuf<uint32_t> rayList;
     DevBuf<uint2> gridDesc;
     DevBuf<uint32_t> gridCells, gridTemp;
-    DevBuf<uint2> gridCode;
-    DevBuf<uint32_t> gridTri;
+    DevBuf<uint3> gridEnt;
 
     size_t frontCap = 0, candCap = 0, gridCap = 0;
     uint32_t spillPerWave = kBatchSpillInit;
@@ -136,8 +135,7 @@ int size_buffers(sbg_batch b)
     TRY(b->hist.ensure(std::max(sort_hist_entries(nt), sort_hist_entries(b->candCap))));
     TRY(b->gridCells.ensure((size_t)b->ncells + 1));
     TRY(b->gridTemp.ensure(scan_large_temp((size_t)b->ncells + 1)));
-    TRY(b->gridCode.ensure(b->gridCap));
-    TRY(b->gridTri.ensure(b->gridCap));
+    TRY(b->gridEnt.ensure(b->gridCap));
     return SBG_OK;
 }
 
@@ -194,21 +192,20 @@ int enqueue(sbg_batch b, int precision)
     // classification of every non-intersected triangle against its partner
     launch_fill_u32((uint32_t *)b->flags.p, 0u, (nt + 3) / 4, s);
     launch_mark_both(sorted, b->status.p, candCount, ccap, b->flags.p, s);
-    launch_centroids(b->tri64.p, b->flags.p, nt, b->pts.p, s);
+    launch_centroids_ordered(b->tri64.p, b->flags.p, b->ids.p, b->target.p, nt, b->pts.p, s);
     GridSet gs = {};
     gs.meshOf = b->meshOf.p;
     gs.target = b->target.p;
     gs.desc = b->gridDesc.p;
     gs.boxes = b->meshBox.p;
     gs.cells = b->gridCells.p;
-    gs.gcode = b->gridCode.p;
-    gs.gtri = b->gridTri.p;
+    gs.gent = b->gridEnt.p;
     gs.cap = (uint32_t)b->gridCap;
     gs.ncells = b->ncells;
     gs.order = b->ids.p;
-    launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, s);
-    launch_rays(gs, b->tri64.p, b->pts.p, b->ids.p, b->flags.p, nt, b->bits.p, b->rayList.p,
-                w + kRayListW, w + kRayFlagW, s);
+    launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridEnt.p, s);
+    launch_rays_ordered(gs, b->tri64.p, b->pts.p, b->ids.p, nt, b->bits.p, b->rayList.p, w + kRayListW,
+                        w + kRayFlagW, s);
     launch_vote(b->bits.p, b->flags.p, nt, b->votes.p, s);
     launch_copy_u32(b->gridCells.p + b->ncells, w + kGridW, 1, s);
     TRY(check_launch("batch classify"));
@@ -288,7 +285,7 @@ int sbg_batch_create(sbg_context ctx, const double *xyz, const uint32_t *tris, c
         (rc = b->refitScratch.ensure(refit_scratch_floats((uint32_t)nt))) ||
         (rc = b->meshRoot.ensure(nmeshes)) || (rc = b->nodes.ensure(nt > 1 ? nt - 1 : 1)) ||
         (rc = b->words.ensure(kWords)) || (rc = b->flags.ensure(nt + 4)) || (rc = b->bits.ensure(3 * nt + 4)) ||
-        (rc = b->votes.ensure(nt)) || (rc = b->pts.ensure(3 * nt)) || (rc = b->rayList.ensure(3 * nt)))
+        (rc = b->votes.ensure(nt)) || (rc = b->pts.ensure(4 * nt)) || (rc = b->rayList.ensure(3 * nt)))
         return bail(rc);
     if (hipHostMalloc((void **)&b->host, sizeof(uint32_t) * kWords, hipHostMallocDefault) != hipSuccess) {
         b->host = nullptr;
@@ -613,8 +610,7 @@ int sbg_batch_destroy(sbg_batch b)
     b->front[1].release();
     b->spill.release();
     b->gridDesc.release();
-    b->gridCode.release();
-    b->gridTri.release();
+    b->gridEnt.release();
     delete b;
     context_release(c);
     return SBG_OK;

@@ -100,6 +100,8 @@ struct sbg_context_s {
     };
     std::vector<Ev> events;
     double acc[SBG_STAGE_COUNT] = {0};
+    // the buffers of the last destroyed pair list, handed to the next one
+    struct sbg_pairs_s *sparePairs = nullptr;
 };
 
 namespace sbg {
