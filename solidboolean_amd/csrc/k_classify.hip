@@ -17,7 +17,7 @@
 // triangle boxes instead of a tree walk: a ray's candidates are the entries of
 // the one cell holding its origin, a superset of the reference's set that the
 // exact FP64 box test then cuts down to it.  k_rays_grid scans a cell of up
-// to 48 entries with up to 4 distinct keys in registers; a ray with a crowded
+// to 48 entries with up to 3 distinct keys in registers; a ray with a crowded
 // cell or more hits goes to k_rays_wave, where one wave splits the cell and
 // counts distinct keys in LDS (up to 256 hits).  A ray past that raises
 // `overflow` and the call fails: nothing is guessed.
@@ -30,7 +30,7 @@
 
 namespace sbg {
 
-constexpr int kFastKeys = 4;     // distinct keys a fast lane keeps
+constexpr int kFastKeys = 3;     // distinct keys a fast lane keeps
 constexpr int kFastCands = 48;   // cell entries a fast lane scans alone
 constexpr int kCodeGroup = 4;    // entry codes a fast lane loads at once
 constexpr int kWaveHits = 256;   // hits a wave gathers for one deferred ray
@@ -90,17 +90,22 @@ __global__ __launch_bounds__(256) void k_centroids_ordered(const double *__restr
 // A ray is the segment from p to e = p + dir (src/solidboolean.cpp:60-66):
 // dir is DBL_MAX along the axis and DBL_EPSILON along the other two, so
 // e >= p in every component and the segment's box (:67-69) is exactly [p, e].
+// Only p is kept; e is recomputed where it is needed.
 struct Ray {
-    V3 p, e;
+    V3 p;
+    uint32_t axis;
+    __device__ __forceinline__ V3 end() const
+    {
+        const double eps = 2.220446049250313080847e-16, big = 1.7976931348623157e308;
+        return {p.x + (axis == 0 ? big : eps), p.y + (axis == 1 ? big : eps), p.z + (axis == 2 ? big : eps)};
+    }
 };
 
 __device__ __forceinline__ Ray ray_from(const V3 &p, uint32_t axis)
 {
-    const double eps = 2.220446049250313080847e-16, big = 1.7976931348623157e308;
     Ray r;
     r.p = p;
-    const V3 dir = {axis == 0 ? big : eps, axis == 1 ? big : eps, axis == 2 ? big : eps};
-    r.e = {r.p.x + dir.x, r.p.y + dir.y, r.p.z + dir.z};
+    r.axis = axis;
     return r;
 }
 
@@ -183,9 +188,10 @@ __device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ t
     const double2 *x = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * t);
     const double2 r0 = x[0], r1 = x[1], r2 = x[2], r3 = x[3], r4 = x[4], r5 = x[5];
     const V3 v0 = {r0.x, r0.y, r1.x}, v1 = {r2.x, r2.y, r3.x}, v2 = {r4.x, r4.y, r5.x};
-    if (!box64_overlap(tri_box64(v0, v1, v2), Box64{r.p, r.e}))
+    const V3 e = r.end();
+    if (!box64_overlap(tri_box64(v0, v1, v2), Box64{r.p, e}))
         return false;
-    const V3 u = vsub(r.e, r.p);
+    const V3 u = vsub(e, r.p);
     const V3 n = {r1.y, r3.y, r5.y};  // the record's pad slots hold the FP64 normal
     const V3 w = vsub(r.p, v0);
     const double d = n.x * u.x + n.y * u.y + n.z * u.z;
@@ -215,7 +221,8 @@ __device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ t
 // meshes (a batch) share one cell array: mesh m owns 3 G_m^2 cells from its
 // first cell on.
 //
-// An entry is 12 bytes: two code words and the triangle index.  The codes are
+// An entry is two 4-byte code words (one array) and the triangle index (a
+// second array, read only for entries that pass the code test).  The codes are
 // the triangle's FP64 box bounds lo_u, hi_u, lo_v, hi_v relative to the
 // entry's cell (12 bits each, in 1/4096 of a cell, clamped to the cell) and
 // hi_a relative to the mesh box along the axis (16 bits, in 1/65536 of its
@@ -282,12 +289,11 @@ __device__ __forceinline__ uint32_t cell_code(double t, uint32_t c)
     return code_of((t - (double)c) * kCellSteps, kCellMax);
 }
 
-// entry: x = lo_u | hi_u << 12 | (hi_a & 255) << 24,
-//        y = lo_v | hi_v << 12 | (hi_a >> 8) << 24, z = triangle
-__device__ __forceinline__ uint3 pack_entry(uint32_t lu, uint32_t hu, uint32_t lv, uint32_t hv, uint32_t ha,
-                                            uint32_t t)
+// code words: x = lo_u | hi_u << 12 | (hi_a & 255) << 24,
+//             y = lo_v | hi_v << 12 | (hi_a >> 8) << 24
+__device__ __forceinline__ uint2 pack_code(uint32_t lu, uint32_t hu, uint32_t lv, uint32_t hv, uint32_t ha)
 {
-    return make_uint3(lu | hu << 12 | (ha & 255u) << 24, lv | hv << 12 | (ha >> 8) << 24, t);
+    return make_uint2(lu | hu << 12 | (ha & 255u) << 24, lv | hv << 12 | (ha >> 8) << 24);
 }
 
 __device__ __forceinline__ double comp(const V3 &a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
@@ -322,9 +328,9 @@ __device__ __forceinline__ GridSpan grid_span(const Box64 &b, int axis, const Gr
 }
 
 // the entry of span s in cell (cu, cv)
-__device__ __forceinline__ uint3 span_entry(const GridSpan &s, uint32_t cu, uint32_t cv, uint32_t t)
+__device__ __forceinline__ uint2 span_code(const GridSpan &s, uint32_t cu, uint32_t cv)
 {
-    return pack_entry(cell_code(s.lu, cu), cell_code(s.hu, cu), cell_code(s.lv, cv), cell_code(s.hv, cv), s.ha, t);
+    return pack_code(cell_code(s.lu, cu), cell_code(s.hu, cu), cell_code(s.lv, cv), cell_code(s.hv, cv), s.ha);
 }
 
 // Grid build: a workgroup takes kGridBlockTris consecutive triangles of the
@@ -428,7 +434,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const double *__r
 // afterwards cell c spans [cells[c], cells[c + 1]).
 __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt,
                                                               GridSet gs, uint32_t *__restrict__ cells,
-                                                              uint3 *__restrict__ gent)
+                                                              uint2 *__restrict__ gcode, uint32_t *__restrict__ gtri)
 {
     __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
     const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
@@ -465,7 +471,8 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__re
                 const int sl = table_slot(s_key, key);
                 const uint32_t slot = sl >= 0 ? atomicAdd(&s_cnt[sl], 1u) : atomicAdd(&cells[key], 1u);
                 if (slot < gs.cap) {
-                    gent[slot] = span_entry(x.s[axis], cu, cv, x.t);
+                    gcode[slot] = span_code(x.s[axis], cu, cv);
+                    gtri[slot] = x.t;
                 }
             }
 }
@@ -478,7 +485,8 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__re
 // seen edge-on.
 struct RayCell {
     uint2 range;
-    uint32_t pu, eu, pv, ev, pa;
+    uint32_t u, v;  // code of p | code of e << 16, along u and along v
+    uint32_t pa;    // code of p along the axis
 };
 
 __device__ __forceinline__ RayCell ray_cell(const GridSet &gs, uint32_t m, const Ray &r, int axis)
@@ -486,8 +494,9 @@ __device__ __forceinline__ RayCell ray_cell(const GridSet &gs, uint32_t m, const
     RayCell c;
     c.range = make_uint2(0u, 0u);
     const float *bx = gs.boxes + 6 * (size_t)m;
-    if (!(bx[0] <= f_up(r.e.x) && bx[3] >= f_down(r.p.x) && bx[1] <= f_up(r.e.y) && bx[4] >= f_down(r.p.y) &&
-          bx[2] <= f_up(r.e.z) && bx[5] >= f_down(r.p.z)))
+    const V3 e = r.end();
+    if (!(bx[0] <= f_up(e.x) && bx[3] >= f_down(r.p.x) && bx[1] <= f_up(e.y) && bx[4] >= f_down(r.p.y) &&
+          bx[2] <= f_up(e.z) && bx[5] >= f_down(r.p.z)))
         return c;
     const uint2 d = grid_desc(gs, m);
     const uint32_t G = d.x;
@@ -495,10 +504,8 @@ __device__ __forceinline__ RayCell ray_cell(const GridSet &gs, uint32_t m, const
     const int u = axis_u(axis), v = axis_v(axis);
     const double tu = cell_coord(comp(r.p, u), f.lu, f.iu), tv = cell_coord(comp(r.p, v), f.lv, f.iv);
     const uint32_t cu = cell_of(tu, G), cv = cell_of(tv, G);
-    c.pu = cell_code(tu, cu);
-    c.eu = cell_code(cell_coord(comp(r.e, u), f.lu, f.iu), cu);
-    c.pv = cell_code(tv, cv);
-    c.ev = cell_code(cell_coord(comp(r.e, v), f.lv, f.iv), cv);
+    c.u = cell_code(tu, cu) | cell_code(cell_coord(comp(e, u), f.lu, f.iu), cu) << 16;
+    c.v = cell_code(tv, cv) | cell_code(cell_coord(comp(e, v), f.lv, f.iv), cv) << 16;
     c.pa = code_of(cell_coord(comp(r.p, axis), f.la, f.ia), kAxisMax);
     const size_t k = (size_t)d.y + (size_t)axis * G * G + (size_t)cu * G + cv;
     c.range = make_uint2(gs.cells[k], gs.cells[k + 1]);
@@ -506,19 +513,20 @@ __device__ __forceinline__ RayCell ray_cell(const GridSet &gs, uint32_t m, const
 }
 
 // code test of an entry against the ray (a superset of the FP64 box test)
-__device__ __forceinline__ bool code_may_hit(const uint3 &e, const RayCell &c)
+__device__ __forceinline__ bool code_may_hit(const uint2 &e, const RayCell &c)
 {
     const uint32_t ha = e.x >> 24 | (e.y >> 24) << 8;
-    return (e.x & kCellMax) <= c.eu && ((e.x >> 12) & kCellMax) >= c.pu && (e.y & kCellMax) <= c.ev &&
-           ((e.y >> 12) & kCellMax) >= c.pv && ha >= c.pa;
+    return (e.x & kCellMax) <= (c.u >> 16) && ((e.x >> 12) & kCellMax) >= (c.u & 0xffffu) &&
+           (e.y & kCellMax) <= (c.v >> 16) && ((e.y >> 12) & kCellMax) >= (c.v & 0xffffu) && ha >= c.pa;
 }
 
 // code test of grid entry k; on success `t` is the triangle
 __device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, const RayCell &c, uint32_t &t)
 {
-    const uint3 e = gs.gent[k];
-    t = e.z;
-    return code_may_hit(e, c);
+    if (!code_may_hit(gs.gcode[k], c))
+        return false;
+    t = gs.gtri[k];
+    return true;
 }
 
 // One lane per (axis, query).  A lane whose cell holds at most kFastCands
@@ -558,21 +566,17 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
     // range, the excess masked off), then the FP64 test of the entries passing
     for (uint32_t k0 = c.range.x; !defer && k0 < c.range.y; k0 += kCodeGroup) {
         uint32_t pass = 0;
-        uint32_t tri[kCodeGroup];
 #pragma unroll
         for (int j = 0; j < kCodeGroup; ++j) {
             const uint32_t k = k0 + j < c.range.y ? k0 + j : c.range.y - 1;
-            const uint3 e = gs.gent[k];
-            tri[j] = e.z;
-            if (code_may_hit(e, c) && k0 + j < c.range.y)
+            if (code_may_hit(gs.gcode[k], c) && k0 + j < c.range.y)
                 pass |= 1u << j;
         }
-#pragma unroll
-        for (int j = 0; j < kCodeGroup; ++j) {
-            if (!(pass >> j & 1u) || defer)
-                continue;
+        while (pass && !defer) {
+            const int j = __builtin_ctz(pass);
+            pass &= pass - 1;
             Key3 key;
-            if (ray_hit(tri[j], tri64, r, key)) {
+            if (ray_hit(gs.gtri[k0 + j], tri64, r, key)) {
                 keys.add(key, (int)axis);
                 defer = keys.full;
             }
@@ -687,7 +691,7 @@ size_t grid_cells(uint32_t G) { return 3 * (size_t)G * G + 1; }
 
 
 void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
-                       uint3 *gent, hipStream_t s)
+                       uint2 *gcode, uint32_t *gtri, hipStream_t s)
 {
     const size_t m = (size_t)gs.ncells + 1;
     launch_fill_u32(cells, 0u, m, s);
@@ -696,7 +700,7 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
     const dim3 grid(xcd_grid((nt + kGridBlockTris - 1) / kGridBlockTris));
     hipLaunchKernelGGL(k_grid_count, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
-    hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gent);
+    hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gcode, gtri);
 }
 
 void launch_rays(const GridSet &gs, const double *tri64, const double *pts,

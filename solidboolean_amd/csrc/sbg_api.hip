@@ -76,7 +76,8 @@ struct sbg_mesh_s {
     bool prepared = false;
     // per-axis ray grid, built on first use as a classification target
     DevBuf<uint32_t> gridCells, gridTemp;
-    DevBuf<uint3> gridEnt;
+    DevBuf<uint2> gridCode;
+    DevBuf<uint32_t> gridTri;
     size_t gridCap = 0;
     uint32_t gridG = 0;
     int gridDensity = 0;
@@ -336,7 +337,8 @@ int sbg_mesh_destroy(sbg_mesh m)
     m->nodes.release();
     m->gridCells.release();
     m->gridTemp.release();
-    m->gridEnt.release();
+    m->gridCode.release();
+    m->gridTri.release();
     sbg_context c = m->ctx;
     delete m;
     context_release(c);
@@ -851,7 +853,8 @@ static GridSet grid_set(sbg_mesh m)
     GridSet gs = {};
     gs.boxes = m->rootBox.p;
     gs.cells = m->gridCells.p;
-    gs.gent = m->gridEnt.p;
+    gs.gcode = m->gridCode.p;
+    gs.gtri = m->gridTri.p;
     gs.G0 = m->gridG;
     gs.cap = (uint32_t)m->gridCap;
     gs.ncells = (uint32_t)(grid_cells(m->gridG) - 1);
@@ -872,12 +875,13 @@ static int ensure_grid(sbg_mesh m)
         return fail(SBG_ERR_OVERFLOW, "ray grid needs %zu entries", m->gridCap);
     TRY(m->gridCells.ensure(grid_cells(G)));
     TRY(m->gridTemp.ensure(scan_large_temp(grid_cells(G))));
-    TRY(m->gridEnt.ensure(m->gridCap));
+    TRY(m->gridCode.ensure(m->gridCap));
+    TRY(m->gridTri.ensure(m->gridCap));
     m->gridG = G;
     {
         Stage st(c, SBG_STAGE_GRID);
-        launch_grid_build(m->tri64.p, (uint32_t)m->nt, grid_set(m), m->gridCells.p, m->gridTemp.p, m->gridEnt.p,
-                          c->stream);
+        launch_grid_build(m->tri64.p, (uint32_t)m->nt, grid_set(m), m->gridCells.p, m->gridTemp.p, m->gridCode.p,
+                          m->gridTri.p, c->stream);
         TRY(check_launch("ray grid"));
     }
     m->gridReady = true;

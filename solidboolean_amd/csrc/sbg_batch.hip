@@ -60,7 +60,8 @@ struct sbg_batch_s {
     DevBuf<uint32_t> rayList;
     DevBuf<uint2> gridDesc;
     DevBuf<uint32_t> gridCells, gridTemp;
-    DevBuf<uint3> gridEnt;
+    DevBuf<uint2> gridCode;
+    DevBuf<uint32_t> gridTri;
 
     size_t frontCap = 0, candCap = 0, gridCap = 0;
     uint32_t spillPerWave = kBatchSpillInit;
@@ -135,7 +136,8 @@ int size_buffers(sbg_batch b)
     TRY(b->hist.ensure(std::max(sort_hist_entries(nt), sort_hist_entries(b->candCap))));
     TRY(b->gridCells.ensure((size_t)b->ncells + 1));
     TRY(b->gridTemp.ensure(scan_large_temp((size_t)b->ncells + 1)));
-    TRY(b->gridEnt.ensure(b->gridCap));
+    TRY(b->gridCode.ensure(b->gridCap));
+    TRY(b->gridTri.ensure(b->gridCap));
     return SBG_OK;
 }
 
@@ -199,11 +201,12 @@ int enqueue(sbg_batch b, int precision)
     gs.desc = b->gridDesc.p;
     gs.boxes = b->meshBox.p;
     gs.cells = b->gridCells.p;
-    gs.gent = b->gridEnt.p;
+    gs.gcode = b->gridCode.p;
+    gs.gtri = b->gridTri.p;
     gs.cap = (uint32_t)b->gridCap;
     gs.ncells = b->ncells;
     gs.order = b->ids.p;
-    launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridEnt.p, s);
+    launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, s);
     launch_rays_ordered(gs, b->tri64.p, b->pts.p, b->ids.p, nt, b->bits.p, b->rayList.p, w + kRayListW,
                         w + kRayFlagW, s);
     launch_vote(b->bits.p, b->flags.p, nt, b->votes.p, s);
@@ -610,7 +613,8 @@ int sbg_batch_destroy(sbg_batch b)
     b->front[1].release();
     b->spill.release();
     b->gridDesc.release();
-    b->gridEnt.release();
+    b->gridCode.release();
+    b->gridTri.release();
     delete b;
     context_release(c);
     return SBG_OK;

@@ -127,14 +127,15 @@ struct GridSet {
     const uint2 *desc;       // mesh -> (G, first cell) (null: (G0, 0))
     const float *boxes;      // mesh -> FP32 root box (lo xyz, hi xyz)
     const uint32_t *cells;
-    const uint3 *gent;       // per entry: two box code words, triangle
+    const uint2 *gcode;      // per entry: two box code words
+    const uint32_t *gtri;    // per entry: the triangle
     const uint32_t *order;  // grid build's triangle visiting order, e.g. Morton (null: index order)
     uint32_t G0, cap, ncells;
 };
 uint32_t grid_dim(uint32_t nt, int density);
 size_t grid_cells(uint32_t G);  // 3 G^2 + 1
 void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
-                       uint3 *gent, hipStream_t s);
+                       uint2 *gcode, uint32_t *gtri, hipStream_t s);
 // order (may be null): query q uses point order[q]; ovfList holds 3n entries,
 // *ovfCount must be zero on entry; overflow bit 0: a ray has too many distinct
 // hits, bit 1: the grid was truncated (cells[ncells] > cap)
