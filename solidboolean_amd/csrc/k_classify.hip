@@ -534,16 +534,19 @@ __device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, con
 // lane (a crowded cell, or more distinct hits) hands (axis, query) to
 // k_rays_wave.  Query i's rays go to mesh target[i] (mesh 0 without a target
 // list); a query without a target mesh (~0u) counts as outside.
-template <bool Q4>
+template <bool Q4, bool LIST>
 __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__restrict__ tri64,
                                                    const double *__restrict__ pts,
                                                    const uint32_t *__restrict__ order, const uint8_t *__restrict__ skip,
-                                                   uint32_t n, uint8_t *__restrict__ bits, uint32_t *__restrict__ ovfList,
+                                                   uint32_t n, uint32_t lanes, const uint32_t *__restrict__ laneList,
+                                                   const uint32_t *__restrict__ laneCount,
+                                                   uint8_t *__restrict__ bits, uint32_t *__restrict__ ovfList,
                                                    uint32_t *ovfCount, uint32_t *overflow)
 {
-    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= 3u * n)
+    const uint32_t idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (LIST ? *laneCount : lanes))
         return;
+    const uint32_t g = LIST ? laneList[idx] : idx;
     if (gs.cells[gs.ncells] > gs.cap) {  // grid truncated: the host rebuilds it larger
         if ((threadIdx.x & 63) == 0)
             atomicOr(overflow, 2u);
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
     }
     if (defer) {
         const uint32_t slot = atomicAdd(ovfCount, 1u);
-        ovfList[slot] = g;  // slot < 3n: every lane appends at most once
+        ovfList[slot] = g;  // slot < lanes of this launch: every lane appends at most once
         return;
     }
     bits[(size_t)axis * n + i] = (uint8_t)(keys.n & 1);
@@ -649,8 +652,40 @@ __global__ __launch_bounds__(64) void k_rays_wave(GridSet gs, const double *__re
     }
 }
 
+// The vote needs the z ray only where the x and y rays disagree: lists the
+// lanes 2n + q of those queries (one atomic per wave).
+template <bool Q4>
+__global__ __launch_bounds__(256) void k_third_axis(const double *__restrict__ pts,
+                                                    const uint32_t *__restrict__ order,
+                                                    const uint8_t *__restrict__ skip, uint32_t n,
+                                                    const uint8_t *__restrict__ bits, uint32_t *__restrict__ list,
+                                                    uint32_t *count)
+{
+    const uint32_t q = blockIdx.x * 256 + threadIdx.x;
+    bool need = false;
+    if (q < n) {
+        const uint32_t i = order ? order[q] : q;
+        const bool skipped = Q4 ? reinterpret_cast<const double2 *>(pts)[2 * (size_t)q + 1].y == kTagSkip
+                                : (skip && skip[i]);
+        need = !skipped && bits[i] != bits[(size_t)n + i];
+    }
+    const unsigned long long mask = __ballot(need);
+    if (!mask)
+        return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader)
+        base = atomicAdd(count, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (need)
+        list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = 2 * n + q;
+}
+
 // decideGroupSide's vote (src/solidboolean.cpp:492-508): inside iff
-// (float)inside / 3 > 0.5, i.e. at least 2 of the 3 rays.
+// (float)inside / 3 > 0.5, i.e. at least 2 of the 3 rays.  The z bit is read
+// only where the x and y bits differ (it is not cast elsewhere), which gives
+// the same majority.
 __global__ __launch_bounds__(256) void k_vote(const uint8_t *__restrict__ bits, const uint8_t *__restrict__ skip,
                                               uint32_t n, uint8_t *__restrict__ out)
 {
@@ -661,7 +696,8 @@ __global__ __launch_bounds__(256) void k_vote(const uint8_t *__restrict__ bits, 
         out[i] = 2;
         return;
     }
-    const int inside = bits[i] + bits[n + i] + bits[2 * (size_t)n + i];
+    const int bx = bits[i], by = bits[n + i];
+    const int inside = bx == by ? 2 * bx : bx + by + bits[2 * (size_t)n + i];
     out[i] = ((float)inside / 3.0f > 0.5f) ? 1 : 0;
 }
 
@@ -703,17 +739,40 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
     hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gcode, gtri);
 }
 
-void launch_rays(const GridSet &gs, const double *tri64, const double *pts,
-                 const uint32_t *order, const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList,
-                 uint32_t *ovfCount, uint32_t *overflow, hipStream_t s)
+// Rays of n queries.  allAxes: every query casts x, y and z; otherwise x and
+// y, then z only where they disagree (enough for the vote; the per-axis bits
+// of z are then valid only there).  lists: 4n u32; counters: 4 u32, zero on
+// entry ([0] overflow bits, [1] deferred x/y rays, [2] z lanes, [3] deferred
+// z rays).
+template <bool Q4>
+static void rays(const GridSet &gs, const double *tri64, const double *pts, const uint32_t *order,
+                 const uint8_t *skip, uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists,
+                 uint32_t *counters, hipStream_t s)
 {
     if (n == 0)
         return;
-    const uint64_t lanes = 3ull * n;
-    hipLaunchKernelGGL(k_rays_grid<false>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, gs, tri64, pts,
-                       order, skip, n, axisBits, ovfList, ovfCount, overflow);
-    hipLaunchKernelGGL(k_rays_wave<false>, dim3(8192), dim3(64), 0, s, gs, tri64, pts, order, n, axisBits, ovfList,
-                       ovfCount, overflow);
+    uint32_t *overflow = counters;
+    const uint32_t lanes = allAxes ? 3u * n : 2u * n;
+    uint32_t *deferred = lists, *zLanes = lists + 3 * (size_t)n, *zDeferred = lists + 2 * (size_t)n;
+    hipLaunchKernelGGL((k_rays_grid<Q4, false>), dim3((lanes + 255) / 256), dim3(256), 0, s, gs, tri64, pts, order,
+                       skip, n, lanes, nullptr, nullptr, axisBits, deferred, counters + 1, overflow);
+    hipLaunchKernelGGL(k_rays_wave<Q4>, dim3(8192), dim3(64), 0, s, gs, tri64, pts, order, n, axisBits, deferred,
+                       counters + 1, overflow);
+    if (allAxes)
+        return;
+    hipLaunchKernelGGL(k_third_axis<Q4>, dim3((n + 255) / 256), dim3(256), 0, s, pts, order, skip, n, axisBits,
+                       zLanes, counters + 2);
+    hipLaunchKernelGGL((k_rays_grid<Q4, true>), dim3((n + 255) / 256), dim3(256), 0, s, gs, tri64, pts, order, skip,
+                       n, n, zLanes, counters + 2, axisBits, zDeferred, counters + 3, overflow);
+    hipLaunchKernelGGL(k_rays_wave<Q4>, dim3(8192), dim3(64), 0, s, gs, tri64, pts, order, n, axisBits, zDeferred,
+                       counters + 3, overflow);
+}
+
+void launch_rays(const GridSet &gs, const double *tri64, const double *pts, const uint32_t *order,
+                 const uint8_t *skip, uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists,
+                 uint32_t *counters, hipStream_t s)
+{
+    rays<false>(gs, tri64, pts, order, skip, n, allAxes, axisBits, lists, counters, s);
 }
 
 void launch_centroids_ordered(const double *tri64, const uint8_t *flags, const uint32_t *order,
@@ -726,16 +785,10 @@ void launch_centroids_ordered(const double *tri64, const uint8_t *flags, const u
 }
 
 void launch_rays_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
-                         uint32_t n, uint8_t *axisBits, uint32_t *ovfList, uint32_t *ovfCount, uint32_t *overflow,
+                         uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists, uint32_t *counters,
                          hipStream_t s)
 {
-    if (n == 0)
-        return;
-    const uint64_t lanes = 3ull * n;
-    hipLaunchKernelGGL(k_rays_grid<true>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, s, gs, tri64, q4,
-                       order, nullptr, n, axisBits, ovfList, ovfCount, overflow);
-    hipLaunchKernelGGL(k_rays_wave<true>, dim3(8192), dim3(64), 0, s, gs, tri64, q4, order, n, axisBits, ovfList,
-                       ovfCount, overflow);
+    rays<true>(gs, tri64, q4, order, nullptr, n, allAxes, axisBits, lists, counters, s);
 }
 
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s)

@@ -898,11 +898,11 @@ static int ensure_grid(sbg_mesh m)
 // copied out under the same stream synchronisation that reads the overflow
 // flags.
 static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder, const uint8_t *dskip, size_t n,
-                    bool ordered, uint8_t *hostVotes)
+                    bool ordered, bool allAxes, uint8_t *hostVotes)
 {
     sbg_context c = target->ctx;
-    TRY(c->ovf.ensure(2));
-    TRY(c->rayList.ensure(3 * n));
+    TRY(c->ovf.ensure(4));
+    TRY(c->rayList.ensure(4 * n));
     if (target->nt == 0) {
         {
             Stage st(c, SBG_STAGE_CLASSIFY);
@@ -920,13 +920,13 @@ static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder,
         TRY(ensure_grid(target));
         {
             Stage st(c, SBG_STAGE_CLASSIFY);
-            launch_fill_u32(c->ovf.p, 0u, 2, c->stream);
+            launch_fill_u32(c->ovf.p, 0u, 4, c->stream);
             if (ordered)
-                launch_rays_ordered(grid_set(target), target->tri64.p, dpts, dorder, (uint32_t)n, c->bits.p,
-                                    c->rayList.p, c->ovf.p + 1, c->ovf.p, c->stream);
+                launch_rays_ordered(grid_set(target), target->tri64.p, dpts, dorder, (uint32_t)n, allAxes, c->bits.p,
+                                    c->rayList.p, c->ovf.p, c->stream);
             else
-                launch_rays(grid_set(target), target->tri64.p, dpts, dorder, dskip, (uint32_t)n, c->bits.p,
-                            c->rayList.p, c->ovf.p + 1, c->ovf.p, c->stream);
+                launch_rays(grid_set(target), target->tri64.p, dpts, dorder, dskip, (uint32_t)n, allAxes, c->bits.p,
+                            c->rayList.p, c->ovf.p, c->stream);
             launch_vote(c->bits.p, dskip, (uint32_t)n, c->votes.p, c->stream);
             TRY(check_launch("rays"));
         }
@@ -965,7 +965,7 @@ int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *a
     TRY(c->bits.ensure(3 * n + 4));
     TRY(c->votes.ensure(n));
     HIP_TRY(hipMemcpyAsync(c->pts.p, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-    TRY(run_rays(target, c->pts.p, nullptr, nullptr, n, false, inside));
+    TRY(run_rays(target, c->pts.p, nullptr, nullptr, n, false, axis_inside != nullptr, inside));
     std::vector<uint8_t> bits;
     if (axis_inside) {
         bits.resize(3 * n);
@@ -1011,7 +1011,7 @@ int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side,
                                  c->stream);
         TRY(check_launch("classify"));
     }
-    return run_rays(other, c->pts.p, self->sortedIds, c->flags.p, nt, true, out);
+    return run_rays(other, c->pts.p, self->sortedIds, c->flags.p, nt, true, false, out);
 }
 
 int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t)

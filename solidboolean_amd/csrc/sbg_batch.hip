@@ -19,10 +19,10 @@ namespace {
 constexpr int kBatchMaxLevels = 32;
 constexpr uint32_t kBatchSpillInit = 2048, kBatchSpillMax = 1u << 15;
 // readback words: [0, 33) frontier sizes, 33 candidates, 34 search flags,
-// 35 depth-first claims, 36 ray flags, 37 deferred rays, 38 grid entries
+// 35 depth-first claims, [36, 40) ray counters (36 ray flags), 40 grid entries
 constexpr int kCandW = kBatchMaxLevels + 1, kFlagW = kBatchMaxLevels + 2, kHeadW = kBatchMaxLevels + 3;
-constexpr int kRayFlagW = kBatchMaxLevels + 4, kRayListW = kBatchMaxLevels + 5, kGridW = kBatchMaxLevels + 6;
-constexpr int kWords = kBatchMaxLevels + 8;
+constexpr int kRayFlagW = kBatchMaxLevels + 4, kGridW = kBatchMaxLevels + 8;
+constexpr int kWords = kBatchMaxLevels + 10;
 
 int bits_for(uint64_t n)
 {
@@ -207,8 +207,7 @@ int enqueue(sbg_batch b, int precision)
     gs.ncells = b->ncells;
     gs.order = b->ids.p;
     launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, s);
-    launch_rays_ordered(gs, b->tri64.p, b->pts.p, b->ids.p, nt, b->bits.p, b->rayList.p, w + kRayListW,
-                        w + kRayFlagW, s);
+    launch_rays_ordered(gs, b->tri64.p, b->pts.p, b->ids.p, nt, false, b->bits.p, b->rayList.p, w + kRayFlagW, s);
     launch_vote(b->bits.p, b->flags.p, nt, b->votes.p, s);
     launch_copy_u32(b->gridCells.p + b->ncells, w + kGridW, 1, s);
     TRY(check_launch("batch classify"));
@@ -288,7 +287,7 @@ int sbg_batch_create(sbg_context ctx, const double *xyz, const uint32_t *tris, c
         (rc = b->refitScratch.ensure(refit_scratch_floats((uint32_t)nt))) ||
         (rc = b->meshRoot.ensure(nmeshes)) || (rc = b->nodes.ensure(nt > 1 ? nt - 1 : 1)) ||
         (rc = b->words.ensure(kWords)) || (rc = b->flags.ensure(nt + 4)) || (rc = b->bits.ensure(3 * nt + 4)) ||
-        (rc = b->votes.ensure(nt)) || (rc = b->pts.ensure(4 * nt)) || (rc = b->rayList.ensure(3 * nt)))
+        (rc = b->votes.ensure(nt)) || (rc = b->pts.ensure(4 * nt)) || (rc = b->rayList.ensure(4 * nt)))
         return bail(rc);
     if (hipHostMalloc((void **)&b->host, sizeof(uint32_t) * kWords, hipHostMallocDefault) != hipSuccess) {
         b->host = nullptr;

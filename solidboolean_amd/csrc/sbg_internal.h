@@ -136,20 +136,23 @@ uint32_t grid_dim(uint32_t nt, int density);
 size_t grid_cells(uint32_t G);  // 3 G^2 + 1
 void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
                        uint2 *gcode, uint32_t *gtri, hipStream_t s);
-// order (may be null): query q uses point order[q]; ovfList holds 3n entries,
-// *ovfCount must be zero on entry; overflow bit 0: a ray has too many distinct
-// hits, bit 1: the grid was truncated (cells[ncells] > cap)
-// tri64 records carry the FP64 normal in their pad slots
-void launch_rays(const GridSet &gs, const double *tri64, const double *pts,
-                 const uint32_t *order, const uint8_t *skip, uint32_t n, uint8_t *axisBits, uint32_t *ovfList,
-                 uint32_t *ovfCount, uint32_t *overflow, hipStream_t s);
+// order (may be null): query q uses point order[q].  allAxes: cast x, y and z
+// for every query; otherwise x and y, and z only where they disagree (the
+// vote's answer is the same; z's bits are then valid only there).  lists:
+// 4n u32 of scratch; counters: 4 u32, zero on entry, counters[0] gets the
+// overflow bits (bit 0: a ray has too many distinct hits, bit 1: the grid was
+// truncated, cells[ncells] > cap).  tri64 records carry the FP64 normal in
+// their pad slots.
+void launch_rays(const GridSet &gs, const double *tri64, const double *pts, const uint32_t *order,
+                 const uint8_t *skip, uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists,
+                 uint32_t *counters, hipStream_t s);
 // Visiting-order queries: q4 holds 4 nt doubles, (x, y, z, tag) of triangle
 // order[q] at 4q (tag: target mesh, or skip / no target); the rays write
 // axisBits by triangle like launch_rays.
 void launch_centroids_ordered(const double *tri64, const uint8_t *flags, const uint32_t *order,
                               const uint32_t *target, uint32_t nt, double *q4, hipStream_t s);
 void launch_rays_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
-                         uint32_t n, uint8_t *axisBits, uint32_t *ovfList, uint32_t *ovfCount, uint32_t *overflow,
+                         uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists, uint32_t *counters,
                          hipStream_t s);
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);
 
