@@ -348,14 +348,33 @@ constexpr int kGridSlots = 1 << kGridSlotBits;
 constexpr int kProbe = 64;
 
 // slot holding `key` in the LDS table (inserting it), or -1.  Slots only fill,
-// so a key that once found no slot never finds one later in the same pass.
+// so a key that once found no slot never finds one later in the same pass.  A
+// plain read comes first: most entries find their cell already there, and
+// reads of one slot by many lanes do not serialise the way atomics do.
 __device__ __forceinline__ int table_slot(uint32_t *keys, uint32_t key)
 {
     uint32_t h = (key * 2654435761u) >> (32 - kGridSlotBits);
     for (int probe = 0; probe < kProbe; ++probe) {
-        const uint32_t cur = atomicCAS(&keys[h], 0u, key);
+        uint32_t cur = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == 0u)
+            cur = atomicCAS(&keys[h], 0u, key);
         if (cur == 0u || cur == key)
             return (int)h;
+        h = (h + 1) & (kGridSlots - 1);
+    }
+    return -1;
+}
+
+// slot of `key` in a table that no longer changes, or -1 when it is not there
+__device__ __forceinline__ int table_find(const uint32_t *keys, uint32_t key)
+{
+    uint32_t h = (key * 2654435761u) >> (32 - kGridSlotBits);
+    for (int probe = 0; probe < kProbe; ++probe) {
+        const uint32_t cur = keys[h];
+        if (cur == key)
+            return (int)h;
+        if (cur == 0u)
+            return -1;
         h = (h + 1) & (kGridSlots - 1);
     }
     return -1;
@@ -468,7 +487,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__re
         for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
             for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
                 const uint32_t key = cell_key(x, axis, cu, cv);
-                const int sl = table_slot(s_key, key);
+                const int sl = table_find(s_key, key);
                 const uint32_t slot = sl >= 0 ? atomicAdd(&s_cnt[sl], 1u) : atomicAdd(&cells[key], 1u);
                 if (slot < gs.cap) {
                     gcode[slot] = span_code(x.s[axis], cu, cv);
