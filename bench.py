@@ -8,7 +8,8 @@ One step = the whole GPU hot path on mesh data already in HBM:
   prepare both meshes (records, normals, leaf boxes, Morton sort, LBVH),
   tree-vs-tree candidate search, Guigue-Devillers on every candidate pair
   (FP32-filtered), and the inside/outside vote of every non-intersected
-  triangle of both meshes (3 rays each), results copied to the host.
+  triangle of both meshes (x and y rays, z where they disagree: the same
+  2-of-3 vote as three rays), results copied to the host.
 
 `value` is the metric's first number: candidate tri-pair tests per second of
 the Guigue-Devillers kernel (FP32-filtered), i.e. candidates per step divided
@@ -16,8 +17,10 @@ by the kernel's HIP-event time in a repeat of the timed steps with stage
 events on.  `ms_per_step` is the second: end-to-end hot-path milliseconds,
 wall clock over the timed steps (no events recorded).  The plain-FP64 kernel, the
 saturated (2^24-test) kernel rate, the stage breakdown, the roofline of the
-dominant kernel and the CPU baseline (the compiled reference, or the oracle
-port, on this host) are reported in the same JSON line.
+step's dominant kernel (`roofline`: the x/y ray kernel, timed alone with HIP
+events) and of the GD kernel (`roofline_gd`), and the CPU baseline (the
+compiled reference, or the oracle port, on this host) are reported in the
+same JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -58,15 +61,31 @@ def sha(*arrays) -> str:
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_1310720_pair.md")
+RAYS_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_rays_1310720_pair_density{}.md")
+
+# algorithmic bytes of the x/y ray kernel (k_rays_grid): per lane the query
+# record (x, y, z, tag) and its output index; per cast ray the cell range and
+# the result bit; per scanned grid entry its code words; per entry passing the
+# code test its triangle index and FP64 record; per deferred ray a list slot
+RAY_LANE_B = 32 + 4
+RAY_CAST_B = 8 + 1
+RAY_ENTRY_B = 8
+RAY_PASS_B = 4 + TRI64_B
+RAY_DEFER_B = 4
 
 
-def measured_traffic(kernels, grid: int):
-    """HBM bytes per launch of `kernels` at `grid` work-items from the committed
+def ray_bytes(lanes: int, work: dict) -> float:
+    return (lanes * RAY_LANE_B + work["rays"] * RAY_CAST_B + work["scanned"] * RAY_ENTRY_B
+            + work["passed"] * RAY_PASS_B + work["deferred"] * RAY_DEFER_B)
+
+
+def measured_traffic(kernels, grid: int, profile: str = TRAFFIC_PROFILE):
+    """HBM bytes per launch of `kernels` at `grid` work-items from a committed
     rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE passes scaled by the copy
     kernel's calibration), or None when the summary does not cover them."""
     try:
         rows = {}
-        for line in open(TRAFFIC_PROFILE):
+        for line in open(profile):
             cells = [c.strip().strip("`") for c in line.split("|")]
             if len(cells) == 7 and cells[2].isdigit():
                 rows[(cells[1].replace("sbg::", ""), int(cells[2]))] = float(cells[5]) * 1e6
@@ -350,6 +369,9 @@ def main():
     ap.add_argument("--no-stress", action="store_true", help="skip the torus / cube filter stress cases")
     ap.add_argument("--gd-only", type=int, default=0, help="profiling aid: only N back-to-back GD launches")
     ap.add_argument("--step-only", type=int, default=0, help="profiling aid: only N steps with default knobs")
+    ap.add_argument("--rays-only", type=int, default=0,
+                    help="profiling aid: one step, then N launches of the x/y ray kernel and a 1 GiB copy")
+    ap.add_argument("--grid-density", type=int, default=0, help="grid density for --step-only / --rays-only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the hot path of one mesh pair "
@@ -387,6 +409,16 @@ def main():
         log(f"gd-only: {len(P)} pairs, filtered {ms * 1e3:.2f} us, fp64 {ms64 * 1e3:.2f} us per launch; "
             f"saturated {len(Psat)} pairs: filtered {sat:.3f} ms, fp64 {sat64:.3f} ms; copy 1 GiB {cms:.3f} ms")
         del Psat, P
+        return
+
+    if args.grid_density:
+        ctx.set_tuning(grid_density=args.grid_density)
+    if args.rays_only:
+        P, _, _ = step()
+        ms, work = A.bench_rays(B, P, 0, args.rays_only)
+        cms = ctx.calibrate_copy(1 << 30, 5)
+        log(f"rays-only: grid density {ctx.tuning()['grid_density']}, x/y ray kernel {ms * 1e3:.1f} us per launch, "
+            f"{work}; copy 1 GiB {cms:.3f} ms")
         return
 
     if args.step_only:
@@ -446,9 +478,30 @@ def main():
     fb = stats["fallback_pairs"]
     tests_per_s = n_pairs / (stages["tri_tri"] * 1e-3) if stages["tri_tri"] > 0 else n_pairs / (gd_ms * 1e-3)
 
-    # roofline of the dominant kernel of the step
+    # roofline of the dominant kernel of the step: the x/y ray kernel of the
+    # classification stage, timed alone with HIP events on the context stream
     dom = max(("records", "sort", "bvh", "traverse", "pair_sort", "tri_tri", "compact", "classify", "grid"),
               key=lambda k: stages[k])
+    ray_ms, ray_work = A.bench_rays(B, P, 0, 20)
+    lanes = 2 * A.nt
+    rbytes = ray_bytes(lanes, ray_work)
+    density = ctx.tuning()["grid_density"]
+    roof_rays = {
+        "kernel": "k_rays_grid (x and y rays of mesh A's 1,310,720 triangle centroids into mesh B's grid)",
+        "bound": "hbm",
+        "achieved": rbytes / (ray_ms * 1e-3) / 1e9,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": rbytes / (ray_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "traffic": measured_traffic(("void k_rays_grid<true, false>",), lanes, RAYS_TRAFFIC_PROFILE.format(density)),
+        "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
+                           f"profiles/hbm_traffic_pmc_rays_1310720_pair_density{density}.md"),
+        "bytes_per_launch": rbytes,
+        "avg_launch_ms": ray_ms,
+        "work_per_launch": ray_work,
+        "grid_density": density,
+    }
+    log(f"x/y ray kernel: {ray_ms * 1e3:.1f} us per launch, {ray_work}, {rbytes / 1e6:.0f} MB algorithmic")
     sat_bytes = gd_bytes(n_sat, hits * reps, fb * reps, 1)
     roof_gd = {
         "kernel": "k_tritri<FP32-filtered> (saturated list)",
@@ -504,7 +557,8 @@ def main():
             "exits": {k: stats[k] for k in ("exit_plane1", "exit_plane2", "exit_interval", "hits", "coplanar")},
         },
         "dominant_stage": dom,
-        "roofline": roof_gd,
+        "roofline": roof_rays,
+        "roofline_gd": roof_gd,
         "copy_kernel_gbs": copy_gbs,
         "parity": parity,
     }

@@ -146,6 +146,15 @@ int sbg_context_stage_ms(sbg_context ctx, double *ms, int reset);
  * between two HIP events; *avg_ms = elapsed / iters. */
 int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, int precision, int iters, double *avg_ms);
 
+/* Measurement aid (no counterpart in the reference): sets up the rays of
+ * sbg_classify_triangles(self, other, pairs, side) and runs the fast x/y ray
+ * kernel alone `iters` times, HIP events around each launch on the context
+ * stream; *avg_ms = mean launch time.  work (4 values, may be null) gets the
+ * work of one launch: rays cast, grid entries scanned by the fast path,
+ * entries passing the code test, rays deferred to the wave path. */
+int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs pairs, int side, int iters, double *avg_ms,
+                   uint64_t *work);
+
 /* Measurement aid (no counterpart in the reference): a plain device-to-device
  * copy kernel of `bytes` bytes run `iters` times, for calibrating the HBM
  * traffic counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */

@@ -701,6 +701,44 @@ __global__ __launch_bounds__(256) void k_third_axis(const double *__restrict__ p
         list[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = 2 * n + q;
 }
 
+// Work of one k_rays_grid launch over `lanes` lanes, for measurement:
+// work[0] rays cast, [1] grid entries scanned by the fast path, [2] entries
+// passing the code test there, [3] rays deferred to the wave path.
+template <bool Q4>
+__global__ __launch_bounds__(256) void k_ray_work(GridSet gs, const double *__restrict__ pts,
+                                                  const uint32_t *__restrict__ order,
+                                                  const uint8_t *__restrict__ skip, uint32_t n, uint32_t lanes,
+                                                  unsigned long long *work)
+{
+    const uint32_t g = blockIdx.x * 256 + threadIdx.x;
+    unsigned long long cast = 0, scanned = 0, passed = 0, deferred = 0;
+    if (g < lanes) {
+        const uint32_t axis = g / n, q = g - axis * n;
+        uint32_t i, m = 0;
+        Ray r;
+        if (load_query<Q4>(gs, pts, order, skip, q, axis, i, m, r) == 2) {
+            cast = 1;
+            const RayCell c = ray_cell(gs, m, r, (int)axis);
+            if (c.range.y - c.range.x > (uint32_t)kFastCands) {
+                deferred = 1;
+            } else {
+                scanned = c.range.y - c.range.x;
+                for (uint32_t k = c.range.x; k < c.range.y; ++k)
+                    passed += code_may_hit(gs.gcode[k], c) ? 1 : 0;
+            }
+        }
+    }
+    unsigned long long v[4] = {cast, scanned, passed, deferred};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        unsigned long long x = v[j];
+        for (int off = 32; off > 0; off >>= 1)
+            x += __shfl_down(x, off);
+        if ((threadIdx.x & 63) == 0 && x)
+            atomicAdd(work + j, x);
+    }
+}
+
 // decideGroupSide's vote (src/solidboolean.cpp:492-508): inside iff
 // (float)inside / 3 > 0.5, i.e. at least 2 of the 3 rays.  The z bit is read
 // only where the x and y bits differ (it is not cast elsewhere), which gives
@@ -801,6 +839,26 @@ void launch_centroids_ordered(const double *tri64, const uint8_t *flags, const u
         return;
     hipLaunchKernelGGL(k_centroids_ordered, dim3((nt + 255) / 256), dim3(256), 0, s, tri64, flags, order, target,
                        nt, q4);
+}
+
+void launch_rays_xy_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
+                            uint32_t n, uint8_t *axisBits, uint32_t *lists, uint32_t *counters, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    const uint32_t lanes = 2u * n;
+    hipLaunchKernelGGL((k_rays_grid<true, false>), dim3((lanes + 255) / 256), dim3(256), 0, s, gs, tri64, q4, order,
+                       nullptr, n, lanes, nullptr, nullptr, axisBits, lists, counters + 1, counters);
+}
+
+void launch_ray_work_ordered(const GridSet &gs, const double *q4, const uint32_t *order, uint32_t n,
+                             unsigned long long *work, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    const uint32_t lanes = 2u * n;
+    hipLaunchKernelGGL(k_ray_work<true>, dim3((lanes + 255) / 256), dim3(256), 0, s, gs, q4, order, nullptr, n, lanes,
+                       work);
 }
 
 void launch_rays_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,

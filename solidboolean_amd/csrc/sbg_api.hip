@@ -195,6 +195,7 @@ void context_free(sbg_context ctx)
     ctx->votes.release();
     ctx->ovf.release();
     ctx->rayList.release();
+    ctx->workCounts.release();
     ctx->rawA.release();
     ctx->rawB.release();
     ctx->rawSeg.release();
@@ -1206,6 +1207,71 @@ int sbg_calibrate_copy(sbg_context ctx, size_t bytes, int iters, double *avg_ms)
         return fail(SBG_ERR_HIP, "calibration copy failed: %s", hipGetErrorString(e));
     *avg_ms = (double)ms / iters;
     return check_launch("copy");
+}
+
+int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int iters, double *avg_ms,
+                   uint64_t *work)
+{
+    if (!self || !other || !p || !avg_ms || iters <= 0)
+        return fail(SBG_ERR_INVALID, "bad argument");
+    if (self->ctx != other->ctx || self->ctx != p->ctx)
+        return fail(SBG_ERR_INVALID, "handles belong to different contexts");
+    if (!self->prepared || !other->prepared)
+        return fail(SBG_ERR_STATE, "mesh not prepared");
+    if (!p->tested && p->n)
+        return fail(SBG_ERR_STATE, "sbg_tri_tri has not run on this pair list");
+    if (side != 0 && side != 1)
+        return fail(SBG_ERR_INVALID, "side must be 0 or 1");
+    if (p->n && (side == 0 ? p->maxA : p->maxB) >= self->nt)
+        return fail(SBG_ERR_INVALID, "pair list does not index this mesh on side %d", side);
+    sbg_context c = self->ctx;
+    TRY(use(c));
+    const size_t nt = self->nt;
+    *avg_ms = 0.0;
+    if (work)
+        for (int k = 0; k < 4; ++k)
+            work[k] = 0;
+    if (!nt || !other->nt)
+        return SBG_OK;
+    TRY(c->flags.ensure(nt + 4));
+    TRY(c->pts.ensure(4 * nt));
+    TRY(c->bits.ensure(3 * nt + 4));
+    TRY(c->ovf.ensure(4));
+    TRY(c->rayList.ensure(4 * nt));
+    TRY(c->workCounts.ensure(4));
+    launch_fill_u32((uint32_t *)c->flags.p, 0u, (nt + 3) / 4, c->stream);
+    launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, c->flags.p, c->stream);
+    launch_centroids_ordered(self->tri64.p, c->flags.p, self->sortedIds, nullptr, (uint32_t)nt, c->pts.p, c->stream);
+    TRY(ensure_grid(other));
+    const GridSet gs = grid_set(other);
+    std::vector<hipEvent_t> ev(2 * (size_t)iters, nullptr);
+    for (auto &e : ev)
+        HIP_TRY(hipEventCreate(&e));
+    for (int i = 0; i < iters; ++i) {
+        launch_fill_u32(c->ovf.p, 0u, 4, c->stream);
+        HIP_TRY(hipEventRecord(ev[2 * i], c->stream));
+        launch_rays_xy_ordered(gs, other->tri64.p, c->pts.p, self->sortedIds, (uint32_t)nt, c->bits.p, c->rayList.p,
+                               c->ovf.p, c->stream);
+        HIP_TRY(hipEventRecord(ev[2 * i + 1], c->stream));
+    }
+    launch_fill_u32((uint32_t *)c->workCounts.p, 0u, 8, c->stream);
+    launch_ray_work_ordered(gs, c->pts.p, self->sortedIds, (uint32_t)nt, c->workCounts.p, c->stream);
+    uint64_t h[4] = {0, 0, 0, 0};
+    HIP_TRY(hipMemcpyAsync(h, c->workCounts.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double total = 0.0;
+    for (int i = 0; i < iters; ++i) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+        total += ms;
+    }
+    for (auto &e : ev)
+        (void)hipEventDestroy(e);
+    *avg_ms = total / iters;
+    if (work)
+        for (int k = 0; k < 4; ++k)
+            work[k] = h[k];
+    return check_launch("bench rays");
 }
 
 int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, int iters, double *avg_ms)

@@ -87,6 +87,7 @@ struct sbg_context_s {
     DevBuf<double> pts;
     DevBuf<uint8_t> flags, bits, votes;
     DevBuf<uint32_t> ovf, rayList;
+    DevBuf<unsigned long long> workCounts;  // sbg_bench_rays
     DevBuf<double> rawA, rawB, rawSeg;
     DevBuf<uint8_t> rawStatus;
     DevBuf<double> adjXyz, adjAll, adjGradD, adjGall, adjVol;  // sbg_adjoint

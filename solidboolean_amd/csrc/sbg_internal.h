@@ -154,6 +154,13 @@ void launch_centroids_ordered(const double *tri64, const uint8_t *flags, const u
 void launch_rays_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
                          uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists, uint32_t *counters,
                          hipStream_t s);
+// measurement: the x/y fast ray kernel alone (lists: 2n u32, counters: 4 u32
+// zero on entry), and the work counts of one such launch (work: 4 u64, zero on
+// entry: rays cast, entries scanned, entries passing the code test, deferred)
+void launch_rays_xy_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
+                            uint32_t n, uint8_t *axisBits, uint32_t *lists, uint32_t *counters, hipStream_t s);
+void launch_ray_work_ordered(const GridSet &gs, const double *q4, const uint32_t *order, uint32_t n,
+                             unsigned long long *work, hipStream_t s);
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);
 
 // k_adjoint.hip

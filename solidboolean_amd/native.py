@@ -33,7 +33,7 @@ EXPORTS = [
     "sbg_mesh_normals", "sbg_mesh_boxes", "sbg_mesh_bvh", "sbg_find_pairs", "sbg_pairs_create", "sbg_pairs_count",
     "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
     "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_context_set_timing",
-    "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_calibrate_copy",
+    "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_bench_rays", "sbg_calibrate_copy",
     "sbg_context_set_tuning", "sbg_context_get_tuning", "sbg_autotune_pair",
     "sbg_adjoint",
     "sbg_batch_create", "sbg_batch_set_vertices", "sbg_batch_set_tuning", "sbg_batch_get_tuning", "sbg_batch_run",
@@ -121,6 +121,7 @@ def load(path: str = LIB_PATH):
         "sbg_context_set_timing": [_P, C.c_int],
         "sbg_context_stage_ms": [_P, _D, C.c_int],
         "sbg_bench_tri_tri": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)],
+        "sbg_bench_rays": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double), _U64],
         "sbg_calibrate_copy": [_P, _SZ, C.c_int, C.POINTER(C.c_double)],
         "sbg_context_set_tuning": [_P, C.POINTER(Tuning)],
         "sbg_context_get_tuning": [_P, C.POINTER(Tuning)],
@@ -330,6 +331,14 @@ class Mesh:
         out = np.zeros(max(self.nt, 1), np.uint8)
         _check(self.L.sbg_classify_triangles(self.h, other.h, pairs.h, side, out))
         return out[: self.nt]
+
+    def bench_rays(self, other: "Mesh", pairs: "Pairs", side: int, iters: int):
+        """sbg_bench_rays: mean launch ms of the x/y ray kernel of
+        classify_triangles, and the work of one launch."""
+        ms = C.c_double(0)
+        work = np.zeros(4, np.uint64)
+        _check(self.L.sbg_bench_rays(self.h, other.h, pairs.h, side, iters, C.byref(ms), work))
+        return ms.value, dict(zip(("rays", "scanned", "passed", "deferred"), (int(x) for x in work)))
 
 
 class Pairs:
