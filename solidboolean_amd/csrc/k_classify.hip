@@ -32,7 +32,7 @@ namespace sbg {
 
 constexpr int kFastKeys = 3;     // distinct keys a fast lane keeps
 constexpr int kFastCands = 48;   // cell entries a fast lane scans alone
-constexpr int kCodeGroup = 4;    // entry codes a fast lane loads at once
+constexpr int kCodeGroup = 8;    // entry codes a fast lane loads at once
 constexpr int kWaveHits = 256;   // hits a wave gathers for one deferred ray
 
 __global__ __launch_bounds__(256) void k_mark_intersected(const uint64_t *__restrict__ pairs,
@@ -142,22 +142,30 @@ __device__ __forceinline__ int load_query(const GridSet &gs, const double *__res
 
 // Distinct PositionKeys of one ray, CAP of them in registers.  Along an axis
 // ray the hit points share (nearly always) the key's two off-axis components,
-// so only the on-axis component is kept per key; a key whose off-axis
-// components differ from the first key's, or one more than CAP, sets `full`
-// and the ray is finished by the wave path, which compares whole keys.  New
-// keys enter at slot 0 and shift the others up, so every slot index is a
-// compile-time constant and the set never lands in scratch memory.
+// so only the on-axis component is kept per key, as 32 bits.  A key with a
+// component outside the 32-bit range (|coordinate| above about 21474), one
+// whose off-axis components differ from the first key's, or one more than
+// CAP, sets `full` and the ray is finished by the wave path, which compares
+// whole 64-bit keys.  New keys enter at slot 0 and shift the others up, so
+// every slot index is a compile-time constant and the set never lands in
+// scratch memory.
 template <int CAP>
 struct AxisKeys {
-    long long a[CAP];
-    long long bu, bv;
+    int a[CAP];
+    int bu, bv;
     int n = 0;
     bool full = false;
+    __device__ __forceinline__ static bool fits(long long k) { return k >= -2147483647ll - 1 && k <= 2147483647ll; }
     __device__ __forceinline__ void add(const Key3 &key, int axis)
     {
-        const long long ka = axis == 0 ? key.x : (axis == 1 ? key.y : key.z);
-        const long long ku = axis == 0 ? key.y : key.x;
-        const long long kv = axis == 2 ? key.y : key.z;
+        const long long la = axis == 0 ? key.x : (axis == 1 ? key.y : key.z);
+        const long long lu = axis == 0 ? key.y : key.x;
+        const long long lv = axis == 2 ? key.y : key.z;
+        if (!fits(la) || !fits(lu) || !fits(lv)) {
+            full = true;
+            return;
+        }
+        const int ka = (int)la, ku = (int)lu, kv = (int)lv;
         if (n == 0) {
             bu = ku;
             bv = kv;
