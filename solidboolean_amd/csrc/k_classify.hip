@@ -32,7 +32,7 @@ namespace sbg {
 
 constexpr int kFastKeys = 3;     // distinct keys a fast lane keeps
 constexpr int kFastCands = 48;   // cell entries a fast lane scans alone
-constexpr int kCodeGroup = 8;    // entry codes a fast lane loads at once
+constexpr int kCodeGroup = 12;   // entry codes a fast lane loads at once
 constexpr int kWaveHits = 256;   // hits a wave gathers for one deferred ray
 
 __global__ __launch_bounds__(256) void k_mark_intersected(const uint64_t *__restrict__ pairs,
