@@ -613,14 +613,19 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
         }
     }
     if (defer) {
+        // slot < lanes of this launch: every lane appends at most once.  The
+        // record carries what the wave path would otherwise recompute.
         const uint32_t slot = atomicAdd(ovfCount, 1u);
-        ovfList[slot] = g;  // slot < lanes of this launch: every lane appends at most once
+        uint4 *rec = reinterpret_cast<uint4 *>(ovfList) + 2 * (size_t)slot;
+        rec[0] = make_uint4(g, c.range.x, c.range.y, i);
+        rec[1] = make_uint4(c.u, c.v, c.pa, 0u);
         return;
     }
     bits[(size_t)axis * n + i] = (uint8_t)(keys.n & 1);
 }
 
-// One wave per deferred (axis, query), g = axis * n + q: the 64 lanes split
+// One wave per deferred ray (record of 8 u32 written by k_rays_grid: lane
+// g = axis * n + q, cell range, output index, the ray's codes): the 64 lanes split
 // the cell's entries, hit keys are gathered in LDS (up to kWaveHits) and the
 // distinct ones counted; more hits than that raise `overflow`.
 template <bool Q4>
@@ -635,13 +640,25 @@ __global__ __launch_bounds__(64) void k_rays_wave(GridSet gs, const double *__re
     const unsigned long long below = (1ull << lane) - 1ull;
     const uint32_t total = *count;
     for (uint32_t k = blockIdx.x; k < total; k += gridDim.x) {
-        const uint32_t g = list[k];
+        // the fast path's record of the ray: lane, cell range, output index, codes
+        const uint4 *rec = reinterpret_cast<const uint4 *>(list) + 2 * (size_t)k;
+        const uint4 r0 = rec[0], r1 = rec[1];
+        const uint32_t g = r0.x, i = r0.w;
         const uint32_t axis = g / n;
         const uint32_t q = g - axis * n;
-        uint32_t i, m = 0;
+        RayCell c;
+        c.range = make_uint2(r0.y, r0.z);
+        c.u = r1.x;
+        c.v = r1.y;
+        c.pa = r1.z;
         Ray r;
-        load_query<Q4>(gs, pts, order, nullptr, q, axis, i, m, r);  // deferred lanes always cast
-        const RayCell c = ray_cell(gs, m, r, (int)axis);
+        if (Q4) {
+            const double2 *x = reinterpret_cast<const double2 *>(pts) + 2 * (size_t)q;
+            const double2 a = x[0], b = x[1];
+            r = ray_from({a.x, a.y, b.x}, axis);
+        } else {
+            r = ray_from({pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]}, axis);
+        }
         uint32_t nh = 0;
         for (uint32_t k0 = c.range.x; k0 < c.range.y; k0 += 64) {
             const uint32_t e = k0 + lane;
@@ -806,9 +823,10 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
 
 // Rays of n queries.  allAxes: every query casts x, y and z; otherwise x and
 // y, then z only where they disagree (enough for the vote; the per-axis bits
-// of z are then valid only there).  lists: 4n u32; counters: 4 u32, zero on
-// entry ([0] overflow bits, [1] deferred x/y rays, [2] z lanes, [3] deferred
-// z rays).
+// of z are then valid only there).  lists: ray_list_u32(n) u32 (deferred-ray
+// records of 8 u32 for up to 3n rays, then n z lanes); counters: 4 u32, zero
+// on entry ([0] overflow bits, [1] deferred x/y rays, [2] z lanes, [3]
+// deferred z rays).
 template <bool Q4>
 static void rays(const GridSet &gs, const double *tri64, const double *pts, const uint32_t *order,
                  const uint8_t *skip, uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists,
@@ -818,7 +836,7 @@ static void rays(const GridSet &gs, const double *tri64, const double *pts, cons
         return;
     uint32_t *overflow = counters;
     const uint32_t lanes = allAxes ? 3u * n : 2u * n;
-    uint32_t *deferred = lists, *zLanes = lists + 3 * (size_t)n, *zDeferred = lists + 2 * (size_t)n;
+    uint32_t *deferred = lists, *zDeferred = lists + 8 * 2 * (size_t)n, *zLanes = lists + 8 * 3 * (size_t)n;
     hipLaunchKernelGGL((k_rays_grid<Q4, false>), dim3((lanes + 255) / 256), dim3(256), 0, s, gs, tri64, pts, order,
                        skip, n, lanes, nullptr, nullptr, axisBits, deferred, counters + 1, overflow);
     hipLaunchKernelGGL(k_rays_wave<Q4>, dim3(8192), dim3(64), 0, s, gs, tri64, pts, order, n, axisBits, deferred,
@@ -832,6 +850,8 @@ static void rays(const GridSet &gs, const double *tri64, const double *pts, cons
     hipLaunchKernelGGL(k_rays_wave<Q4>, dim3(8192), dim3(64), 0, s, gs, tri64, pts, order, n, axisBits, zDeferred,
                        counters + 3, overflow);
 }
+
+size_t ray_list_u32(size_t n) { return 8 * 3 * n + n; }
 
 void launch_rays(const GridSet &gs, const double *tri64, const double *pts, const uint32_t *order,
                  const uint8_t *skip, uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists,

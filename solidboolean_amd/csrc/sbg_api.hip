@@ -903,7 +903,7 @@ static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder,
 {
     sbg_context c = target->ctx;
     TRY(c->ovf.ensure(4));
-    TRY(c->rayList.ensure(4 * n));
+    TRY(c->rayList.ensure(ray_list_u32(n)));
     if (target->nt == 0) {
         {
             Stage st(c, SBG_STAGE_CLASSIFY);
@@ -1237,7 +1237,7 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
     TRY(c->pts.ensure(4 * nt));
     TRY(c->bits.ensure(3 * nt + 4));
     TRY(c->ovf.ensure(4));
-    TRY(c->rayList.ensure(4 * nt));
+    TRY(c->rayList.ensure(ray_list_u32(nt)));
     TRY(c->workCounts.ensure(4));
     launch_fill_u32((uint32_t *)c->flags.p, 0u, (nt + 3) / 4, c->stream);
     launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, c->flags.p, c->stream);

@@ -287,7 +287,7 @@ int sbg_batch_create(sbg_context ctx, const double *xyz, const uint32_t *tris, c
         (rc = b->refitScratch.ensure(refit_scratch_floats((uint32_t)nt))) ||
         (rc = b->meshRoot.ensure(nmeshes)) || (rc = b->nodes.ensure(nt > 1 ? nt - 1 : 1)) ||
         (rc = b->words.ensure(kWords)) || (rc = b->flags.ensure(nt + 4)) || (rc = b->bits.ensure(3 * nt + 4)) ||
-        (rc = b->votes.ensure(nt)) || (rc = b->pts.ensure(4 * nt)) || (rc = b->rayList.ensure(4 * nt)))
+        (rc = b->votes.ensure(nt)) || (rc = b->pts.ensure(4 * nt)) || (rc = b->rayList.ensure(ray_list_u32(nt))))
         return bail(rc);
     if (hipHostMalloc((void **)&b->host, sizeof(uint32_t) * kWords, hipHostMallocDefault) != hipSuccess) {
         b->host = nullptr;

@@ -139,10 +139,11 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
 // order (may be null): query q uses point order[q].  allAxes: cast x, y and z
 // for every query; otherwise x and y, and z only where they disagree (the
 // vote's answer is the same; z's bits are then valid only there).  lists:
-// 4n u32 of scratch; counters: 4 u32, zero on entry, counters[0] gets the
+// ray_list_u32(n) u32 of scratch; counters: 4 u32, zero on entry, counters[0] gets the
 // overflow bits (bit 0: a ray has too many distinct hits, bit 1: the grid was
 // truncated, cells[ncells] > cap).  tri64 records carry the FP64 normal in
 // their pad slots.
+size_t ray_list_u32(size_t n);
 void launch_rays(const GridSet &gs, const double *tri64, const double *pts, const uint32_t *order,
                  const uint8_t *skip, uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists,
                  uint32_t *counters, hipStream_t s);
@@ -154,7 +155,7 @@ void launch_centroids_ordered(const double *tri64, const uint8_t *flags, const u
 void launch_rays_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
                          uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists, uint32_t *counters,
                          hipStream_t s);
-// measurement: the x/y fast ray kernel alone (lists: 2n u32, counters: 4 u32
+// measurement: the x/y fast ray kernel alone (lists: ray_list_u32(n) u32, counters: 4 u32
 // zero on entry), and the work counts of one such launch (work: 4 u64, zero on
 // entry: rays cast, entries scanned, entries passing the code test, deferred)
 void launch_rays_xy_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
