@@ -144,6 +144,67 @@ def test_oracle_random_seeds(gpu, seed):
     assert np.array_equal(A.classify_triangles(B, P, 0), OA.classify_triangles(OB, ia))
 
 
+def _rays_vs_oracle(A, OA, pts):
+    """Per-axis ray answers and votes of points against a mesh, GPU vs oracle."""
+    axis, vote = A.classify_points(pts)
+    ref = np.stack([OA.points_inside(pts, k) for k in range(3)], 1)
+    assert np.array_equal(axis, ref)
+    assert np.array_equal(vote, (ref.sum(1) >= 2).astype(np.uint8))
+
+
+def test_large_coordinates(gpu):
+    """Meshes far from the origin: PositionKey components beyond 32 bits
+    (|x| * 1e5 > 2^31), which the fast ray path hands to the wave path."""
+    a, b = mg.icosphere_pair(3, 5, (0.3, 0.2, -0.1), noise=1e-2, radius_b=0.9)
+    off = np.array([5.0e4, -3.0e4, 1.0e5])
+    a = mg.Mesh(a.vertices * 1000.0 + off, a.triangles)
+    b = mg.Mesh(b.vertices * 1000.0 + off, b.triangles)
+    OA, OB = OracleMesh(a.vertices, a.triangles), OracleMesh(b.vertices, b.triangles)
+    A, B = gpu.mesh(a.vertices, a.triangles), gpu.mesh(b.vertices, b.triangles)
+    ref = sorted_pairs(OA.find_pairs(OB))
+    P = A.find_pairs(B)
+    assert np.array_equal(P.array().astype(np.int64), ref)
+    P.tri_tri(A, B, N.FP32_FILTERED)
+    st, _ = P.results()
+    ost, _ = OA.tri_tri_pairs(OB, ref)
+    assert np.array_equal(st, ost)
+    ia = np.zeros(a.nt, np.uint8)
+    ia[ref[ost == 1, 0]] = 1
+    ib = np.zeros(b.nt, np.uint8)
+    ib[ref[ost == 1, 1]] = 1
+    assert np.array_equal(A.classify_triangles(B, P, 0), OA.classify_triangles(OB, ia))
+    assert np.array_equal(B.classify_triangles(A, P, 1), OB.classify_triangles(OA, ib))
+    pts = a.vertices[a.triangles].mean(1)[::7]
+    _rays_vs_oracle(B, OB, pts)
+
+
+def test_nested_shells_many_hits(gpu):
+    """A target of four concentric shells (one mesh): rays cross up to eight
+    faces, more distinct hits than the fast path keeps, so those rays finish
+    in the wave path; votes and per-axis answers must still equal the oracle's."""
+    shells = [mg.icosphere(3, r) for r in (0.4, 0.6, 0.8, 1.0)]
+    V, T, base = [], [], 0
+    for m in shells:
+        V.append(m.vertices)
+        T.append(m.triangles + base)
+        base += m.vertices.shape[0]
+    target = mg.Mesh(np.concatenate(V), np.concatenate(T))
+    query = mg.transform(mg.icosphere(3, 0.7), None, (0.15, -0.05, 0.1))
+    OT, OQ = OracleMesh(target.vertices, target.triangles), OracleMesh(query.vertices, query.triangles)
+    Tm, Q = gpu.mesh(target.vertices, target.triangles), gpu.mesh(query.vertices, query.triangles)
+    ref = sorted_pairs(OQ.find_pairs(OT))
+    P = Q.find_pairs(Tm)
+    assert np.array_equal(P.array().astype(np.int64), ref)
+    P.tri_tri(Q, Tm, N.FP64)
+    st, _ = P.results()
+    iq = np.zeros(query.nt, np.uint8)
+    iq[ref[st == 1, 0]] = 1
+    assert np.array_equal(Q.classify_triangles(Tm, P, 0), OQ.classify_triangles(OT, iq))
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(-1.1, 1.1, (3000, 3))
+    _rays_vs_oracle(Tm, OT, pts)
+
+
 @pytest.mark.parametrize("name", ["sub6_s1", "sub6_s3"])
 def test_sub6_digests(gpu, name):
     g = load_golden(name)
