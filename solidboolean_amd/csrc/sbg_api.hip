@@ -145,6 +145,60 @@ int sbg_context_create(int device, sbg_context *out)
 }  // extern "C"
 
 namespace sbg {
+constexpr size_t kSpareMeshes = 2;
+
+static void mesh_free_buffers(sbg_mesh m)
+{
+    m->V.release();
+    m->T.release();
+    m->tri64.release();
+    m->normals.release();
+    m->tri32.release();
+    m->box32.release();
+    m->blockBounds.release();
+    m->bounds.release();
+    m->rootBox.release();
+    m->refitScratch.release();
+    m->keys.release();
+    m->vals.release();
+    m->keysAlt.release();
+    m->valsAlt.release();
+    m->leafParent.release();
+    m->flags.release();
+    m->nodes.release();
+    m->gridCells.release();
+    m->gridTemp.release();
+    m->gridCode.release();
+    m->gridTri.release();
+}
+
+// a new mesh, on the buffers of a destroyed one when there is one (a caller
+// that makes and drops meshes per boolean then allocates nothing)
+static sbg_mesh mesh_new(sbg_context c)
+{
+    sbg_mesh m = nullptr;
+    if (!c->spareMeshes.empty()) {
+        // work queued on the old mesh may still read these buffers, and the
+        // new mesh's uploads do not go through the context stream
+        (void)hipStreamSynchronize(c->stream);
+        m = c->spareMeshes.back();
+        c->spareMeshes.pop_back();
+    } else {
+        m = new sbg_mesh_s;
+    }
+    m->ctx = c;
+    m->nv = m->nt = 0;
+    m->root = 0;
+    m->sortedIds = nullptr;
+    m->prepared = false;
+    m->gridCap = 0;
+    m->gridG = 0;
+    m->gridDensity = 0;
+    m->gridReady = false;
+    ++c->refs;
+    return m;
+}
+
 static void pairs_free_buffers(sbg_pairs p)
 {
     p->pairs.release();
@@ -159,13 +213,14 @@ static void pairs_free_buffers(sbg_pairs p)
 }
 
 // a new pair list, on the buffers of the last destroyed one when there is one
-// (stream order makes the reuse safe, and a step that makes and drops one
-// list per call allocates nothing)
+// (a step that makes and drops one list per call then allocates nothing)
 static sbg_pairs pairs_new(sbg_context c)
 {
     sbg_pairs p = c->sparePairs;
     c->sparePairs = nullptr;
-    if (!p)
+    if (p)
+        (void)hipStreamSynchronize(c->stream);  // as for meshes: host uploads bypass the stream
+    else
         p = new sbg_pairs_s;
     p->ctx = c;
     p->n = 0;
@@ -212,6 +267,11 @@ void context_free(sbg_context ctx)
         delete ctx->sparePairs;
         ctx->sparePairs = nullptr;
     }
+    for (sbg_mesh m : ctx->spareMeshes) {
+        mesh_free_buffers(m);
+        delete m;
+    }
+    ctx->spareMeshes.clear();
     (void)hipHostFree(ctx->pinned);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -289,9 +349,7 @@ int sbg_mesh_create(sbg_context ctx, const double *xyz, size_t nverts, const uin
     for (size_t i = 0; i < 3 * ntris; ++i)
         if ((size_t)tris[i] >= nverts)
             return fail(SBG_ERR_INVALID, "triangle %zu references vertex %u of %zu", i / 3, tris[i], nverts);
-    sbg_mesh m = new sbg_mesh_s;
-    m->ctx = ctx;
-    ++ctx->refs;
+    sbg_mesh m = mesh_new(ctx);
     m->nv = nverts;
     m->nt = ntris;
     int r = m->V.ensure(3 * nverts);
@@ -317,31 +375,15 @@ int sbg_mesh_destroy(sbg_mesh m)
 {
     if (!m)
         return SBG_OK;
-    (void)hipSetDevice(m->ctx->device);
-    (void)hipStreamSynchronize(m->ctx->stream);
-    m->V.release();
-    m->T.release();
-    m->tri64.release();
-    m->normals.release();
-    m->tri32.release();
-    m->box32.release();
-    m->blockBounds.release();
-    m->bounds.release();
-    m->rootBox.release();
-    m->refitScratch.release();
-    m->keys.release();
-    m->vals.release();
-    m->keysAlt.release();
-    m->valsAlt.release();
-    m->leafParent.release();
-    m->flags.release();
-    m->nodes.release();
-    m->gridCells.release();
-    m->gridTemp.release();
-    m->gridCode.release();
-    m->gridTri.release();
     sbg_context c = m->ctx;
-    delete m;
+    (void)hipSetDevice(c->device);
+    if (c->spareMeshes.size() < kSpareMeshes && !c->destroyed) {
+        c->spareMeshes.push_back(m);  // keeps its buffers for the next mesh
+    } else {
+        (void)hipStreamSynchronize(c->stream);
+        mesh_free_buffers(m);
+        delete m;
+    }
     context_release(c);
     return SBG_OK;
 }

@@ -101,8 +101,10 @@ struct sbg_context_s {
     };
     std::vector<Ev> events;
     double acc[SBG_STAGE_COUNT] = {0};
-    // the buffers of the last destroyed pair list, handed to the next one
+    // the buffers of the last destroyed pair list and meshes, handed to the
+    // next ones
     struct sbg_pairs_s *sparePairs = nullptr;
+    std::vector<struct sbg_mesh_s *> spareMeshes;
 };
 
 namespace sbg {

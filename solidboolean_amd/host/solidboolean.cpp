@@ -27,50 +27,46 @@ void SolidBoolean::HalfEdgeMap::reserve(size_t n)
     size_t cap = 16;
     while (cap < 2 * n)
         cap <<= 1;
-    if (cap <= m_keys.size())
+    if (cap <= m_slots.size())
         return;
-    std::vector<uint64_t> keys(cap, kEmpty);
-    std::vector<size_t> values(cap);
+    std::vector<Slot> slots(cap, Slot{kEmpty, 0});
     const size_t mask = cap - 1;
-    for (size_t i = 0; i < m_keys.size(); ++i)
-        if (m_keys[i] != kEmpty) {
-            size_t s = slot_of(m_keys[i], mask);
-            while (keys[s] != kEmpty)
+    for (const Slot &x : m_slots)
+        if (x.key != kEmpty) {
+            size_t s = slot_of(x.key, mask);
+            while (slots[s].key != kEmpty)
                 s = (s + 1) & mask;
-            keys[s] = m_keys[i];
-            values[s] = m_values[i];
+            slots[s] = x;
         }
-    m_keys.swap(keys);
-    m_values.swap(values);
+    m_slots.swap(slots);
     m_mask = mask;
 }
 
-void SolidBoolean::HalfEdgeMap::grow() { reserve(m_keys.empty() ? 8 : m_keys.size()); }
+void SolidBoolean::HalfEdgeMap::grow() { reserve(m_slots.empty() ? 8 : m_slots.size()); }
 
 bool SolidBoolean::HalfEdgeMap::insert(uint64_t key, size_t value)
 {
-    if (2 * (m_count + 1) > m_keys.size())
+    if (2 * (m_count + 1) > m_slots.size())
         grow();
     size_t s = slot_of(key, m_mask);
-    while (m_keys[s] != kEmpty) {
-        if (m_keys[s] == key)
+    while (m_slots[s].key != kEmpty) {
+        if (m_slots[s].key == key)
             return false;
         s = (s + 1) & m_mask;
     }
-    m_keys[s] = key;
-    m_values[s] = value;
+    m_slots[s] = Slot{key, value};
     ++m_count;
     return true;
 }
 
 const size_t *SolidBoolean::HalfEdgeMap::find(uint64_t key) const
 {
-    if (m_keys.empty())
+    if (m_slots.empty())
         return nullptr;
     size_t s = slot_of(key, m_mask);
-    while (m_keys[s] != kEmpty) {
-        if (m_keys[s] == key)
-            return &m_values[s];
+    while (m_slots[s].key != kEmpty) {
+        if (m_slots[s].key == key)
+            return &m_slots[s].value;
         s = (s + 1) & m_mask;
     }
     return nullptr;
@@ -148,12 +144,11 @@ bool SolidBoolean::reTriangulate(const ContextMap &contexts, const SolidMesh *me
             const size_t index = m_newTriangles.size();
             const size_t a = newIndices[nt[0]], b = newIndices[nt[1]], c = newIndices[nt[2]];
             m_newTriangles.push_back({a, b, c});
-            if (!halfEdges.insert(halfEdgeKey(a, b), index))
-                sbhost::diag("Found repeated halfedge");
-            if (!halfEdges.insert(halfEdgeKey(b, c), index))
-                sbhost::diag("Found repeated halfedge");
-            if (!halfEdges.insert(halfEdgeKey(c, a), index))
-                sbhost::diag("Found repeated halfedge");
+            for (const uint64_t he : {halfEdgeKey(a, b), halfEdgeKey(b, c), halfEdgeKey(c, a)})
+                if (!halfEdges.insert(he, index)) {
+                    m_repeatedHalfEdge = true;
+                    sbhost::diag("Found repeated halfedge");
+                }
         }
         for (const auto &nb : kv.second.neighborMap) {
             const size_t from = newIndices[nb.first];
@@ -212,10 +207,15 @@ void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &inter
                                    std::vector<std::vector<size_t>> &groups)
 {
     // (reference: src/solidboolean.cpp:167-239) seeds on both sides of every
-    // loop edge, flood fill across half-edges that are not loop edges
+    // loop edge, flood fill across half-edges that are not loop edges.  The
+    // reference also records every half-edge a filled triangle owns, but a
+    // triangle is filled once, so such a record is only ever looked up again
+    // when another triangle has the same directed edge; unless that happened
+    // (m_repeatedHalfEdge), only the loop edges are kept, in a small table.
     HalfEdgeMap groupOf;
     size_t groupIndex = 0;
-    std::deque<std::pair<size_t, size_t>> queue;
+    std::vector<std::pair<size_t, size_t>> queue;  // FIFO: [head, end)
+    size_t head = 0;
     for (const auto &loop : intersections) {
         for (size_t i = 0; i < loop.size(); ++i) {
             const size_t j = (i + 1) % loop.size();
@@ -231,11 +231,11 @@ void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &inter
         groupIndex += 2;
     }
     groups.resize(groupIndex);
+    const bool recordAll = m_repeatedHalfEdge;
     std::vector<uint8_t> visited(m_newTriangles.size(), 0);
     auto drain = [&]() {
-        while (!queue.empty()) {
-            const auto tg = queue.front();
-            queue.pop_front();
+        while (head < queue.size()) {
+            const auto tg = queue[head++];
             if (visited[tg.first])
                 continue;
             visited[tg.first] = 1;
@@ -246,11 +246,14 @@ void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &inter
                 const uint64_t he = halfEdgeKey(t[i], t[j]);
                 if (groupOf.find(he))
                     continue;
-                groupOf.insert(he, tg.second);
+                if (recordAll)
+                    groupOf.insert(he, tg.second);
                 if (const size_t *o = halfEdges.find(halfEdgeKey(t[j], t[i])))
                     queue.push_back({*o, tg.second});
             }
         }
+        queue.clear();
+        head = 0;
     };
     drain();
     const size_t end = remainingStart + remainingCount;
@@ -410,11 +413,11 @@ void SolidBoolean::fetchUnion(std::vector<std::vector<size_t>> &out)
     for (size_t i = 0; i < m_firstGroupSides.size(); ++i)
         if (!m_firstGroupSides[i])
             for (size_t t : m_firstTriangleGroups[i])
-                out.push_back(m_newTriangles[t]);
+                out.push_back({m_newTriangles[t][0], m_newTriangles[t][1], m_newTriangles[t][2]});
     for (size_t i = 0; i < m_secondGroupSides.size(); ++i)
         if (!m_secondGroupSides[i])
             for (size_t t : m_secondTriangleGroups[i])
-                out.push_back(m_newTriangles[t]);
+                out.push_back({m_newTriangles[t][0], m_newTriangles[t][1], m_newTriangles[t][2]});
 }
 
 void SolidBoolean::fetchDiff(std::vector<std::vector<size_t>> &out)
@@ -422,7 +425,7 @@ void SolidBoolean::fetchDiff(std::vector<std::vector<size_t>> &out)
     for (size_t i = 0; i < m_firstGroupSides.size(); ++i)
         if (!m_firstGroupSides[i])
             for (size_t t : m_firstTriangleGroups[i])
-                out.push_back(m_newTriangles[t]);
+                out.push_back({m_newTriangles[t][0], m_newTriangles[t][1], m_newTriangles[t][2]});
     for (size_t i = 0; i < m_secondGroupSides.size(); ++i)
         if (m_secondGroupSides[i])
             for (size_t t : m_secondTriangleGroups[i]) {
@@ -436,11 +439,11 @@ void SolidBoolean::fetchIntersect(std::vector<std::vector<size_t>> &out)
     for (size_t i = 0; i < m_firstGroupSides.size(); ++i)
         if (m_firstGroupSides[i])
             for (size_t t : m_firstTriangleGroups[i])
-                out.push_back(m_newTriangles[t]);
+                out.push_back({m_newTriangles[t][0], m_newTriangles[t][1], m_newTriangles[t][2]});
     for (size_t i = 0; i < m_secondGroupSides.size(); ++i)
         if (m_secondGroupSides[i])
             for (size_t t : m_secondTriangleGroups[i])
-                out.push_back(m_newTriangles[t]);
+                out.push_back({m_newTriangles[t][0], m_newTriangles[t][1], m_newTriangles[t][2]});
 }
 
 const std::vector<Vector3> &SolidBoolean::resultVertices() { return m_newVertices; }

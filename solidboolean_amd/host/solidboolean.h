@@ -12,6 +12,7 @@
 #pragma once
 #include <chrono>
 #include <cstdint>
+#include <array>
 #include <map>
 #include <set>
 #include <vector>
@@ -66,8 +67,11 @@ public:
         const size_t *find(uint64_t key) const;
 
     private:
-        std::vector<uint64_t> m_keys;
-        std::vector<size_t> m_values;
+        struct Slot {
+            uint64_t key;
+            size_t value;
+        };
+        std::vector<Slot> m_slots;  // key and value side by side: one cache line per probe
         size_t m_count = 0;
         size_t m_mask = 0;
         void grow();
@@ -86,7 +90,8 @@ private:
     const SolidMesh *m_firstMesh;
     const SolidMesh *m_secondMesh;
     std::vector<Vector3> m_newVertices;
-    std::vector<std::vector<size_t>> m_newTriangles;
+    std::vector<std::array<size_t, 3>> m_newTriangles;
+    bool m_repeatedHalfEdge = false;  // some directed edge belongs to two new triangles
     std::map<PositionKey, size_t> m_newPositionMap;
     std::vector<std::vector<size_t>> m_firstTriangleGroups, m_secondTriangleGroups;
     std::vector<bool> m_firstGroupSides, m_secondGroupSides;
