@@ -18,7 +18,6 @@ constexpr int kRecBlock = 256;
 __global__ __launch_bounds__(kRecBlock) void k_records(const double *__restrict__ V,
                                                        const uint32_t *__restrict__ T, uint32_t nt,
                                                        double *__restrict__ tri64, float *__restrict__ tri32,
-                                                       double *__restrict__ normals,
                                                        float *__restrict__ box32, float *__restrict__ blockBounds)
 {
     const uint32_t i = blockIdx.x * kRecBlock + threadIdx.x;
@@ -39,9 +38,6 @@ __global__ __launch_bounds__(kRecBlock) void k_records(const double *__restrict_
         r64[3] = make_double2(b.z, n.y);
         r64[4] = make_double2(c.x, c.y);
         r64[5] = make_double2(c.z, n.z);
-        normals[3 * (size_t)i] = n.x;
-        normals[3 * (size_t)i + 1] = n.y;
-        normals[3 * (size_t)i + 2] = n.z;
         // exact FP64 min/max (AxisAlignedBoudingBox::update), then rounded outward
         Box64 bb = tri_box64(a, b, c);
         // FP32 record; its pad slots carry upper bounds of the triangle's
@@ -139,6 +135,26 @@ __global__ __launch_bounds__(256) void k_boxes64(const double *__restrict__ tri6
     o[5] = b.mx.z;
 }
 
+// FP64 normals (the records' pad slots) as xyz triples
+__global__ __launch_bounds__(256) void k_normals64(const double *__restrict__ tri64, uint32_t nt,
+                                                   double *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nt)
+        return;
+    const double *r = tri64 + (size_t)kTri64Stride * i;
+    out[3 * (size_t)i] = r[3];
+    out[3 * (size_t)i + 1] = r[7];
+    out[3 * (size_t)i + 2] = r[11];
+}
+
+void launch_normals64(const double *tri64, uint32_t nt, double *out, hipStream_t s)
+{
+    if (nt == 0)
+        return;
+    hipLaunchKernelGGL(k_normals64, dim3((nt + 255) / 256), dim3(256), 0, s, tri64, nt, out);
+}
+
 void launch_boxes64(const double *tri64, uint32_t nt, double *out, hipStream_t s)
 {
     if (nt == 0)
@@ -148,13 +164,13 @@ void launch_boxes64(const double *tri64, uint32_t nt, double *out, hipStream_t s
 
 int records_blocks(uint32_t nt) { return (int)((nt + kRecBlock - 1) / kRecBlock); }
 
-void launch_records(const double *V, const uint32_t *T, uint32_t nt, double *tri64, float *tri32,
-                    double *normals, float *box32, float *blockBounds, int nblocks, hipStream_t s)
+void launch_records(const double *V, const uint32_t *T, uint32_t nt, double *tri64, float *tri32, float *box32,
+                    float *blockBounds, int nblocks, hipStream_t s)
 {
     if (nt == 0)
         return;
-    hipLaunchKernelGGL(k_records, dim3(nblocks), dim3(kRecBlock), 0, s, V, T, nt, tri64, tri32, normals,
-                       box32, blockBounds);
+    hipLaunchKernelGGL(k_records, dim3(nblocks), dim3(kRecBlock), 0, s, V, T, nt, tri64, tri32, box32,
+                       blockBounds);
 }
 
 void launch_bounds_reduce(const float *blockBounds, int nblocks, float *bounds, hipStream_t s)

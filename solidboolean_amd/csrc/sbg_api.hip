@@ -67,7 +67,7 @@ struct sbg_mesh_s {
     size_t nv = 0, nt = 0;
     DevBuf<double> V;
     DevBuf<uint32_t> T;
-    DevBuf<double> tri64, normals;
+    DevBuf<double> tri64;
     DevBuf<float> tri32, box32, blockBounds, bounds, rootBox, refitScratch;
     DevBuf<uint32_t> keys, vals, keysAlt, valsAlt, leafParent, flags;
     DevBuf<BNode> nodes;
@@ -152,7 +152,6 @@ static void mesh_free_buffers(sbg_mesh m)
     m->V.release();
     m->T.release();
     m->tri64.release();
-    m->normals.release();
     m->tri32.release();
     m->box32.release();
     m->blockBounds.release();
@@ -409,7 +408,6 @@ int sbg_mesh_prepare(sbg_mesh m)
     const int nb = records_blocks((uint32_t)nt);
     TRY(m->tri64.ensure((size_t)kTri64Stride * nt));
     TRY(m->tri32.ensure((size_t)kTri32Stride * nt));
-    TRY(m->normals.ensure(3 * nt));
     TRY(m->box32.ensure((size_t)kBox32Stride * nt));
     TRY(m->blockBounds.ensure(6 * (size_t)(nb > 0 ? nb : 1)));
     TRY(m->bounds.ensure(6));
@@ -431,7 +429,7 @@ int sbg_mesh_prepare(sbg_mesh m)
     }
     {
         Stage st(c, SBG_STAGE_RECORDS);
-        launch_records(m->V.p, m->T.p, (uint32_t)nt, m->tri64.p, m->tri32.p, m->normals.p, m->box32.p,
+        launch_records(m->V.p, m->T.p, (uint32_t)nt, m->tri64.p, m->tri32.p, m->box32.p,
                        m->blockBounds.p, nb, c->stream);
         launch_bounds_reduce(m->blockBounds.p, nb, m->bounds.p, c->stream);
         launch_morton(m->box32.p, m->bounds.p, (uint32_t)nt, m->keys.p, m->vals.p, c->stream);
@@ -474,8 +472,12 @@ int sbg_mesh_normals(sbg_mesh m, double *out)
         return fail(SBG_ERR_STATE, "mesh not prepared");
     TRY(use(m->ctx));
     if (m->nt) {
-        HIP_TRY(hipMemcpyAsync(out, m->normals.p, sizeof(double) * 3 * m->nt, hipMemcpyDeviceToHost, m->ctx->stream));
-        HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+        sbg_context c = m->ctx;
+        TRY(c->pts.ensure(3 * m->nt));
+        launch_normals64(m->tri64.p, (uint32_t)m->nt, c->pts.p, c->stream);
+        TRY(check_launch("normals64"));
+        HIP_TRY(hipMemcpyAsync(out, c->pts.p, sizeof(double) * 3 * m->nt, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
     }
     return SBG_OK;
 }

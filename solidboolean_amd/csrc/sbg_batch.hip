@@ -43,7 +43,7 @@ struct sbg_batch_s {
 
     DevBuf<double> V;
     DevBuf<uint32_t> T, meshOf, target, triBegin;
-    DevBuf<double> tri64, normals;
+    DevBuf<double> tri64;
     DevBuf<float> tri32, box32, blockBounds, rootBox, meshBox, refitScratch;
     DevBuf<int> meshBounds, waveBounds;
     DevBuf<uint64_t> keys, keysAlt, codes;
@@ -150,7 +150,7 @@ int enqueue(sbg_batch b, int precision)
     uint32_t *w = b->words.p;
     const int nb = records_blocks(nt);
     launch_fill_u32(w, 0u, kWords, s);
-    launch_records(b->V.p, b->T.p, nt, b->tri64.p, b->tri32.p, b->normals.p, b->box32.p, b->blockBounds.p, nb, s);
+    launch_records(b->V.p, b->T.p, nt, b->tri64.p, b->tri32.p, b->box32.p, b->blockBounds.p, nb, s);
     // combined LBVH: every mesh a subtree
     launch_batch_bounds_init(b->meshBounds.p, (uint32_t)b->nmeshes, s);
     launch_batch_bounds(b->box32.p, b->meshOf.p, b->triBegin.p, (uint32_t)b->nmeshes, nt, b->meshBounds.p,
@@ -278,7 +278,7 @@ int sbg_batch_create(sbg_context ctx, const double *xyz, const uint32_t *tris, c
     if ((rc = b->V.ensure(3 * nv)) || (rc = b->T.ensure(3 * nt)) || (rc = b->meshOf.ensure(nt)) ||
         (rc = b->target.ensure(nt)) || (rc = b->triBegin.ensure(nmeshes + 1)) ||
         (rc = b->tri64.ensure((size_t)kTri64Stride * nt)) || (rc = b->tri32.ensure((size_t)kTri32Stride * nt)) ||
-        (rc = b->normals.ensure(3 * nt)) || (rc = b->box32.ensure((size_t)kBox32Stride * nt)) ||
+        (rc = b->box32.ensure((size_t)kBox32Stride * nt)) ||
         (rc = b->blockBounds.ensure(6 * (size_t)records_blocks((uint32_t)nt))) || (rc = b->rootBox.ensure(6)) ||
         (rc = b->meshBox.ensure(6 * nmeshes)) || (rc = b->meshBounds.ensure(6 * nmeshes)) ||
         (rc = b->waveBounds.ensure(6 * ((nt + 63) / 64))) ||
@@ -588,7 +588,7 @@ int sbg_batch_destroy(sbg_batch b)
         (void)hipEventDestroy(b->ev1);
     if (b->host)
         (void)hipHostFree(b->host);
-    DevBuf<double> *d64[] = {&b->V, &b->tri64, &b->normals, &b->seg, &b->pts};
+    DevBuf<double> *d64[] = {&b->V, &b->tri64, &b->seg, &b->pts};
     for (auto *p : d64)
         p->release();
     DevBuf<float> *f32[] = {&b->tri32, &b->box32, &b->blockBounds, &b->rootBox, &b->meshBox, &b->refitScratch};

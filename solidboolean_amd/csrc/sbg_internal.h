@@ -32,8 +32,9 @@ inline unsigned xcd_grid(uint64_t blocks) { return (unsigned)((blocks + 7) / 8 *
 
 // ---- launchers (each .hip file owns its kernels) --------------------------
 // k_prepare.hip
-void launch_records(const double *V, const uint32_t *T, uint32_t nt, double *tri64, float *tri32,
-                    double *normals, float *box32, float *blockBounds, int nblocks, hipStream_t s);
+void launch_records(const double *V, const uint32_t *T, uint32_t nt, double *tri64, float *tri32, float *box32,
+                    float *blockBounds, int nblocks, hipStream_t s);
+void launch_normals64(const double *tri64, uint32_t nt, double *out, hipStream_t s);
 int records_blocks(uint32_t nt);
 void launch_boxes64(const double *tri64, uint32_t nt, double *out, hipStream_t s);
 void launch_bounds_reduce(const float *blockBounds, int nblocks, float *bounds, hipStream_t s);
