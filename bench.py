@@ -277,8 +277,7 @@ def stress_section(ctx, mg, N, steps: int) -> dict:
             B.prepare()
             P = A.find_pairs(B)
             P.tri_tri(A, B, N.FP32_FILTERED, stats=False)
-            A.classify_triangles(B, P, 0)
-            B.classify_triangles(A, P, 1)
+            A.classify_pair(B, P)
             return P
 
         step()
@@ -393,8 +392,7 @@ def main():
         B.prepare()
         P = A.find_pairs(B)
         P.tri_tri(A, B, precision, stats=False)
-        ca = A.classify_triangles(B, P, 0)
-        cb = B.classify_triangles(A, P, 1)
+        ca, cb = A.classify_pair(B, P)
         return P, ca, cb
 
     if args.gd_only:

@@ -136,6 +136,13 @@ int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *a
  * 2 intersected (not classified).  Requires sbg_tri_tri on `pairs`. */
 int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs pairs, int side, uint8_t *out);
 
+/* Both meshes of one boolean at once (the vote of src/solidboolean.cpp:497-508
+ * for every uncut triangle of each): out_a for a against b (pairs side 0) and
+ * out_b for b against a (side 1), exactly as two sbg_classify_triangles calls
+ * give them.  Both sides are queued before one wait, so the host's launches
+ * for the second overlap the first's kernels. */
+int sbg_classify_pair(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, uint8_t *out_a, uint8_t *out_b);
+
 /* Stage timers: when enabled, every stage is bracketed by HIP events.
  * sbg_context_stage_ms synchronises and returns the milliseconds accumulated
  * per stage (SBG_STAGE_COUNT doubles) since the last reset. */

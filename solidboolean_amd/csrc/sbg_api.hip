@@ -133,10 +133,19 @@ int sbg_context_create(int device, sbg_context *out)
         delete c;
         return fail(SBG_ERR_HIP, "hipStreamCreateWithFlags failed");
     }
-    if (hipHostMalloc((void **)&c->pinned, sizeof(uint32_t) * (kMaxLevels + 8), hipHostMallocDefault) != hipSuccess) {
+    if (hipHostMalloc((void **)&c->pinned, sizeof(uint32_t) * kPinnedWords, hipHostMallocDefault) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
         return fail(SBG_ERR_OOM, "hipHostMalloc failed");
+    }
+    if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->auxReady, hipEventDisableTiming) != hipSuccess) {
+        if (c->aux)
+            (void)hipStreamDestroy(c->aux);
+        (void)hipHostFree(c->pinned);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return fail(SBG_ERR_HIP, "second stream creation failed");
     }
     *out = c;
     return SBG_OK;
@@ -249,6 +258,14 @@ void context_free(sbg_context ctx)
     ctx->votes.release();
     ctx->ovf.release();
     ctx->rayList.release();
+    ctx->pts2.release();
+    ctx->flags2.release();
+    ctx->bits2.release();
+    ctx->votes2.release();
+    ctx->ovf2.release();
+    ctx->rayList2.release();
+    ctx->votesHost.release();
+    ctx->votesHost2.release();
     ctx->workCounts.release();
     ctx->rawA.release();
     ctx->rawB.release();
@@ -272,6 +289,9 @@ void context_free(sbg_context ctx)
     }
     ctx->spareMeshes.clear();
     (void)hipHostFree(ctx->pinned);
+    (void)hipStreamSynchronize(ctx->aux);
+    (void)hipEventDestroy(ctx->auxReady);
+    (void)hipStreamDestroy(ctx->aux);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -563,6 +583,8 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
     uint32_t nCand = 0;
     int rc = SBG_OK;
     bool found = false;
+    size_t preSorted = 0;  // slots of the sort queued before the count was read
+    bool preInAlt = false;
     for (int attempt = 0; attempt < 16; ++attempt) {
         if ((rc = c->front[0].ensure(c->frontCap)) != SBG_OK || (rc = c->front[1].ensure(c->frontCap)) != SBG_OK ||
             (rc = c->counters.ensure(kMaxLevels + 8)) != SBG_OK || (rc = p->pairs.ensure(c->candCap)) != SBG_OK) {
@@ -619,6 +641,21 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
                     done = true;
             }
         }
+        // queue the pair sort before the count comes back (over sortHint
+        // padded slots), so the host's launches overlap the search
+        preSorted = 0;
+        preInAlt = false;
+        if (rc == SBG_OK && dfs && c->sortHint > 1 && c->sortHint <= ccap) {
+            const size_t ns = c->sortHint;
+            if ((rc = p->alt.ensure(ns)) == SBG_OK && (rc = c->hist.ensure(sort_hist_entries(ns))) == SBG_OK) {
+                Stage st(c, SBG_STAGE_PAIR_SORT);
+                launch_pad_pairs(p->pairs.p, candCount, (uint32_t)ns, c->stream);
+                const int ranges[4] = {0, ceil_log2(nb), 32, 32 + ceil_log2(na)};
+                preInAlt = radix_sort_u64(p->pairs.p, p->alt.p, ns, ranges, 2, c->hist.p, c->stream) != 0;
+                preSorted = ns;
+                rc = check_launch("pair sort");
+            }
+        }
         if (rc != SBG_OK) {
             sbg_pairs_destroy(p);
             return rc;
@@ -669,7 +706,11 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
     p->n = nCand;
     p->maxA = na - 1;
     p->maxB = nb - 1;
-    if (nCand > 1) {
+    c->sortHint = nCand > 1 ? nCand + nCand / 4 + 1024 : 0;
+    if (nCand > 1 && preSorted >= nCand) {
+        if (preInAlt)
+            std::swap(p->pairs, p->alt);
+    } else if (nCand > 1) {
         Stage st(c, SBG_STAGE_PAIR_SORT);
         int rc2 = p->alt.ensure(nCand);
         if (rc2 == SBG_OK)
@@ -907,7 +948,7 @@ static GridSet grid_set(sbg_mesh m)
     return gs;
 }
 
-static int ensure_grid(sbg_mesh m)
+static int ensure_grid(sbg_mesh m, hipStream_t stream = nullptr)
 {
     if (m->gridReady && m->gridDensity == m->ctx->tune.grid_density)
         return SBG_OK;
@@ -924,72 +965,108 @@ static int ensure_grid(sbg_mesh m)
     TRY(m->gridTri.ensure(m->gridCap));
     m->gridG = G;
     {
-        Stage st(c, SBG_STAGE_GRID);
+        const hipStream_t s = stream ? stream : c->stream;
+        Stage st(c, SBG_STAGE_GRID, s);
         launch_grid_build(m->tri64.p, (uint32_t)m->nt, grid_set(m), m->gridCells.p, m->gridTemp.p, m->gridCode.p,
-                          m->gridTri.p, c->stream);
+                          m->gridTri.p, s);
         TRY(check_launch("ray grid"));
     }
     m->gridReady = true;
     return SBG_OK;
 }
 
-// Rays from n points against `target` and the 2-of-3 vote into c->votes;
-// ovf[0]: overflow flags, ovf[1]: number of rays handed to the list kernel.
-// A grid that turned out too small is rebuilt at its exact size and the rays
-// run again.
-// Casts the 3 n rays of the queries into target's grid and votes.  ordered:
-// dpts is the visiting-order (x, y, z, tag) layout over dorder; otherwise xyz
-// triples by index with optional dorder / dskip.  With hostVotes the votes are
-// copied out under the same stream synchronisation that reads the overflow
-// flags.
-static int run_rays(sbg_mesh target, const double *dpts, const uint32_t *dorder, const uint8_t *dskip, size_t n,
-                    bool ordered, bool allAxes, uint8_t *hostVotes)
+// One classification side's scratch buffers and the words of c->pinned its
+// readbacks land in.
+struct RayBufs {
+    DevBuf<double> &pts;
+    DevBuf<uint8_t> &flags, &bits, &votes;
+    DevBuf<uint32_t> &ovf, &rayList;
+    uint32_t *host;  // 4 words: overflow flags, grid entries, -, -
+    hipStream_t s;   // the stream this side's work goes on
+    PinnedBuf &votesHost;
+};
+
+static RayBufs ray_bufs(sbg_context c, int set)
+{
+    if (set)
+        return {c->pts2,   c->flags2, c->bits2, c->votes2, c->ovf2, c->rayList2, c->pinned + kRayPinned + 4,
+                c->aux,    c->votesHost2};
+    return {c->pts,    c->flags,  c->bits,  c->votes,  c->ovf,  c->rayList,  c->pinned + kRayPinned,
+            c->stream, c->votesHost};
+}
+
+// Queues the rays of n queries into target's grid, the vote into rb.votes,
+// and the readbacks (overflow flags and the grid's entry count into rb.host,
+// with `votes` the votes into rb.votesHost); nothing waits.  ordered: dpts is the
+// visiting-order (x, y, z, tag) layout over dorder; otherwise xyz triples by
+// index with optional dorder / dskip.
+static int enqueue_rays(sbg_mesh target, RayBufs rb, const double *dpts, const uint32_t *dorder,
+                        const uint8_t *dskip, size_t n, bool ordered, bool allAxes, bool votes)
 {
     sbg_context c = target->ctx;
-    TRY(c->ovf.ensure(4));
-    TRY(c->rayList.ensure(ray_list_u32(n)));
+    TRY(rb.ovf.ensure(4));
+    if (votes)
+        TRY(rb.votesHost.ensure(n));
+    TRY(rb.rayList.ensure(ray_list_u32(n)));
+    rb.host[0] = rb.host[1] = 0;
     if (target->nt == 0) {
-        {
-            Stage st(c, SBG_STAGE_CLASSIFY);
-            launch_fill_u32((uint32_t *)c->bits.p, 0u, (3 * n + 3) / 4, c->stream);
-            launch_vote(c->bits.p, dskip, (uint32_t)n, c->votes.p, c->stream);
-            TRY(check_launch("rays"));
-        }
-        if (hostVotes) {
-            HIP_TRY(hipMemcpyAsync(hostVotes, c->votes.p, n, hipMemcpyDeviceToHost, c->stream));
-            HIP_TRY(hipStreamSynchronize(c->stream));
-        }
+        Stage st(c, SBG_STAGE_CLASSIFY, rb.s);
+        launch_fill_u32((uint32_t *)rb.bits.p, 0u, (3 * n + 3) / 4, rb.s);
+        launch_vote(rb.bits.p, dskip, (uint32_t)n, rb.votes.p, rb.s);
+        TRY(check_launch("rays"));
+    } else {
+        TRY(ensure_grid(target, rb.s));
+        Stage st(c, SBG_STAGE_CLASSIFY, rb.s);
+        launch_fill_u32(rb.ovf.p, 0u, 4, rb.s);
+        if (ordered)
+            launch_rays_ordered(grid_set(target), target->tri64.p, dpts, dorder, (uint32_t)n, allAxes, rb.bits.p,
+                                rb.rayList.p, rb.ovf.p, rb.s);
+        else
+            launch_rays(grid_set(target), target->tri64.p, dpts, dorder, dskip, (uint32_t)n, allAxes, rb.bits.p,
+                        rb.rayList.p, rb.ovf.p, rb.s);
+        launch_vote(rb.bits.p, dskip, (uint32_t)n, rb.votes.p, rb.s);
+        TRY(check_launch("rays"));
+        HIP_TRY(hipMemcpyAsync(rb.host, rb.ovf.p, sizeof(uint32_t), hipMemcpyDeviceToHost, rb.s));
+        HIP_TRY(hipMemcpyAsync(rb.host + 1, target->gridCells.p + (grid_cells(target->gridG) - 1),
+                               sizeof(uint32_t), hipMemcpyDeviceToHost, rb.s));
+    }
+    if (votes)
+        HIP_TRY(hipMemcpyAsync(rb.votesHost.p, rb.votes.p, n, hipMemcpyDeviceToHost, rb.s));
+    return SBG_OK;
+}
+
+// After the stream has drained: SBG_OK, or `retry` when the grid turned out
+// too small (it is marked for a rebuild at its exact size), or an error.
+static int check_rays(sbg_mesh target, RayBufs rb, bool &retry)
+{
+    retry = false;
+    if (rb.host[0] & 2u) {
+        target->gridCap = (size_t)rb.host[1] + rb.host[1] / 16 + 1024;
+        target->gridReady = false;
+        retry = true;
         return SBG_OK;
     }
+    if (rb.host[0])
+        return fail(SBG_ERR_OVERFLOW, "a ray has more distinct hits than its key list holds");
+    return SBG_OK;
+}
+
+// enqueue, wait, check; a grid that turned out too small is rebuilt at its
+// exact size and the rays run again
+static int run_rays(sbg_mesh target, RayBufs rb, const double *dpts, const uint32_t *dorder, const uint8_t *dskip,
+                    size_t n, bool ordered, bool allAxes, uint8_t *hostVotes)
+{
+    sbg_context c = target->ctx;
     for (int attempt = 0; attempt < 2; ++attempt) {
-        TRY(ensure_grid(target));
-        {
-            Stage st(c, SBG_STAGE_CLASSIFY);
-            launch_fill_u32(c->ovf.p, 0u, 4, c->stream);
-            if (ordered)
-                launch_rays_ordered(grid_set(target), target->tri64.p, dpts, dorder, (uint32_t)n, allAxes, c->bits.p,
-                                    c->rayList.p, c->ovf.p, c->stream);
-            else
-                launch_rays(grid_set(target), target->tri64.p, dpts, dorder, dskip, (uint32_t)n, allAxes, c->bits.p,
-                            c->rayList.p, c->ovf.p, c->stream);
-            launch_vote(c->bits.p, dskip, (uint32_t)n, c->votes.p, c->stream);
-            TRY(check_launch("rays"));
+        TRY(enqueue_rays(target, rb, dpts, dorder, dskip, n, ordered, allAxes, hostVotes != nullptr));
+        HIP_TRY(hipStreamSynchronize(rb.s));
+        bool retry = false;
+        TRY(check_rays(target, rb, retry));
+        if (!retry) {
+            if (hostVotes)
+                std::memcpy(hostVotes, rb.votesHost.p, n);
+            return SBG_OK;
         }
-        uint32_t h[2] = {0, 0};
-        HIP_TRY(hipMemcpyAsync(h, c->ovf.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipMemcpyAsync(h + 1, target->gridCells.p + (grid_cells(target->gridG) - 1),
-                               sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-        if (hostVotes)
-            HIP_TRY(hipMemcpyAsync(hostVotes, c->votes.p, n, hipMemcpyDeviceToHost, c->stream));
-        HIP_TRY(hipStreamSynchronize(c->stream));
-        if (h[0] & 2u) {
-            target->gridCap = (size_t)h[1] + h[1] / 16 + 1024;
-            target->gridReady = false;
-            continue;
-        }
-        if (h[0])
-            return fail(SBG_ERR_OVERFLOW, "a ray has more distinct hits than its key list holds");
-        return SBG_OK;
     }
     return fail(SBG_ERR_OVERFLOW, "ray grid did not fit after a resize");
 }
@@ -1010,7 +1087,7 @@ int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *a
     TRY(c->bits.ensure(3 * n + 4));
     TRY(c->votes.ensure(n));
     HIP_TRY(hipMemcpyAsync(c->pts.p, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-    TRY(run_rays(target, c->pts.p, nullptr, nullptr, n, false, axis_inside != nullptr, inside));
+    TRY(run_rays(target, ray_bufs(c, 0), c->pts.p, nullptr, nullptr, n, false, axis_inside != nullptr, inside));
     std::vector<uint8_t> bits;
     if (axis_inside) {
         bits.resize(3 * n);
@@ -1024,10 +1101,8 @@ int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *a
     return SBG_OK;
 }
 
-int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, uint8_t *out)
+static int check_classify_args(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side)
 {
-    if (!self || !other || !p || !out)
-        return fail(SBG_ERR_INVALID, "null argument");
     if (self->ctx != other->ctx || self->ctx != p->ctx)
         return fail(SBG_ERR_INVALID, "handles belong to different contexts");
     if (!self->prepared || !other->prepared)
@@ -1039,24 +1114,80 @@ int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side,
     if (p->n && (side == 0 ? p->maxA : p->maxB) >= self->nt)
         return fail(SBG_ERR_INVALID, "pair list indexes triangle %llu of a %zu-triangle mesh (wrong side?)",
                     (unsigned long long)(side == 0 ? p->maxA : p->maxB), self->nt);
+    return SBG_OK;
+}
+
+// the queries of sbg_classify_triangles(self, ..., side): intersected flags,
+// then the centroids of the others in visiting order, into rb
+static int enqueue_queries(sbg_mesh self, sbg_pairs p, int side, RayBufs rb)
+{
+    sbg_context c = self->ctx;
+    const size_t nt = self->nt;
+    TRY(rb.flags.ensure(nt + 4));
+    TRY(rb.pts.ensure(4 * nt));
+    TRY(rb.bits.ensure(3 * nt + 4));
+    TRY(rb.votes.ensure(nt));
+    Stage st(c, SBG_STAGE_CLASSIFY, rb.s);
+    launch_fill_u32((uint32_t *)rb.flags.p, 0u, (nt + 3) / 4, rb.s);
+    launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, rb.flags.p, rb.s);
+    launch_centroids_ordered(self->tri64.p, rb.flags.p, self->sortedIds, nullptr, (uint32_t)nt, rb.pts.p, rb.s);
+    return check_launch("classify");
+}
+
+int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, uint8_t *out)
+{
+    if (!self || !other || !p || !out)
+        return fail(SBG_ERR_INVALID, "null argument");
+    TRY(check_classify_args(self, other, p, side));
     sbg_context c = self->ctx;
     TRY(use(c));
-    const size_t nt = self->nt;
-    if (!nt)
+    if (!self->nt)
         return SBG_OK;
-    TRY(c->flags.ensure(nt + 4));
-    TRY(c->pts.ensure(4 * nt));
-    TRY(c->bits.ensure(3 * nt + 4));
-    TRY(c->votes.ensure(nt));
-    {
-        Stage st(c, SBG_STAGE_CLASSIFY);
-        launch_fill_u32((uint32_t *)c->flags.p, 0u, (nt + 3) / 4, c->stream);
-        launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, c->flags.p, c->stream);
-        launch_centroids_ordered(self->tri64.p, c->flags.p, self->sortedIds, nullptr, (uint32_t)nt, c->pts.p,
-                                 c->stream);
-        TRY(check_launch("classify"));
+    const RayBufs rb = ray_bufs(c, 0);
+    TRY(enqueue_queries(self, p, side, rb));
+    return run_rays(other, rb, rb.pts.p, self->sortedIds, rb.flags.p, self->nt, true, false, out);
+}
+
+int sbg_classify_pair(sbg_mesh a, sbg_mesh b, sbg_pairs p, uint8_t *out_a, uint8_t *out_b)
+{
+    if (!a || !b || !p || !out_a || !out_b)
+        return fail(SBG_ERR_INVALID, "null argument");
+    TRY(check_classify_args(a, b, p, 0));
+    TRY(check_classify_args(b, a, p, 1));
+    sbg_context c = a->ctx;
+    TRY(use(c));
+    // side 0 on the context stream, side 1 on the second stream and scratch
+    // set, ordered after everything queued so far; one wait for both
+    const RayBufs ra = ray_bufs(c, 0), rb = ray_bufs(c, 1);
+    HIP_TRY(hipEventRecord(c->auxReady, c->stream));
+    HIP_TRY(hipStreamWaitEvent(c->aux, c->auxReady, 0));
+    if (a->nt) {
+        TRY(enqueue_queries(a, p, 0, ra));
+        TRY(enqueue_rays(b, ra, ra.pts.p, a->sortedIds, ra.flags.p, a->nt, true, false, true));
     }
-    return run_rays(other, c->pts.p, self->sortedIds, c->flags.p, nt, true, false, out);
+    if (b->nt) {
+        TRY(enqueue_queries(b, p, 1, rb));
+        TRY(enqueue_rays(a, rb, rb.pts.p, b->sortedIds, rb.flags.p, b->nt, true, false, true));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipStreamSynchronize(c->aux));
+    // a side whose target grid turned out too small runs again on its own
+    bool retry = false;
+    if (a->nt) {
+        TRY(check_rays(b, ra, retry));
+        if (retry)
+            TRY(run_rays(b, ra, ra.pts.p, a->sortedIds, ra.flags.p, a->nt, true, false, out_a));
+        else
+            std::memcpy(out_a, ra.votesHost.p, a->nt);
+    }
+    if (b->nt) {
+        TRY(check_rays(a, rb, retry));
+        if (retry)
+            TRY(run_rays(a, rb, rb.pts.p, b->sortedIds, rb.flags.p, b->nt, true, false, out_b));
+        else
+            std::memcpy(out_b, rb.votesHost.p, b->nt);
+    }
+    return SBG_OK;
 }
 
 int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t)
@@ -1090,9 +1221,7 @@ static int time_pair_chain(sbg_mesh a, sbg_mesh b, int reps, double &ms)
         TRY(sbg_find_pairs(a, b, &p));
         int rc = sbg_tri_tri(a, b, p, SBG_PRECISION_FP32_FILTERED, nullptr);
         if (rc == SBG_OK)
-            rc = sbg_classify_triangles(a, b, p, 0, ca.data());
-        if (rc == SBG_OK)
-            rc = sbg_classify_triangles(b, a, p, 1, cb.data());
+            rc = sbg_classify_pair(a, b, p, ca.data(), cb.data());
         sbg_pairs_destroy(p);
         return rc;
     };

@@ -30,6 +30,34 @@ int fail(int code, const char *fmt, ...);
     } while (0)
 
 // growable device buffer (contents are not preserved on growth)
+// page-locked host staging (hipHostMalloc), grown on demand: device-to-host
+// copies into it are plain DMA on the stream and never block the host
+struct PinnedBuf {
+    uint8_t *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t n)
+    {
+        if (n <= cap && p)
+            return SBG_OK;
+        release();
+        const size_t count = n ? n : 1;
+        if (hipHostMalloc((void **)&p, count, hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(SBG_ERR_OOM, "hipHostMalloc of %zu bytes failed", count);
+        }
+        cap = count;
+        return SBG_OK;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
@@ -59,6 +87,8 @@ struct DevBuf {
 };
 
 constexpr int kMaxLevels = 256;
+constexpr int kRayPinned = kMaxLevels + 8;      // 4 words per classification side
+constexpr int kPinnedWords = kRayPinned + 8;
 constexpr int kExpandGrid = 1024;
 constexpr int kLevelsPerSync = 8;
 // breadth-first levels before the persistent depth-first kernel takes the
@@ -74,7 +104,9 @@ using namespace sbg;
 struct sbg_context_s {
     int device = 0;
     hipStream_t stream = nullptr;
-    uint32_t *pinned = nullptr;  // host-visible counters (kMaxLevels + 8)
+    hipStream_t aux = nullptr;        // second stream (default priority): side 1 of sbg_classify_pair
+    hipEvent_t auxReady = nullptr;    // orders the aux stream after the main one
+    uint32_t *pinned = nullptr;  // host-visible counters (kPinnedWords): search, then two ray sides
     DevBuf<uint2> front[2];
     DevBuf<uint2> spill;              // depth-first spill regions
     uint32_t spillPerWave = kSpillInit;
@@ -82,11 +114,18 @@ struct sbg_context_s {
     sbg_tuning tune = {kBfsLevels, kDfsWaves, 4, 1};
     DevBuf<uint32_t> counters;
     size_t frontCap = 0, candCap = 0;
+    // candidates expected from the next search (the last count plus a margin):
+    // the pair sort is queued over that many padded slots before the host
+    // reads the count, and redone on the exact count if more were found
+    size_t sortHint = 0;
     DevBuf<uint32_t> hist;
     DevBuf<GdStats> stats;
-    DevBuf<double> pts;
-    DevBuf<uint8_t> flags, bits, votes;
-    DevBuf<uint32_t> ovf, rayList;
+    // classification scratch, two sets (side 0 and side 1 of sbg_classify_pair;
+    // single calls and other scratch uses take set 0)
+    DevBuf<double> pts, pts2;
+    DevBuf<uint8_t> flags, bits, votes, flags2, bits2, votes2;
+    DevBuf<uint32_t> ovf, rayList, ovf2, rayList2;
+    PinnedBuf votesHost, votesHost2;  // vote readbacks of the two sets
     DevBuf<unsigned long long> workCounts;  // sbg_bench_rays
     DevBuf<double> rawA, rawB, rawSeg;
     DevBuf<uint8_t> rawStatus;
@@ -120,11 +159,12 @@ void context_free(sbg_context ctx);
 struct Stage {
     sbg_context c;
     int stage;
+    hipStream_t s;
     hipEvent_t a = nullptr;
-    Stage(sbg_context ctx, int s) : c(ctx), stage(s)
+    Stage(sbg_context ctx, int st, hipStream_t stream = nullptr) : c(ctx), stage(st), s(stream ? stream : ctx->stream)
     {
         if (c->timing && hipEventCreate(&a) == hipSuccess)
-            (void)hipEventRecord(a, c->stream);
+            (void)hipEventRecord(a, s);
         else
             a = nullptr;
     }
@@ -134,7 +174,7 @@ struct Stage {
             return;
         hipEvent_t b = nullptr;
         if (hipEventCreate(&b) == hipSuccess) {
-            (void)hipEventRecord(b, c->stream);
+            (void)hipEventRecord(b, s);
             c->events.push_back({stage, a, b});
         } else {
             (void)hipEventDestroy(a);

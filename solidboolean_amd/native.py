@@ -32,7 +32,8 @@ EXPORTS = [
     "sbg_context_stream", "sbg_mesh_create", "sbg_mesh_prepare", "sbg_mesh_destroy", "sbg_mesh_counts",
     "sbg_mesh_normals", "sbg_mesh_boxes", "sbg_mesh_bvh", "sbg_find_pairs", "sbg_pairs_create", "sbg_pairs_count",
     "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
-    "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_context_set_timing",
+    "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_classify_pair",
+    "sbg_context_set_timing",
     "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_bench_rays", "sbg_calibrate_copy",
     "sbg_context_set_tuning", "sbg_context_get_tuning", "sbg_autotune_pair",
     "sbg_adjoint",
@@ -118,6 +119,7 @@ def load(path: str = LIB_PATH):
         "sbg_tri_tri_batch": [_P, _D, _D, _SZ, _U8, _D],
         "sbg_classify_points": [_P, _D, _SZ, _P, _U8],
         "sbg_classify_triangles": [_P, _P, _P, C.c_int, _U8],
+        "sbg_classify_pair": [_P, _P, _P, _U8, _U8],
         "sbg_context_set_timing": [_P, C.c_int],
         "sbg_context_stage_ms": [_P, _D, C.c_int],
         "sbg_bench_tri_tri": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)],
@@ -331,6 +333,14 @@ class Mesh:
         out = np.zeros(max(self.nt, 1), np.uint8)
         _check(self.L.sbg_classify_triangles(self.h, other.h, pairs.h, side, out))
         return out[: self.nt]
+
+    def classify_pair(self, other: "Mesh", pairs: "Pairs"):
+        """sbg_classify_pair: (votes of self's triangles, votes of other's),
+        self being the pairs' first mesh."""
+        oa = np.zeros(max(self.nt, 1), np.uint8)
+        ob = np.zeros(max(other.nt, 1), np.uint8)
+        _check(self.L.sbg_classify_pair(self.h, other.h, pairs.h, oa, ob))
+        return oa[: self.nt], ob[: other.nt]
 
     def bench_rays(self, other: "Mesh", pairs: "Pairs", side: int, iters: int):
         """sbg_bench_rays: mean launch ms of the x/y ray kernel of

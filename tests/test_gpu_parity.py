@@ -106,6 +106,9 @@ def test_classify_triangles(case):
     P.tri_tri(A, B, N.FP32_FILTERED, stats=False)
     assert np.array_equal(A.classify_triangles(B, P, 0), g["cls_a"]), name
     assert np.array_equal(B.classify_triangles(A, P, 1), g["cls_b"]), name
+    ca, cb = A.classify_pair(B, P)
+    assert np.array_equal(ca, g["cls_a"]), name
+    assert np.array_equal(cb, g["cls_b"]), name
     if "side_a" in g and name != "cube_n8":
         # where the reference's combine() succeeds on a well-conditioned pair,
         # the per-triangle vote agrees with the side of the group each kept
@@ -237,6 +240,8 @@ def _check_digests(gpu, g, a, b):
     assert np.array_equal(np.bincount(ca, minlength=3), g["cls_count_a"])
     assert np.array_equal(np.bincount(cb, minlength=3), g["cls_count_b"])
     assert sha(ca) == str(g["cls_digest_a"]) and sha(cb) == str(g["cls_digest_b"])
+    pa, pb = A.classify_pair(B, P)
+    assert sha(pa) == str(g["cls_digest_a"]) and sha(pb) == str(g["cls_digest_b"])
     return P, st  # st: statistics of the FP32-filtered run
 
 
