@@ -65,6 +65,14 @@ int ceil_log2(size_t n)
 struct sbg_mesh_s {
     sbg_context ctx = nullptr;
     size_t nv = 0, nt = 0;
+    // sbg_mesh_prepare runs on this stream (meshes alternate between the
+    // context's two, so the two meshes of a boolean prepare concurrently);
+    // `used` marks the last work queued on the mesh, which everything that
+    // reads or rewrites the mesh waits for
+    hipStream_t stream = nullptr;
+    hipEvent_t used = nullptr;
+    bool pendingOnAux = false;  // `used` was last recorded on the second stream and not yet waited for
+    DevBuf<uint32_t> hist;  // the Morton sort's histograms (per mesh: prepares overlap)
     DevBuf<double> V;
     DevBuf<uint32_t> T;
     DevBuf<double> tri64;
@@ -96,6 +104,37 @@ struct sbg_pairs_s {
     // largest first / second index in the list (host-built lists are checked
     // against the meshes before any kernel indexes them)
     uint64_t maxA = 0, maxB = 0;
+};
+
+// Work queued on the context stream for up to two meshes: waits for what was
+// queued on them elsewhere (a prepare on the other stream), and on leaving
+// marks the context stream's work as their latest use, which a later prepare
+// waits for.
+struct MeshUse {
+    sbg_mesh m[2];
+    explicit MeshUse(sbg_mesh a, sbg_mesh b = nullptr) : m{a, b}
+    {
+        for (sbg_mesh x : m)
+            if (x && x->pendingOnAux) {
+                (void)hipStreamWaitEvent(x->ctx->stream, x->used, 0);
+                x->pendingOnAux = false;
+            }
+    }
+    ~MeshUse()
+    {
+        // only a mesh that prepares on the second stream needs the mark: one
+        // that prepares on the context stream is ordered after this work anyway
+        for (sbg_mesh x : m)
+            if (x && x->stream != x->ctx->stream)
+                (void)hipEventRecord(x->used, x->ctx->stream);
+    }
+};
+
+// waits for the second stream when leaving (work queued there must not
+// outlive the call, whatever path it returns by)
+struct AuxJoin {
+    sbg_context c;
+    ~AuxJoin() { (void)hipStreamSynchronize(c->aux); }
 };
 
 extern "C" {
@@ -158,6 +197,7 @@ constexpr size_t kSpareMeshes = 2;
 
 static void mesh_free_buffers(sbg_mesh m)
 {
+    m->hist.release();
     m->V.release();
     m->T.release();
     m->tri64.release();
@@ -187,13 +227,22 @@ static sbg_mesh mesh_new(sbg_context c)
     sbg_mesh m = nullptr;
     if (!c->spareMeshes.empty()) {
         // work queued on the old mesh may still read these buffers, and the
-        // new mesh's uploads do not go through the context stream
+        // new mesh's uploads do not go through the streams
         (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->aux);
         m = c->spareMeshes.back();
         c->spareMeshes.pop_back();
     } else {
         m = new sbg_mesh_s;
     }
+    if (!m->used && hipEventCreateWithFlags(&m->used, hipEventDisableTiming) != hipSuccess) {
+        m->used = nullptr;
+        delete m;
+        return nullptr;
+    }
+    m->stream = (c->meshCount++ & 1) ? c->aux : c->stream;
+    (void)hipEventRecord(m->used, c->stream);
+    m->pendingOnAux = false;
     m->ctx = c;
     m->nv = m->nt = 0;
     m->root = 0;
@@ -222,14 +271,19 @@ static void pairs_free_buffers(sbg_pairs p)
 
 // a new pair list, on the buffers of the last destroyed one when there is one
 // (a step that makes and drops one list per call then allocates nothing)
-static sbg_pairs pairs_new(sbg_context c)
+static sbg_pairs pairs_new(sbg_context c, bool hostUpload)
 {
     sbg_pairs p = c->sparePairs;
     c->sparePairs = nullptr;
-    if (p)
-        (void)hipStreamSynchronize(c->stream);  // as for meshes: host uploads bypass the stream
-    else
+    // a list filled by kernels on the context stream follows every earlier
+    // user of these buffers there (the second stream is joined before any
+    // call returns); a host upload bypasses the streams and must wait
+    if (!p) {
         p = new sbg_pairs_s;
+    } else if (hostUpload) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->aux);
+    }
     p->ctx = c;
     p->n = 0;
     p->tested = false;
@@ -285,6 +339,8 @@ void context_free(sbg_context ctx)
     }
     for (sbg_mesh m : ctx->spareMeshes) {
         mesh_free_buffers(m);
+        if (m->used)
+            (void)hipEventDestroy(m->used);
         delete m;
     }
     ctx->spareMeshes.clear();
@@ -369,6 +425,8 @@ int sbg_mesh_create(sbg_context ctx, const double *xyz, size_t nverts, const uin
         if ((size_t)tris[i] >= nverts)
             return fail(SBG_ERR_INVALID, "triangle %zu references vertex %u of %zu", i / 3, tris[i], nverts);
     sbg_mesh m = mesh_new(ctx);
+    if (!m)
+        return fail(SBG_ERR_HIP, "event creation failed");
     m->nv = nverts;
     m->nt = ntris;
     int r = m->V.ensure(3 * nverts);
@@ -400,7 +458,10 @@ int sbg_mesh_destroy(sbg_mesh m)
         c->spareMeshes.push_back(m);  // keeps its buffers for the next mesh
     } else {
         (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->aux);
         mesh_free_buffers(m);
+        if (m->used)
+            (void)hipEventDestroy(m->used);
         delete m;
     }
     context_release(c);
@@ -440,7 +501,11 @@ int sbg_mesh_prepare(sbg_mesh m)
     TRY(m->nodes.ensure(nt > 1 ? nt - 1 : 1));
     TRY(m->flags.ensure(2 * nt + 1));
     TRY(m->refitScratch.ensure(refit_scratch_floats((uint32_t)nt)));
-    TRY(c->hist.ensure(sort_hist_entries(nt)));
+    TRY(m->hist.ensure(sort_hist_entries(nt)));
+    const hipStream_t s = m->stream;
+    const bool onAux = s != c->stream;
+    if (onAux)
+        HIP_TRY(hipStreamWaitEvent(s, m->used, 0));  // earlier context-stream work on this mesh
     m->prepared = false;
     m->gridReady = false;
     if (nt == 0) {
@@ -448,17 +513,17 @@ int sbg_mesh_prepare(sbg_mesh m)
         return SBG_OK;
     }
     {
-        Stage st(c, SBG_STAGE_RECORDS);
+        Stage st(c, SBG_STAGE_RECORDS, s);
         launch_records(m->V.p, m->T.p, (uint32_t)nt, m->tri64.p, m->tri32.p, m->box32.p,
-                       m->blockBounds.p, nb, c->stream);
-        launch_bounds_reduce(m->blockBounds.p, nb, m->bounds.p, c->stream);
-        launch_morton(m->box32.p, m->bounds.p, (uint32_t)nt, m->keys.p, m->vals.p, c->stream);
+                       m->blockBounds.p, nb, s);
+        launch_bounds_reduce(m->blockBounds.p, nb, m->bounds.p, s);
+        launch_morton(m->box32.p, m->bounds.p, (uint32_t)nt, m->keys.p, m->vals.p, s);
         TRY(check_launch("records"));
     }
     uint32_t *sk = m->keys.p, *sv = m->vals.p;
     {
-        Stage st(c, SBG_STAGE_SORT);
-        if (radix_sort_u32(m->keys.p, m->vals.p, m->keysAlt.p, m->valsAlt.p, nt, 0, 30, c->hist.p, c->stream)) {
+        Stage st(c, SBG_STAGE_SORT, s);
+        if (radix_sort_u32(m->keys.p, m->vals.p, m->keysAlt.p, m->valsAlt.p, nt, 0, 30, m->hist.p, s)) {
             sk = m->keysAlt.p;
             sv = m->valsAlt.p;
         }
@@ -466,19 +531,23 @@ int sbg_mesh_prepare(sbg_mesh m)
     }
     m->sortedIds = sv;
     {
-        Stage st(c, SBG_STAGE_BVH);
+        Stage st(c, SBG_STAGE_BVH, s);
         if (nt == 1) {
             m->root = kLeafBit;
-            HIP_TRY(hipMemcpyAsync(m->rootBox.p, m->box32.p, 3 * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            HIP_TRY(hipMemcpyAsync(m->rootBox.p, m->box32.p, 3 * sizeof(float), hipMemcpyDeviceToDevice, s));
             HIP_TRY(hipMemcpyAsync(m->rootBox.p + 3, m->box32.p + 4, 3 * sizeof(float), hipMemcpyDeviceToDevice,
-                                   c->stream));
+                                   s));
         } else {
             m->root = 0;
-            launch_hierarchy(sk, sv, (uint32_t)nt, m->nodes.p, m->leafParent.p, m->flags.p, c->stream);
+            launch_hierarchy(sk, sv, (uint32_t)nt, m->nodes.p, m->leafParent.p, m->flags.p, s);
             launch_refit(sv, m->box32.p, (uint32_t)nt, m->nodes.p, m->leafParent.p, m->flags.p, m->refitScratch.p,
-                         m->rootBox.p, c->stream);
+                         m->rootBox.p, s);
         }
         TRY(check_launch("bvh"));
+    }
+    if (onAux) {
+        HIP_TRY(hipEventRecord(m->used, s));
+        m->pendingOnAux = true;
     }
     m->prepared = true;
     return SBG_OK;
@@ -491,6 +560,7 @@ int sbg_mesh_normals(sbg_mesh m, double *out)
     if (!m->prepared)
         return fail(SBG_ERR_STATE, "mesh not prepared");
     TRY(use(m->ctx));
+    MeshUse mu(m);
     if (m->nt) {
         sbg_context c = m->ctx;
         TRY(c->pts.ensure(3 * m->nt));
@@ -510,6 +580,7 @@ int sbg_mesh_boxes(sbg_mesh m, double *out)
         return fail(SBG_ERR_STATE, "mesh not prepared");
     sbg_context c = m->ctx;
     TRY(use(c));
+    MeshUse mu(m);
     if (!m->nt)
         return SBG_OK;
     TRY(c->pts.ensure(6 * m->nt));
@@ -528,6 +599,7 @@ int sbg_mesh_bvh(sbg_mesh m, void *nodes_out, uint32_t *root_ref, float *root_bo
         return fail(SBG_ERR_STATE, "mesh not prepared");
     sbg_context c = m->ctx;
     TRY(use(c));
+    MeshUse mu(m);
     if (root_ref)
         *root_ref = m->root;
     if (nodes_out && m->nt > 1)
@@ -569,7 +641,8 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
         return fail(SBG_ERR_STATE, "mesh not prepared");
     sbg_context c = a->ctx;
     TRY(use(c));
-    sbg_pairs p = pairs_new(c);
+    MeshUse mu(a, b);
+    sbg_pairs p = pairs_new(c, false);
     if (a->nt == 0 || b->nt == 0) {
         *out = p;
         return SBG_OK;
@@ -647,7 +720,8 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
         preInAlt = false;
         if (rc == SBG_OK && dfs && c->sortHint > 1 && c->sortHint <= ccap) {
             const size_t ns = c->sortHint;
-            if ((rc = p->alt.ensure(ns)) == SBG_OK && (rc = c->hist.ensure(sort_hist_entries(ns))) == SBG_OK) {
+            // alt as large as pairs: the two swap, and neither may shrink
+            if ((rc = p->alt.ensure(ccap)) == SBG_OK && (rc = c->hist.ensure(sort_hist_entries(ns))) == SBG_OK) {
                 Stage st(c, SBG_STAGE_PAIR_SORT);
                 launch_pad_pairs(p->pairs.p, candCount, (uint32_t)ns, c->stream);
                 const int ranges[4] = {0, ceil_log2(nb), 32, 32 + ceil_log2(na)};
@@ -712,7 +786,7 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
             std::swap(p->pairs, p->alt);
     } else if (nCand > 1) {
         Stage st(c, SBG_STAGE_PAIR_SORT);
-        int rc2 = p->alt.ensure(nCand);
+        int rc2 = p->alt.ensure(std::max<size_t>(nCand, p->pairs.cap));
         if (rc2 == SBG_OK)
             rc2 = c->hist.ensure(sort_hist_entries(nCand));
         if (rc2 != SBG_OK) {
@@ -739,7 +813,7 @@ int sbg_pairs_create(sbg_context ctx, const uint32_t *ab, size_t n, sbg_pairs *o
     TRY(use(ctx));
     if (n && !ab)
         return fail(SBG_ERR_INVALID, "null pair array");
-    sbg_pairs p = pairs_new(ctx);
+    sbg_pairs p = pairs_new(ctx, true);
     p->n = n;
     int rc = p->pairs.ensure(n);
     if (rc != SBG_OK) {
@@ -834,6 +908,7 @@ int sbg_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, sbg_tri_tri_
         return fail(SBG_ERR_INVALID, "unknown precision %d", precision);
     sbg_context c = p->ctx;
     TRY(use(c));
+    MeshUse mu(a, b);
     TRY(check_pairs_in_range(a, b, p));
     GdStats *dstats = nullptr;
     if (stats) {
@@ -1081,6 +1156,7 @@ int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *a
         return fail(SBG_ERR_INVALID, "too many points");
     sbg_context c = target->ctx;
     TRY(use(c));
+    MeshUse mu(target);
     if (!n)
         return SBG_OK;
     TRY(c->pts.ensure(3 * n));
@@ -1141,6 +1217,7 @@ int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side,
     TRY(check_classify_args(self, other, p, side));
     sbg_context c = self->ctx;
     TRY(use(c));
+    MeshUse mu(self, other);
     if (!self->nt)
         return SBG_OK;
     const RayBufs rb = ray_bufs(c, 0);
@@ -1156,6 +1233,8 @@ int sbg_classify_pair(sbg_mesh a, sbg_mesh b, sbg_pairs p, uint8_t *out_a, uint8
     TRY(check_classify_args(b, a, p, 1));
     sbg_context c = a->ctx;
     TRY(use(c));
+    MeshUse mu(a, b);
+    AuxJoin join{c};
     // side 0 on the context stream, side 1 on the second stream and scratch
     // set, ordered after everything queued so far; one wait for both
     const RayBufs ra = ray_bufs(c, 0), rb = ray_bufs(c, 1);
@@ -1399,6 +1478,7 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
         return fail(SBG_ERR_INVALID, "pair list does not index this mesh on side %d", side);
     sbg_context c = self->ctx;
     TRY(use(c));
+    MeshUse mu(self, other);
     const size_t nt = self->nt;
     *avg_ms = 0.0;
     if (work)
@@ -1455,6 +1535,7 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, int it
         return fail(SBG_ERR_INVALID, "unknown precision %d", precision);
     sbg_context c = p->ctx;
     TRY(use(c));
+    MeshUse mu(a, b);
     TRY(check_pairs_in_range(a, b, p));
     TRY(p->status.ensure(p->n));
     TRY(p->seg.ensure(6 * p->n));

@@ -144,6 +144,7 @@ struct sbg_context_s {
     // next ones
     struct sbg_pairs_s *sparePairs = nullptr;
     std::vector<struct sbg_mesh_s *> spareMeshes;
+    unsigned meshCount = 0;  // meshes made so far: even ones prepare on `stream`, odd ones on `aux`
 };
 
 namespace sbg {
