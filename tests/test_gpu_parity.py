@@ -290,6 +290,40 @@ def test_empty_and_single_triangle(gpu):
     assert inside.tolist() == [0]
 
 
+def test_stream_ordering(gpu):
+    """Meshes alternate between the context's two streams for their prepare;
+    whatever is asked of a mesh right after its prepare (normals, boxes, tree,
+    pairs, votes), with no synchronisation in between, equals the oracle's,
+    step after step."""
+    a, b = mg.icosphere_pair(4, 21, (0.3, 0.1, -0.2), noise=1e-2, radius_b=0.85)
+    A = gpu.mesh(a.vertices, a.triangles, prepare=False)
+    B = gpu.mesh(b.vertices, b.triangles, prepare=False)
+    OA, OB = OracleMesh(a.vertices, a.triangles), OracleMesh(b.vertices, b.triangles)
+    ref = sorted_pairs(OA.find_pairs(OB))
+    ost, _ = OA.tri_tri_pairs(OB, ref)
+    ia = np.zeros(a.nt, np.uint8)
+    ia[ref[ost == 1, 0]] = 1
+    ib = np.zeros(b.nt, np.uint8)
+    ib[ref[ost == 1, 1]] = 1
+    cls_a, cls_b = OA.classify_triangles(OB, ia), OB.classify_triangles(OA, ib)
+    from oracle.oracle import normal
+    nb = np.array([normal(*b.vertices[t]) for t in b.triangles])
+    for step in range(3):
+        A.prepare()
+        B.prepare()
+        assert np.array_equal(bits(B.normals()), bits(nb)), step
+        check_bvh(B, B.boxes())
+        P = A.find_pairs(B)
+        assert np.array_equal(P.array().astype(np.int64), ref), step
+        P.tri_tri(A, B, N.FP32_FILTERED)
+        ca, cb = A.classify_pair(B, P)
+        assert np.array_equal(ca, cls_a) and np.array_equal(cb, cls_b), step
+    # a mesh made later lands on the other stream; its answers follow too
+    C = gpu.mesh(b.vertices, b.triangles)
+    P = A.find_pairs(C)
+    assert np.array_equal(P.array().astype(np.int64), ref)
+
+
 def test_disjoint_meshes(gpu):
     a, b = mg.icosphere_pair(3, 1, (5.0, 0.0, 0.0))
     A, B = gpu.mesh(a.vertices, a.triangles), gpu.mesh(b.vertices, b.triangles)
