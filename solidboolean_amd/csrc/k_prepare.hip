@@ -113,7 +113,9 @@ __global__ __launch_bounds__(256) void k_morton(const float *__restrict__ box32,
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= nt)
         return;
-    keys[i] = morton30(box_centre(box32, i), bounds);
+    // 8 bits per axis: a 3-pass sort, and the tree's order is as good at
+    // these sizes (equal codes keep triangle order, which is local too)
+    keys[i] = morton30(box_centre(box32, i), bounds) >> (30 - kMortonBits);
     vals[i] = i;
 }
 

@@ -523,7 +523,7 @@ int sbg_mesh_prepare(sbg_mesh m)
     uint32_t *sk = m->keys.p, *sv = m->vals.p;
     {
         Stage st(c, SBG_STAGE_SORT, s);
-        if (radix_sort_u32(m->keys.p, m->vals.p, m->keysAlt.p, m->valsAlt.p, nt, 0, 30, m->hist.p, s)) {
+        if (radix_sort_u32(m->keys.p, m->vals.p, m->keysAlt.p, m->valsAlt.p, nt, 0, kMortonBits, m->hist.p, s)) {
             sk = m->keysAlt.p;
             sv = m->valsAlt.p;
         }

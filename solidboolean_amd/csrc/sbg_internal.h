@@ -8,6 +8,8 @@
 
 namespace sbg {
 
+constexpr int kMortonBits = 24;  // single-mesh Morton keys (top bits of the 30-bit code)
+
 // ---- device data layouts (HBM) -------------------------------------------
 // Triangle record, FP64: 3 vertices x (x, y, z, pad) = 96 B, 16-B aligned.
 constexpr int kTri64Stride = 12;  // doubles
