@@ -143,6 +143,17 @@ int sbg_classify_triangles(sbg_mesh self, sbg_mesh other, sbg_pairs pairs, int s
  * for the second overlap the first's kernels. */
 int sbg_classify_pair(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, uint8_t *out_a, uint8_t *out_b);
 
+/* Face groups of src/solidboolean.cpp:167-239 for triangles that no
+ * intersection loop touches: connected components across opposite half-edges
+ * (triangle t's edge i -> j leads to the triangle owning j -> i), over the
+ * triangles with blocked[t] == 0 (blocked may be NULL).  labels[t] (ntris) =
+ * the smallest triangle index of t's component, which is where the
+ * reference's leftover pass (:230-238) starts that group, or 0xffffffff for a
+ * blocked triangle.  A directed edge shared by two unblocked triangles fails
+ * with SBG_ERR_INVALID, as addUnintersectedTriangles (:250-285) fails on a
+ * repeated half-edge.  Needs no sbg_mesh_prepare. */
+int sbg_components(sbg_mesh m, const uint8_t *blocked, uint32_t *labels);
+
 /* Stage timers: when enabled, every stage is bracketed by HIP events.
  * sbg_context_stage_ms synchronises and returns the milliseconds accumulated
  * per stage (SBG_STAGE_COUNT doubles) since the last reset. */

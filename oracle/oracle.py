@@ -159,3 +159,43 @@ class OracleMesh:
                  else np.ascontiguousarray(intersected, np.uint8))
         self._L.orc_classify_triangles(self.h, other.h, flags, out)
         return out[: self.nt]
+
+
+def face_groups(triangles: np.ndarray, blocked: np.ndarray | None = None) -> np.ndarray:
+    """Restatement of buildFaceGroups' leftover pass (src/solidboolean.cpp:
+    230-238, flood fill :203-226) for triangles no loop edge touches: in index
+    order, each unvisited triangle starts a group that a breadth-first search
+    spreads across half-edges (edge i -> j of a triangle leads to the triangle
+    that owns j -> i).  Only unblocked triangles enter the half-edge map, and
+    a directed edge two of them share raises ValueError, as
+    addUnintersectedTriangles (:250-285) fails on a repeated half-edge.
+    Returns per triangle the index of the triangle that started its group, -1
+    where blocked.  Pure Python: small meshes only."""
+    tris = np.asarray(triangles, np.int64).reshape(-1, 3)
+    nt = tris.shape[0]
+    blk = np.zeros(nt, bool) if blocked is None else np.asarray(blocked, bool)
+    owner: dict = {}
+    for t in range(nt):
+        if blk[t]:
+            continue
+        for k in range(3):
+            key = (int(tris[t, k]), int(tris[t, (k + 1) % 3]))
+            if key in owner:
+                raise ValueError(f"repeated half-edge {key}")
+            owner[key] = t
+    label = np.full(nt, -1, np.int64)
+    for s in range(nt):
+        if blk[s] or label[s] >= 0:
+            continue
+        label[s] = s
+        queue = [s]
+        head = 0
+        while head < len(queue):
+            t = queue[head]
+            head += 1
+            for k in range(3):
+                o = owner.get((int(tris[t, (k + 1) % 3]), int(tris[t, k])))
+                if o is not None and label[o] < 0:
+                    label[o] = s
+                    queue.append(o)
+    return label

@@ -27,7 +27,7 @@ OBJDIR = os.path.join(ROOT, "_build", "obj")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["k_prepare.hip", "k_sort.hip", "k_lbvh.hip", "k_traverse.hip", "k_tritri.hip", "k_classify.hip",
-               "k_batch.hip", "k_adjoint.hip", "sbg_api.hip", "sbg_batch.hip"]
+               "k_batch.hip", "k_adjoint.hip", "k_components.hip", "sbg_api.hip", "sbg_batch.hip"]
 ARCH = os.environ.get("SBG_OFFLOAD_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

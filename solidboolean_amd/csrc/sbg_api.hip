@@ -321,6 +321,11 @@ void context_free(sbg_context ctx)
     ctx->votesHost.release();
     ctx->votesHost2.release();
     ctx->workCounts.release();
+    ctx->ccKeys.release();
+    ctx->ccOwners.release();
+    ctx->ccParent.release();
+    ctx->ccLabels.release();
+    ctx->ccBlocked.release();
     ctx->rawA.release();
     ctx->rawB.release();
     ctx->rawSeg.release();
@@ -588,6 +593,37 @@ int sbg_mesh_boxes(sbg_mesh m, double *out)
     TRY(check_launch("boxes64"));
     HIP_TRY(hipMemcpyAsync(out, c->pts.p, sizeof(double) * 6 * m->nt, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    return SBG_OK;
+}
+
+int sbg_components(sbg_mesh m, const uint8_t *blocked, uint32_t *labels)
+{
+    if (!m || !labels)
+        return fail(SBG_ERR_INVALID, "null argument");
+    sbg_context c = m->ctx;
+    TRY(use(c));
+    MeshUse mu(m);
+    const size_t nt = m->nt;
+    if (!nt)
+        return SBG_OK;
+    const size_t slots = component_table_slots((uint32_t)nt);
+    TRY(c->ccKeys.ensure(slots));
+    TRY(c->ccOwners.ensure(slots + 1));
+    TRY(c->ccParent.ensure(nt));
+    TRY(c->ccLabels.ensure(nt));
+    if (blocked) {
+        TRY(c->ccBlocked.ensure(nt));
+        HIP_TRY(hipMemcpyAsync(c->ccBlocked.p, blocked, nt, hipMemcpyHostToDevice, c->stream));
+    }
+    launch_components(m->T.p, (uint32_t)nt, blocked ? c->ccBlocked.p : nullptr, c->ccKeys.p, c->ccOwners.p,
+                      c->ccParent.p, c->ccLabels.p, c->stream);
+    TRY(check_launch("components"));
+    uint32_t *repeated = c->pinned + kComponentsPinned;
+    HIP_TRY(hipMemcpyAsync(repeated, c->ccOwners.p + slots, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(labels, c->ccLabels.p, sizeof(uint32_t) * nt, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (*repeated)
+        return fail(SBG_ERR_INVALID, "repeated half-edge: two unblocked triangles share a directed edge");
     return SBG_OK;
 }
 

@@ -87,6 +87,7 @@ struct DevBuf {
 };
 
 constexpr int kMaxLevels = 256;
+constexpr int kComponentsPinned = kMaxLevels + 7;  // sbg_components' repeated-half-edge flag
 constexpr int kRayPinned = kMaxLevels + 8;      // 4 words per classification side
 constexpr int kPinnedWords = kRayPinned + 8;
 constexpr int kExpandGrid = 1024;
@@ -127,6 +128,9 @@ struct sbg_context_s {
     DevBuf<uint32_t> ovf, rayList, ovf2, rayList2;
     PinnedBuf votesHost, votesHost2;  // vote readbacks of the two sets
     DevBuf<unsigned long long> workCounts;  // sbg_bench_rays
+    DevBuf<unsigned long long> ccKeys;      // sbg_components: half-edge table
+    DevBuf<uint32_t> ccOwners, ccParent, ccLabels;
+    DevBuf<uint8_t> ccBlocked;
     DevBuf<double> rawA, rawB, rawSeg;
     DevBuf<uint8_t> rawStatus;
     DevBuf<double> adjXyz, adjAll, adjGradD, adjGall, adjVol;  // sbg_adjoint

@@ -167,6 +167,11 @@ void launch_ray_work_ordered(const GridSet &gs, const double *q4, const uint32_t
                              unsigned long long *work, hipStream_t s);
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);
 
+// k_components.hip: triangle components across opposite half-edges
+size_t component_table_slots(uint32_t nt);
+void launch_components(const uint32_t *T, uint32_t nt, const uint8_t *blocked, unsigned long long *keys,
+                       uint32_t *owners, uint32_t *parent, uint32_t *labels, hipStream_t s);
+
 // k_adjoint.hip
 void launch_adjoint(const double *xyz, uint32_t nin, const uint32_t *src, uint32_t nd, const uint32_t *faces,
                     uint32_t nf, double weight, const double *gradDerived, double *all, double *vol, double *gall,

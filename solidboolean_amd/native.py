@@ -32,7 +32,7 @@ EXPORTS = [
     "sbg_context_stream", "sbg_mesh_create", "sbg_mesh_prepare", "sbg_mesh_destroy", "sbg_mesh_counts",
     "sbg_mesh_normals", "sbg_mesh_boxes", "sbg_mesh_bvh", "sbg_find_pairs", "sbg_pairs_create", "sbg_pairs_count",
     "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
-    "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_classify_pair",
+    "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_classify_pair", "sbg_components",
     "sbg_context_set_timing",
     "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_bench_rays", "sbg_calibrate_copy",
     "sbg_context_set_tuning", "sbg_context_get_tuning", "sbg_autotune_pair",
@@ -120,6 +120,7 @@ def load(path: str = LIB_PATH):
         "sbg_classify_points": [_P, _D, _SZ, _P, _U8],
         "sbg_classify_triangles": [_P, _P, _P, C.c_int, _U8],
         "sbg_classify_pair": [_P, _P, _P, _U8, _U8],
+        "sbg_components": [_P, _P, _U32],
         "sbg_context_set_timing": [_P, C.c_int],
         "sbg_context_stage_ms": [_P, _D, C.c_int],
         "sbg_bench_tri_tri": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)],
@@ -332,6 +333,14 @@ class Mesh:
     def classify_triangles(self, other: "Mesh", pairs: "Pairs", side: int) -> np.ndarray:
         out = np.zeros(max(self.nt, 1), np.uint8)
         _check(self.L.sbg_classify_triangles(self.h, other.h, pairs.h, side, out))
+        return out[: self.nt]
+
+    def components(self, blocked=None) -> np.ndarray:
+        """sbg_components: per triangle the smallest triangle index of its
+        component across opposite half-edges (0xffffffff where blocked)."""
+        out = np.zeros(max(self.nt, 1), np.uint32)
+        bl = None if blocked is None else np.ascontiguousarray(blocked, np.uint8)
+        _check(self.L.sbg_components(self.h, bl.ctypes.data_as(C.c_void_p) if bl is not None else None, out))
         return out[: self.nt]
 
     def classify_pair(self, other: "Mesh", pairs: "Pairs"):

@@ -418,3 +418,43 @@ def test_tuning_changes_speed_not_results(gpu):
             assert all(np.array_equal(x, y) for x, y in zip(ref, got)), kw
     finally:
         gpu.set_tuning(**saved)
+
+
+def test_components_vs_oracle(gpu):
+    """sbg_components against the oracle's restatement of the reference's face
+    grouping: whole meshes, blocked bands, random blocked sets, several
+    disjoint shells in one mesh, and the repeated-half-edge failure of a
+    reversed triangle (none once that triangle is blocked)."""
+    from oracle.oracle import face_groups
+
+    rng = np.random.default_rng(9)
+    m = mg.icosphere(3)
+    c = m.vertices[m.triangles].mean(1)
+    shells = [mg.icosphere(2, r) for r in (0.5, 1.0)] + [mg.transform(mg.icosphere(1), None, (3, 0, 0))]
+    V, T, base = [], [], 0
+    for s in shells:
+        V.append(s.vertices)
+        T.append(s.triangles + base)
+        base += s.vertices.shape[0]
+    multi = mg.Mesh(np.concatenate(V), np.concatenate(T))
+    flipped = mg.Mesh(m.vertices, m.triangles.copy())
+    flipped.triangles[17] = flipped.triangles[17][::-1]
+    b17 = np.zeros(m.nt, bool)
+    b17[17] = True
+    cases = [(m, None), (m, np.abs(c[:, 2]) < 0.2), (m, rng.random(m.nt) < 0.3), (multi, None),
+             (multi, rng.random(multi.nt) < 0.1), (flipped, b17 | (rng.random(m.nt) < 0.05))]
+    for mesh, blocked in cases:
+        G = gpu.mesh(mesh.vertices, mesh.triangles, prepare=False)
+        got = G.components(None if blocked is None else blocked.astype(np.uint8)).astype(np.int64)
+        got[got == 0xFFFFFFFF] = -1
+        assert np.array_equal(got, face_groups(mesh.triangles, blocked))
+    G = gpu.mesh(flipped.vertices, flipped.triangles, prepare=False)
+    with pytest.raises(N.SbgError, match="repeated half-edge"):
+        G.components()
+    # the same context keeps working after the failure
+    G = gpu.mesh(m.vertices, m.triangles, prepare=False)
+    assert np.all(G.components() == 0)
+    # at a large size, the closed form: one group per closed shell
+    big = mg.icosphere(7)
+    G = gpu.mesh(big.vertices, big.triangles, prepare=False)
+    assert np.all(G.components() == 0)
