@@ -513,6 +513,23 @@ def main():
         "bytes_per_launch": sat_bytes,
         "avg_launch_ms": sat_ms,
     }
+    # face groups of mesh A's uncut triangles (sbg_components), host arrays in
+    # and out as the call takes them
+    blocked = np.zeros(A.nt, np.uint8)
+    blocked[np.unique(hp[:, 0])] = 1
+    labels = A.components(blocked)
+    ct = []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        A.components(blocked)
+        ct.append(time.perf_counter() - t1)
+    comp = {
+        "ms": float(np.median(ct)) * 1e3,
+        "triangles": A.nt,
+        "blocked": int(blocked.sum()),
+        "groups": int(np.unique(labels[labels != 0xFFFFFFFF]).size),
+    }
+    log(f"components: {comp}")
     cal_bytes = 1 << 30
     copy_ms = ctx.calibrate_copy(cal_bytes, 10)
     copy_gbs = 2 * cal_bytes / (copy_ms * 1e-3) / 1e9
@@ -557,6 +574,7 @@ def main():
         "dominant_stage": dom,
         "roofline": roof_rays,
         "roofline_gd": roof_gd,
+        "components": comp,
         "copy_kernel_gbs": copy_gbs,
         "parity": parity,
     }

@@ -187,6 +187,8 @@ typedef struct sbg_tuning {
     int dfs_waves;    /* one-wave workgroups of the depth-first search (64..16384) */
     int grid_density; /* ray-grid cells per axis plane = triangles * grid_density / 16 (1..64) */
     int depth_first;  /* 0: breadth-first levels only, with a host check every 8 levels */
+    int early_grids;  /* 1: sbg_find_pairs queues both meshes' ray grids on the second stream, beside
+                         the search; 0: each grid is built when a classification first needs it */
 } sbg_tuning;
 int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t);
 int sbg_context_get_tuning(sbg_context ctx, sbg_tuning *t);

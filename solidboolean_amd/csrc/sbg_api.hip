@@ -90,6 +90,11 @@ struct sbg_mesh_s {
     uint32_t gridG = 0;
     int gridDensity = 0;
     bool gridReady = false;
+    // the grid was last built on the second stream; gridDone follows that build,
+    // and work on the other stream that reads the grid or rewrites its inputs
+    // waits for it
+    bool gridOnAux = false;
+    hipEvent_t gridDone = nullptr;
 };
 
 struct sbg_pairs_s {
@@ -235,8 +240,12 @@ static sbg_mesh mesh_new(sbg_context c)
     } else {
         m = new sbg_mesh_s;
     }
-    if (!m->used && hipEventCreateWithFlags(&m->used, hipEventDisableTiming) != hipSuccess) {
-        m->used = nullptr;
+    if ((!m->used && hipEventCreateWithFlags(&m->used, hipEventDisableTiming) != hipSuccess) ||
+        (!m->gridDone && hipEventCreateWithFlags(&m->gridDone, hipEventDisableTiming) != hipSuccess)) {
+        if (m->used)
+            (void)hipEventDestroy(m->used);
+        if (m->gridDone)
+            (void)hipEventDestroy(m->gridDone);
         delete m;
         return nullptr;
     }
@@ -252,6 +261,7 @@ static sbg_mesh mesh_new(sbg_context c)
     m->gridG = 0;
     m->gridDensity = 0;
     m->gridReady = false;
+    m->gridOnAux = false;
     ++c->refs;
     return m;
 }
@@ -346,6 +356,8 @@ void context_free(sbg_context ctx)
         mesh_free_buffers(m);
         if (m->used)
             (void)hipEventDestroy(m->used);
+        if (m->gridDone)
+            (void)hipEventDestroy(m->gridDone);
         delete m;
     }
     ctx->spareMeshes.clear();
@@ -467,6 +479,8 @@ int sbg_mesh_destroy(sbg_mesh m)
         mesh_free_buffers(m);
         if (m->used)
             (void)hipEventDestroy(m->used);
+        if (m->gridDone)
+            (void)hipEventDestroy(m->gridDone);
         delete m;
     }
     context_release(c);
@@ -511,6 +525,9 @@ int sbg_mesh_prepare(sbg_mesh m)
     const bool onAux = s != c->stream;
     if (onAux)
         HIP_TRY(hipStreamWaitEvent(s, m->used, 0));  // earlier context-stream work on this mesh
+    if (m->gridOnAux && s != c->aux)
+        HIP_TRY(hipStreamWaitEvent(s, m->gridDone, 0));  // the last grid build still reads the records
+    m->gridOnAux = false;
     m->prepared = false;
     m->gridReady = false;
     if (nt == 0) {
@@ -666,6 +683,8 @@ int sbg_pairs_destroy(sbg_pairs p)
     return SBG_OK;
 }
 
+static int queue_grids(sbg_mesh a, sbg_mesh b);
+
 int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
 {
     if (!out || !a || !b)
@@ -678,6 +697,12 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
     sbg_context c = a->ctx;
     TRY(use(c));
     MeshUse mu(a, b);
+    const bool earlyGrids = c->tune.early_grids && a->nt && b->nt;
+    if (earlyGrids) {
+        // the second stream after everything queued so far (the prepares)
+        HIP_TRY(hipEventRecord(c->auxReady, c->stream));
+        HIP_TRY(hipStreamWaitEvent(c->aux, c->auxReady, 0));
+    }
     sbg_pairs p = pairs_new(c, false);
     if (a->nt == 0 || b->nt == 0) {
         *out = p;
@@ -766,6 +791,10 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
                 rc = check_launch("pair sort");
             }
         }
+        // the ray grids go beside the depth-first search and the pair sort,
+        // which leave most of the chip idle
+        if (rc == SBG_OK && attempt == 0 && earlyGrids)
+            rc = queue_grids(a, b);
         if (rc != SBG_OK) {
             sbg_pairs_destroy(p);
             return rc;
@@ -1059,15 +1088,27 @@ static GridSet grid_set(sbg_mesh m)
     return gs;
 }
 
+// Builds m's ray grid on `stream` unless it is current; a grid built on the
+// second stream is waited for by a reader on the other one.
 static int ensure_grid(sbg_mesh m, hipStream_t stream = nullptr)
 {
-    if (m->gridReady && m->gridDensity == m->ctx->tune.grid_density)
-        return SBG_OK;
     sbg_context c = m->ctx;
+    const hipStream_t s = stream ? stream : c->stream;
+    if (m->gridReady && m->gridDensity == c->tune.grid_density) {
+        if (m->gridOnAux && s != c->aux)
+            HIP_TRY(hipStreamWaitEvent(s, m->gridDone, 0));
+        return SBG_OK;
+    }
+    // a rebuild must not overwrite a grid that the other stream may still read
+    if (m->gridOnAux && s != c->aux)
+        HIP_TRY(hipStreamWaitEvent(s, m->gridDone, 0));
     const uint32_t G = grid_dim((uint32_t)m->nt, c->tune.grid_density);
     m->gridDensity = c->tune.grid_density;
+    // entries per triangle grow with the density: about 9 at density 2 and
+    // 12 at 4 over the three axes on closed meshes; a grid that turns out too
+    // small is rebuilt at its exact size
     if (!m->gridCap)
-        m->gridCap = 6 * m->nt + 1024;
+        m->gridCap = (6 + 3 * (size_t)c->tune.grid_density) * m->nt + 1024;
     if (m->gridCap > 0xffffffffu)
         return fail(SBG_ERR_OVERFLOW, "ray grid needs %zu entries", m->gridCap);
     TRY(m->gridCells.ensure(grid_cells(G)));
@@ -1076,13 +1117,29 @@ static int ensure_grid(sbg_mesh m, hipStream_t stream = nullptr)
     TRY(m->gridTri.ensure(m->gridCap));
     m->gridG = G;
     {
-        const hipStream_t s = stream ? stream : c->stream;
         Stage st(c, SBG_STAGE_GRID, s);
         launch_grid_build(m->tri64.p, (uint32_t)m->nt, grid_set(m), m->gridCells.p, m->gridTemp.p, m->gridCode.p,
                           m->gridTri.p, s);
         TRY(check_launch("ray grid"));
     }
+    m->gridOnAux = s == c->aux;
+    if (m->gridOnAux)
+        HIP_TRY(hipEventRecord(m->gridDone, s));
     m->gridReady = true;
+    return SBG_OK;
+}
+
+// The ray grids of both meshes of a boolean, queued on the second stream
+// during the pair search: they depend on the meshes only, and the search
+// leaves most of the chip idle.  The second stream must already be ordered
+// after the meshes' prepares (sbg_find_pairs forks it before the search).
+static int queue_grids(sbg_mesh a, sbg_mesh b)
+{
+    sbg_context c = a->ctx;
+    // b's first: the context stream's side classifies a's triangles in b's grid
+    for (sbg_mesh m : {b, a})
+        if (m->nt && !(m->gridReady && m->gridDensity == c->tune.grid_density))
+            TRY(ensure_grid(m, c->aux));
     return SBG_OK;
 }
 
@@ -1315,6 +1372,7 @@ int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t)
         return fail(SBG_ERR_INVALID, "tuning out of range");
     ctx->tune = *t;
     ctx->tune.depth_first = t->depth_first ? 1 : 0;
+    ctx->tune.early_grids = t->early_grids ? 1 : 0;
     return SBG_OK;
 }
 
@@ -1395,6 +1453,9 @@ int sbg_autotune_pair(sbg_mesh a, sbg_mesh b, int reps, sbg_tuning *best, double
                 t.grid_density = v;
                 TRY(trial(t));
             }
+        sbg_tuning t = cur;
+        t.early_grids = !cur.early_grids;
+        TRY(trial(t));
     }
     c->tune = cur;
     if (best)

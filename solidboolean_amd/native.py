@@ -57,7 +57,8 @@ class BatchTuning(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("bfs_levels", C.c_int), ("dfs_waves", C.c_int), ("grid_density", C.c_int), ("depth_first", C.c_int)]
+    _fields_ = [("bfs_levels", C.c_int), ("dfs_waves", C.c_int), ("grid_density", C.c_int), ("depth_first", C.c_int),
+                ("early_grids", C.c_int)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

@@ -308,7 +308,15 @@ def test_stream_ordering(gpu):
     cls_a, cls_b = OA.classify_triangles(OB, ia), OB.classify_triangles(OA, ib)
     from oracle.oracle import normal
     nb = np.array([normal(*b.vertices[t]) for t in b.triangles])
-    for step in range(3):
+    saved = gpu.tuning()
+    for step in range(4):
+        # odd steps build the ray grids when classification asks for them,
+        # even ones queue them on the second stream beside the pair search
+        gpu.set_tuning(early_grids=1 - step % 2)
+        if step == 2:
+            # a search whose grids are still being built when the meshes
+            # prepare again
+            A.find_pairs(B)
         A.prepare()
         B.prepare()
         assert np.array_equal(bits(B.normals()), bits(nb)), step
@@ -318,6 +326,8 @@ def test_stream_ordering(gpu):
         P.tri_tri(A, B, N.FP32_FILTERED)
         ca, cb = A.classify_pair(B, P)
         assert np.array_equal(ca, cls_a) and np.array_equal(cb, cls_b), step
+        assert np.array_equal(A.classify_triangles(B, P, 0), cls_a), step
+    gpu.set_tuning(**saved)
     # a mesh made later lands on the other stream; its answers follow too
     C = gpu.mesh(b.vertices, b.triangles)
     P = A.find_pairs(C)
