@@ -72,6 +72,12 @@ const size_t *SolidBoolean::HalfEdgeMap::find(uint64_t key) const
     return nullptr;
 }
 
+void SolidBoolean::HalfEdgeMap::prefetch(uint64_t key) const
+{
+    if (!m_slots.empty())
+        __builtin_prefetch(&m_slots[slot_of(key, m_mask)]);
+}
+
 // ---------------------------------------------------------------------------
 
 SolidBoolean::SolidBoolean(const SolidMesh *firstMesh, const SolidMesh *secondMesh)
@@ -103,7 +109,15 @@ bool SolidBoolean::addUnintersectedTriangles(const SolidMesh *mesh, const std::v
     const auto &tris = *mesh->triangles();
     halfEdges->reserve(3 * tris.size());
     m_newTriangles.reserve(m_newTriangles.size() + tris.size());
+    constexpr size_t kAhead = 16;  // slots prefetched this many triangles ahead
     for (size_t i = 0; i < tris.size(); ++i) {
+        if (i + kAhead < tris.size() && !used[i + kAhead]) {
+            const auto &f = tris[i + kAhead];
+            const size_t fa = f[0] + oldVertexCount, fb = f[1] + oldVertexCount, fc = f[2] + oldVertexCount;
+            halfEdges->prefetch(halfEdgeKey(fa, fb));
+            halfEdges->prefetch(halfEdgeKey(fb, fc));
+            halfEdges->prefetch(halfEdgeKey(fc, fa));
+        }
         if (used[i])
             continue;
         const size_t index = m_newTriangles.size();
@@ -204,14 +218,15 @@ bool SolidBoolean::buildPolygonsFromEdges(const EdgeMap &edges, std::vector<std:
 
 void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &intersections,
                                    const HalfEdgeMap &halfEdges, size_t remainingStart, size_t remainingCount,
-                                   std::vector<std::vector<size_t>> &groups)
+                                   size_t retriStart, size_t retriEnd, std::vector<std::vector<size_t>> &groups)
 {
     // (reference: src/solidboolean.cpp:167-239) seeds on both sides of every
     // loop edge, flood fill across half-edges that are not loop edges.  The
     // reference also records every half-edge a filled triangle owns, but a
     // triangle is filled once, so such a record is only ever looked up again
-    // when another triangle has the same directed edge; unless that happened
-    // (m_repeatedHalfEdge), only the loop edges are kept, in a small table.
+    // when another triangle has the same directed edge.  Unless that happened
+    // (m_repeatedHalfEdge), the fill reads a per-half-edge array instead of
+    // the maps: the triangle across the edge, or "loop edge", or nothing.
     HalfEdgeMap groupOf;
     size_t groupIndex = 0;
     std::vector<std::pair<size_t, size_t>> queue;  // FIFO: [head, end)
@@ -231,8 +246,47 @@ void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &inter
         groupIndex += 2;
     }
     groups.resize(groupIndex);
-    const bool recordAll = m_repeatedHalfEdge;
+    constexpr uint32_t kNone = ~0u, kLoop = ~1u;
+    const bool recordAll = m_repeatedHalfEdge || m_newTriangles.size() >= kLoop;
     std::vector<uint8_t> visited(m_newTriangles.size(), 0);
+    // Every triangle the fill can reach lies in this mesh's two ranges (its
+    // untouched triangles, then its retriangulated ones).  Without repeated
+    // half-edges the triangle across each half-edge is looked up once, in
+    // triangle order with the slots prefetched ahead, so the misses in the
+    // large table overlap instead of following the fill's order one by one.
+    if (!recordAll) {
+        m_opposite.resize(3 * m_newTriangles.size());
+        constexpr size_t kAhead = 16;
+        for (const auto &r : {std::make_pair(remainingStart, remainingStart + remainingCount),
+                              std::make_pair(retriStart, retriEnd)})
+            for (size_t t = r.first; t < r.second; ++t) {
+                if (t + kAhead < r.second) {
+                    const auto &f = m_newTriangles[t + kAhead];
+                    halfEdges.prefetch(halfEdgeKey(f[1], f[0]));
+                    halfEdges.prefetch(halfEdgeKey(f[2], f[1]));
+                    halfEdges.prefetch(halfEdgeKey(f[0], f[2]));
+                }
+                const auto &tri = m_newTriangles[t];
+                for (size_t i = 0; i < 3; ++i) {
+                    const size_t *o = halfEdges.find(halfEdgeKey(tri[(i + 1) % 3], tri[i]));
+                    m_opposite[3 * t + i] = o ? (uint32_t)*o : kNone;
+                }
+            }
+        // A loop half-edge, when some triangle has it, belongs to exactly that
+        // triangle, so the fill's test "is this half-edge a loop edge" is a
+        // flag on the owner's edge.  Cleared edges point nowhere.
+        for (const auto &loop : intersections)
+            for (size_t i = 0; i < loop.size(); ++i) {
+                const size_t j = (i + 1) % loop.size();
+                for (const uint64_t he : {halfEdgeKey(loop[i], loop[j]), halfEdgeKey(loop[j], loop[i])})
+                    if (const size_t *o = halfEdges.find(he)) {
+                        const auto &tri = m_newTriangles[*o];
+                        for (size_t k = 0; k < 3; ++k)
+                            if (halfEdgeKey(tri[k], tri[(k + 1) % 3]) == he)
+                                m_opposite[3 * *o + k] = kLoop;
+                    }
+            }
+    }
     auto drain = [&]() {
         while (head < queue.size()) {
             const auto tg = queue[head++];
@@ -240,14 +294,21 @@ void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &inter
                 continue;
             visited[tg.first] = 1;
             groups[tg.second].push_back(tg.first);
+            if (!recordAll) {
+                for (size_t i = 0; i < 3; ++i) {
+                    const uint32_t o = m_opposite[3 * tg.first + i];
+                    if (o < kLoop)
+                        queue.push_back({o, tg.second});
+                }
+                continue;
+            }
             const auto &t = m_newTriangles[tg.first];
             for (size_t i = 0; i < 3; ++i) {
                 const size_t j = (i + 1) % 3;
                 const uint64_t he = halfEdgeKey(t[i], t[j]);
                 if (groupOf.find(he))
                     continue;
-                if (recordAll)
-                    groupOf.insert(he, tg.second);
+                groupOf.insert(he, tg.second);
                 if (const size_t *o = halfEdges.find(halfEdgeKey(t[j], t[i])))
                     queue.push_back({*o, tg.second});
             }
@@ -375,14 +436,17 @@ bool SolidBoolean::combine()
     benchEnd_addUnintersectedTriangles = Clock::now();
 
     benchBegin_reTriangulate = Clock::now();
+    const size_t firstRetriStart = m_newTriangles.size();
     if (!reTriangulate(firstCtx, m_firstMesh, 0, firstEdges, firstHalfEdges)) {
         sbhost::diag("Retriangulate first mesh failed");
         return false;
     }
+    const size_t secondRetriStart = m_newTriangles.size();
     if (!reTriangulate(secondCtx, m_secondMesh, m_firstMesh->vertices()->size(), secondEdges, secondHalfEdges)) {
         sbhost::diag("Retriangulate second mesh failed");
         return false;
     }
+    const size_t secondRetriEnd = m_newTriangles.size();
     benchEnd_reTriangulate = Clock::now();
 
     benchBegin_buildPolygonsFromEdges = Clock::now();
@@ -396,9 +460,9 @@ bool SolidBoolean::combine()
     // (src/solidboolean.cpp:451-462)
     benchBegin_buildFaceGroups = Clock::now();
     buildFaceGroups(firstIntersections, firstHalfEdges, m_firstRemainingStart, firstRemainingCount,
-                    m_firstTriangleGroups);
+                    firstRetriStart, secondRetriStart, m_firstTriangleGroups);
     buildFaceGroups(firstIntersections, secondHalfEdges, m_secondRemainingStart, secondRemainingCount,
-                    m_secondTriangleGroups);
+                    secondRetriStart, secondRetriEnd, m_secondTriangleGroups);
     benchEnd_buildFaceGroups = Clock::now();
 
     benchBegin_decideGroupSide = Clock::now();

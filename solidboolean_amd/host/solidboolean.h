@@ -65,6 +65,7 @@ public:
         void reserve(size_t n);
         bool insert(uint64_t key, size_t value);  // false when the key is present
         const size_t *find(uint64_t key) const;
+        void prefetch(uint64_t key) const;  // the first slot `key` probes, into cache
 
     private:
         struct Slot {
@@ -112,7 +113,9 @@ private:
                        HalfEdgeMap &halfEdges);
     bool buildPolygonsFromEdges(const EdgeMap &edges, std::vector<std::vector<size_t>> &polygons);
     void buildFaceGroups(const std::vector<std::vector<size_t>> &intersections, const HalfEdgeMap &halfEdges,
-                         size_t remainingStart, size_t remainingCount, std::vector<std::vector<size_t>> &groups);
+                         size_t remainingStart, size_t remainingCount, size_t retriStart, size_t retriEnd,
+                         std::vector<std::vector<size_t>> &groups);
+    std::vector<uint32_t> m_opposite;  // buildFaceGroups: per half-edge (3 t + i), the triangle across it
     void decideGroupSide(const std::vector<std::vector<size_t>> &groups, const SolidMesh *mesh,
                          std::vector<bool> &groupSides);
 };
