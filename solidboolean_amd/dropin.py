@@ -5,6 +5,7 @@ SolidBoolean with the reference's calling pattern
     b = SolidMesh(...); b.prepare()
     sb = SolidBoolean(a, b); ok = sb.combine()
     sb.fetch_union(), sb.fetch_diff(), sb.fetch_intersect(), sb.result_vertices()
+    v, t = load_obj("a.obj"); export_obj("out.obj", v, t)   # the test driver's OBJ I/O
 
 Raises if the native libraries are missing or no GPU is usable.
 """
@@ -55,8 +56,7 @@ def load():
     L.sbh_boolean_vertex_sources.restype = C.c_size_t
     L.sbh_boolean_vertex_sources.argtypes = [_P, np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS"),
                                              C.c_size_t]
-    L.sbh_earclip.restype = C.c_size_t
-    L.sbh_earclip.argtypes = [_D, _I, C.c_size_t, _I, C.c_size_t]
+    _host_only_signatures(L)
     _lib = L
     return L
 
@@ -78,10 +78,45 @@ def load_earclip_only():
     global _lib
     if _lib is None:
         L = C.CDLL(HOST_LIB)
-        L.sbh_earclip.restype = C.c_size_t
-        L.sbh_earclip.argtypes = [_D, _I, C.c_size_t, _I, C.c_size_t]
+        _host_only_signatures(L)
         return L
     return _lib
+
+
+def _host_only_signatures(L):
+    L.sbh_last_error.restype = C.c_char_p
+    L.sbh_earclip.restype = C.c_size_t
+    L.sbh_earclip.argtypes = [_D, _I, C.c_size_t, _I, C.c_size_t]
+    L.sbh_obj_load.restype = C.c_int
+    L.sbh_obj_load.argtypes = [C.c_char_p, _D, C.c_size_t, C.POINTER(C.c_size_t), _I, C.c_size_t,
+                               C.POINTER(C.c_size_t)]
+    L.sbh_obj_export.restype = C.c_int
+    L.sbh_obj_export.argtypes = [C.c_char_p, _D, C.c_size_t, _I, C.c_size_t]
+
+
+def load_obj(path: str):
+    """(vertices (n, 3) float64, triangles (m, 3) int64) of an OBJ file, read as
+    the reference's test driver reads it (tinyobjloader, triangulating);
+    raises IOError when the file cannot be read or a face index is bad."""
+    L = load_earclip_only()
+    nv, nt = C.c_size_t(0), C.c_size_t(0)
+    if not L.sbh_obj_load(os.fsencode(path), np.zeros(3), 0, C.byref(nv), np.zeros(3, np.int64), 0, C.byref(nt)):
+        raise IOError(f"{path}: {L.sbh_last_error().decode(errors='replace')}")
+    v = np.zeros((max(nv.value, 1), 3))
+    t = np.zeros((max(nt.value, 1), 3), np.int64)
+    L.sbh_obj_load(os.fsencode(path), v.reshape(-1), nv.value, C.byref(nv), t.reshape(-1), nt.value, C.byref(nt))
+    return v[: nv.value], t[: nt.value]
+
+
+def export_obj(path: str, vertices, triangles) -> None:
+    """Writes "v %f %f %f" and 1-based "f" lines (test/main.cpp:9-28)."""
+    L = load_earclip_only()
+    v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.int64).reshape(-1, 3)
+    vv = v.reshape(-1) if v.size else np.zeros(3)
+    tt = t.reshape(-1) if t.size else np.zeros(3, np.int64)
+    if not L.sbh_obj_export(os.fsencode(path), vv, v.shape[0], tt, t.shape[0]):
+        raise IOError(f"{path}: {L.sbh_last_error().decode(errors='replace')}")
 
 
 class SolidMesh:

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "earclip.h"
+#include "objfile.h"
 #include "solidboolean.h"
 #include "solidmesh.h"
 
@@ -147,6 +148,58 @@ size_t sbh_earclip(const double *xy, const int64_t *sizes, size_t nrings, int64_
     for (size_t i = 0; i < idx.size() && i < cap; ++i)
         out[i] = (int64_t)idx[i];
     return idx.size();
+}
+
+// Reads an OBJ file as the reference's test driver does (objfile.h).  Call
+// with capv = capt = 0 for the counts, then with room for them (xyz: 3 nv
+// doubles, tris: 3 nt indices).  1 on success; 0 on failure, with the message
+// in sbh_last_error().
+int sbh_obj_load(const char *path, double *xyz, size_t capv, size_t *nv, int64_t *tris, size_t capt, size_t *nt)
+{
+    try {
+        std::vector<Vector3> v;
+        std::vector<std::vector<size_t>> t;
+        std::string err;
+        if (!path || !sbobj::loadObj(path, v, t, &err)) {
+            g_hostErr = path ? err : "null path";
+            return 0;
+        }
+        if (nv)
+            *nv = v.size();
+        if (nt)
+            *nt = t.size();
+        for (size_t i = 0; i < v.size() && i < capv; ++i)
+            for (int k = 0; k < 3; ++k)
+                xyz[3 * i + k] = v[i].constData()[k];
+        for (size_t i = 0; i < t.size() && i < capt; ++i)
+            for (int k = 0; k < 3; ++k)
+                tris[3 * i + k] = (int64_t)t[i][k];
+        return 1;
+    } catch (const std::exception &e) {
+        g_hostErr = e.what();
+        return 0;
+    }
+}
+
+// Writes nv vertices and nf triangles as test/main.cpp:9-28 does.  1 / 0.
+int sbh_obj_export(const char *path, const double *xyz, size_t nv, const int64_t *tris, size_t nf)
+{
+    try {
+        std::vector<Vector3> v(nv);
+        for (size_t i = 0; i < nv; ++i)
+            v[i] = Vector3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+        std::vector<std::vector<size_t>> f(nf);
+        for (size_t i = 0; i < nf; ++i)
+            f[i] = {(size_t)tris[3 * i], (size_t)tris[3 * i + 1], (size_t)tris[3 * i + 2]};
+        if (!path || !sbobj::exportObject(path, v, f)) {
+            g_hostErr = "cannot write the file";
+            return 0;
+        }
+        return 1;
+    } catch (const std::exception &e) {
+        g_hostErr = e.what();
+        return 0;
+    }
 }
 
 }  // extern "C"
