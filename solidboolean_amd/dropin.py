@@ -45,6 +45,8 @@ def load():
     L.sbh_mesh_free.argtypes = [_P]
     L.sbh_boolean_new.restype = C.c_int
     L.sbh_boolean_new.argtypes = [_P, _P, C.POINTER(C.c_void_p)]
+    L.sbh_boolean_new_ordered.restype = C.c_int
+    L.sbh_boolean_new_ordered.argtypes = [_P, _P, C.c_int, C.POINTER(C.c_void_p)]
     L.sbh_boolean_free.argtypes = [_P]
     L.sbh_boolean_vertices.restype = C.c_size_t
     L.sbh_boolean_vertices.argtypes = [_P, _D, C.c_size_t]
@@ -92,6 +94,30 @@ def _host_only_signatures(L):
                                C.POINTER(C.c_size_t)]
     L.sbh_obj_export.restype = C.c_int
     L.sbh_obj_export.argtypes = [C.c_char_p, _D, C.c_size_t, _I, C.c_size_t]
+    _U32 = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+    _U64 = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+    L.sbh_reference_pair_order.restype = C.c_int
+    L.sbh_reference_pair_order.argtypes = [_D, C.c_size_t, _I, C.c_size_t, _D, C.c_size_t, _I, C.c_size_t, _U32,
+                                           C.c_size_t, _U64]
+
+
+def reference_pair_order(va, ta, vb, tb, ab) -> np.ndarray:
+    """The permutation that puts pairs ab (n x 2: a of mesh A, b of mesh B)
+    into the order in which the reference's own tree search reports them."""
+    L = load_earclip_only()
+    va = np.ascontiguousarray(va, np.float64).reshape(-1, 3)
+    vb = np.ascontiguousarray(vb, np.float64).reshape(-1, 3)
+    ta = np.ascontiguousarray(ta, np.int64).reshape(-1, 3)
+    tb = np.ascontiguousarray(tb, np.int64).reshape(-1, 3)
+    ab = np.ascontiguousarray(ab, np.uint32).reshape(-1, 2)
+    perm = np.zeros(max(len(ab), 1), np.uint64)
+    z3, zi = np.zeros(3), np.zeros(3, np.int64)
+    if not L.sbh_reference_pair_order(va.reshape(-1) if va.size else z3, len(va), ta.reshape(-1) if ta.size else zi,
+                                      len(ta), vb.reshape(-1) if vb.size else z3, len(vb),
+                                      tb.reshape(-1) if tb.size else zi, len(tb),
+                                      ab.reshape(-1) if ab.size else np.zeros(2, np.uint32), len(ab), perm):
+        raise ValueError(L.sbh_last_error().decode(errors="replace"))
+    return perm[: len(ab)].astype(np.int64)
 
 
 def load_obj(path: str):
@@ -142,14 +168,18 @@ class SolidMesh:
 
 
 class SolidBoolean:
-    def __init__(self, a: SolidMesh, b: SolidMesh):
+    def __init__(self, a: SolidMesh, b: SolidMesh, reference_pair_order: bool = False):
+        """reference_pair_order: weld the intersection points in the order the
+        reference's own tree search reports the pairs (see
+        SolidBoolean::setReferencePairOrder)."""
         self.L = load()
         self.a, self.b = a, b
         self.h = None
+        self.reference_pair_order = reference_pair_order
 
     def combine(self) -> bool:
         h = C.c_void_p()
-        r = self.L.sbh_boolean_new(self.a.h, self.b.h, C.byref(h))
+        r = self.L.sbh_boolean_new_ordered(self.a.h, self.b.h, int(self.reference_pair_order), C.byref(h))
         self.h = h
         if r < 0:
             raise RuntimeError(self.L.sbh_last_error().decode())

@@ -4,11 +4,13 @@
 #include <cstdint>
 #include <cstring>
 #include <exception>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "earclip.h"
 #include "objfile.h"
+#include "referencetree.h"
 #include "solidboolean.h"
 #include "solidmesh.h"
 
@@ -63,11 +65,13 @@ void *sbh_mesh_new(const double *xyz, size_t nv, const int64_t *tri, size_t nt)
 
 void sbh_mesh_free(void *p) { delete (HostMesh *)p; }
 
-// SolidBoolean::combine() + fetch*; returns 1 / 0 as combine(), -1 on a GPU error.
-int sbh_boolean_new(void *a, void *b, void **out)
+// SolidBoolean::combine() + fetch*; returns 1 / 0 as combine(), -1 on a GPU
+// error.  referenceOrder: SolidBoolean::setReferencePairOrder.
+int sbh_boolean_new_ordered(void *a, void *b, int referenceOrder, void **out)
 {
     HostBoolean *h = new HostBoolean(&((HostMesh *)a)->mesh, &((HostMesh *)b)->mesh);
     *out = h;
+    h->sb.setReferencePairOrder(referenceOrder != 0);
     try {
         h->ok = h->sb.combine();
         if (h->ok) {
@@ -81,6 +85,8 @@ int sbh_boolean_new(void *a, void *b, void **out)
         return -1;
     }
 }
+
+int sbh_boolean_new(void *a, void *b, void **out) { return sbh_boolean_new_ordered(a, b, 0, out); }
 
 void sbh_boolean_free(void *p) { delete (HostBoolean *)p; }
 
@@ -195,6 +201,45 @@ int sbh_obj_export(const char *path, const double *xyz, size_t nv, const int64_t
             g_hostErr = "cannot write the file";
             return 0;
         }
+        return 1;
+    } catch (const std::exception &e) {
+        g_hostErr = e.what();
+        return 0;
+    }
+}
+
+// The permutation that puts the n pairs ab (a of mesh A, b of mesh B) into
+// the order the reference's search reports them (referencetree.h).  Host only.
+// 1 on success, 0 on failure.
+int sbh_reference_pair_order(const double *xyzA, size_t nvA, const int64_t *triA, size_t ntA, const double *xyzB,
+                             size_t nvB, const int64_t *triB, size_t ntB, const uint32_t *ab, size_t n,
+                             uint64_t *perm)
+{
+    try {
+        auto mesh = [](const double *xyz, size_t nv, const int64_t *tri, size_t nt, std::vector<Vector3> &v,
+                       std::vector<std::vector<size_t>> &t) {
+            v.resize(nv);
+            for (size_t i = 0; i < nv; ++i)
+                v[i] = Vector3(xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+            t.resize(nt);
+            for (size_t i = 0; i < nt; ++i) {
+                t[i] = {(size_t)tri[3 * i], (size_t)tri[3 * i + 1], (size_t)tri[3 * i + 2]};
+                for (size_t k : t[i])
+                    if (k >= nv)
+                        throw std::out_of_range("triangle index out of range");
+            }
+        };
+        std::vector<Vector3> va, vb;
+        std::vector<std::vector<size_t>> ta, tb;
+        mesh(xyzA, nvA, triA, ntA, va, ta);
+        mesh(xyzB, nvB, triB, ntB, vb, tb);
+        for (size_t i = 0; i < n; ++i)
+            if (ab[2 * i] >= ntA || ab[2 * i + 1] >= ntB)
+                throw std::out_of_range("pair index out of range");
+        const sbhost::ReferenceTree A(va, ta), B(vb, tb);
+        const std::vector<size_t> order = sbhost::referencePairOrder(A, B, ab, n);
+        for (size_t i = 0; i < n; ++i)
+            perm[i] = order[i];
         return 1;
     } catch (const std::exception &e) {
         g_hostErr = e.what();

@@ -1,0 +1,199 @@
+// See referencetree.h.  The split rules follow
+// src/axisalignedboundingboxtree.cpp:27-141 over the boxes and centers of
+// src/solidmesh.cpp:42-76 and src/axisalignedboundingbox.h:31-53; the walk
+// follows src/axisalignedboundingboxtree.h:54-95.  Every value the shape
+// depends on is computed with the same FP64 operations in the same order
+// (sums of centers in member order, a division by the member count narrowed to
+// float, spans narrowed to float).
+#include "referencetree.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <utility>
+
+namespace sbhost {
+
+namespace {
+
+constexpr size_t kLeafMax = 20;  // src/axisalignedboundingboxtree.cpp:25
+
+struct Box {
+    double lo[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    double hi[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    void add(const double *p)
+    {
+        for (int k = 0; k < 3; ++k) {
+            if (p[k] > hi[k])
+                hi[k] = p[k];
+            if (p[k] < lo[k])
+                lo[k] = p[k];
+        }
+    }
+};
+
+}  // namespace
+
+ReferenceTree::ReferenceTree(const std::vector<Vector3> &vertices, const std::vector<std::vector<size_t>> &triangles)
+{
+    const size_t nt = triangles.size();
+    m_leafOf.assign(nt, 0);
+    m_place.assign(nt, 0);
+    // per triangle: box and center (the running sum from 0, divided by 3)
+    std::vector<Box> box(nt);
+    std::vector<Vector3> center(nt);
+    Box all;
+    Vector3 rootCenter(0.0, 0.0, 0.0);
+    for (size_t t = 0; t < nt; ++t) {
+        Vector3 sum(0.0, 0.0, 0.0);
+        for (size_t k = 0; k < 3; ++k) {
+            const Vector3 &p = vertices[triangles[t][k]];
+            box[t].add(p.constData());
+            all.add(p.constData());
+            sum += p;
+        }
+        center[t] = sum /= (double)(float)3;
+        rootCenter += center[t];
+    }
+    if (nt)
+        rootCenter /= (double)(float)nt;
+
+    // nodes own ranges of `members`; a split rewrites its range as the left
+    // side's members in reverse scan order, then the right side's in order
+    std::vector<uint32_t> members(nt), scratch(nt);
+    for (size_t t = 0; t < nt; ++t)
+        members[t] = (uint32_t)t;
+    struct Work {
+        uint32_t node;
+        size_t begin, end;
+        Box box;
+        Vector3 center;
+    };
+    m_nodes.emplace_back();
+    m_nodes[0].count = (uint32_t)nt;
+    std::vector<Work> stack{{0, 0, nt, all, rootCenter}};
+    while (!stack.empty()) {
+        Work w = stack.back();
+        stack.pop_back();
+        const size_t n = w.end - w.begin;
+        if (n <= kLeafMax) {
+            for (size_t i = w.begin; i < w.end; ++i) {
+                m_leafOf[members[i]] = w.node;
+                m_place[members[i]] = (uint32_t)(i - w.begin);
+            }
+            continue;
+        }
+        // the longest axis of the node's box (spans narrowed to float; the
+        // first of equal ones), split at the node's center on it
+        float span[3];
+        for (int k = 0; k < 3; ++k)
+            span[k] = (float)(w.box.hi[k] - w.box.lo[k]);
+        int axis = 0;
+        for (int k = 1; k < 3; ++k)
+            if (span[axis] < span[k])
+                axis = k;
+        const double split = w.center.constData()[axis];
+        // left members fill scratch from the top down (so in reverse scan
+        // order once read upwards), right ones from the bottom up
+        size_t nl = 0, nr = 0;
+        uint32_t *lo = scratch.data() + w.begin;
+        for (size_t i = w.begin; i < w.end; ++i) {
+            const uint32_t t = members[i];
+            if (center[t].constData()[axis] < split)
+                lo[n - 1 - nl++] = t;
+            else
+                lo[nr++] = t;
+        }
+        // members = left (reverse scan order), then right (scan order)
+        for (size_t j = 0; j < nl; ++j)
+            members[w.begin + j] = lo[n - nl + j];
+        for (size_t j = 0; j < nr; ++j)
+            members[w.begin + nl + j] = lo[j];
+        size_t cut = nl;
+        if (nl == 0)
+            cut = n / 2;
+        else if (nr == 0)
+            cut = n - n / 2;
+        Work child[2];
+        const size_t bounds[3] = {w.begin, w.begin + cut, w.end};
+        for (int c = 0; c < 2; ++c) {
+            Work &x = child[c];
+            x.begin = bounds[c];
+            x.end = bounds[c + 1];
+            x.center = Vector3(0.0, 0.0, 0.0);
+            for (size_t j = x.begin; j < x.end; ++j) {
+                const uint32_t t = members[j];
+                x.box.add(box[t].lo);
+                x.box.add(box[t].hi);
+                x.center += center[t];
+            }
+            x.center /= (double)(float)(x.end - x.begin);
+            x.node = (uint32_t)m_nodes.size();
+            m_nodes.emplace_back();
+            m_nodes.back().count = (uint32_t)(x.end - x.begin);
+        }
+        m_nodes[w.node].left = child[0].node;
+        m_nodes[w.node].right = child[1].node;
+        stack.push_back(child[1]);
+        stack.push_back(child[0]);
+    }
+    // the turns down to every leaf
+    m_paths.assign(m_nodes.size(), std::string());
+    std::vector<uint32_t> todo{0};
+    while (!todo.empty()) {
+        const uint32_t v = todo.back();
+        todo.pop_back();
+        const Node &x = m_nodes[v];
+        if (x.leaf())
+            continue;
+        m_paths[x.left] = m_paths[v] + '0';
+        m_paths[x.right] = m_paths[v] + '1';
+        todo.push_back(x.left);
+        todo.push_back(x.right);
+    }
+    for (size_t v = 0; v < m_nodes.size(); ++v)
+        if (!m_nodes[v].leaf())
+            std::string().swap(m_paths[v]);
+}
+
+std::vector<size_t> referencePairOrder(const ReferenceTree &first, const ReferenceTree &second, const uint32_t *ab,
+                                       size_t n)
+{
+    // A pair's key is the walk's turns from the two roots to its two leaves
+    // (which tree turns at each step is fixed by the node pair: the inner
+    // node, or of two inner nodes the one with more triangles, the first on a
+    // tie), then the two places in the leaves, which the walk's nested loop
+    // takes first-major.  Keys of different leaf pairs differ before either
+    // ends, so plain string order is the walk's order.
+    const auto &A = first.nodes();
+    const auto &B = second.nodes();
+    std::vector<std::string> key(n);
+    for (size_t i = 0; i < n; ++i) {
+        const uint32_t a = ab[2 * i], b = ab[2 * i + 1];
+        const std::string &pa = first.pathTo(a), &pb = second.pathTo(b);
+        std::string &k = key[i];
+        k.reserve(pa.size() + pb.size() + 2);
+        uint32_t na = 0, nb = 0;
+        size_t ia = 0, ib = 0;
+        while (!(A[na].leaf() && B[nb].leaf())) {
+            const bool turnB = A[na].leaf() || (!B[nb].leaf() && A[na].count < B[nb].count);
+            if (turnB) {
+                const char c = pb[ib++];
+                nb = c == '0' ? B[nb].left : B[nb].right;
+                k += c;
+            } else {
+                const char c = pa[ia++];
+                na = c == '0' ? A[na].left : A[na].right;
+                k += c;
+            }
+        }
+        k += (char)('A' + first.placeInLeaf(a));
+        k += (char)('A' + second.placeInLeaf(b));
+    }
+    std::vector<size_t> order(n);
+    for (size_t i = 0; i < n; ++i)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return key[x] < key[y]; });
+    return order;
+}
+
+}  // namespace sbhost

@@ -7,6 +7,7 @@
 #include <deque>
 #include <stdexcept>
 
+#include "referencetree.h"
 #include "retriangulator.h"
 #include "sbhost_log.h"
 
@@ -383,6 +384,20 @@ bool SolidBoolean::combine()
         }
         sbg_pairs_destroy(pairs);
         sbhost::check(rc, "sbg_tri_tri");
+    }
+    if (m_referencePairOrder && ab.size() > 2) {
+        const sbhost::ReferenceTree first(*m_firstMesh->vertices(), *m_firstMesh->triangles());
+        const sbhost::ReferenceTree second(*m_secondMesh->vertices(), *m_secondMesh->triangles());
+        const std::vector<size_t> order = sbhost::referencePairOrder(first, second, ab.data(), ab.size() / 2);
+        std::vector<uint32_t> ab2(ab.size());
+        std::vector<double> seg2(seg.size());
+        for (size_t i = 0; i < order.size(); ++i) {
+            ab2[2 * i] = ab[2 * order[i]];
+            ab2[2 * i + 1] = ab[2 * order[i] + 1];
+            std::memcpy(&seg2[6 * i], &seg[6 * order[i]], 6 * sizeof(double));
+        }
+        ab.swap(ab2);
+        seg.swap(seg2);
     }
     m_firstIntersected.assign(na, 0);
     m_secondIntersected.assign(nb, 0);

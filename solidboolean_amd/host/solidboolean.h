@@ -7,8 +7,9 @@
 // welding of intersection points (first inserted wins), re-triangulation of
 // cut triangles, intersection loops, half-edge flood fill into groups.  Host
 // maps are ordered (or keyed hash tables whose iteration order is never
-// observed), so results do not depend on hash iteration order; candidate pairs
-// are processed in (a, b) order.
+// observed), so results do not depend on hash iteration order; intersecting
+// pairs are processed in (a, b) order, or in the reference's search order
+// (setReferencePairOrder).
 #pragma once
 #include <chrono>
 #include <cstdint>
@@ -58,6 +59,14 @@ public:
     // p0, p1, p2 (the other triangle).  False if some endpoint matches no edge.
     bool vertexSources(std::vector<uint32_t> &src5) const;
     std::vector<uint8_t> triangleSides(int which) const;
+    // Before combine() (no counterpart in the reference): take the
+    // intersecting pairs in the order the reference's own tree search reports
+    // them instead of (a, b) order.  The weld keeps the first point it meets
+    // in each PositionKey cell, so with this on, the kept points are the
+    // reference's even where two intersection points share a cell.  Costs a
+    // host build of the reference's tree for both meshes (about 20-30 ms per
+    // 82k-triangle mesh).
+    void setReferencePairOrder(bool on) { m_referencePairOrder = on; }
 
     // open-addressed map from a half-edge key (first << 32 | second) to a triangle
     class HalfEdgeMap {
@@ -93,6 +102,7 @@ private:
     std::vector<Vector3> m_newVertices;
     std::vector<std::array<size_t, 3>> m_newTriangles;
     bool m_repeatedHalfEdge = false;  // some directed edge belongs to two new triangles
+    bool m_referencePairOrder = false;
     std::map<PositionKey, size_t> m_newPositionMap;
     std::vector<std::vector<size_t>> m_firstTriangleGroups, m_secondTriangleGroups;
     std::vector<bool> m_firstGroupSides, m_secondGroupSides;
