@@ -45,8 +45,8 @@ def load():
     L.sbh_mesh_free.argtypes = [_P]
     L.sbh_boolean_new.restype = C.c_int
     L.sbh_boolean_new.argtypes = [_P, _P, C.POINTER(C.c_void_p)]
-    L.sbh_boolean_new_ordered.restype = C.c_int
-    L.sbh_boolean_new_ordered.argtypes = [_P, _P, C.c_int, C.POINTER(C.c_void_p)]
+    L.sbh_boolean_new_opts.restype = C.c_int
+    L.sbh_boolean_new_opts.argtypes = [_P, _P, C.c_uint, C.POINTER(C.c_void_p)]
     L.sbh_boolean_free.argtypes = [_P]
     L.sbh_boolean_vertices.restype = C.c_size_t
     L.sbh_boolean_vertices.argtypes = [_P, _D, C.c_size_t]
@@ -168,18 +168,20 @@ class SolidMesh:
 
 
 class SolidBoolean:
-    def __init__(self, a: SolidMesh, b: SolidMesh, reference_pair_order: bool = False):
+    def __init__(self, a: SolidMesh, b: SolidMesh, reference_pair_order: bool = False, robust_weld: bool = False):
         """reference_pair_order: weld the intersection points in the order the
-        reference's own tree search reports the pairs (see
-        SolidBoolean::setReferencePairOrder)."""
+        reference's own tree search reports the pairs
+        (SolidBoolean::setReferencePairOrder).  robust_weld: weld them by the
+        edge-triangle crossing each one is, not by PositionKey cell
+        (SolidBoolean::setRobustWeld; not the reference's behaviour)."""
         self.L = load()
         self.a, self.b = a, b
         self.h = None
-        self.reference_pair_order = reference_pair_order
+        self.flags = (1 if reference_pair_order else 0) | (2 if robust_weld else 0)
 
     def combine(self) -> bool:
         h = C.c_void_p()
-        r = self.L.sbh_boolean_new_ordered(self.a.h, self.b.h, int(self.reference_pair_order), C.byref(h))
+        r = self.L.sbh_boolean_new_opts(self.a.h, self.b.h, self.flags, C.byref(h))
         self.h = h
         if r < 0:
             raise RuntimeError(self.L.sbh_last_error().decode())

@@ -66,12 +66,13 @@ void *sbh_mesh_new(const double *xyz, size_t nv, const int64_t *tri, size_t nt)
 void sbh_mesh_free(void *p) { delete (HostMesh *)p; }
 
 // SolidBoolean::combine() + fetch*; returns 1 / 0 as combine(), -1 on a GPU
-// error.  referenceOrder: SolidBoolean::setReferencePairOrder.
-int sbh_boolean_new_ordered(void *a, void *b, int referenceOrder, void **out)
+// error.  flags: 1 SolidBoolean::setReferencePairOrder, 2 setRobustWeld.
+int sbh_boolean_new_opts(void *a, void *b, unsigned flags, void **out)
 {
     HostBoolean *h = new HostBoolean(&((HostMesh *)a)->mesh, &((HostMesh *)b)->mesh);
     *out = h;
-    h->sb.setReferencePairOrder(referenceOrder != 0);
+    h->sb.setReferencePairOrder((flags & 1u) != 0);
+    h->sb.setRobustWeld((flags & 2u) != 0);
     try {
         h->ok = h->sb.combine();
         if (h->ok) {
@@ -86,7 +87,7 @@ int sbh_boolean_new_ordered(void *a, void *b, int referenceOrder, void **out)
     }
 }
 
-int sbh_boolean_new(void *a, void *b, void **out) { return sbh_boolean_new_ordered(a, b, 0, out); }
+int sbh_boolean_new(void *a, void *b, void **out) { return sbh_boolean_new_opts(a, b, 0, out); }
 
 void sbh_boolean_free(void *p) { delete (HostBoolean *)p; }
 

@@ -72,8 +72,11 @@ void ReTriangulator::lookupPolylines()
     }
 }
 
-int ReTriangulator::attachPointToTriangleEdge(const Vector2 &p) const
+int ReTriangulator::attachPointToTriangleEdge(size_t index) const
 {
+    if (m_edgeHints && index >= 3 && index - 3 < m_edgeHints->size() && (*m_edgeHints)[index - 3] >= 0)
+        return (*m_edgeHints)[index - 3];
+    const Vector2 &p = m_points[index];
     for (int i = 0; i < 3; ++i)
         if (p.isOnLine(m_points[i], m_points[(i + 1) % 3]))
             return i;
@@ -97,8 +100,8 @@ bool ReTriangulator::buildPolygons()
     std::array<std::vector<EdgePoint>, 3> edgePoints;
     for (int pi = 0; pi < (int)m_polylines.size(); ++pi) {
         const auto &line = m_polylines[pi];
-        const int fe = attachPointToTriangleEdge(m_points[line.front()]);
-        const int be = attachPointToTriangleEdge(m_points[line.back()]);
+        const int fe = attachPointToTriangleEdge(line.front());
+        const int be = attachPointToTriangleEdge(line.back());
         if (fe == -1 || be == -1) {
             sbhost::diag("Attach point to triangle edge failed");
             return false;

@@ -20,6 +20,9 @@ public:
     ReTriangulator(const std::vector<Vector3> &points, const Vector3 &normal);
     // points: the cut points (index 3 + k in neighborMapFrom3)
     void setEdges(const std::vector<Vector3> &points, const std::map<size_t, std::set<size_t>> *neighborMapFrom3);
+    // hints[k] (cut point 3 + k): the corner edge it lies on (0: 0-1, 1: 1-2,
+    // 2: 2-0) or -1 to test it; kept by pointer until reTriangulate()
+    void setEdgeHints(const std::vector<int> *hints) { m_edgeHints = hints; }
     bool reTriangulate();
     const std::vector<std::vector<size_t>> &polygons() const { return m_polygons; }
     const std::vector<std::vector<size_t>> &triangles() const { return m_triangles; }
@@ -28,6 +31,7 @@ private:
     Vector3 m_projectAxis, m_projectOrigin, m_projectNormal;
     std::vector<Vector2> m_points;
     const std::map<size_t, std::set<size_t>> *m_neighborMap = nullptr;
+    const std::vector<int> *m_edgeHints = nullptr;
     std::vector<std::vector<size_t>> m_polylines;
     std::vector<std::vector<size_t>> m_innerPolygons;
     std::vector<std::vector<size_t>> m_polygons;
@@ -37,7 +41,7 @@ private:
     std::vector<std::vector<size_t>> m_triangles;
 
     void lookupPolylines();
-    int attachPointToTriangleEdge(const Vector2 &p) const;
+    int attachPointToTriangleEdge(size_t index) const;
     bool buildPolygons();
     void buildPolygonHierarchy();
     void triangulate();

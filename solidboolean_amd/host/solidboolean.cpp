@@ -2,6 +2,7 @@
 // src/solidboolean.cpp:288-480.
 #include "solidboolean.h"
 
+#include <algorithm>
 #include <cstring>
 
 #include <deque>
@@ -90,6 +91,14 @@ SolidBoolean::~SolidBoolean() = default;
 
 size_t SolidBoolean::addNewPoint(const Vector3 &position, int64_t origin)
 {
+    if (m_robustWeld && origin >= 0 && (size_t)origin < m_endpointKeys.size()) {
+        auto r = m_newKeyMap.insert({m_endpointKeys[(size_t)origin], m_newVertices.size()});
+        if (r.second) {
+            m_newVertices.push_back(position);
+            m_vertexSource.push_back(-origin - 1);
+        }
+        return r.first->second;
+    }
     auto r = m_newPositionMap.insert({PositionKey(position), m_newVertices.size()});
     if (r.second) {
         m_newVertices.push_back(position);
@@ -144,6 +153,23 @@ bool SolidBoolean::reTriangulate(const ContextMap &contexts, const SolidMesh *me
         const auto &t = tris[kv.first];
         ReTriangulator rt({verts[t[0]], verts[t[1]], verts[t[2]]}, (*normals)[kv.first]);
         rt.setEdges(kv.second.points, &kv.second.neighborMap);
+        std::vector<int> hints;
+        if (m_robustWeld) {
+            // a cut point made on one of this triangle's own edges lies on it
+            hints.assign(kv.second.points.size(), -1);
+            for (size_t k = 0; k < hints.size(); ++k) {
+                const int64_t o = kv.second.origin[k];
+                if (o < 0 || (size_t)o >= m_endpointKeys.size() || m_endpointKeys[(size_t)o].edge == ~0ull)
+                    continue;
+                const uint64_t e = m_endpointKeys[(size_t)o].edge;
+                for (int i = 0; i < 3; ++i) {
+                    const uint64_t x = startOldVertex + t[i], y = startOldVertex + t[(i + 1) % 3];
+                    if (e == (std::min(x, y) << 32 | std::max(x, y)))
+                        hints[k] = i;
+                }
+            }
+            rt.setEdgeHints(&hints);
+        }
         if (!rt.reTriangulate()) {
             sbhost::diag("Retriangle failed");
             return false;
@@ -401,8 +427,38 @@ bool SolidBoolean::combine()
     }
     m_firstIntersected.assign(na, 0);
     m_secondIntersected.assign(nb, 0);
+    if (m_robustWeld) {
+        // every endpoint's crossing: an edge of one mesh with a triangle of the
+        // other; endpoints that match none are keyed by their PositionKey cell
+        const size_t nh = ab.size() / 2;
+        m_endpointKeys.assign(2 * nh, WeldKey{~0ull, 0});
+        std::map<PositionKey, uint64_t> cells;
+        const uint32_t offB = (uint32_t)m_firstMesh->vertices()->size();
+        for (size_t h = 0; h < nh; ++h)
+            for (size_t e = 0; e < 2; ++e) {
+                uint32_t d[5];
+                WeldKey &k = m_endpointKeys[2 * h + e];
+                if (endpointSource(ab.data(), seg.data(), h, e, d)) {
+                    const uint64_t x = d[0], y = d[1];
+                    const bool edgeOfA = x < offB;
+                    k.edge = std::min(x, y) << 32 | std::max(x, y);
+                    k.other = edgeOfA ? na + ab[2 * h + 1] : ab[2 * h];
+                } else {
+                    const double *p = &seg[6 * h + 3 * e];
+                    k.other = cells.insert({PositionKey(Vector3(p[0], p[1], p[2])), cells.size()}).first->second;
+                }
+            }
+    }
     ContextMap firstCtx, secondCtx;
-    auto addPoint = [](IntersectedContext &c, const Vector3 &p, int64_t origin) {
+    auto addPoint = [this](IntersectedContext &c, const Vector3 &p, int64_t origin) {
+        if (m_robustWeld) {
+            auto r = c.keyMap.insert({m_endpointKeys[(size_t)origin], c.points.size()});
+            if (r.second) {
+                c.points.push_back(p);
+                c.origin.push_back(origin);
+            }
+            return r.first->second;
+        }
         auto r = c.positionMap.insert({PositionKey(p), c.points.size()});
         if (r.second) {
             c.points.push_back(p);
@@ -554,12 +610,48 @@ bool same_bits(const D3 &a, const double *b)
 
 }  // namespace
 
-bool SolidBoolean::vertexSources(std::vector<uint32_t> &src5) const
+bool SolidBoolean::endpointSource(const uint32_t *ab, const double *seg, size_t h, size_t e, uint32_t d[5]) const
 {
-    const auto &va = *m_firstMesh->vertices();
     const auto &ta = *m_firstMesh->triangles();
     const auto &tb = *m_secondMesh->triangles();
-    const uint32_t offB = (uint32_t)va.size();
+    const uint32_t offB = (uint32_t)m_firstMesh->vertices()->size();
+    const size_t a = ab[2 * h], b = ab[2 * h + 1];
+    const double *pt = &seg[6 * h + 3 * e];
+    // the two triangles, ids into the input vertices, as the GD test saw them
+    uint32_t id[2][3];
+    D3 v[2][3];
+    for (int k = 0; k < 3; ++k) {
+        id[0][k] = (uint32_t)ta[a][k];
+        id[1][k] = offB + (uint32_t)tb[b][k];
+        const Vector3 &pa = (*m_firstMesh->vertices())[ta[a][k]];
+        const Vector3 &pb = (*m_secondMesh->vertices())[tb[b][k]];
+        v[0][k] = {pa.x(), pa.y(), pa.z()};
+        v[1][k] = {pb.x(), pb.y(), pb.z()};
+    }
+    // N1 = (q1 - p1) x (r1 - p1), N2 = (p2 - r2) x (q2 - r2) (tri_tri_intersect.c:403-431)
+    const D3 N[2] = {dcross(dsub(v[0][1], v[0][0]), dsub(v[0][2], v[0][0])),
+                     dcross(dsub(v[1][0], v[1][2]), dsub(v[1][1], v[1][2]))};
+    for (int owner = 0; owner < 2; ++owner)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                if (i == j)
+                    continue;
+                for (int k = 0; k < 3; ++k) {
+                    if (!same_bits(edge_plane(v[owner][i], v[1 - owner][k], v[owner][j], N[1 - owner]), pt))
+                        continue;
+                    d[0] = id[owner][i];
+                    d[1] = id[owner][j];
+                    d[2] = id[1 - owner][0];
+                    d[3] = id[1 - owner][1];
+                    d[4] = id[1 - owner][2];
+                    return true;
+                }
+            }
+    return false;
+}
+
+bool SolidBoolean::vertexSources(std::vector<uint32_t> &src5) const
+{
     const uint32_t none = 0xffffffffu;
     src5.assign(5 * m_vertexSource.size(), none);
     for (size_t r = 0; r < m_vertexSource.size(); ++r) {
@@ -569,41 +661,7 @@ bool SolidBoolean::vertexSources(std::vector<uint32_t> &src5) const
             continue;
         }
         const size_t h = (size_t)((-o - 1) / 2), e = (size_t)((-o - 1) % 2);
-        const size_t a = m_hitPairs[2 * h], b = m_hitPairs[2 * h + 1];
-        const double *pt = &m_hitSegments[6 * h + 3 * e];
-        // the two triangles, ids into the input vertices, as the GD test saw them
-        uint32_t id[2][3];
-        D3 v[2][3];
-        for (int k = 0; k < 3; ++k) {
-            id[0][k] = (uint32_t)ta[a][k];
-            id[1][k] = offB + (uint32_t)tb[b][k];
-            const Vector3 &pa = (*m_firstMesh->vertices())[ta[a][k]];
-            const Vector3 &pb = (*m_secondMesh->vertices())[tb[b][k]];
-            v[0][k] = {pa.x(), pa.y(), pa.z()};
-            v[1][k] = {pb.x(), pb.y(), pb.z()};
-        }
-        // N1 = (q1 - p1) x (r1 - p1), N2 = (p2 - r2) x (q2 - r2) (tri_tri_intersect.c:403-431)
-        const D3 N[2] = {dcross(dsub(v[0][1], v[0][0]), dsub(v[0][2], v[0][0])),
-                         dcross(dsub(v[1][0], v[1][2]), dsub(v[1][1], v[1][2]))};
-        bool found = false;
-        for (int owner = 0; owner < 2 && !found; ++owner)
-            for (int i = 0; i < 3 && !found; ++i)
-                for (int j = 0; j < 3 && !found; ++j) {
-                    if (i == j)
-                        continue;
-                    for (int k = 0; k < 3 && !found; ++k) {
-                        if (!same_bits(edge_plane(v[owner][i], v[1 - owner][k], v[owner][j], N[1 - owner]), pt))
-                            continue;
-                        found = true;
-                        uint32_t *d = &src5[5 * r];
-                        d[0] = id[owner][i];
-                        d[1] = id[owner][j];
-                        d[2] = id[1 - owner][0];
-                        d[3] = id[1 - owner][1];
-                        d[4] = id[1 - owner][2];
-                    }
-                }
-        if (!found)
+        if (!endpointSource(m_hitPairs.data(), m_hitSegments.data(), h, e, &src5[5 * r]))
             return false;
     }
     return true;

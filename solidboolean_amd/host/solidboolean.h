@@ -67,6 +67,17 @@ public:
     // host build of the reference's tree for both meshes (about 20-30 ms per
     // 82k-triangle mesh).
     void setReferencePairOrder(bool on) { m_referencePairOrder = on; }
+    // Before combine() (not the reference's behaviour, off by default): weld
+    // intersection points by what they are, the crossing of one mesh's edge
+    // with the other mesh's triangle, recovered bit for bit from the segment
+    // endpoints, instead of by 1e-5 PositionKey cells; and tell the
+    // re-triangulation which of its triangle's edges a cut point lies on
+    // instead of testing it.  Distinct points closer than 1e-5 then stay
+    // distinct, and points on an edge are never rejected, which is where the
+    // reference's combine() fails on dense meshes ("Attach point to triangle
+    // edge failed").  An endpoint that matches no edge-plane crossing (a
+    // degenerate touch) falls back to its PositionKey cell.
+    void setRobustWeld(bool on) { m_robustWeld = on; }
 
     // open-addressed map from a half-edge key (first << 32 | second) to a triangle
     class HalfEdgeMap {
@@ -88,10 +99,18 @@ public:
     };
 
 private:
+    // robust weld: per hit endpoint (2 h + e) its key; (edge (lo << 32 | hi) of
+    // input vertex ids, other triangle (first mesh's, then second's)), or
+    // (~0, PositionKey cell number) when no crossing matched
+    struct WeldKey {
+        uint64_t edge, other;
+        bool operator<(const WeldKey &o) const { return edge != o.edge ? edge < o.edge : other < o.other; }
+    };
     struct IntersectedContext {
         std::vector<Vector3> points;
         std::vector<int64_t> origin;  // per point: 2 * hit + endpoint of its first insertion
         std::map<PositionKey, size_t> positionMap;
+        std::map<WeldKey, size_t> keyMap;  // robust weld
         std::map<size_t, std::set<size_t>> neighborMap;
     };
     using ContextMap = std::map<size_t, IntersectedContext>;
@@ -103,6 +122,10 @@ private:
     std::vector<std::array<size_t, 3>> m_newTriangles;
     bool m_repeatedHalfEdge = false;  // some directed edge belongs to two new triangles
     bool m_referencePairOrder = false;
+    bool m_robustWeld = false;
+    std::vector<WeldKey> m_endpointKeys;
+    std::map<WeldKey, size_t> m_newKeyMap;
+    bool endpointSource(const uint32_t *ab, const double *seg, size_t h, size_t e, uint32_t d[5]) const;
     std::map<PositionKey, size_t> m_newPositionMap;
     std::vector<std::vector<size_t>> m_firstTriangleGroups, m_secondTriangleGroups;
     std::vector<bool> m_firstGroupSides, m_secondGroupSides;

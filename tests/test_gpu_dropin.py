@@ -105,3 +105,62 @@ def test_fetch_appends_and_diff_reverses():
     b_in = len(d) - k
     assert b_in > 0 and k > 0
     assert np.array_equal(d[k:], i[len(i) - b_in:][:, ::-1])
+
+
+def signed_volume(v, t):
+    t = np.asarray(t, np.int64)
+    a, b, c = v[t[:, 0]], v[t[:, 1]], v[t[:, 2]]
+    return float(np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0)
+
+
+def watertight(t):
+    """Every directed edge once, and its reverse present: a closed, consistently
+    oriented surface."""
+    t = np.asarray(t, np.int64)
+    e = np.concatenate([t[:, [0, 1]], t[:, [1, 2]], t[:, [2, 0]]])
+    fwd = e[:, 0] * (1 << 32) + e[:, 1]
+    u, c = np.unique(fwd, return_counts=True)
+    return bool(np.all(c == 1) and np.all(np.isin(e[:, 1] * (1 << 32) + e[:, 0], u)))
+
+
+@pytest.mark.parametrize("name", [n for n in FULL_CASES if n != "cube_n8"])
+def test_robust_weld_agrees_where_reference_succeeds(name):
+    g = load_golden(name)
+    a = dropin.SolidMesh(g["va"], g["ta"])
+    b = dropin.SolidMesh(g["vb"], g["tb"])
+    a.prepare()
+    b.prepare()
+    sb = dropin.SolidBoolean(a, b, robust_weld=True)
+    assert sb.combine() == bool(g["combine_ok"])
+    counts = [len(sb.fetch_union()), len(sb.fetch_diff()), len(sb.fetch_intersect())]
+    assert counts == list(g["face_counts"]), name
+    assert np.array_equal(sb.triangle_sides(0), g["side_a"]), name
+    assert np.array_equal(sb.triangle_sides(1), g["side_b"]), name
+
+
+@pytest.mark.parametrize("which", ["sub6_s3", "large_s1"])
+def test_robust_weld_succeeds_where_reference_fails(which):
+    """Pairs on which the reference's combine() fails ("Attach point to
+    triangle edge failed"), as ours does by default: with the robust weld the
+    union, difference and intersection are closed, consistently oriented
+    surfaces, and their volumes obey inclusion-exclusion:
+    V(A u B) + V(A n B) = V(A) + V(B) and V(A - B) = V(A) - V(A n B)."""
+    g = load_golden(which)
+    assert not bool(g["combine_ok"])
+    pa, pb = mg.icosphere_pair(6, 3, (0.5, 0.2, 0.1)) if which == "sub6_s3" else mg.large_pair(1)
+    assert mg.mesh_digest(pa) == str(g["digest_a"]) and mg.mesh_digest(pb) == str(g["digest_b"])
+    a = dropin.SolidMesh(pa.vertices, pa.triangles)
+    b = dropin.SolidMesh(pb.vertices, pb.triangles)
+    a.prepare()
+    b.prepare()
+    assert not dropin.SolidBoolean(a, b).combine()
+    sb = dropin.SolidBoolean(a, b, robust_weld=True)
+    assert sb.combine()
+    v = sb.result_vertices()
+    u, d, i = sb.fetch_union(), sb.fetch_diff(), sb.fetch_intersect()
+    assert watertight(u) and watertight(d) and watertight(i)
+    va, vb = signed_volume(pa.vertices, pa.triangles), signed_volume(pb.vertices, pb.triangles)
+    vu, vd, vi = signed_volume(v, u), signed_volume(v, d), signed_volume(v, i)
+    assert vi > 0 and vu > max(va, vb)
+    assert abs(vu + vi - (va + vb)) < 1e-9 * (va + vb)
+    assert abs(vd - (va - vi)) < 1e-9 * va
