@@ -259,6 +259,29 @@ def dropin_section(mg, steps: int) -> dict:
     return out
 
 
+def dropin_large_section(a, b, reps: int = 3) -> dict:
+    """The whole boolean on the 1.31M-triangle pair through the drop-in with
+    the robust weld (the reference's combine() fails on this pair, and so
+    does the drop-in's default weld, which follows it)."""
+    from solidboolean_amd import dropin
+
+    times, sb = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        A, B = dropin.SolidMesh(a.vertices, a.triangles), dropin.SolidMesh(b.vertices, b.triangles)
+        A.prepare()
+        B.prepare()
+        sb = dropin.SolidBoolean(A, B, robust_weld=True)
+        ok = sb.combine()
+        times.append(time.perf_counter() - t0)
+    out = {"workload": "the 1,310,720-triangle pair, robust weld", "combine_ok": ok,
+           "ms": float(np.median(times)) * 1e3, "stage_seconds": sb.stage_seconds(),
+           "face_counts": [len(sb.fetch_union()), len(sb.fetch_diff()), len(sb.fetch_intersect())] if ok else None,
+           "reference_combine_ok": False}
+    log(f"drop-in robust combine (1.31M pair): {out['ms']:.1f} ms, ok {ok}")
+    return out
+
+
 def stress_section(ctx, mg, N, steps: int) -> dict:
     """The filter stress cases: per case the candidate count, hits, coplanar
     pairs, the FP32 filter's fallback rate, both GD precisions per launch, and
@@ -581,6 +604,7 @@ def main():
     if not args.no_stress:
         line["stress"] = stress_section(ctx, mg, N, args.steps)
         line["dropin_combine"] = dropin_section(mg, args.steps)
+        line["dropin_combine_1310720_robust"] = dropin_large_section(a, b)
     if not args.no_batch:
         line["fp64_vs_filtered_batched"] = fp64_vs_batched(ctx, a, b, step, N, args.steps, not args.no_graph)
         line["batch"] = batch_section(ctx, mg, N, args.steps, not args.no_graph)
