@@ -154,6 +154,19 @@ int sbg_classify_pair(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, uint8_t *out_a, u
  * repeated half-edge.  Needs no sbg_mesh_prepare. */
 int sbg_components(sbg_mesh m, const uint8_t *blocked, uint32_t *labels);
 
+/* The half-edge map of src/solidboolean.cpp:250-286 and :386-396 (directed
+ * edge (from << 32 | to) -> the triangle that has it), built over a host
+ * triangle list in index order, answering what buildFaceGroups (:167-239)
+ * asks of it.  opposite[3 t + i] (ntris * 3) = the triangle that has the
+ * reverse of t's edge from corner i to corner i + 1 (the earliest one), or
+ * 0xffffffff; query_owner[q] (nqueries) = the triangle that has directed edge
+ * queries[q], or 0xffffffff; *first_repeat = the first triangle whose
+ * insertion would meet a directed edge already present (where
+ * addUnintersectedTriangles, :271-282, fails), or 0xffffffff.  Vertex indices
+ * must be below 0xffffffff. */
+int sbg_half_edges(sbg_context ctx, const uint32_t *tris, size_t ntris, const uint64_t *queries, size_t nqueries,
+                   uint32_t *opposite, uint32_t *query_owner, uint32_t *first_repeat);
+
 /* Stage timers: when enabled, every stage is bracketed by HIP events.
  * sbg_context_stage_ms synchronises and returns the milliseconds accumulated
  * per stage (SBG_STAGE_COUNT doubles) since the last reset. */

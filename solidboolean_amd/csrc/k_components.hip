@@ -141,6 +141,40 @@ __global__ __launch_bounds__(kComponentBlock) void k_cc_labels(const uint8_t *__
     labels[t] = (blocked && blocked[t]) ? ~0u : cc_find(parent, t);
 }
 
+// Half-edge table queries for the host's face grouping.  repeat: triangle t
+// repeats a directed edge when an earlier triangle owns one of its half-edges
+// or two of its own are equal (where inserting its half-edges in order would
+// first fail); *firstRepeat = the smallest such t.
+__global__ __launch_bounds__(kComponentBlock) void k_he_opposite(const uint32_t *__restrict__ T, uint32_t nt,
+                                                                 const unsigned long long *__restrict__ keys,
+                                                                 const uint32_t *__restrict__ owners, uint32_t mask,
+                                                                 uint32_t *__restrict__ opposite,
+                                                                 uint32_t *__restrict__ firstRepeat)
+{
+    const uint32_t e = blockIdx.x * kComponentBlock + threadIdx.x;
+    if (e >= 3u * nt)
+        return;
+    const uint32_t t = e / 3, k = e - 3 * t;
+    const uint32_t from = T[3 * (size_t)t + k], to = T[3 * (size_t)t + (k + 1) % 3];
+    opposite[e] = he_owner(keys, owners, mask, (unsigned long long)to << 32 | from);
+    bool repeat = he_owner(keys, owners, mask, (unsigned long long)from << 32 | to) < t;
+    for (uint32_t j = 0; j < k; ++j)
+        repeat |= T[3 * (size_t)t + j] == from && T[3 * (size_t)t + (j + 1) % 3] == to;
+    if (repeat)
+        atomicMin(firstRepeat, t);
+}
+
+__global__ __launch_bounds__(kComponentBlock) void k_he_query(const unsigned long long *__restrict__ queries,
+                                                              uint32_t nq,
+                                                              const unsigned long long *__restrict__ keys,
+                                                              const uint32_t *__restrict__ owners, uint32_t mask,
+                                                              uint32_t *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * kComponentBlock + threadIdx.x;
+    if (i < nq)
+        out[i] = he_owner(keys, owners, mask, queries[i]);
+}
+
 size_t component_table_slots(uint32_t nt)
 {
     size_t cap = 1024;
@@ -170,6 +204,37 @@ void launch_components(const uint32_t *T, uint32_t nt, const uint8_t *blocked, u
                        T, nt, blocked, keys, owners, mask, parent);
     hipLaunchKernelGGL(k_cc_labels, dim3((nt + kComponentBlock - 1) / kComponentBlock), dim3(kComponentBlock), 0, s,
                        blocked, nt, parent, labels);
+}
+
+}  // namespace sbg
+
+namespace sbg {
+
+// keys: component_table_slots(nt) entries; owners: that many + 1 (the last
+// word takes the repeat flag of the insert and is unused here); opposite: 3 nt;
+// firstRepeat: 1 word; queries / queryOwner: nq
+void launch_half_edges(const uint32_t *T, uint32_t nt, const unsigned long long *queries, uint32_t nq,
+                       unsigned long long *keys, uint32_t *owners, uint32_t *opposite, uint32_t *queryOwner,
+                       uint32_t *firstRepeat, hipStream_t s)
+{
+    const size_t slots = component_table_slots(nt);
+    const uint32_t mask = (uint32_t)(slots - 1);
+    launch_fill_u32(firstRepeat, ~0u, 1, s);
+    if (nt) {
+        launch_fill_u32(reinterpret_cast<uint32_t *>(keys), ~0u, 2 * slots, s);
+        launch_fill_u32(owners, ~0u, slots, s);
+        launch_fill_u32(owners + slots, 0u, 1, s);
+        const uint32_t he = 3u * nt;
+        hipLaunchKernelGGL(k_he_insert, dim3((he + kComponentBlock - 1) / kComponentBlock), dim3(kComponentBlock), 0,
+                           s, T, nt, nullptr, keys, owners, mask, owners + slots);
+        hipLaunchKernelGGL(k_he_opposite, dim3((he + kComponentBlock - 1) / kComponentBlock), dim3(kComponentBlock),
+                           0, s, T, nt, keys, owners, mask, opposite, firstRepeat);
+    } else {
+        launch_fill_u32(reinterpret_cast<uint32_t *>(keys), ~0u, 2 * slots, s);
+    }
+    if (nq)
+        hipLaunchKernelGGL(k_he_query, dim3((nq + kComponentBlock - 1) / kComponentBlock), dim3(kComponentBlock), 0, s,
+                           queries, nq, keys, owners, mask, queryOwner);
 }
 
 }  // namespace sbg

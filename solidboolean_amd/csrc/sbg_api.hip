@@ -644,6 +644,44 @@ int sbg_components(sbg_mesh m, const uint8_t *blocked, uint32_t *labels)
     return SBG_OK;
 }
 
+int sbg_half_edges(sbg_context c, const uint32_t *tris, size_t ntris, const uint64_t *queries, size_t nqueries,
+                   uint32_t *opposite, uint32_t *query_owner, uint32_t *first_repeat)
+{
+    if ((ntris && (!tris || !opposite)) || (nqueries && (!queries || !query_owner)) || !first_repeat)
+        return fail(SBG_ERR_INVALID, "null argument");
+    if (ntris >= 0x50000000u || nqueries >= 0xffffffffu)
+        return fail(SBG_ERR_INVALID, "too many triangles or queries");
+    TRY(use(c));
+    for (size_t i = 0; i < 3 * ntris; ++i)
+        if (tris[i] == 0xffffffffu)
+            return fail(SBG_ERR_INVALID, "vertex index 0xffffffff");
+    const size_t slots = component_table_slots((uint32_t)ntris);
+    TRY(c->ccKeys.ensure(slots));
+    TRY(c->ccOwners.ensure(slots + 1));
+    TRY(c->heTris.ensure(3 * ntris));
+    TRY(c->heOpposite.ensure(3 * ntris));
+    TRY(c->heQueries.ensure(nqueries));
+    TRY(c->heQueryOwner.ensure(nqueries));
+    TRY(c->heRepeat.ensure(1));
+    if (ntris)
+        HIP_TRY(hipMemcpyAsync(c->heTris.p, tris, sizeof(uint32_t) * 3 * ntris, hipMemcpyHostToDevice, c->stream));
+    if (nqueries)
+        HIP_TRY(hipMemcpyAsync(c->heQueries.p, queries, sizeof(uint64_t) * nqueries, hipMemcpyHostToDevice,
+                               c->stream));
+    launch_half_edges(c->heTris.p, (uint32_t)ntris, c->heQueries.p, (uint32_t)nqueries, c->ccKeys.p, c->ccOwners.p,
+                      c->heOpposite.p, c->heQueryOwner.p, c->heRepeat.p, c->stream);
+    TRY(check_launch("half edges"));
+    if (ntris)
+        HIP_TRY(hipMemcpyAsync(opposite, c->heOpposite.p, sizeof(uint32_t) * 3 * ntris, hipMemcpyDeviceToHost,
+                               c->stream));
+    if (nqueries)
+        HIP_TRY(hipMemcpyAsync(query_owner, c->heQueryOwner.p, sizeof(uint32_t) * nqueries, hipMemcpyDeviceToHost,
+                               c->stream));
+    HIP_TRY(hipMemcpyAsync(first_repeat, c->heRepeat.p, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return SBG_OK;
+}
+
 int sbg_mesh_bvh(sbg_mesh m, void *nodes_out, uint32_t *root_ref, float *root_box6)
 {
     if (!m)

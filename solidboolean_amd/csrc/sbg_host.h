@@ -131,6 +131,8 @@ struct sbg_context_s {
     DevBuf<unsigned long long> ccKeys;      // sbg_components: half-edge table
     DevBuf<uint32_t> ccOwners, ccParent, ccLabels;
     DevBuf<uint8_t> ccBlocked;
+    DevBuf<uint32_t> heTris, heOpposite, heQueryOwner, heRepeat;  // sbg_half_edges
+    DevBuf<unsigned long long> heQueries;
     DevBuf<double> rawA, rawB, rawSeg;
     DevBuf<uint8_t> rawStatus;
     DevBuf<double> adjXyz, adjAll, adjGradD, adjGall, adjVol;  // sbg_adjoint

@@ -171,6 +171,9 @@ void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8
 size_t component_table_slots(uint32_t nt);
 void launch_components(const uint32_t *T, uint32_t nt, const uint8_t *blocked, unsigned long long *keys,
                        uint32_t *owners, uint32_t *parent, uint32_t *labels, hipStream_t s);
+void launch_half_edges(const uint32_t *T, uint32_t nt, const unsigned long long *queries, uint32_t nq,
+                       unsigned long long *keys, uint32_t *owners, uint32_t *opposite, uint32_t *queryOwner,
+                       uint32_t *firstRepeat, hipStream_t s);
 
 // k_adjoint.hip
 void launch_adjoint(const double *xyz, uint32_t nin, const uint32_t *src, uint32_t nd, const uint32_t *faces,

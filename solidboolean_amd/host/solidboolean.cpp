@@ -117,8 +117,16 @@ bool SolidBoolean::addUnintersectedTriangles(const SolidMesh *mesh, const std::v
     for (size_t i = 0; i < vertices.size(); ++i)
         m_vertexSource.push_back((int64_t)(inputOffset + i));
     const auto &tris = *mesh->triangles();
-    halfEdges->reserve(3 * tris.size());
     m_newTriangles.reserve(m_newTriangles.size() + tris.size());
+    if (!halfEdges) {
+        // the GPU builds the half-edge table later
+        for (size_t i = 0; i < tris.size(); ++i)
+            if (!used[i])
+                m_newTriangles.push_back(
+                    {tris[i][0] + oldVertexCount, tris[i][1] + oldVertexCount, tris[i][2] + oldVertexCount});
+        return true;
+    }
+    halfEdges->reserve(3 * tris.size());
     constexpr size_t kAhead = 16;  // slots prefetched this many triangles ahead
     for (size_t i = 0; i < tris.size(); ++i) {
         if (i + kAhead < tris.size() && !used[i + kAhead]) {
@@ -143,7 +151,7 @@ bool SolidBoolean::addUnintersectedTriangles(const SolidMesh *mesh, const std::v
 }
 
 bool SolidBoolean::reTriangulate(const ContextMap &contexts, const SolidMesh *mesh, size_t startOldVertex,
-                                 EdgeMap &edges, HalfEdgeMap &halfEdges)
+                                 EdgeMap &edges, HalfEdgeMap *halfEdges)
 {
     // (reference: src/solidboolean.cpp:352-407)
     const auto &tris = *mesh->triangles();
@@ -185,8 +193,10 @@ bool SolidBoolean::reTriangulate(const ContextMap &contexts, const SolidMesh *me
             const size_t index = m_newTriangles.size();
             const size_t a = newIndices[nt[0]], b = newIndices[nt[1]], c = newIndices[nt[2]];
             m_newTriangles.push_back({a, b, c});
+            if (!halfEdges)
+                continue;
             for (const uint64_t he : {halfEdgeKey(a, b), halfEdgeKey(b, c), halfEdgeKey(c, a)})
-                if (!halfEdges.insert(he, index)) {
+                if (!halfEdges->insert(he, index)) {
                     m_repeatedHalfEdge = true;
                     sbhost::diag("Found repeated halfedge");
                 }
@@ -243,9 +253,59 @@ bool SolidBoolean::buildPolygonsFromEdges(const EdgeMap &edges, std::vector<std:
     return true;
 }
 
+// Half-edge tables of one mesh's new triangles (its untouched ones, then its
+// re-triangulated ones) built on the GPU: per local triangle the local
+// triangle across each edge, the owners of the loop half-edges (forward, then
+// backward, per loop edge in loop order), and the first triangle at which a
+// directed edge repeats.  False when the indices do not fit the device table.
+bool SolidBoolean::gpuHalfEdges(const std::vector<std::vector<size_t>> &intersections, size_t remainingStart,
+                                size_t remainingCount, size_t retriStart, size_t retriEnd, GpuTables &out) const
+{
+    const size_t n = remainingCount + (retriEnd - retriStart);
+    std::vector<uint32_t> tris(3 * n);
+    size_t l = 0;
+    for (const auto &r : {std::make_pair(remainingStart, remainingStart + remainingCount),
+                          std::make_pair(retriStart, retriEnd)})
+        for (size_t t = r.first; t < r.second; ++t, ++l)
+            for (size_t k = 0; k < 3; ++k) {
+                if (m_newTriangles[t][k] >= 0xffffffffu)
+                    return false;
+                tris[3 * l + k] = (uint32_t)m_newTriangles[t][k];
+            }
+    std::vector<uint64_t> queries;
+    for (const auto &loop : intersections)
+        for (size_t i = 0; i < loop.size(); ++i) {
+            const size_t j = (i + 1) % loop.size();
+            if (loop[i] >= 0xffffffffu || loop[j] >= 0xffffffffu)
+                return false;
+            queries.push_back(halfEdgeKey(loop[i], loop[j]));
+            queries.push_back(halfEdgeKey(loop[j], loop[i]));
+        }
+    out.opposite.assign(3 * n, 0);
+    out.loopOwner.assign(queries.size(), 0);
+    sbhost::check(sbg_half_edges(sbhost::gpu(), tris.data(), n, queries.data(), queries.size(), out.opposite.data(),
+                                 out.loopOwner.data(), &out.firstRepeat),
+                  "sbg_half_edges");
+    // local -> global triangle indices
+    auto global = [&](uint32_t x) -> uint32_t {
+        if (x == ~0u)
+            return x;
+        return (uint32_t)(x < remainingCount ? remainingStart + x : retriStart + (x - remainingCount));
+    };
+    for (uint32_t &x : out.opposite)
+        x = global(x);
+    for (uint32_t &x : out.loopOwner)
+        x = global(x);
+    out.remainingStart = remainingStart;
+    out.remainingCount = remainingCount;
+    out.retriStart = retriStart;
+    return true;
+}
+
 void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &intersections,
-                                   const HalfEdgeMap &halfEdges, size_t remainingStart, size_t remainingCount,
-                                   size_t retriStart, size_t retriEnd, std::vector<std::vector<size_t>> &groups)
+                                   const HalfEdgeMap &halfEdges, const GpuTables *tables, size_t remainingStart,
+                                   size_t remainingCount, size_t retriStart, size_t retriEnd,
+                                   std::vector<std::vector<size_t>> &groups)
 {
     // (reference: src/solidboolean.cpp:167-239) seeds on both sides of every
     // loop edge, flood fill across half-edges that are not loop edges.  The
@@ -254,8 +314,17 @@ void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &inter
     // when another triangle has the same directed edge.  Unless that happened
     // (m_repeatedHalfEdge), the fill reads a per-half-edge array instead of
     // the maps: the triangle across the edge, or "loop edge", or nothing.
+    // the owner of loop half-edge number q (forward, backward per loop edge,
+    // in loop order), from the GPU tables or the host map
+    constexpr uint32_t kNone = ~0u, kLoop = ~1u;
+    auto loopOwner = [&](size_t q, uint64_t key) -> uint32_t {
+        if (tables)
+            return tables->loopOwner[q];
+        const size_t *t = halfEdges.find(key);
+        return t ? (uint32_t)*t : kNone;
+    };
     HalfEdgeMap groupOf;
-    size_t groupIndex = 0;
+    size_t groupIndex = 0, q = 0;
     std::vector<std::pair<size_t, size_t>> queue;  // FIFO: [head, end)
     size_t head = 0;
     for (const auto &loop : intersections) {
@@ -263,18 +332,19 @@ void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &inter
             const size_t j = (i + 1) % loop.size();
             const uint64_t fwd = halfEdgeKey(loop[i], loop[j]);
             groupOf.insert(fwd, groupIndex);
-            if (const size_t *t = halfEdges.find(fwd))
-                queue.push_back({*t, groupIndex});
+            const uint32_t tf = loopOwner(q++, fwd);
+            if (tf != kNone)
+                queue.push_back({tf, groupIndex});
             const uint64_t rev = halfEdgeKey(loop[j], loop[i]);
             groupOf.insert(rev, groupIndex + 1);
-            if (const size_t *t = halfEdges.find(rev))
-                queue.push_back({*t, groupIndex + 1});
+            const uint32_t tr = loopOwner(q++, rev);
+            if (tr != kNone)
+                queue.push_back({tr, groupIndex + 1});
         }
         groupIndex += 2;
     }
     groups.resize(groupIndex);
-    constexpr uint32_t kNone = ~0u, kLoop = ~1u;
-    const bool recordAll = m_repeatedHalfEdge || m_newTriangles.size() >= kLoop;
+    const bool recordAll = !tables && (m_repeatedHalfEdge || m_newTriangles.size() >= kLoop);
     std::vector<uint8_t> visited(m_newTriangles.size(), 0);
     // Every triangle the fill can reach lies in this mesh's two ranges (its
     // untouched triangles, then its retriangulated ones).  Without repeated
@@ -284,9 +354,16 @@ void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &inter
     if (!recordAll) {
         m_opposite.resize(3 * m_newTriangles.size());
         constexpr size_t kAhead = 16;
+        size_t l = 0;
         for (const auto &r : {std::make_pair(remainingStart, remainingStart + remainingCount),
                               std::make_pair(retriStart, retriEnd)})
             for (size_t t = r.first; t < r.second; ++t) {
+                if (tables) {
+                    for (size_t i = 0; i < 3; ++i)
+                        m_opposite[3 * t + i] = tables->opposite[3 * l + i];
+                    ++l;
+                    continue;
+                }
                 if (t + kAhead < r.second) {
                     const auto &f = m_newTriangles[t + kAhead];
                     halfEdges.prefetch(halfEdgeKey(f[1], f[0]));
@@ -302,16 +379,19 @@ void SolidBoolean::buildFaceGroups(const std::vector<std::vector<size_t>> &inter
         // A loop half-edge, when some triangle has it, belongs to exactly that
         // triangle, so the fill's test "is this half-edge a loop edge" is a
         // flag on the owner's edge.  Cleared edges point nowhere.
+        q = 0;
         for (const auto &loop : intersections)
             for (size_t i = 0; i < loop.size(); ++i) {
                 const size_t j = (i + 1) % loop.size();
-                for (const uint64_t he : {halfEdgeKey(loop[i], loop[j]), halfEdgeKey(loop[j], loop[i])})
-                    if (const size_t *o = halfEdges.find(he)) {
-                        const auto &tri = m_newTriangles[*o];
-                        for (size_t k = 0; k < 3; ++k)
-                            if (halfEdgeKey(tri[k], tri[(k + 1) % 3]) == he)
-                                m_opposite[3 * *o + k] = kLoop;
-                    }
+                for (const uint64_t he : {halfEdgeKey(loop[i], loop[j]), halfEdgeKey(loop[j], loop[i])}) {
+                    const uint32_t o = loopOwner(q++, he);
+                    if (o == kNone)
+                        continue;
+                    const auto &tri = m_newTriangles[o];
+                    for (size_t k = 0; k < 3; ++k)
+                        if (halfEdgeKey(tri[k], tri[(k + 1) % 3]) == he)
+                            m_opposite[3 * (size_t)o + k] = kLoop;
+                }
             }
     }
     auto drain = [&]() {
@@ -488,18 +568,43 @@ bool SolidBoolean::combine()
     m_hitSegments = seg;
     benchEnd_processPotentialIntersectedPairs = Clock::now();
 
+    // the half-edge tables are built on the GPU unless a directed edge
+    // repeats, in which case the host maps, with the reference's insertion
+    // order semantics, are built instead and the host stages rerun
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const int r = finishCombine(firstCtx, secondCtx, attempt == 0);
+        if (r >= 0)
+            return r == 1;
+        m_newVertices.clear();
+        m_vertexSource.clear();
+        m_newTriangles.clear();
+        m_newPositionMap.clear();
+        m_newKeyMap.clear();
+        m_repeatedHalfEdge = false;
+        m_firstTriangleGroups.clear();
+        m_secondTriangleGroups.clear();
+    }
+    return false;
+}
+
+// The host stages after the pairs: 1 / 0 as combine() returns, or -1 when the
+// GPU half-edge tables found a repeated directed edge (rerun with host maps).
+int SolidBoolean::finishCombine(const ContextMap &firstCtx, const ContextMap &secondCtx, bool gpuTables)
+{
     EdgeMap firstEdges, secondEdges;
     std::vector<std::vector<size_t>> firstIntersections;
     HalfEdgeMap firstHalfEdges, secondHalfEdges;
+    HalfEdgeMap *firstMap = gpuTables ? nullptr : &firstHalfEdges;
+    HalfEdgeMap *secondMap = gpuTables ? nullptr : &secondHalfEdges;
 
     benchBegin_addUnintersectedTriangles = Clock::now();
     m_firstRemainingStart = m_newTriangles.size();
-    if (!addUnintersectedTriangles(m_firstMesh, m_firstIntersected, &firstHalfEdges, 0))
+    if (!addUnintersectedTriangles(m_firstMesh, m_firstIntersected, firstMap, 0))
         sbhost::diag("Add first mesh remaining triangles failed");
     const size_t firstRemainingCount = m_newTriangles.size() - m_firstRemainingStart;
     m_firstRemainingCount = firstRemainingCount;
     m_secondRemainingStart = m_newTriangles.size();
-    if (!addUnintersectedTriangles(m_secondMesh, m_secondIntersected, &secondHalfEdges,
+    if (!addUnintersectedTriangles(m_secondMesh, m_secondIntersected, secondMap,
                                    m_firstMesh->vertices()->size()))
         sbhost::diag("Add second mesh remaining triangles failed");
     const size_t secondRemainingCount = m_newTriangles.size() - m_secondRemainingStart;
@@ -508,14 +613,14 @@ bool SolidBoolean::combine()
 
     benchBegin_reTriangulate = Clock::now();
     const size_t firstRetriStart = m_newTriangles.size();
-    if (!reTriangulate(firstCtx, m_firstMesh, 0, firstEdges, firstHalfEdges)) {
+    if (!reTriangulate(firstCtx, m_firstMesh, 0, firstEdges, firstMap)) {
         sbhost::diag("Retriangulate first mesh failed");
-        return false;
+        return 0;
     }
     const size_t secondRetriStart = m_newTriangles.size();
-    if (!reTriangulate(secondCtx, m_secondMesh, m_firstMesh->vertices()->size(), secondEdges, secondHalfEdges)) {
+    if (!reTriangulate(secondCtx, m_secondMesh, m_firstMesh->vertices()->size(), secondEdges, secondMap)) {
         sbhost::diag("Retriangulate second mesh failed");
-        return false;
+        return 0;
     }
     const size_t secondRetriEnd = m_newTriangles.size();
     benchEnd_reTriangulate = Clock::now();
@@ -523,24 +628,34 @@ bool SolidBoolean::combine()
     benchBegin_buildPolygonsFromEdges = Clock::now();
     if (!buildPolygonsFromEdges(firstEdges, firstIntersections)) {
         sbhost::diag("Build polygons from edges failed");
-        return false;
+        return 0;
     }
     benchEnd_buildPolygonsFromEdges = Clock::now();
 
     // both meshes are grouped by the first mesh's loops, as the reference does
     // (src/solidboolean.cpp:451-462)
     benchBegin_buildFaceGroups = Clock::now();
-    buildFaceGroups(firstIntersections, firstHalfEdges, m_firstRemainingStart, firstRemainingCount,
-                    firstRetriStart, secondRetriStart, m_firstTriangleGroups);
-    buildFaceGroups(firstIntersections, secondHalfEdges, m_secondRemainingStart, secondRemainingCount,
-                    secondRetriStart, secondRetriEnd, m_secondTriangleGroups);
+    GpuTables firstTables, secondTables;
+    if (gpuTables) {
+        if (!gpuHalfEdges(firstIntersections, m_firstRemainingStart, firstRemainingCount, firstRetriStart,
+                          secondRetriStart, firstTables) ||
+            !gpuHalfEdges(firstIntersections, m_secondRemainingStart, secondRemainingCount, secondRetriStart,
+                          secondRetriEnd, secondTables) ||
+            firstTables.firstRepeat != ~0u || secondTables.firstRepeat != ~0u)
+            return -1;
+    }
+    buildFaceGroups(firstIntersections, firstHalfEdges, gpuTables ? &firstTables : nullptr, m_firstRemainingStart,
+                    firstRemainingCount, firstRetriStart, secondRetriStart, m_firstTriangleGroups);
+    buildFaceGroups(firstIntersections, secondHalfEdges, gpuTables ? &secondTables : nullptr,
+                    m_secondRemainingStart, secondRemainingCount, secondRetriStart, secondRetriEnd,
+                    m_secondTriangleGroups);
     benchEnd_buildFaceGroups = Clock::now();
 
     benchBegin_decideGroupSide = Clock::now();
     decideGroupSide(m_firstTriangleGroups, m_secondMesh, m_firstGroupSides);
     decideGroupSide(m_secondTriangleGroups, m_firstMesh, m_secondGroupSides);
     benchEnd_decideGroupSide = Clock::now();
-    return true;
+    return 1;
 }
 
 void SolidBoolean::fetchUnion(std::vector<std::vector<size_t>> &out)

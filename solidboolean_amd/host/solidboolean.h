@@ -143,11 +143,20 @@ private:
     bool addUnintersectedTriangles(const SolidMesh *mesh, const std::vector<uint8_t> &used, HalfEdgeMap *halfEdges,
                                    size_t inputOffset);
     bool reTriangulate(const ContextMap &contexts, const SolidMesh *mesh, size_t startOldVertex, EdgeMap &edges,
-                       HalfEdgeMap &halfEdges);
+                       HalfEdgeMap *halfEdges);
+    struct GpuTables {
+        std::vector<uint32_t> opposite;   // per half-edge of the mesh's triangles, local order
+        std::vector<uint32_t> loopOwner;  // per loop half-edge
+        uint32_t firstRepeat = ~0u;
+        size_t remainingStart = 0, remainingCount = 0, retriStart = 0;
+    };
+    bool gpuHalfEdges(const std::vector<std::vector<size_t>> &intersections, size_t remainingStart,
+                      size_t remainingCount, size_t retriStart, size_t retriEnd, GpuTables &out) const;
+    int finishCombine(const ContextMap &firstCtx, const ContextMap &secondCtx, bool gpuTables);
     bool buildPolygonsFromEdges(const EdgeMap &edges, std::vector<std::vector<size_t>> &polygons);
     void buildFaceGroups(const std::vector<std::vector<size_t>> &intersections, const HalfEdgeMap &halfEdges,
-                         size_t remainingStart, size_t remainingCount, size_t retriStart, size_t retriEnd,
-                         std::vector<std::vector<size_t>> &groups);
+                         const GpuTables *tables, size_t remainingStart, size_t remainingCount, size_t retriStart,
+                         size_t retriEnd, std::vector<std::vector<size_t>> &groups);
     std::vector<uint32_t> m_opposite;  // buildFaceGroups: per half-edge (3 t + i), the triangle across it
     void decideGroupSide(const std::vector<std::vector<size_t>> &groups, const SolidMesh *mesh,
                          std::vector<bool> &groupSides);

@@ -33,6 +33,7 @@ EXPORTS = [
     "sbg_mesh_normals", "sbg_mesh_boxes", "sbg_mesh_bvh", "sbg_find_pairs", "sbg_pairs_create", "sbg_pairs_count",
     "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
     "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_classify_pair", "sbg_components",
+    "sbg_half_edges",
     "sbg_context_set_timing",
     "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_bench_rays", "sbg_calibrate_copy",
     "sbg_context_set_tuning", "sbg_context_get_tuning", "sbg_autotune_pair",
@@ -122,6 +123,7 @@ def load(path: str = LIB_PATH):
         "sbg_classify_triangles": [_P, _P, _P, C.c_int, _U8],
         "sbg_classify_pair": [_P, _P, _P, _U8, _U8],
         "sbg_components": [_P, _P, _U32],
+        "sbg_half_edges": [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, C.POINTER(C.c_uint32)],
         "sbg_context_set_timing": [_P, C.c_int],
         "sbg_context_stage_ms": [_P, _D, C.c_int],
         "sbg_bench_tri_tri": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)],
@@ -207,6 +209,19 @@ class Context:
         t = Tuning()
         _check(self.L.sbg_context_get_tuning(self.h, C.byref(t)))
         return t.as_dict()
+
+    def half_edges(self, triangles, queries=None):
+        """sbg_half_edges: (opposite (n, 3) uint32, query owners, first repeat)
+        of a triangle list; 0xffffffff for none."""
+        t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+        q = np.ascontiguousarray(np.zeros(0) if queries is None else queries, np.uint64).reshape(-1)
+        opp = np.zeros((max(len(t), 1), 3), np.uint32)
+        qo = np.zeros(max(len(q), 1), np.uint32)
+        first = C.c_uint32(0)
+        _check(self.L.sbg_half_edges(self.h, t.ctypes.data_as(C.c_void_p) if len(t) else None, len(t),
+                                     q.ctypes.data_as(C.c_void_p) if len(q) else None, len(q),
+                                     opp.ctypes.data_as(C.c_void_p), qo.ctypes.data_as(C.c_void_p), C.byref(first)))
+        return opp[: len(t)], qo[: len(q)], first.value
 
     def set_tuning(self, **kw):
         t = Tuning()

@@ -164,3 +164,19 @@ def test_robust_weld_succeeds_where_reference_fails(which):
     assert vi > 0 and vu > max(va, vb)
     assert abs(vu + vi - (va + vb)) < 1e-9 * (va + vb)
     assert abs(vd - (va - vi)) < 1e-9 * va
+
+
+def test_repeated_half_edge_falls_back_to_host_maps():
+    """A directed edge that repeats (a copy of one of A's triangles appended):
+    the GPU half-edge tables find it, and the host maps with the reference's
+    insertion-order rules give the compiled reference's face counts."""
+    g = load_golden("repeat_s1")
+    a, b = mg.small_pair(1)
+    for dup in g["dups"]:
+        ta = np.concatenate([a.triangles, a.triangles[dup:dup + 1]])
+        ok, sb, keep = run(a.vertices, ta, b.vertices, b.triangles)
+        assert ok == bool(g[f"dup{dup}_ok"]), dup
+        if ok:
+            counts = [len(sb.fetch_union()), len(sb.fetch_diff()), len(sb.fetch_intersect())]
+            assert counts == list(g[f"dup{dup}_face_counts"]), dup
+            assert len(sb.result_vertices()) == int(g[f"dup{dup}_n_vertices"]), dup

@@ -468,3 +468,37 @@ def test_components_vs_oracle(gpu):
     big = mg.icosphere(7)
     G = gpu.mesh(big.vertices, big.triangles, prepare=False)
     assert np.all(G.components() == 0)
+
+
+def test_half_edges_vs_dict(gpu):
+    """sbg_half_edges against a dictionary half-edge map built in index order
+    (the reference's unordered_map keyed (from << 32) | to, first insert
+    kept): opposite triangles, loop-edge owners, and the first triangle whose
+    insertion meets a repeated directed edge."""
+    def expect(T, queries):
+        owner, first = {}, 0xFFFFFFFF
+        for t, tri in enumerate(T):
+            for k in range(3):
+                key = (int(tri[k]) << 32) | int(tri[(k + 1) % 3])
+                if key in owner:
+                    first = min(first, t)
+                else:
+                    owner[key] = t
+        opp = np.array([[owner.get((int(tri[(k + 1) % 3]) << 32) | int(tri[k]), 0xFFFFFFFF) for k in range(3)]
+                        for tri in T], np.uint32).reshape(-1, 3)
+        qo = np.array([owner.get(int(q), 0xFFFFFFFF) for q in queries], np.uint32)
+        return opp, qo, first
+
+    rng = np.random.default_rng(4)
+    m = mg.icosphere(3)
+    shells = mg.Mesh(np.concatenate([m.vertices, m.vertices + 3]), np.concatenate([m.triangles, m.triangles + m.nv]))
+    T = shells.triangles.astype(np.uint32)
+    q = np.array([(int(a) << 32) | int(b) for a, b in rng.integers(0, shells.nv, (50, 2))]
+                 + [(int(T[i, 0]) << 32) | int(T[i, 1]) for i in range(0, len(T), 37)], np.uint64)
+    cases = [(T, q), (T[rng.permutation(len(T))][:1000], q), (np.concatenate([T, T[5:6]]), q),
+             (np.concatenate([T[:10], T[3:4][:, ::-1], T[10:]]), q), (np.array([[1, 1, 2], [3, 4, 5]], np.uint32), q),
+             (np.zeros((0, 3), np.uint32), q)]
+    for tris, queries in cases:
+        opp, qo, first = gpu.half_edges(tris, queries)
+        eo, eq, ef = expect(tris, queries)
+        assert np.array_equal(opp, eo) and np.array_equal(qo, eq) and first == ef
