@@ -27,7 +27,8 @@ struct HostMesh {
 struct HostBoolean {
     SolidBoolean sb;
     bool ok = false;
-    std::vector<std::vector<size_t>> fetched[3];
+    std::vector<std::vector<size_t>> fetched[3];  // filled by the first sbh_boolean_fetch of each
+    bool haveFetched[3] = {false, false, false};
     HostBoolean(const SolidMesh *a, const SolidMesh *b) : sb(a, b) {}
 };
 
@@ -75,11 +76,6 @@ int sbh_boolean_new_opts(void *a, void *b, unsigned flags, void **out)
     h->sb.setRobustWeld((flags & 2u) != 0);
     try {
         h->ok = h->sb.combine();
-        if (h->ok) {
-            h->sb.fetchUnion(h->fetched[0]);
-            h->sb.fetchDiff(h->fetched[1]);
-            h->sb.fetchIntersect(h->fetched[2]);
-        }
         return h->ok ? 1 : 0;
     } catch (const std::exception &e) {
         g_hostErr = e.what();
@@ -102,7 +98,20 @@ size_t sbh_boolean_vertices(void *p, double *out, size_t cap)
 
 size_t sbh_boolean_fetch(void *p, int which, int64_t *out, size_t cap)
 {
-    const auto &f = ((HostBoolean *)p)->fetched[which];
+    HostBoolean *h = (HostBoolean *)p;
+    if (which < 0 || which > 2)
+        return 0;
+    if (h->ok && !h->haveFetched[which]) {
+        // fetch* appends, as the reference's do
+        if (which == 0)
+            h->sb.fetchUnion(h->fetched[0]);
+        else if (which == 1)
+            h->sb.fetchDiff(h->fetched[1]);
+        else
+            h->sb.fetchIntersect(h->fetched[2]);
+        h->haveFetched[which] = true;
+    }
+    const auto &f = h->fetched[which];
     for (size_t i = 0; i < f.size() && i < cap; ++i)
         for (int k = 0; k < 3; ++k)
             out[3 * i + k] = (int64_t)f[i][k];
