@@ -73,6 +73,7 @@ struct sbg_batch_s {
     hipGraphExec_t exec = nullptr;
     int graphPrecision = -1;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;  // the ray grid's branch on the second stream
     double lastMs = 0.0;
     bool ran = false;
     uint32_t nCand = 0;
@@ -169,6 +170,27 @@ int enqueue(sbg_batch b, int precision)
     launch_mesh_roots(b->nodes.p, b->codes.p, b->ids.p, nt, b->mortonBits, b->box32.p, b->rootBox.p, b->meshRoot.p,
                       b->meshBox.p, s);
     TRY(check_launch("batch tree"));
+    // The ray grids depend on the meshes alone: they are built on a branch
+    // (the context's second stream, default priority) beside the search and
+    // the GD, and joined before the rays.  Captured, the branch becomes
+    // parallel nodes of the graph.
+    GridSet gs = {};
+    gs.meshOf = b->meshOf.p;
+    gs.target = b->target.p;
+    gs.desc = b->gridDesc.p;
+    gs.boxes = b->meshBox.p;
+    gs.cells = b->gridCells.p;
+    gs.gcode = b->gridCode.p;
+    gs.gtri = b->gridTri.p;
+    gs.cap = (uint32_t)b->gridCap;
+    gs.ncells = b->ncells;
+    gs.order = b->ids.p;
+    const hipStream_t aux = b->ctx->aux;
+    HIP_TRY(hipEventRecord(b->fork, s));
+    HIP_TRY(hipStreamWaitEvent(aux, b->fork, 0));
+    launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, aux);
+    HIP_TRY(hipEventRecord(b->join, aux));
+    TRY(check_launch("batch grid"));
     // candidate search: every pair's root pair, breadth-first levels, then the
     // persistent depth-first kernel over what is left
     const TraverseArgs t = {b->nodes.p, b->box32.p, b->tri64.p, b->nodes.p, b->box32.p, b->tri64.p};
@@ -195,18 +217,7 @@ int enqueue(sbg_batch b, int precision)
     launch_fill_u32((uint32_t *)b->flags.p, 0u, (nt + 3) / 4, s);
     launch_mark_both(sorted, b->status.p, candCount, ccap, b->flags.p, s);
     launch_centroids_ordered(b->tri64.p, b->flags.p, b->ids.p, b->target.p, nt, b->pts.p, s);
-    GridSet gs = {};
-    gs.meshOf = b->meshOf.p;
-    gs.target = b->target.p;
-    gs.desc = b->gridDesc.p;
-    gs.boxes = b->meshBox.p;
-    gs.cells = b->gridCells.p;
-    gs.gcode = b->gridCode.p;
-    gs.gtri = b->gridTri.p;
-    gs.cap = (uint32_t)b->gridCap;
-    gs.ncells = b->ncells;
-    gs.order = b->ids.p;
-    launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, s);
+    HIP_TRY(hipStreamWaitEvent(s, b->join, 0));
     launch_rays_ordered(gs, b->tri64.p, b->pts.p, b->ids.p, nt, false, b->bits.p, b->rayList.p, w + kRayFlagW, s);
     launch_vote(b->bits.p, b->flags.p, nt, b->votes.p, s);
     launch_copy_u32(b->gridCells.p + b->ncells, w + kGridW, 1, s);
@@ -293,7 +304,9 @@ int sbg_batch_create(sbg_context ctx, const double *xyz, const uint32_t *tris, c
         b->host = nullptr;
         return bail(fail(SBG_ERR_OOM, "hipHostMalloc failed"));
     }
-    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+    if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
         return bail(fail(SBG_ERR_HIP, "hipEventCreate failed"));
     if (hipMemcpy(b->V.p, xyz, sizeof(double) * 3 * nv, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->T.p, T.data(), sizeof(uint32_t) * 3 * nt, hipMemcpyHostToDevice) != hipSuccess ||
@@ -586,6 +599,10 @@ int sbg_batch_destroy(sbg_batch b)
         (void)hipEventDestroy(b->ev0);
     if (b->ev1)
         (void)hipEventDestroy(b->ev1);
+    if (b->fork)
+        (void)hipEventDestroy(b->fork);
+    if (b->join)
+        (void)hipEventDestroy(b->join);
     if (b->host)
         (void)hipHostFree(b->host);
     DevBuf<double> *d64[] = {&b->V, &b->tri64, &b->seg, &b->pts};
