@@ -466,6 +466,32 @@ bool SolidBoolean::combine()
 {
     if (!m_firstMesh->gpuMesh() || !m_secondMesh->gpuMesh())
         throw std::logic_error("SolidBoolean::combine: both meshes must be prepared");
+    if (combineOnce())
+        return true;
+    if (!m_robustWeld)
+        return false;
+    // the robust weld failed (exact touches it could not key): the default
+    // weld, from the start
+    sbhost::diag("robust weld failed; combining with the PositionKey weld");
+    m_robustWeld = false;
+    m_newVertices.clear();
+    m_vertexSource.clear();
+    m_newTriangles.clear();
+    m_newPositionMap.clear();
+    m_newKeyMap.clear();
+    m_endpointKeys.clear();
+    m_repeatedHalfEdge = false;
+    m_firstTriangleGroups.clear();
+    m_secondTriangleGroups.clear();
+    m_firstGroupSides.clear();
+    m_secondGroupSides.clear();
+    const bool ok = combineOnce();
+    m_robustWeld = true;
+    return ok;
+}
+
+bool SolidBoolean::combineOnce()
+{
     const size_t na = m_firstMesh->triangles()->size();
     const size_t nb = m_secondMesh->triangles()->size();
 

@@ -76,7 +76,9 @@ public:
     // distinct, and points on an edge are never rejected, which is where the
     // reference's combine() fails on dense meshes ("Attach point to triangle
     // edge failed").  An endpoint that matches no edge-plane crossing (a
-    // degenerate touch) falls back to its PositionKey cell.
+    // degenerate touch) falls back to its PositionKey cell, and when the
+    // robust weld still fails, combine() runs again with the PositionKey
+    // weld, so it succeeds wherever the default does.
     void setRobustWeld(bool on) { m_robustWeld = on; }
 
     // open-addressed map from a half-edge key (first << 32 | second) to a triangle
@@ -153,6 +155,7 @@ private:
     bool gpuHalfEdges(const std::vector<std::vector<size_t>> &intersections, size_t remainingStart,
                       size_t remainingCount, size_t retriStart, size_t retriEnd, GpuTables &out) const;
     int finishCombine(const ContextMap &firstCtx, const ContextMap &secondCtx, bool gpuTables);
+    bool combineOnce();
     bool buildPolygonsFromEdges(const EdgeMap &edges, std::vector<std::vector<size_t>> &polygons);
     void buildFaceGroups(const std::vector<std::vector<size_t>> &intersections, const HalfEdgeMap &halfEdges,
                          const GpuTables *tables, size_t remainingStart, size_t remainingCount, size_t retriStart,

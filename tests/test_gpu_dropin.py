@@ -180,3 +180,22 @@ def test_repeated_half_edge_falls_back_to_host_maps():
             counts = [len(sb.fetch_union()), len(sb.fetch_diff()), len(sb.fetch_intersect())]
             assert counts == list(g[f"dup{dup}_face_counts"]), dup
             assert len(sb.result_vertices()) == int(g[f"dup{dup}_n_vertices"]), dup
+
+
+def test_robust_weld_falls_back_to_default_weld():
+    """On the near-coplanar cube pair (exact ties) the robust weld cannot key
+    every point and fails; combine() then reruns with the PositionKey weld and
+    returns exactly what the default does."""
+    g = load_golden("cube_n8")
+    res = []
+    for robust in (False, True):
+        a = dropin.SolidMesh(g["va"], g["ta"])
+        b = dropin.SolidMesh(g["vb"], g["tb"])
+        a.prepare()
+        b.prepare()
+        sb = dropin.SolidBoolean(a, b, robust_weld=robust)
+        assert sb.combine()
+        res.append(([len(sb.fetch_union()), len(sb.fetch_diff()), len(sb.fetch_intersect())],
+                    sorted_rows(sb.result_vertices())))
+    assert res[0][0] == res[1][0]
+    assert np.array_equal(res[0][1], res[1][1])
