@@ -91,7 +91,7 @@ SolidBoolean::~SolidBoolean() = default;
 
 size_t SolidBoolean::addNewPoint(const Vector3 &position, int64_t origin)
 {
-    if (m_robustWeld && origin >= 0 && (size_t)origin < m_endpointKeys.size()) {
+    if (m_weld == Weld::Crossings && origin >= 0 && (size_t)origin < m_endpointKeys.size()) {
         auto r = m_newKeyMap.insert({m_endpointKeys[(size_t)origin], m_newVertices.size()});
         if (r.second) {
             m_newVertices.push_back(position);
@@ -162,17 +162,30 @@ bool SolidBoolean::reTriangulate(const ContextMap &contexts, const SolidMesh *me
         ReTriangulator rt({verts[t[0]], verts[t[1]], verts[t[2]]}, (*normals)[kv.first]);
         rt.setEdges(kv.second.points, &kv.second.neighborMap);
         std::vector<int> hints;
-        if (m_robustWeld) {
+        if (m_weld != Weld::Cells) {
             // a cut point made on one of this triangle's own edges lies on it
+            // (with hinted cells: any crossing welded into the point; the first
+            // edge of the triangle that one lies on)
             hints.assign(kv.second.points.size(), -1);
-            for (size_t k = 0; k < hints.size(); ++k) {
-                const int64_t o = kv.second.origin[k];
+            auto edgeOf = [&](int64_t o) {
                 if (o < 0 || (size_t)o >= m_endpointKeys.size() || m_endpointKeys[(size_t)o].edge == ~0ull)
-                    continue;
+                    return -1;
                 const uint64_t e = m_endpointKeys[(size_t)o].edge;
                 for (int i = 0; i < 3; ++i) {
                     const uint64_t x = startOldVertex + t[i], y = startOldVertex + t[(i + 1) % 3];
                     if (e == (std::min(x, y) << 32 | std::max(x, y)))
+                        return i;
+                }
+                return -1;
+            };
+            for (size_t k = 0; k < hints.size(); ++k) {
+                if (m_weld == Weld::Crossings) {
+                    hints[k] = edgeOf(kv.second.origin[k]);
+                    continue;
+                }
+                for (int64_t o : kv.second.merged[k]) {
+                    const int i = edgeOf(o);
+                    if (i >= 0 && (hints[k] < 0 || i < hints[k]))
                         hints[k] = i;
                 }
             }
@@ -466,28 +479,30 @@ bool SolidBoolean::combine()
 {
     if (!m_firstMesh->gpuMesh() || !m_secondMesh->gpuMesh())
         throw std::logic_error("SolidBoolean::combine: both meshes must be prepared");
-    if (combineOnce())
-        return true;
-    if (!m_robustWeld)
-        return false;
-    // the robust weld failed (exact touches it could not key): the default
-    // weld, from the start
-    sbhost::diag("robust weld failed; combining with the PositionKey weld");
-    m_robustWeld = false;
-    m_newVertices.clear();
-    m_vertexSource.clear();
-    m_newTriangles.clear();
-    m_newPositionMap.clear();
-    m_newKeyMap.clear();
-    m_endpointKeys.clear();
-    m_repeatedHalfEdge = false;
-    m_firstTriangleGroups.clear();
-    m_secondTriangleGroups.clear();
-    m_firstGroupSides.clear();
-    m_secondGroupSides.clear();
-    const bool ok = combineOnce();
-    m_robustWeld = true;
-    return ok;
+    if (!m_robustWeld) {
+        m_weld = Weld::Cells;
+        return combineOnce();
+    }
+    // robust: crossings; if they cannot close the loops, PositionKey cells
+    // with edge hints; then plain PositionKey cells, each from the start
+    for (Weld w : {Weld::Crossings, Weld::HintedCells, Weld::Cells}) {
+        m_weld = w;
+        m_newVertices.clear();
+        m_vertexSource.clear();
+        m_newTriangles.clear();
+        m_newPositionMap.clear();
+        m_newKeyMap.clear();
+        m_endpointKeys.clear();
+        m_repeatedHalfEdge = false;
+        m_firstTriangleGroups.clear();
+        m_secondTriangleGroups.clear();
+        m_firstGroupSides.clear();
+        m_secondGroupSides.clear();
+        if (combineOnce())
+            return true;
+        sbhost::diag("robust weld: attempt failed, trying the next weld");
+    }
+    return false;
 }
 
 bool SolidBoolean::combineOnce()
@@ -533,7 +548,7 @@ bool SolidBoolean::combineOnce()
     }
     m_firstIntersected.assign(na, 0);
     m_secondIntersected.assign(nb, 0);
-    if (m_robustWeld) {
+    if (m_weld != Weld::Cells) {
         // every endpoint's crossing: an edge of one mesh with a triangle of the
         // other; endpoints that match none are keyed by their PositionKey cell
         const size_t nh = ab.size() / 2;
@@ -557,7 +572,7 @@ bool SolidBoolean::combineOnce()
     }
     ContextMap firstCtx, secondCtx;
     auto addPoint = [this](IntersectedContext &c, const Vector3 &p, int64_t origin) {
-        if (m_robustWeld) {
+        if (m_weld == Weld::Crossings) {
             auto r = c.keyMap.insert({m_endpointKeys[(size_t)origin], c.points.size()});
             if (r.second) {
                 c.points.push_back(p);
@@ -569,7 +584,11 @@ bool SolidBoolean::combineOnce()
         if (r.second) {
             c.points.push_back(p);
             c.origin.push_back(origin);
+            if (m_weld == Weld::HintedCells)
+                c.merged.emplace_back();
         }
+        if (m_weld == Weld::HintedCells)
+            c.merged[r.first->second].push_back(origin);
         return r.first->second;
     };
     auto addSegment = [&](IntersectedContext &c, const Vector3 &s, const Vector3 &t, size_t hit) {

@@ -76,9 +76,11 @@ public:
     // distinct, and points on an edge are never rejected, which is where the
     // reference's combine() fails on dense meshes ("Attach point to triangle
     // edge failed").  An endpoint that matches no edge-plane crossing (a
-    // degenerate touch) falls back to its PositionKey cell, and when the
-    // robust weld still fails, combine() runs again with the PositionKey
-    // weld, so it succeeds wherever the default does.
+    // degenerate touch) falls back to its PositionKey cell.  When that fails,
+    // combine() runs again welding by PositionKey cell as the reference does
+    // but still hinting each cut point's edge (from every crossing welded into
+    // it), and then with the plain PositionKey weld, so it succeeds wherever
+    // the default does.
     void setRobustWeld(bool on) { m_robustWeld = on; }
 
     // open-addressed map from a half-edge key (first << 32 | second) to a triangle
@@ -113,6 +115,7 @@ private:
         std::vector<int64_t> origin;  // per point: 2 * hit + endpoint of its first insertion
         std::map<PositionKey, size_t> positionMap;
         std::map<WeldKey, size_t> keyMap;  // robust weld
+        std::vector<std::vector<int64_t>> merged;  // hinted cells: every endpoint welded into each point
         std::map<size_t, std::set<size_t>> neighborMap;
     };
     using ContextMap = std::map<size_t, IntersectedContext>;
@@ -125,6 +128,10 @@ private:
     bool m_repeatedHalfEdge = false;  // some directed edge belongs to two new triangles
     bool m_referencePairOrder = false;
     bool m_robustWeld = false;
+    // the weld of the current attempt: PositionKey cells; crossings (keys);
+    // or PositionKey cells with the edge hints of every endpoint in the cell
+    enum class Weld { Cells, Crossings, HintedCells };
+    Weld m_weld = Weld::Cells;
     std::vector<WeldKey> m_endpointKeys;
     std::map<WeldKey, size_t> m_newKeyMap;
     bool endpointSource(const uint32_t *ab, const double *seg, size_t h, size_t e, uint32_t d[5]) const;
