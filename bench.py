@@ -319,6 +319,21 @@ def stress_section(ctx, mg, N, steps: int) -> dict:
                      "filtered_us_per_launch": f_us, "fp64_us_per_launch": d_us, "end_to_end_ms": ms}
         log(f"stress {name}: {st['tested']} pairs, fallback {st['fallback_pairs']}, {ms:.2f} ms/step")
         del P, A, B
+        # the whole boolean through the drop-in, default weld (as the
+        # reference) and the robust weld chain
+        from solidboolean_amd import dropin
+
+        for robust in (False, True):
+            t0 = time.perf_counter()
+            DA, DB = dropin.SolidMesh(a.vertices, a.triangles), dropin.SolidMesh(b.vertices, b.triangles)
+            DA.prepare()
+            DB.prepare()
+            ok = dropin.SolidBoolean(DA, DB, robust_weld=robust).combine()
+            key = "dropin_combine_robust" if robust else "dropin_combine_default"
+            out[name][key] = {"ok": ok, "ms": (time.perf_counter() - t0) * 1e3}
+            del DA, DB
+        log(f"stress {name}: drop-in default {out[name]['dropin_combine_default']}, "
+            f"robust {out[name]['dropin_combine_robust']}")
     return out
 
 
