@@ -21,6 +21,12 @@ void check(int rc, const char *what)
         throw std::runtime_error(std::string(what) + ": " + sbg_last_error());
 }
 
+std::recursive_mutex &gpuMutex()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+
 sbg_context gpu()
 {
     static std::once_flag once;

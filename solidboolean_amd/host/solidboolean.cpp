@@ -296,6 +296,7 @@ bool SolidBoolean::gpuHalfEdges(const std::vector<std::vector<size_t>> &intersec
         }
     out.opposite.assign(3 * n, 0);
     out.loopOwner.assign(queries.size(), 0);
+    sbhost::GpuLock lock(sbhost::gpuMutex());
     sbhost::check(sbg_half_edges(sbhost::gpu(), tris.data(), n, queries.data(), queries.size(), out.opposite.data(),
                                  out.loopOwner.data(), &out.firstRepeat),
                   "sbg_half_edges");
@@ -469,8 +470,11 @@ void SolidBoolean::decideGroupSide(const std::vector<std::vector<size_t>> &group
     if (which.empty())
         return;
     std::vector<uint8_t> inside(which.size());
-    sbhost::check(sbg_classify_points(mesh->gpuMesh(), pts.data(), which.size(), nullptr, inside.data()),
-                  "sbg_classify_points");
+    {
+        sbhost::GpuLock lock(sbhost::gpuMutex());
+        sbhost::check(sbg_classify_points(mesh->gpuMesh(), pts.data(), which.size(), nullptr, inside.data()),
+                      "sbg_classify_points");
+    }
     for (size_t k = 0; k < which.size(); ++k)
         groupSides[which[k]] = inside[k] != 0;
 }
@@ -511,6 +515,7 @@ bool SolidBoolean::combineOnce()
     const size_t nb = m_secondMesh->triangles()->size();
 
     benchBegin_searchPotentialIntersectedPairs = Clock::now();
+    std::unique_lock<std::recursive_mutex> gpuLock(sbhost::gpuMutex());  // until the hits are on the host
     sbg_pairs pairs = nullptr;
     sbhost::check(sbg_find_pairs(m_firstMesh->gpuMesh(), m_secondMesh->gpuMesh(), &pairs), "sbg_find_pairs");
     benchEnd_searchPotentialIntersectedPairs = Clock::now();
@@ -532,6 +537,7 @@ bool SolidBoolean::combineOnce()
         sbg_pairs_destroy(pairs);
         sbhost::check(rc, "sbg_tri_tri");
     }
+    gpuLock.unlock();
     if (m_referencePairOrder && ab.size() > 2) {
         const sbhost::ReferenceTree first(*m_firstMesh->vertices(), *m_firstMesh->triangles());
         const sbhost::ReferenceTree second(*m_secondMesh->vertices(), *m_secondMesh->triangles());

@@ -15,8 +15,10 @@ void SolidMesh::release()
     m_boxes = nullptr;
     delete m_tree;
     m_tree = nullptr;
-    if (m_gpu)
+    if (m_gpu) {
+        sbhost::GpuLock lock(sbhost::gpuMutex());
         sbg_mesh_destroy(m_gpu);
+    }
     m_gpu = nullptr;
 }
 
@@ -46,6 +48,7 @@ void SolidMesh::prepare()
             tri[3 * i + k] = (uint32_t)t[k];
         }
     }
+    sbhost::GpuLock lock(sbhost::gpuMutex());
     sbhost::check(sbg_mesh_create(sbhost::gpu(), xyz.data(), nv, tri.data(), nt, &m_gpu), "sbg_mesh_create");
     sbhost::check(sbg_mesh_prepare(m_gpu), "sbg_mesh_prepare");
 }
@@ -57,7 +60,10 @@ const std::vector<Vector3> *SolidMesh::triangleNormals() const
     if (!m_triangleNormals) {
         const size_t nt = m_triangles->size();
         std::vector<double> n(3 * nt);
-        sbhost::check(sbg_mesh_normals(m_gpu, n.data()), "sbg_mesh_normals");
+        {
+            sbhost::GpuLock lock(sbhost::gpuMutex());
+            sbhost::check(sbg_mesh_normals(m_gpu, n.data()), "sbg_mesh_normals");
+        }
         auto *v = new std::vector<Vector3>(nt);
         for (size_t i = 0; i < nt; ++i)
             (*v)[i] = Vector3(n[3 * i], n[3 * i + 1], n[3 * i + 2]);
@@ -96,14 +102,18 @@ const AxisAlignedBoudingBoxTree *SolidMesh::axisAlignedBoundingBoxTree() const
 void AxisAlignedBoudingBoxTree::test(const AxisAlignedBoudingBoxTree *other,
                                      std::vector<std::pair<size_t, size_t>> *pairs) const
 {
-    sbg_pairs p = nullptr;
-    sbhost::check(sbg_find_pairs(m_mesh->gpuMesh(), other->m_mesh->gpuMesh(), &p), "sbg_find_pairs");
     size_t n = 0;
-    sbg_pairs_count(p, &n);
-    std::vector<uint32_t> ab(2 * n);
-    int rc = sbg_pairs_copy(p, ab.data());
-    sbg_pairs_destroy(p);
-    sbhost::check(rc, "sbg_pairs_copy");
+    std::vector<uint32_t> ab;
+    {
+        sbhost::GpuLock lock(sbhost::gpuMutex());
+        sbg_pairs p = nullptr;
+        sbhost::check(sbg_find_pairs(m_mesh->gpuMesh(), other->m_mesh->gpuMesh(), &p), "sbg_find_pairs");
+        sbg_pairs_count(p, &n);
+        ab.resize(2 * n);
+        int rc = sbg_pairs_copy(p, ab.data());
+        sbg_pairs_destroy(p);
+        sbhost::check(rc, "sbg_pairs_copy");
+    }
     pairs->reserve(pairs->size() + n);
     for (size_t i = 0; i < n; ++i)
         pairs->push_back({ab[2 * i], ab[2 * i + 1]});

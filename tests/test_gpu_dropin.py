@@ -218,3 +218,36 @@ def test_robust_weld_torus_stress_pair():
     assert vi > 0 and vu > max(va, vb)
     assert abs(vu + vi - (va + vb)) < 1e-7 * (va + vb)
     assert abs(vd - (va - vi)) < 1e-7 * va
+
+
+def test_threads_share_the_gpu():
+    """Distinct SolidBoolean instances on different host threads (the
+    reference allows it): each combine() gives what it gives alone."""
+    import threading
+
+    names = ["small_s1", "small_s2", "small_s3", "case_complex"]
+    expect = {}
+    for n in names:
+        g = load_golden(n)
+        ok, sb, keep = run(g["va"], g["ta"], g["vb"], g["tb"])
+        expect[n] = (ok, [len(sb.fetch_union()), len(sb.fetch_diff()), len(sb.fetch_intersect())])
+    got, errors = {}, []
+
+    def work(n, k):
+        try:
+            g = load_golden(n)
+            for r in range(3):
+                ok, sb, keep = run(g["va"], g["ta"], g["vb"], g["tb"])
+                got[(n, k, r)] = (ok, [len(sb.fetch_union()), len(sb.fetch_diff()), len(sb.fetch_intersect())])
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=work, args=(n, k)) for k in range(2) for n in names]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(120)
+    assert not errors, errors
+    assert len(got) == 2 * 3 * len(names)
+    for (n, k, r), v in got.items():
+        assert v == expect[n], (n, k, r)
