@@ -1371,14 +1371,17 @@ int sbg_classify_pair(sbg_mesh a, sbg_mesh b, sbg_pairs p, uint8_t *out_a, uint8
     const RayBufs ra = ray_bufs(c, 0), rb = ray_bufs(c, 1);
     HIP_TRY(hipEventRecord(c->auxReady, c->stream));
     HIP_TRY(hipStreamWaitEvent(c->aux, c->auxReady, 0));
-    if (a->nt) {
+    // both sides' queries first, then both sides' rays: the second stream
+    // starts its work after a few launches instead of after the first side's
+    // whole sequence
+    if (a->nt)
         TRY(enqueue_queries(a, p, 0, ra));
-        TRY(enqueue_rays(b, ra, ra.pts.p, a->sortedIds, ra.flags.p, a->nt, true, false, true));
-    }
-    if (b->nt) {
+    if (b->nt)
         TRY(enqueue_queries(b, p, 1, rb));
+    if (a->nt)
+        TRY(enqueue_rays(b, ra, ra.pts.p, a->sortedIds, ra.flags.p, a->nt, true, false, true));
+    if (b->nt)
         TRY(enqueue_rays(a, rb, rb.pts.p, b->sortedIds, rb.flags.p, b->nt, true, false, true));
-    }
     HIP_TRY(hipStreamSynchronize(c->stream));
     HIP_TRY(hipStreamSynchronize(c->aux));
     // a side whose target grid turned out too small runs again on its own
