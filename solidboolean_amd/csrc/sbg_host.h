@@ -94,7 +94,7 @@ constexpr int kExpandGrid = 1024;
 constexpr int kLevelsPerSync = 8;
 // breadth-first levels before the persistent depth-first kernel takes the
 // frontier, its wave count, and the initial / largest spill region per wave
-constexpr int kBfsLevels = 20;
+constexpr int kBfsLevels = 24;
 constexpr int kDfsWaves = 512;
 constexpr uint32_t kSpillInit = 2048, kSpillMax = 1u << 15;
 
