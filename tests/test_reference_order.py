@@ -53,3 +53,42 @@ def test_bad_index_is_an_error():
     g = load_golden("small_s1")
     with pytest.raises(ValueError, match="out of range"):
         dropin.reference_pair_order(g["va"], g["ta"], g["vb"], g["tb"], np.array([[10 ** 6, 0]]))
+
+
+def _mesh(v, t):
+    return np.ascontiguousarray(v, np.float64), np.ascontiguousarray(t, np.int64)
+
+
+@pytest.mark.parametrize("case", ["duplicates", "collinear_centers", "tiny", "clustered"])
+def test_order_matches_oracle_dfs_on_degenerate_trees(case):
+    """Shapes that take the tree build's special branches (every center on one
+    side: halving by count; ties of spans; single-leaf trees), against the
+    oracle's reference-shaped tree and depth-first search (itself pinned to
+    the compiled reference's pair lists)."""
+    from oracle.oracle import OracleMesh
+
+    rng = np.random.default_rng(3)
+    if case == "duplicates":
+        # many copies of a few triangles: equal centers, so splits halve
+        base = rng.normal(size=(4, 3, 3))
+        tri = base[rng.integers(0, 4, 300)]
+        va, ta = _mesh(tri.reshape(-1, 3), np.arange(900).reshape(-1, 3))
+        vb, tb = _mesh(tri.reshape(-1, 3) + 0.05, np.arange(900).reshape(-1, 3))
+    elif case == "collinear_centers":
+        x = np.repeat(np.arange(200, dtype=float), 3).reshape(-1, 3, 1)
+        tri = np.concatenate([x * 0.01, rng.normal(scale=0.02, size=(200, 3, 2))], 2)
+        va, ta = _mesh(tri.reshape(-1, 3), np.arange(600).reshape(-1, 3))
+        vb, tb = _mesh(tri.reshape(-1, 3)[:, [1, 0, 2]], np.arange(600).reshape(-1, 3))
+    elif case == "tiny":
+        m = mg.icosphere(0)
+        va, ta = _mesh(m.vertices, m.triangles[:7])
+        vb, tb = _mesh(m.vertices * 0.9 + 0.1, m.triangles)
+    else:
+        tri = np.concatenate([rng.normal(scale=1e-3, size=(150, 3, 3)), rng.normal(size=(150, 3, 3))])
+        va, ta = _mesh(tri.reshape(-1, 3), np.arange(900).reshape(-1, 3))
+        vb, tb = _mesh(tri.reshape(-1, 3) * 1.01, np.arange(900).reshape(-1, 3))
+    ref = OracleMesh(va, ta).find_pairs(OracleMesh(vb, tb))
+    assert len(ref) > 0
+    c = canonical(ref)
+    perm = dropin.reference_pair_order(va, ta, vb, tb, c)
+    assert np.array_equal(c[perm], ref)
