@@ -438,9 +438,15 @@ int sbg_mesh_create(sbg_context ctx, const double *xyz, size_t nverts, const uin
         return fail(SBG_ERR_INVALID, "null vertex or triangle array");
     if (ntris >= (size_t)kLeafBit)
         return fail(SBG_ERR_INVALID, "too many triangles (%zu)", ntris);
+    // the largest index first (a loop without an early exit vectorizes), the
+    // first offending one only when there is one
+    uint32_t top = 0;
     for (size_t i = 0; i < 3 * ntris; ++i)
-        if ((size_t)tris[i] >= nverts)
-            return fail(SBG_ERR_INVALID, "triangle %zu references vertex %u of %zu", i / 3, tris[i], nverts);
+        top = tris[i] > top ? tris[i] : top;
+    if (ntris && (size_t)top >= nverts)
+        for (size_t i = 0; i < 3 * ntris; ++i)
+            if ((size_t)tris[i] >= nverts)
+                return fail(SBG_ERR_INVALID, "triangle %zu references vertex %u of %zu", i / 3, tris[i], nverts);
     sbg_mesh m = mesh_new(ctx);
     if (!m)
         return fail(SBG_ERR_HIP, "event creation failed");

@@ -290,8 +290,14 @@ class Mesh:
         self.L = ctx.L
         v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
         t = np.asarray(triangles).reshape(-1, 3)
-        if t.size and (t.min() < 0 or t.max() >= 2 ** 32):
-            raise ValueError("triangle index out of uint32 range")
+        if t.size and t.dtype != np.uint32:
+            if t.dtype.kind in "iu" and t.dtype.itemsize == 8:
+                # one pass: a negative or >= 2^32 value sets a high bit of the OR
+                wide = (int(np.bitwise_or.reduce(t.view(np.uint64), axis=None)) >> 32) != 0
+            else:
+                wide = t.min() < 0 or t.max() >= 2 ** 32
+            if wide:
+                raise ValueError("triangle index out of uint32 range")
         t = np.ascontiguousarray(t, np.uint32)
         self.nv, self.nt = v.shape[0], t.shape[0]
         h = C.c_void_p()

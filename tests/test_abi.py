@@ -22,3 +22,24 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(native.LIB_PATH)
     missing = [s for s in declared() if not hasattr(lib, s)]
     assert not missing, missing
+
+
+def test_mesh_index_range_checked_before_the_library():
+    """Indices that do not fit the ABI's uint32 are refused in Python, before
+    any library call (the stand-in context has no library)."""
+    import types
+
+    import numpy as np
+    import pytest
+
+    ctx = types.SimpleNamespace(L=None, h=None)
+    v = np.zeros((4, 3))
+    for dtype in (np.int64, np.uint64, np.int32, np.float64):
+        for bad in (-1, 2 ** 32, 2 ** 40):
+            if np.dtype(dtype).kind == "u" and bad < 0 or np.dtype(dtype).itemsize == 4 and bad >= 2 ** 31:
+                continue
+            t = np.array([[0, 1, 2], [1, 2, 3], [0, 2, bad]], dtype)
+            with pytest.raises(ValueError, match="uint32 range"):
+                native.Mesh(ctx, v, t)
+            with pytest.raises(ValueError, match="uint32 range"):
+                native.Mesh(ctx, v, t[:, ::-1])  # strided view

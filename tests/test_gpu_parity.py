@@ -390,6 +390,12 @@ def test_host_pair_list_validation(gpu):
         bad.tri_tri(A, B, N.FP64)
     with pytest.raises(N.SbgError):
         gpu.mesh(a.vertices, np.array([[0, 1, a.nv]]))
+    # the message names the first offending triangle, not the largest index
+    t = np.array(a.triangles, np.uint32)
+    t[7, 2] = a.nv + 5
+    t[3, 0] = a.nv
+    with pytest.raises(N.SbgError, match=f"triangle 3 references vertex {a.nv} of {a.nv}"):
+        gpu.mesh(a.vertices, t)
 
 
 def test_determinism(gpu):
