@@ -261,6 +261,11 @@ int sbg_batch_candidates(sbg_batch b, uint32_t *ab, uint8_t *status, double *seg
 int sbg_batch_sides(sbg_batch b, uint8_t *side);
 /* coordinate search over the tuning knobs, `reps` timed runs per setting; leaves the best set */
 int sbg_batch_autotune(sbg_batch b, int precision, int reps, sbg_batch_tuning *best, double *best_ms);
+/* Testing aid (no counterpart in the reference): sets the depth-first search's
+ * spill region per wave (0..32768 node pairs) and drops the captured graph.  A
+ * run whose search fills the region grows it and repeats, captured runs
+ * included, so results never depend on it. */
+int sbg_batch_set_spill(sbg_batch b, uint32_t per_wave);
 int sbg_batch_destroy(sbg_batch b);
 
 #ifdef __cplusplus

@@ -90,10 +90,10 @@ struct sbg_mesh_s {
     uint32_t gridG = 0;
     int gridDensity = 0;
     bool gridReady = false;
-    // the grid was last built on the second stream; gridDone follows that build,
-    // and work on the other stream that reads the grid or rewrites its inputs
-    // waits for it
-    bool gridOnAux = false;
+    // the stream of the last grid build (null: none since the last prepare);
+    // gridDone follows that build, and work on any other stream that reads the
+    // grid or rewrites its inputs waits for it
+    hipStream_t gridStream = nullptr;
     hipEvent_t gridDone = nullptr;
 };
 
@@ -261,7 +261,7 @@ static sbg_mesh mesh_new(sbg_context c)
     m->gridG = 0;
     m->gridDensity = 0;
     m->gridReady = false;
-    m->gridOnAux = false;
+    m->gridStream = nullptr;
     ++c->refs;
     return m;
 }
@@ -531,9 +531,9 @@ int sbg_mesh_prepare(sbg_mesh m)
     const bool onAux = s != c->stream;
     if (onAux)
         HIP_TRY(hipStreamWaitEvent(s, m->used, 0));  // earlier context-stream work on this mesh
-    if (m->gridOnAux && s != c->aux)
+    if (m->gridStream && m->gridStream != s)
         HIP_TRY(hipStreamWaitEvent(s, m->gridDone, 0));  // the last grid build still reads the records
-    m->gridOnAux = false;
+    m->gridStream = nullptr;
     m->prepared = false;
     m->gridReady = false;
     if (nt == 0) {
@@ -1132,19 +1132,19 @@ static GridSet grid_set(sbg_mesh m)
     return gs;
 }
 
-// Builds m's ray grid on `stream` unless it is current; a grid built on the
-// second stream is waited for by a reader on the other one.
+// Builds m's ray grid on `stream` unless it is current; a reader on another
+// stream than the build's waits for the build.
 static int ensure_grid(sbg_mesh m, hipStream_t stream = nullptr)
 {
     sbg_context c = m->ctx;
     const hipStream_t s = stream ? stream : c->stream;
     if (m->gridReady && m->gridDensity == c->tune.grid_density) {
-        if (m->gridOnAux && s != c->aux)
+        if (m->gridStream && m->gridStream != s)
             HIP_TRY(hipStreamWaitEvent(s, m->gridDone, 0));
         return SBG_OK;
     }
-    // a rebuild must not overwrite a grid that the other stream may still read
-    if (m->gridOnAux && s != c->aux)
+    // a rebuild must not overwrite a grid that another stream may still read
+    if (m->gridStream && m->gridStream != s)
         HIP_TRY(hipStreamWaitEvent(s, m->gridDone, 0));
     const uint32_t G = grid_dim((uint32_t)m->nt, c->tune.grid_density);
     m->gridDensity = c->tune.grid_density;
@@ -1166,9 +1166,8 @@ static int ensure_grid(sbg_mesh m, hipStream_t stream = nullptr)
                           m->gridTri.p, s);
         TRY(check_launch("ray grid"));
     }
-    m->gridOnAux = s == c->aux;
-    if (m->gridOnAux)
-        HIP_TRY(hipEventRecord(m->gridDone, s));
+    m->gridStream = s;
+    HIP_TRY(hipEventRecord(m->gridDone, s));
     m->gridReady = true;
     return SBG_OK;
 }

@@ -400,10 +400,14 @@ int sbg_batch_run(sbg_batch b, int precision)
         return fail(SBG_ERR_INVALID, "unknown precision %d", precision);
     TRY(use(b->ctx));
     b->ran = false;
-    // a graph is captured only once a run has fitted its buffers
+    // A graph is captured only once a plain run has fitted its buffers.  Every
+    // run, the capturing one and every replay included, goes through the same
+    // flag check: how the depth-first search spills depends on how waves claim
+    // seeds, so a replay can overflow where the plain run did not.
     bool fitted = b->exec != nullptr;
-    for (int attempt = 0; attempt < 12; ++attempt) {
-        TRY(run_once(b, precision, fitted));
+    for (int attempt = 0; attempt < 16; ++attempt) {
+        const bool viaGraph = fitted && b->tune.use_graph;
+        TRY(run_once(b, precision, viaGraph));
         const uint32_t *h = b->host;
         uint32_t maxFront = 0;
         for (int l = 0; l <= kBatchMaxLevels; ++l)
@@ -420,7 +424,7 @@ int sbg_batch_run(sbg_batch b, int precision)
         if (sflags & 12u) {
             if (b->spillPerWave >= kBatchSpillMax)
                 return fail(SBG_ERR_OVERFLOW, "depth-first search exceeded its largest spill regions");
-            b->spillPerWave *= 4;
+            b->spillPerWave = std::min(kBatchSpillMax, std::max(kBatchSpillInit, 4 * b->spillPerWave));
             grow = true;
         }
         if (rflags & 2u) {
@@ -429,26 +433,39 @@ int sbg_batch_run(sbg_batch b, int precision)
         }
         if (!grow && (rflags & 1u))
             return fail(SBG_ERR_OVERFLOW, "a ray has more distinct hits than a wave gathers");
-        if (!grow) {
-            b->nCand = nCand;
-            b->ran = true;
+        if (grow) {
+            drop_graph(b);
+            fitted = false;
+            TRY(size_buffers(b));
+            continue;
+        }
+        b->nCand = nCand;
+        if (b->tune.use_graph && !viaGraph) {
             // the pair sort and the GD grid cover the whole capacity: keep it
             // near the real count before a graph fixes it (buffers stay)
             const size_t want = std::max<size_t>((size_t)1 << 14, (size_t)nCand + nCand / 4);
-            if (!fitted && b->candCap > 2 * want)
+            if (b->candCap > 2 * want)
                 b->candCap = want;
-            if (!fitted && b->tune.use_graph) {
-                fitted = true;  // capture now, so the next run is one graph launch
-                TRY(run_once(b, precision, true));
-                b->nCand = b->host[kCandW];
-            }
-            return SBG_OK;
+            fitted = true;  // capture now and check that run like any other
+            continue;
         }
-        drop_graph(b);
-        fitted = false;
-        TRY(size_buffers(b));
+        b->ran = true;
+        return SBG_OK;
     }
     return fail(SBG_ERR_OVERFLOW, "batch buffers did not fit after repeated growth");
+}
+
+int sbg_batch_set_spill(sbg_batch b, uint32_t per_wave)
+{
+    if (!b)
+        return fail(SBG_ERR_INVALID, "null batch");
+    if (per_wave > kBatchSpillMax)
+        return fail(SBG_ERR_INVALID, "spill region of %u pairs exceeds %u", per_wave, kBatchSpillMax);
+    TRY(use(b->ctx));
+    b->spillPerWave = per_wave;
+    drop_graph(b);
+    b->ran = false;
+    return size_buffers(b);
 }
 
 int sbg_batch_last_ms(sbg_batch b, double *ms)

@@ -40,6 +40,7 @@ EXPORTS = [
     "sbg_adjoint",
     "sbg_batch_create", "sbg_batch_set_vertices", "sbg_batch_set_tuning", "sbg_batch_get_tuning", "sbg_batch_run",
     "sbg_batch_last_ms", "sbg_batch_counts", "sbg_batch_candidates", "sbg_batch_sides", "sbg_batch_autotune",
+    "sbg_batch_set_spill",
     "sbg_batch_destroy",
 ]
 
@@ -143,6 +144,7 @@ def load(path: str = LIB_PATH):
         "sbg_batch_candidates": [_P, _P, _P, _P],
         "sbg_batch_sides": [_P, _U8],
         "sbg_batch_autotune": [_P, C.c_int, C.c_int, C.POINTER(BatchTuning), C.POINTER(C.c_double)],
+        "sbg_batch_set_spill": [_P, C.c_uint32],
         "sbg_batch_destroy": [_P],
     }
     for name, args in sig.items():
@@ -493,6 +495,10 @@ class Batch:
         for k, v in kw.items():
             setattr(t, k, int(v))
         _check(self.L.sbg_batch_set_tuning(self.h, C.byref(t)))
+
+    def set_spill(self, per_wave: int):
+        """Testing aid: the depth-first search's spill region per wave."""
+        _check(self.L.sbg_batch_set_spill(self.h, int(per_wave)))
 
     def run(self, precision: int = FP32_FILTERED) -> float:
         _check(self.L.sbg_batch_run(self.h, precision))

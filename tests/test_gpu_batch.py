@@ -159,3 +159,109 @@ def test_autotune_keeps_results(gpu):
     batch.run(N.FP32_FILTERED)
     got = batch.candidates() + (batch.sides(),)
     assert all(np.array_equal(x, y) for x, y in zip(ref, got))
+
+
+def test_small_spill_with_graph_matches_plain(gpu, mixed_pairs):
+    """The depth-first search started at the root pairs with no spill region
+    overflows its per-wave stacks; captured runs must grow and repeat like
+    plain ones, never return a truncated candidate list."""
+    plain = gpu.batch(mixed_pairs)
+    plain.set_tuning(use_graph=0)
+    plain.run(N.FP32_FILTERED)
+    ref = plain.candidates() + (plain.sides(),)
+    plain.close()
+    batch = gpu.batch(mixed_pairs)
+    batch.set_tuning(bfs_levels=0, dfs_waves=64, use_graph=1)
+    for _ in range(3):
+        batch.set_spill(0)
+        batch.run(N.FP32_FILTERED)   # plain run, growth, then a captured run
+        got = batch.candidates() + (batch.sides(),)
+        assert all(np.array_equal(x, y) for x, y in zip(ref, got))
+        batch.run(N.FP32_FILTERED)   # replay
+        got = batch.candidates() + (batch.sides(),)
+        assert all(np.array_equal(x, y) for x, y in zip(ref, got))
+
+
+# ---- the 4,096-pair workload itself (levels 1..6, seed 1) -------------------
+
+@pytest.fixture(scope="module")
+def batch4096(gpu):
+    pairs = mg.batch_pairs(4096, seed=1)
+    print(f"4096-pair batch: {sum(a.nt + b.nt for a, b in pairs)} triangles", flush=True)
+    batch = gpu.batch(pairs)
+    out = {}
+    for prec in (N.FP64, N.FP32_FILTERED):
+        batch.run(prec)
+        batch.run(prec)  # a graph replay
+        ab, st, seg = batch.candidates()
+        out[prec] = (ab, st, seg, batch.sides())
+    batch.close()
+    return pairs, out
+
+
+def _pair_slices(tb, ab):
+    """candidate index range of every pair (candidates are sorted by global a)"""
+    first = np.searchsorted(ab[:, 0].astype(np.int64), tb[0::2].astype(np.int64), side="left")
+    return first[:-1], first[1:]
+
+
+def test_batch4096_precisions_agree(batch4096):
+    pairs, out = batch4096
+    a, b = out[N.FP64], out[N.FP32_FILTERED]
+    assert len(a[0]) > 1_000_000
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    h = a[1] == 1
+    assert np.array_equal(bits(a[2][h]), bits(b[2][h]))
+    assert np.array_equal(a[3], b[3])
+
+
+def test_batch4096_matches_single_pairs(gpu, batch4096):
+    """Every one of the 4,096 pairs against the single-pair path."""
+    pairs, out = batch4096
+    ab, st, seg, sides = out[N.FP32_FILTERED]
+    tb = np.zeros(2 * len(pairs) + 1, np.int64)
+    for k, (a, b) in enumerate(pairs):
+        tb[2 * k + 1] = tb[2 * k] + a.nt
+        tb[2 * k + 2] = tb[2 * k + 1] + b.nt
+    lo, hi = _pair_slices(tb, ab)
+    for k, (a, b) in enumerate(pairs):
+        ref = single_pair(gpu, a, b, N.FP64)
+        loc = ab[lo[k]:hi[k]].astype(np.int64) - np.array([tb[2 * k], tb[2 * k + 1]])
+        assert np.array_equal(loc, ref[0]), k
+        assert np.array_equal(st[lo[k]:hi[k]], ref[1]), k
+        h = ref[1] == 1
+        assert np.array_equal(bits(seg[lo[k]:hi[k]][h]), bits(ref[2][h])), k
+        assert np.array_equal(sides[tb[2 * k]:tb[2 * k + 1]], ref[3]), k
+        assert np.array_equal(sides[tb[2 * k + 1]:tb[2 * k + 2]], ref[4]), k
+
+
+def test_batch4096_matches_oracle(batch4096):
+    """Every level-5 and level-6 pair, plus 12 seeded others, against the oracle."""
+    pairs, out = batch4096
+    ab, st, seg, sides = out[N.FP32_FILTERED]
+    tb = np.zeros(2 * len(pairs) + 1, np.int64)
+    for k, (a, b) in enumerate(pairs):
+        tb[2 * k + 1] = tb[2 * k] + a.nt
+        tb[2 * k + 2] = tb[2 * k + 1] + b.nt
+    lo, hi = _pair_slices(tb, ab)
+    big = [k for k, (a, _) in enumerate(pairs) if a.nt >= 20 * 4 ** 5]
+    assert any(pairs[k][0].nt == 20 * 4 ** 6 for k in big), "the workload holds level-6 pairs"
+    rng = np.random.default_rng(5)
+    pick = sorted(set(big) | set(rng.choice(len(pairs), 12, replace=False).tolist()))
+    for k in pick:
+        a, b = pairs[k]
+        print(f"oracle check of pair {k} ({a.nt} + {b.nt} triangles)", flush=True)
+        OA, OB = OracleMesh(a.vertices, a.triangles), OracleMesh(b.vertices, b.triangles)
+        ref = OA.find_pairs(OB)
+        ref = ref[np.lexsort((ref[:, 1], ref[:, 0]))] if len(ref) else ref.reshape(0, 2)
+        loc = ab[lo[k]:hi[k]].astype(np.int64) - np.array([tb[2 * k], tb[2 * k + 1]])
+        assert np.array_equal(loc, ref.astype(np.int64)), k
+        ost, oseg = OA.tri_tri_pairs(OB, ref)
+        assert np.array_equal(st[lo[k]:hi[k]], ost), k
+        assert np.array_equal(bits(seg[lo[k]:hi[k]][ost == 1]), bits(oseg[ost == 1])), k
+        ia = np.zeros(a.nt, np.uint8)
+        ib = np.zeros(b.nt, np.uint8)
+        ia[ref[ost == 1, 0]] = 1
+        ib[ref[ost == 1, 1]] = 1
+        assert np.array_equal(sides[tb[2 * k]:tb[2 * k + 1]], OA.classify_triangles(OB, ia)), k
+        assert np.array_equal(sides[tb[2 * k + 1]:tb[2 * k + 2]], OB.classify_triangles(OA, ib)), k
