@@ -186,6 +186,19 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, int precision, in
 int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs pairs, int side, int iters, double *avg_ms,
                    uint64_t *work);
 
+/* Testing aids (no counterpart in the reference), run on the host, no device
+ * needed: the FP32 certificate the ray kernels apply to one candidate of
+ * isPointInMesh (src/solidboolean.cpp:64-87), for n candidates: point pts
+ * (n x 3 FP64), axis 0/1/2, triangle tris9 (n x 9 FP64).  verdict: 0 a
+ * certain miss, 1 a certain hit whose FP64 hit point lies within et of
+ * p + t along the axis, 2 not certain (the kernels then run the ray in FP64).
+ * sbg_ray_hits_apart: 1 where two certain hits (t, et) of one ray are
+ * certain to have different PositionKeys. */
+int sbg_ray_certify(const double *pts, const int32_t *axis, const double *tris9, size_t n, int32_t *verdict,
+                    float *t, float *et);
+int sbg_ray_hits_apart(const float *ta, const float *ea, const float *tb, const float *eb, size_t n,
+                       uint8_t *out);
+
 /* Measurement aid (no counterpart in the reference): a plain device-to-device
  * copy kernel of `bytes` bytes run `iters` times, for calibrating the HBM
  * traffic counters (FETCH_SIZE / WRITE_SIZE) against a known byte count. */

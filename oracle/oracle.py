@@ -43,6 +43,11 @@ def load_lib(path: str = ORACLE_SO):
     L.orc_classify_point.argtypes = [_P, _D]
     L.orc_classify_triangles.argtypes = [_P, _P, _U8, _U8]
     L.orc_position_key.argtypes = [_D, _I]
+    L.orc_ray_candidate.restype = C.c_int
+    L.orc_ray_candidate.argtypes = [_D, C.c_int, _D, _D, _D, _D, _D]
+    L.orc_ray_candidates.restype = None
+    L.orc_ray_candidates.argtypes = [C.c_size_t, _D, np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS"), _D,
+                                     np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS"), _D]
     _libs[path] = L
     return L
 
@@ -101,6 +106,31 @@ def position_key(p, L=None) -> np.ndarray:
     out = np.zeros(3, np.int64)
     L.orc_position_key(np.ascontiguousarray(p, np.float64), out)
     return out
+
+
+def ray_candidate(p, axis: int, v0, v1, v2, L=None):
+    """isPointInMesh's test of one candidate triangle (its Vector3::normal
+    computed here): (hit, hit point)."""
+    L = L or lib()
+    f = lambda x: np.ascontiguousarray(x, np.float64)
+    n = normal(v0, v1, v2, L)
+    x = np.zeros(3)
+    h = L.orc_ray_candidate(f(p), int(axis), f(v0), f(v1), f(v2), f(n), x)
+    return bool(h), x
+
+
+def ray_candidates(pts, axis, tris9, L=None):
+    """ray_candidate over arrays: (hit flags, hit points)."""
+    L = L or lib()
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    n = len(pts)
+    ax = np.ascontiguousarray(np.broadcast_to(np.asarray(axis, np.int32), (n,)))
+    tr = np.ascontiguousarray(tris9, np.float64).reshape(n, 9)
+    hit = np.zeros(max(n, 1), np.uint8)
+    x = np.zeros((max(n, 1), 3))
+    L.orc_ray_candidates(n, pts.reshape(-1) if n else np.zeros(3), ax, tr.reshape(-1) if n else np.zeros(9), hit,
+                         x.reshape(-1))
+    return hit[:n].astype(bool), x[:n]
 
 
 class OracleMesh:

@@ -672,42 +672,86 @@ static void collect_ray(const orc_mesh *m, int64_t ni, const double *rmn, const 
     collect_ray(m, f->right, rmn, rmx, ib);
 }
 
-int orc_point_in_mesh(const orc_mesh *m, const double p[3], int axis)
+/* One candidate of isPointInMesh's loop (src/solidboolean.cpp:64-87): the
+ * plane hit of the segment p -> end (vector3.h:264-280) and the three-normal
+ * inside test; 1 with the hit point in x on a hit. */
+static int ray_test(const double p[3], const double end[3], const double u[3], const double *v0, const double *v1,
+                    const double *v2, const double *n, double x[3])
+{
+    double w[3];
+    sub3(w, p, v0);
+    double d = ORC_PLANE_DOT(n, u);
+    double num = (-n[0]) * w[0] + (-n[1]) * w[1] + (-n[2]) * w[2];
+    (void)end;
+    if (fabs(d) <= DBL_EPSILON)
+        return 0;
+    double s = num / d;
+    if (s < 0 || s > 1 || isnan(s) || isinf(s))
+        return 0;
+    x[0] = p[0] + s * u[0];
+    x[1] = p[1] + s * u[1];
+    x[2] = p[2] + s * u[2];
+    const double *tv[3] = {v0, v1, v2};
+    double nn[3][3];
+    for (int i = 0; i < 3; ++i)
+        orc_normal(x, tv[i], tv[(i + 1) % 3], nn[i]);
+    return ORC_INSIDE_GT(dot3(nn[0], nn[1])) && ORC_INSIDE_GT(dot3(nn[0], nn[2]));
+}
+
+static void ray_segment(const double p[3], int axis, double end[3], double u[3], double rmn[3], double rmx[3])
 {
     double dir[3] = {DBL_EPSILON, DBL_EPSILON, DBL_EPSILON};
     dir[axis] = DBL_MAX;
-    double end[3] = {p[0] + dir[0], p[1] + dir[1], p[2] + dir[2]};
-    double rmn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, rmx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    for (int k = 0; k < 3; ++k) {
+        end[k] = p[k] + dir[k];
+        rmn[k] = DBL_MAX;
+        rmx[k] = -DBL_MAX;
+    }
     box_update(rmn, rmx, p);
     box_update(rmn, rmx, end);
+    sub3(u, end, p);
+}
+
+int orc_ray_candidate(const double p[3], int axis, const double v0[3], const double v1[3], const double v2[3],
+                      const double n[3], double hit[3])
+{
+    double end[3], u[3], rmn[3], rmx[3], bmn[3] = {DBL_MAX, DBL_MAX, DBL_MAX};
+    double bmx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+    ray_segment(p, axis, end, u, rmn, rmx);
+    box_update(bmn, bmx, v0);
+    box_update(bmn, bmx, v1);
+    box_update(bmn, bmx, v2);
+    if (!boxes_overlap(bmn, bmx, rmn, rmx))
+        return 0;
+    return ray_test(p, end, u, v0, v1, v2, n, hit);
+}
+
+void orc_ray_candidates(size_t n, const double *pts, const int32_t *axis, const double *tris9, uint8_t *hit,
+                        double *hits)
+{
+    for (size_t i = 0; i < n; ++i) {
+        const double *t = tris9 + 9 * i;
+        double nrm[3];
+        orc_normal(t, t + 3, t + 6, nrm);
+        hits[3 * i] = hits[3 * i + 1] = hits[3 * i + 2] = 0.0;
+        hit[i] = (uint8_t)orc_ray_candidate(pts + 3 * i, axis[i], t, t + 3, t + 6, nrm, hits + 3 * i);
+    }
+}
+
+int orc_point_in_mesh(const orc_mesh *m, const double p[3], int axis)
+{
+    double end[3], u[3], rmn[3], rmx[3];
+    ray_segment(p, axis, end, u, rmn, rmx);
     idxbuf_t ib = {NULL, 0, 0};
     if (m->nnodes)
         collect_ray(m, 0, rmn, rmx, &ib);
     int64_t *keys = (int64_t *)malloc(sizeof(int64_t) * 3 * (ib.n ? ib.n : 1));
     size_t nk = 0;
-    double u[3], w[3];
-    sub3(u, end, p);
     for (size_t c = 0; c < ib.n; ++c) {
         int64_t t = ib.idx[c];
-        const double *v0 = m->V + 3 * m->T[3 * t];
-        const double *v1 = m->V + 3 * m->T[3 * t + 1];
-        const double *v2 = m->V + 3 * m->T[3 * t + 2];
-        const double *n = m->normals + 3 * t;
-        /* intersectSegmentAndPlane (vector3.h:264-280) */
-        sub3(w, p, v0);
-        double d = ORC_PLANE_DOT(n, u);
-        double num = (-n[0]) * w[0] + (-n[1]) * w[1] + (-n[2]) * w[2];
-        if (fabs(d) <= DBL_EPSILON)
-            continue;
-        double s = num / d;
-        if (s < 0 || s > 1 || isnan(s) || isinf(s))
-            continue;
-        double x[3] = {p[0] + s * u[0], p[1] + s * u[1], p[2] + s * u[2]};
-        const double *tv[3] = {v0, v1, v2};
-        double nn[3][3];
-        for (int i = 0; i < 3; ++i)
-            orc_normal(x, tv[i], tv[(i + 1) % 3], nn[i]);
-        if (ORC_INSIDE_GT(dot3(nn[0], nn[1])) && ORC_INSIDE_GT(dot3(nn[0], nn[2]))) {
+        double x[3];
+        if (ray_test(p, end, u, m->V + 3 * m->T[3 * t], m->V + 3 * m->T[3 * t + 1], m->V + 3 * m->T[3 * t + 2],
+                     m->normals + 3 * t, x)) {
             orc_position_key(x, keys + 3 * nk);
             ++nk;
         }

@@ -62,6 +62,18 @@ void orc_tri_tri_pairs(const orc_mesh *a, const orc_mesh *b, const int64_t *pair
 /* isPointInMesh (src/solidboolean.cpp:48-92) along axis 0/1/2 (:31-35). */
 int orc_point_in_mesh(const orc_mesh *m, const double p[3], int axis);
 
+/* One candidate of isPointInMesh (src/solidboolean.cpp:55-87) on its own: the
+ * closed overlap of the triangle's box with the segment p -> p + axis
+ * direction, the plane hit (src/vector3.h:264-280, n = the triangle's
+ * Vector3::normal) and the three-normal inside test.  Returns 1 on a hit with
+ * the FP64 hit point in hit. */
+int orc_ray_candidate(const double p[3], int axis, const double v0[3], const double v1[3], const double v2[3],
+                      const double n[3], double hit[3]);
+
+/* orc_ray_candidate over n candidates (tris9: n x 9 vertices); hits n x 3. */
+void orc_ray_candidates(size_t n, const double *pts, const int32_t *axis, const double *tris9, uint8_t *hit,
+                        double *hits);
+
 /* decideGroupSide's vote (src/solidboolean.cpp:492-508): 1 iff inside along
  * at least 2 of the 3 axes. */
 int orc_classify_point(const orc_mesh *m, const double p[3]);

@@ -5,6 +5,8 @@
 * solidboolean_amd/lib/libsolidboolean.so - the host-side drop-in
   SolidMesh / SolidBoolean C++ API linked against libsbgpu.so (when its
   sources exist).
+* solidboolean_amd/lib/sb_driver - the reference's OBJ test driver on the
+  drop-in API (solidboolean_amd/driver/sb_driver.cpp).
 
 Every FP64 translation unit is compiled with -ffp-contract=off so that no
 a*b+c is fused (the reference's results depend on separate roundings), and
@@ -89,8 +91,15 @@ def build_host(verbose: bool = False) -> str | None:
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libsolidboolean.so")
     tmp = out + ".tmp"
-    _run(["g++", *CXX_FLAGS, "-shared", "-o", tmp, *srcs, f"-L{LIBDIR}", "-lsbgpu", "-Wl,-rpath,$ORIGIN"], verbose)
+    _run(["g++", *CXX_FLAGS, "-shared", "-o", tmp, *srcs, f"-L{LIBDIR}", "-lsbgpu", "-Wl,-rpath,$ORIGIN",
+          "-pthread"], verbose)
     os.replace(tmp, out)
+    driver = os.path.join(HERE, "driver", "sb_driver.cpp")
+    if os.path.exists(driver):
+        exe = os.path.join(LIBDIR, "sb_driver")
+        _run(["g++", *[f for f in CXX_FLAGS if f != "-fPIC"], "-o", exe + ".tmp", driver, f"-L{LIBDIR}",
+              "-lsolidboolean", "-lsbgpu", "-Wl,-rpath,$ORIGIN", "-pthread"], verbose)
+        os.replace(exe + ".tmp", exe)
     return out
 
 

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "sbg_host.h"
+#include "sbg_rayfilter.h"
 
 using namespace sbg;
 
@@ -1668,6 +1669,41 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
         for (int k = 0; k < 4; ++k)
             work[k] = h[k];
     return check_launch("bench rays");
+}
+
+// Testing aid, run on the host: the ray kernels' FP32 certificate for n
+// (point, axis, triangle) candidates, from the same 48-B records they read.
+int sbg_ray_certify(const double *pts, const int32_t *axis, const double *tris9, size_t n, int32_t *verdict,
+                    float *t, float *et)
+{
+    if (n && (!pts || !axis || !tris9 || !verdict || !t || !et))
+        return fail(SBG_ERR_INVALID, "null argument");
+    for (size_t i = 0; i < n; ++i) {
+        if (axis[i] < 0 || axis[i] > 2)
+            return fail(SBG_ERR_INVALID, "axis %d of candidate %zu", axis[i], i);
+        float4 r[3];
+        make_tri32(tris9 + 9 * i, tris9 + 9 * i + 3, tris9 + 9 * i + 6, r);
+        const double *p = pts + 3 * i;
+        const float p32[3] = {(float)p[0], (float)p[1], (float)p[2]};
+        double pm = __builtin_fabs(p[0]);
+        pm = __builtin_fabs(p[1]) > pm ? __builtin_fabs(p[1]) : pm;
+        pm = __builtin_fabs(p[2]) > pm ? __builtin_fabs(p[2]) : pm;
+        const RayCert c = ray_certificate(p32, fl_up(pm), axis[i], r);
+        verdict[i] = c.verdict;
+        t[i] = c.t;
+        et[i] = c.et;
+    }
+    return SBG_OK;
+}
+
+// Testing aid, run on the host: hits_apart() of the ray kernels for n pairs.
+int sbg_ray_hits_apart(const float *ta, const float *ea, const float *tb, const float *eb, size_t n, uint8_t *out)
+{
+    if (n && (!ta || !ea || !tb || !eb || !out))
+        return fail(SBG_ERR_INVALID, "null argument");
+    for (size_t i = 0; i < n; ++i)
+        out[i] = hits_apart(ta[i], ea[i], tb[i], eb[i]) ? 1 : 0;
+    return SBG_OK;
 }
 
 int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, int iters, double *avg_ms)

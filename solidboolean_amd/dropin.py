@@ -99,6 +99,8 @@ def _host_only_signatures(L):
     L.sbh_reference_pair_order.restype = C.c_int
     L.sbh_reference_pair_order.argtypes = [_D, C.c_size_t, _I, C.c_size_t, _D, C.c_size_t, _I, C.c_size_t, _U32,
                                            C.c_size_t, _U64]
+    L.sbh_tree_pairs.restype = C.c_size_t
+    L.sbh_tree_pairs.argtypes = [_D, C.c_size_t, _I, C.c_size_t, _D, C.c_size_t, _I, C.c_size_t, _U64, C.c_size_t]
 
 
 def reference_pair_order(va, ta, vb, tb, ab) -> np.ndarray:
@@ -118,6 +120,25 @@ def reference_pair_order(va, ta, vb, tb, ab) -> np.ndarray:
                                       ab.reshape(-1) if ab.size else np.zeros(2, np.uint32), len(ab), perm):
         raise ValueError(L.sbh_last_error().decode(errors="replace"))
     return perm[: len(ab)].astype(np.int64)
+
+
+def tree_pairs(va, ta, vb, tb) -> np.ndarray:
+    """Candidate pairs (n x 2) from the reference-signature tree on the host:
+    AxisAlignedBoudingBoxTree's public constructor for both meshes, then
+    test(root, root, boxes, pairs), in the order it reports them."""
+    L = load_earclip_only()
+    args = []
+    for v, t in ((va, ta), (vb, tb)):
+        v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+        t = np.ascontiguousarray(t, np.int64).reshape(-1, 3)
+        args += [v.reshape(-1) if v.size else np.zeros(3), len(v), t.reshape(-1) if t.size else np.zeros(3, np.int64),
+                 len(t)]
+    n = L.sbh_tree_pairs(*args, np.zeros(2, np.uint64), 0)
+    if n == 2 ** 64 - 1:
+        raise ValueError(L.sbh_last_error().decode(errors="replace"))
+    out = np.zeros(2 * max(n, 1), np.uint64)
+    L.sbh_tree_pairs(*args, out, n)
+    return out[: 2 * n].reshape(-1, 2).astype(np.int64)
 
 
 def load_obj(path: str):
@@ -168,10 +189,11 @@ class SolidMesh:
 
 
 class SolidBoolean:
-    def __init__(self, a: SolidMesh, b: SolidMesh, reference_pair_order: bool = False, robust_weld: bool = False):
-        """reference_pair_order: weld the intersection points in the order the
-        reference's own tree search reports the pairs
-        (SolidBoolean::setReferencePairOrder).  robust_weld: weld them by the
+    def __init__(self, a: SolidMesh, b: SolidMesh, reference_pair_order: bool = True, robust_weld: bool = False):
+        """reference_pair_order: process the intersecting pairs in the order
+        the reference's own tree search reports them, which makes the results
+        the reference's bit for bit (SolidBoolean::setReferencePairOrder; on by
+        default, off takes (a, b) order).  robust_weld: weld them by the
         edge-triangle crossing each one is, not by PositionKey cell
         (SolidBoolean::setRobustWeld; not the reference's behaviour)."""
         self.L = load()

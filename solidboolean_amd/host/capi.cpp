@@ -83,7 +83,7 @@ int sbh_boolean_new_opts(void *a, void *b, unsigned flags, void **out)
     }
 }
 
-int sbh_boolean_new(void *a, void *b, void **out) { return sbh_boolean_new_opts(a, b, 0, out); }
+int sbh_boolean_new(void *a, void *b, void **out) { return sbh_boolean_new_opts(a, b, 1u, out); }
 
 void sbh_boolean_free(void *p) { delete (HostBoolean *)p; }
 
@@ -254,6 +254,54 @@ int sbh_reference_pair_order(const double *xyzA, size_t nvA, const int64_t *triA
     } catch (const std::exception &e) {
         g_hostErr = e.what();
         return 0;
+    }
+}
+
+// The reference's own search on the host (no GPU): both meshes' trees built
+// by AxisAlignedBoudingBoxTree's public constructor over their triangle boxes
+// and all-vertex boxes (src/solidmesh.cpp:57-75), then test(root, root,
+// boxes, pairs) (src/solidboolean.cpp:94-101).  Writes up to `cap` pairs
+// (a, b) into ab and returns how many there are; (size_t)-1 on bad input.
+size_t sbh_tree_pairs(const double *xyzA, size_t nvA, const int64_t *triA, size_t ntA, const double *xyzB,
+                      size_t nvB, const int64_t *triB, size_t ntB, uint64_t *ab, size_t cap)
+{
+    try {
+        struct Side {
+            std::vector<AxisAlignedBoudingBox> boxes;
+            std::vector<size_t> indices;
+            AxisAlignedBoudingBox all;
+        };
+        auto side = [](const double *xyz, size_t nv, const int64_t *tri, size_t nt, Side &s) {
+            s.boxes.resize(nt);
+            s.indices.resize(nt);
+            for (size_t i = 0; i < nt; ++i) {
+                for (size_t k = 0; k < 3; ++k) {
+                    const int64_t v = tri[3 * i + k];
+                    if (v < 0 || (size_t)v >= nv)
+                        throw std::out_of_range("triangle index out of range");
+                    const Vector3 p(xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]);
+                    s.boxes[i].update(p);
+                    s.all.update(p);
+                }
+                s.boxes[i].updateCenter();
+                s.indices[i] = i;
+            }
+            s.all.updateCenter();
+        };
+        Side a, b;
+        side(xyzA, nvA, triA, ntA, a);
+        side(xyzB, nvB, triB, ntB, b);
+        const AxisAlignedBoudingBoxTree ta(&a.boxes, a.indices, a.all), tb(&b.boxes, b.indices, b.all);
+        std::vector<std::pair<size_t, size_t>> pairs;
+        ta.test(ta.root(), tb.root(), &b.boxes, &pairs);
+        for (size_t i = 0; i < pairs.size() && i < cap; ++i) {
+            ab[2 * i] = pairs[i].first;
+            ab[2 * i + 1] = pairs[i].second;
+        }
+        return pairs.size();
+    } catch (const std::exception &e) {
+        g_hostErr = e.what();
+        return (size_t)-1;
     }
 }
 

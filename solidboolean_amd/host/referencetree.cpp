@@ -155,6 +155,46 @@ ReferenceTree::ReferenceTree(const std::vector<Vector3> &vertices, const std::ve
             std::string().swap(m_paths[v]);
 }
 
+ReferenceTree::ReferenceTree(const AxisAlignedBoudingBoxTree::Node *root, size_t nboxes)
+{
+    m_leafOf.assign(nboxes, 0);
+    m_place.assign(nboxes, 0);
+    m_nodes.emplace_back();
+    m_paths.emplace_back();
+    struct Item {
+        const AxisAlignedBoudingBoxTree::Node *node;
+        uint32_t id;
+    };
+    std::vector<Item> todo;
+    if (root)
+        todo.push_back({root, 0});
+    while (!todo.empty()) {
+        const Item it = todo.back();
+        todo.pop_back();
+        const auto &members = it.node->boxIndices;
+        m_nodes[it.id].count = (uint32_t)members.size();
+        if (it.node->isLeaf()) {
+            for (size_t k = 0; k < members.size(); ++k) {
+                m_leafOf[members[k]] = it.id;
+                m_place[members[k]] = (uint32_t)k;
+            }
+            continue;
+        }
+        const uint32_t l = (uint32_t)m_nodes.size(), r = l + 1;
+        m_nodes.resize(m_nodes.size() + 2);
+        m_paths.resize(m_paths.size() + 2);
+        m_nodes[it.id].left = l;
+        m_nodes[it.id].right = r;
+        m_paths[l] = m_paths[it.id] + '0';
+        m_paths[r] = m_paths[it.id] + '1';
+        todo.push_back({it.node->right, r});
+        todo.push_back({it.node->left, l});
+    }
+    for (size_t v = 0; v < m_nodes.size(); ++v)
+        if (!m_nodes[v].leaf())
+            std::string().swap(m_paths[v]);
+}
+
 std::vector<size_t> referencePairOrder(const ReferenceTree &first, const ReferenceTree &second, const uint32_t *ab,
                                        size_t n)
 {

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "axisalignedboundingboxtree.h"
 #include "vector3.h"
 
 namespace sbhost {
@@ -23,6 +24,9 @@ namespace sbhost {
 class ReferenceTree {
 public:
     ReferenceTree(const std::vector<Vector3> &vertices, const std::vector<std::vector<size_t>> &triangles);
+    // the same tables read off a tree already built with those rules (root
+    // and the number of boxes it indexes)
+    ReferenceTree(const AxisAlignedBoudingBoxTree::Node *root, size_t nboxes);
 
     // the walk's turns (0 left, 1 right) from the root down to triangle t's leaf
     const std::string &pathTo(size_t t) const { return m_paths[m_leafOf[t]]; }

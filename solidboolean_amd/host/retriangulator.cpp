@@ -18,7 +18,7 @@ ReTriangulator::ReTriangulator(const std::vector<Vector3> &points, const Vector3
 }
 
 void ReTriangulator::setEdges(const std::vector<Vector3> &points,
-                              const std::map<size_t, std::set<size_t>> *neighborMapFrom3)
+                              const std::unordered_map<size_t, std::unordered_set<size_t>> *neighborMapFrom3)
 {
     Vector3::project(points, &m_points, m_projectNormal, m_projectAxis, m_projectOrigin);
     m_neighborMap = neighborMapFrom3;
@@ -38,7 +38,7 @@ void ReTriangulator::lookupPolylines()
     for (const auto &kv : nm)
         if (kv.second.size() > 1)
             starts.push_back(kv.first);
-    std::set<size_t> visited;
+    std::unordered_set<size_t> visited;
     for (size_t start : starts) {
         if (visited.count(start))
             continue;
@@ -132,7 +132,7 @@ bool ReTriangulator::buildPolygons()
         for (const auto &e : edgePoints[i])
             ring.push_back(e);
     }
-    std::map<size_t, size_t> ringPos;
+    std::unordered_map<size_t, size_t> ringPos;
     for (size_t i = 0; i < ring.size(); ++i)
         if (ring[i].polylineIndex != -1)
             ringPos.insert({ring[i].pointIndex, i});
@@ -240,7 +240,7 @@ void ReTriangulator::triangulate()
         std::vector<size_t> holes;
         auto c = m_innerChildren.find(pi);
         if (c != m_innerChildren.end()) {
-            std::set<size_t> children = c->second;
+            std::unordered_set<size_t> children = c->second;
             for (size_t child : c->second) {
                 auto g = m_innerChildren.find(child);
                 if (g != m_innerChildren.end())

@@ -229,7 +229,7 @@ bool SolidBoolean::reTriangulate(const ContextMap &contexts, const SolidMesh *me
 bool SolidBoolean::buildPolygonsFromEdges(const EdgeMap &edges, std::vector<std::vector<size_t>> &polygons)
 {
     // (reference: src/solidboolean.cpp:124-165)
-    std::set<size_t> visited;
+    std::unordered_set<size_t> visited;
     for (const auto &kv : edges) {
         const size_t start = kv.first;
         if (visited.count(start))
@@ -539,8 +539,8 @@ bool SolidBoolean::combineOnce()
     }
     gpuLock.unlock();
     if (m_referencePairOrder && ab.size() > 2) {
-        const sbhost::ReferenceTree first(*m_firstMesh->vertices(), *m_firstMesh->triangles());
-        const sbhost::ReferenceTree second(*m_secondMesh->vertices(), *m_secondMesh->triangles());
+        const sbhost::ReferenceTree &first = *m_firstMesh->referenceTree();
+        const sbhost::ReferenceTree &second = *m_secondMesh->referenceTree();
         const std::vector<size_t> order = sbhost::referencePairOrder(first, second, ab.data(), ab.size() / 2);
         std::vector<uint32_t> ab2(ab.size());
         std::vector<double> seg2(seg.size());

@@ -5,17 +5,22 @@
 // (candidate pairs, Guigue-Devillers segments, ray-parity group sides) and the
 // topological stages on the host with the reference's rules: PositionKey
 // welding of intersection points (first inserted wins), re-triangulation of
-// cut triangles, intersection loops, half-edge flood fill into groups.  Host
-// maps are ordered (or keyed hash tables whose iteration order is never
-// observed), so results do not depend on hash iteration order; intersecting
-// pairs are processed in (a, b) order, or in the reference's search order
-// (setReferencePairOrder).
+// cut triangles, intersection loops, half-edge flood fill into groups.  The
+// containers whose iteration order the results depend on (cut-triangle
+// contexts, their cut graphs, the intersection edges) are the reference's own
+// types, unordered_map / unordered_set keyed by triangle or point index, filled
+// by the same operations in the same order; intersecting pairs are processed
+// in (a, b) order, or in the reference's search order (setReferencePairOrder),
+// with which the numbering of new points, the direction of loops and the
+// first triangle of every group are the reference's.
 #pragma once
 #include <chrono>
 #include <cstdint>
 #include <array>
 #include <map>
 #include <set>
+#include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "positionkey.h"
@@ -59,13 +64,16 @@ public:
     // p0, p1, p2 (the other triangle).  False if some endpoint matches no edge.
     bool vertexSources(std::vector<uint32_t> &src5) const;
     std::vector<uint8_t> triangleSides(int which) const;
-    // Before combine() (no counterpart in the reference): take the
-    // intersecting pairs in the order the reference's own tree search reports
-    // them instead of (a, b) order.  The weld keeps the first point it meets
-    // in each PositionKey cell, so with this on, the kept points are the
-    // reference's even where two intersection points share a cell.  Costs a
-    // host build of the reference's tree for both meshes (about 20-30 ms per
-    // 82k-triangle mesh).
+    // Before combine() (no counterpart in the reference; on by default): take
+    // the intersecting pairs in the order the reference's own tree search
+    // reports them (on) or in (a, b) order (off).  The weld keeps the first
+    // point it meets in each PositionKey cell and the first triangle of a
+    // group decides its side, so with this on the results are the
+    // reference's bit for bit and in its order, also where points share a
+    // cell or a group's triangles disagree.  It needs the reference's tree
+    // of both meshes, which SolidMesh::prepare() builds on a host thread
+    // beside the GPU work (about 25 ms per 82k-triangle mesh, 0.4 s per
+    // 1.31M-triangle mesh on one core); off, combine() never waits for it.
     void setReferencePairOrder(bool on) { m_referencePairOrder = on; }
     // Before combine() (not the reference's behaviour, off by default): weld
     // intersection points by what they are, the crossing of one mesh's edge
@@ -116,17 +124,17 @@ private:
         std::map<PositionKey, size_t> positionMap;
         std::map<WeldKey, size_t> keyMap;  // robust weld
         std::vector<std::vector<int64_t>> merged;  // hinted cells: every endpoint welded into each point
-        std::map<size_t, std::set<size_t>> neighborMap;
+        std::unordered_map<size_t, std::unordered_set<size_t>> neighborMap;
     };
-    using ContextMap = std::map<size_t, IntersectedContext>;
-    using EdgeMap = std::map<size_t, std::set<size_t>>;
+    using ContextMap = std::unordered_map<size_t, IntersectedContext>;
+    using EdgeMap = std::unordered_map<size_t, std::unordered_set<size_t>>;
 
     const SolidMesh *m_firstMesh;
     const SolidMesh *m_secondMesh;
     std::vector<Vector3> m_newVertices;
     std::vector<std::array<size_t, 3>> m_newTriangles;
     bool m_repeatedHalfEdge = false;  // some directed edge belongs to two new triangles
-    bool m_referencePairOrder = false;
+    bool m_referencePairOrder = true;
     bool m_robustWeld = false;
     // the weld of the current attempt: PositionKey cells; crossings (keys);
     // or PositionKey cells with the edge hints of every endpoint in the cell

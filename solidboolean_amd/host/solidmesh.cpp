@@ -3,12 +3,19 @@
 #include <cstdint>
 #include <stdexcept>
 
+#include "referencetree.h"
 #include "sbhost_log.h"
+
+SolidMesh::SolidMesh() = default;
 
 SolidMesh::~SolidMesh() { release(); }
 
 void SolidMesh::release()
 {
+    if (m_refTreeBuild.valid())
+        m_refTreeBuild.wait();
+    m_refTreeBuild = {};
+    m_refTree.reset();
     delete m_triangleNormals;
     m_triangleNormals = nullptr;
     delete m_boxes;
@@ -48,9 +55,25 @@ void SolidMesh::prepare()
             tri[3 * i + k] = (uint32_t)t[k];
         }
     }
-    sbhost::GpuLock lock(sbhost::gpuMutex());
-    sbhost::check(sbg_mesh_create(sbhost::gpu(), xyz.data(), nv, tri.data(), nt, &m_gpu), "sbg_mesh_create");
-    sbhost::check(sbg_mesh_prepare(m_gpu), "sbg_mesh_prepare");
+    {
+        sbhost::GpuLock lock(sbhost::gpuMutex());
+        sbhost::check(sbg_mesh_create(sbhost::gpu(), xyz.data(), nv, tri.data(), nt, &m_gpu), "sbg_mesh_create");
+        sbhost::check(sbg_mesh_prepare(m_gpu), "sbg_mesh_prepare");
+    }
+    // the reference's tree, for the pair order, on a host thread while the
+    // GPU prepares and searches (the arrays are the caller's and stay alive)
+    const std::vector<Vector3> *v = m_vertices;
+    const std::vector<std::vector<size_t>> *t = m_triangles;
+    m_refTreeBuild = std::async(std::launch::async, [v, t]() {
+        return std::unique_ptr<sbhost::ReferenceTree>(new sbhost::ReferenceTree(*v, *t));
+    });
+}
+
+const sbhost::ReferenceTree *SolidMesh::referenceTree() const
+{
+    if (!m_refTree && m_refTreeBuild.valid())
+        m_refTree = m_refTreeBuild.get();
+    return m_refTree.get();
 }
 
 const std::vector<Vector3> *SolidMesh::triangleNormals() const
@@ -97,24 +120,4 @@ const AxisAlignedBoudingBoxTree *SolidMesh::axisAlignedBoundingBoxTree() const
     if (!m_tree)
         m_tree = new AxisAlignedBoudingBoxTree(this);
     return m_tree;
-}
-
-void AxisAlignedBoudingBoxTree::test(const AxisAlignedBoudingBoxTree *other,
-                                     std::vector<std::pair<size_t, size_t>> *pairs) const
-{
-    size_t n = 0;
-    std::vector<uint32_t> ab;
-    {
-        sbhost::GpuLock lock(sbhost::gpuMutex());
-        sbg_pairs p = nullptr;
-        sbhost::check(sbg_find_pairs(m_mesh->gpuMesh(), other->m_mesh->gpuMesh(), &p), "sbg_find_pairs");
-        sbg_pairs_count(p, &n);
-        ab.resize(2 * n);
-        int rc = sbg_pairs_copy(p, ab.data());
-        sbg_pairs_destroy(p);
-        sbhost::check(rc, "sbg_pairs_copy");
-    }
-    pairs->reserve(pairs->size() + n);
-    for (size_t i = 0; i < n; ++i)
-        pairs->push_back({ab[2 * i], ab[2 * i + 1]});
 }

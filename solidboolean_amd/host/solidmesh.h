@@ -6,34 +6,23 @@
 // copy results back on first use.
 #pragma once
 #include <cstddef>
+#include <future>
+#include <memory>
 #include <utility>
 #include <vector>
 
 #include "axisalignedboundingbox.h"
+#include "axisalignedboundingboxtree.h"
 #include "sbgpu.h"
 #include "vector3.h"
 
-class SolidMesh;
-
-// The mesh's bounding-volume hierarchy.  In the reference this is a host
-// pointer tree (src/axisalignedboundingboxtree.h); here it is a linear BVH in
-// device memory, so instead of walking nodes a caller asks it for the
-// candidate pairs against another mesh's tree.
-class AxisAlignedBoudingBoxTree {
-public:
-    explicit AxisAlignedBoudingBoxTree(const SolidMesh *mesh) : m_mesh(mesh) {}
-    // every (a in this mesh, b in other) whose triangle boxes closed-overlap,
-    // sorted by (a, b) (the set of axisalignedboundingboxtree.h:54-95)
-    void test(const AxisAlignedBoudingBoxTree *other, std::vector<std::pair<size_t, size_t>> *pairs) const;
-    const SolidMesh *mesh() const { return m_mesh; }
-
-private:
-    const SolidMesh *m_mesh;
-};
+namespace sbhost {
+class ReferenceTree;
+}
 
 class SolidMesh {
 public:
-    SolidMesh() = default;
+    SolidMesh();
     ~SolidMesh();
     SolidMesh(const SolidMesh &) = delete;
     SolidMesh &operator=(const SolidMesh &) = delete;
@@ -52,6 +41,11 @@ public:
 
     // the device mesh (null before prepare())
     sbg_mesh gpuMesh() const { return m_gpu; }
+    // The order tables of the reference's own tree over this mesh (the order
+    // its pair search reports pairs in).  prepare() starts building them on a
+    // host thread beside the GPU work; this waits for that build.  Null
+    // before prepare().
+    const sbhost::ReferenceTree *referenceTree() const;
 
 private:
     const std::vector<Vector3> *m_vertices = nullptr;
@@ -60,5 +54,7 @@ private:
     mutable std::vector<Vector3> *m_triangleNormals = nullptr;
     mutable std::vector<AxisAlignedBoudingBox> *m_boxes = nullptr;
     mutable AxisAlignedBoudingBoxTree *m_tree = nullptr;
+    mutable std::future<std::unique_ptr<sbhost::ReferenceTree>> m_refTreeBuild;
+    mutable std::unique_ptr<sbhost::ReferenceTree> m_refTree;
     void release();
 };

@@ -35,7 +35,8 @@ EXPORTS = [
     "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_classify_pair", "sbg_components",
     "sbg_half_edges",
     "sbg_context_set_timing",
-    "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_bench_rays", "sbg_calibrate_copy",
+    "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_bench_rays", "sbg_calibrate_copy", "sbg_ray_certify",
+    "sbg_ray_hits_apart",
     "sbg_context_set_tuning", "sbg_context_get_tuning", "sbg_autotune_pair",
     "sbg_adjoint",
     "sbg_batch_create", "sbg_batch_set_vertices", "sbg_batch_set_tuning", "sbg_batch_get_tuning", "sbg_batch_run",
@@ -130,6 +131,8 @@ def load(path: str = LIB_PATH):
         "sbg_bench_tri_tri": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)],
         "sbg_bench_rays": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double), _U64],
         "sbg_calibrate_copy": [_P, _SZ, C.c_int, C.POINTER(C.c_double)],
+        "sbg_ray_certify": [_D, _P, _D, _SZ, _P, _F, _F],
+        "sbg_ray_hits_apart": [_F, _F, _F, _F, _SZ, _U8],
         "sbg_context_set_tuning": [_P, C.POINTER(Tuning)],
         "sbg_context_get_tuning": [_P, C.POINTER(Tuning)],
         "sbg_autotune_pair": [_P, _P, C.c_int, C.POINTER(Tuning), C.POINTER(C.c_double)],
@@ -162,6 +165,31 @@ def _check(code: int):
 
 def _ptr(a: np.ndarray | None):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def ray_certify(pts, axis, tris9):
+    """The ray kernels' FP32 certificate on the host (sbg_ray_certify):
+    (verdict, t, et) per candidate."""
+    L = load()
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    n = len(pts)
+    ax = np.ascontiguousarray(np.broadcast_to(np.asarray(axis, np.int32), (n,)))
+    tr = np.ascontiguousarray(tris9, np.float64).reshape(n, 9)
+    v = np.zeros(max(n, 1), np.int32)
+    t = np.zeros(max(n, 1), np.float32)
+    et = np.zeros(max(n, 1), np.float32)
+    _check(L.sbg_ray_certify(pts.reshape(-1) if n else np.zeros(3), ax.ctypes.data_as(C.c_void_p),
+                             tr.reshape(-1) if n else np.zeros(9), n, v.ctypes.data_as(C.c_void_p), t, et))
+    return v[:n], t[:n], et[:n]
+
+
+def ray_hits_apart(ta, ea, tb, eb):
+    L = load()
+    arr = [np.ascontiguousarray(x, np.float32).reshape(-1) for x in (ta, ea, tb, eb)]
+    n = len(arr[0])
+    out = np.zeros(max(n, 1), np.uint8)
+    _check(L.sbg_ray_hits_apart(*[a if n else np.zeros(1, np.float32) for a in arr], n, out))
+    return out[:n].astype(bool)
 
 
 def device_count() -> int:

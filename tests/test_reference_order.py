@@ -49,6 +49,30 @@ def test_hit_order_large():
     assert np.array_equal(c[perm], ref)
 
 
+@pytest.mark.parametrize("name", FULL)
+def test_reference_signature_tree_search(name):
+    """AxisAlignedBoudingBoxTree as the reference's callers use it (public
+    constructor, root(), test(first, second, secondBoxes, pairs)) on the host:
+    the compiled reference's depth-first candidate list, in its order."""
+    g = load_golden(name)
+    got = dropin.tree_pairs(g["va"], g["ta"], g["vb"], g["tb"])
+    assert np.array_equal(got, g["pairs"].astype(np.int64))
+
+
+def test_reference_signature_tree_search_subdivision6():
+    g = load_golden("sub6_s1")
+    a, b = mg.icosphere_pair(6, 1, (0.5, 0.2, 0.1))
+    got = dropin.tree_pairs(a.vertices, a.triangles, b.vertices, b.triangles)
+    assert len(got) == int(g["n_pairs"])
+    import hashlib
+    digest = hashlib.sha256(np.ascontiguousarray(canonical(got).astype("<u4")).tobytes()).hexdigest()
+    assert digest == str(g["pair_digest"])
+    # the hits among them come in the reference's order
+    hits = {(int(x), int(y)) for x, y in g["hit_pairs"]}
+    order = [tuple(p) for p in got.tolist() if tuple(p) in hits]
+    assert np.array_equal(np.array(order), g["hit_pairs"].astype(np.int64))
+
+
 def test_bad_index_is_an_error():
     g = load_golden("small_s1")
     with pytest.raises(ValueError, match="out of range"):
