@@ -129,6 +129,19 @@ int sbg_tri_tri_batch(sbg_context ctx, const double *a9, const double *b9, size_
  * (n x 3, may be NULL) gets each ray's parity; inside gets the vote. */
 int sbg_classify_points(sbg_mesh target, const double *pts, size_t n, uint8_t *axis_inside, uint8_t *inside);
 
+/* sbg_classify_points with no device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results as sbg_classify_points for n <= 65536 points (larger n fails with
+ * SBG_ERR_INVALID).  Every buffer its kernels use beyond the caller's inputs
+ * and outputs lives in one fixed device array of about 8.4 MB per GPU, held
+ * until the process ends; it keeps the last call's contents.  Calls that
+ * overlap on one GPU (from two threads or contexts) race on that array.  The
+ * target's ray grid must already exist: classify against the target once
+ * with an allocating call first (SBG_ERR_STATE otherwise). */
+int sbg_classify_points_noalloc(sbg_mesh target, const double *pts, size_t n, uint8_t *axis_inside,
+                                uint8_t *inside);
+
 /* Every triangle of `self` that no intersecting pair of `pairs` touches is
  * classified by its centroid (v0 + v1 + v2) / 3.0 against `other`
  * (src/solidboolean.cpp:497-508).  side = 0 when self is the pairs' first mesh,

@@ -896,6 +896,25 @@ void launch_rays_ordered(const GridSet &gs, const double *tri64, const double *q
     rays<true>(gs, tri64, q4, order, nullptr, n, allAxes, axisBits, lists, counters, s);
 }
 
+// per-axis bits [axis][n] -> per-point triples [n][3]
+__global__ __launch_bounds__(256) void k_interleave_axes(const uint8_t *__restrict__ bits, uint32_t n,
+                                                         uint8_t *__restrict__ out)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    out[3 * (size_t)i] = bits[i];
+    out[3 * (size_t)i + 1] = bits[(size_t)n + i];
+    out[3 * (size_t)i + 2] = bits[2 * (size_t)n + i];
+}
+
+void launch_interleave_axes(const uint8_t *axisBits, uint32_t n, uint8_t *out, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_interleave_axes, dim3((n + 255) / 256), dim3(256), 0, s, axisBits, n, out);
+}
+
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s)
 {
     if (n == 0)

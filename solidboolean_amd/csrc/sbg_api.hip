@@ -5,11 +5,18 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "sbg_host.h"
 #include "sbg_rayfilter.h"
+
+namespace sbg {
+// resolves this device's copy of the allocation-free scratch array (the
+// device must be current); defined beside the array at the end of this file
+int noalloc_attach(int device);
+}  // namespace sbg
 
 using namespace sbg;
 
@@ -172,6 +179,7 @@ int sbg_context_create(int device, sbg_context *out)
     if (device < 0 || device >= n)
         return fail(SBG_ERR_INVALID, "device %d out of range (%d devices)", device, n);
     HIP_TRY(hipSetDevice(device));
+    TRY(noalloc_attach(device));
     sbg_context c = new sbg_context_s;
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1734,6 +1742,118 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, int it
     (void)hipEventDestroy(e1);
     *avg_ms = (double)ms / iters;
     return check_launch("bench tri_tri");
+}
+
+}  // extern "C"
+
+// ---- allocation-free classification ----------------------------------------
+// sbg_classify_points_noalloc keeps every buffer its kernels use beyond the
+// caller's inputs and outputs in one file-scope device array.  Each device
+// holds its own copy at its own address, resolved once per device by
+// hipGetSymbolAddress when a context is created there; host code uses only
+// that resolved address (the array's name is a host-side stand-in).
+namespace sbg {
+namespace {
+constexpr size_t kNoAllocMaxPoints = (size_t)1 << 16;
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+// layout for up to kNoAllocMaxPoints points: the points (3 doubles each),
+// per-axis ray bits (3 n + 4), votes (n), per-point axis triples (3 n), four
+// counters, and the deferred-ray lists (ray_list_u32(n) = 25 n u32)
+constexpr size_t kNaPts = 0;
+constexpr size_t kNaBits = align256(kNaPts + 24 * kNoAllocMaxPoints);
+constexpr size_t kNaVotes = align256(kNaBits + 3 * kNoAllocMaxPoints + 4);
+constexpr size_t kNaAxes = align256(kNaVotes + kNoAllocMaxPoints);
+constexpr size_t kNaCounters = align256(kNaAxes + 3 * kNoAllocMaxPoints);
+constexpr size_t kNaLists = align256(kNaCounters + 16);
+constexpr size_t kNoAllocBytes = align256(kNaLists + 4 * 25 * kNoAllocMaxPoints);
+constexpr int kNoAllocDevices = 64;
+
+std::mutex g_scratchMu;
+uint8_t *g_scratchPtr[kNoAllocDevices] = {};
+}  // namespace
+}  // namespace sbg
+
+static __device__ __align__(256) unsigned char scratch[sbg::kNoAllocBytes];
+
+namespace sbg {
+int noalloc_attach(int device)
+{
+    if (device < 0 || device >= kNoAllocDevices)
+        return fail(SBG_ERR_INVALID, "device %d beyond the %d the library tracks", device, kNoAllocDevices);
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (g_scratchPtr[device])
+        return SBG_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return fail(SBG_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratchPtr[device] = static_cast<uint8_t *>(p);
+    return SBG_OK;
+}
+}  // namespace sbg
+
+extern "C" {
+
+int sbg_classify_points_noalloc(sbg_mesh target, const double *pts, size_t n, uint8_t *axis_inside,
+                                uint8_t *inside)
+{
+    if (!target || (n && !pts))
+        return fail(SBG_ERR_INVALID, "null argument");
+    if (!target->prepared)
+        return fail(SBG_ERR_STATE, "mesh not prepared");
+    if (n > kNoAllocMaxPoints)
+        return fail(SBG_ERR_INVALID, "%zu points exceed the allocation-free limit of %zu", n, kNoAllocMaxPoints);
+    sbg_context c = target->ctx;
+    TRY(use(c));
+    uint8_t *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratchMu);
+        base = g_scratchPtr[c->device];
+    }
+    if (!base)
+        return fail(SBG_ERR_STATE, "scratch array of device %d not resolved", c->device);
+    if (!n)
+        return SBG_OK;
+    if (target->nt && !(target->gridReady && target->gridDensity == c->tune.grid_density))
+        return fail(SBG_ERR_STATE, "the target's ray grid is not built: classify against it once with an "
+                                   "allocating call (sbg_classify_points) first");
+    const hipStream_t s = c->stream;
+    MeshUse mu(target);
+    if (target->gridStream && target->gridStream != s)
+        HIP_TRY(hipStreamWaitEvent(s, target->gridDone, 0));
+    double *dpts = reinterpret_cast<double *>(base + kNaPts);
+    uint8_t *bits = base + kNaBits, *votes = base + kNaVotes, *axes = base + kNaAxes;
+    uint32_t *counters = reinterpret_cast<uint32_t *>(base + kNaCounters);
+    uint32_t *lists = reinterpret_cast<uint32_t *>(base + kNaLists);
+    const bool allAxes = axis_inside != nullptr;
+    uint32_t *host = c->pinned + kRayPinned;  // page-locked words made with the context
+    host[0] = 0;
+    HIP_TRY(hipMemcpyAsync(dpts, pts, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    if (target->nt == 0) {
+        launch_fill_u32(reinterpret_cast<uint32_t *>(bits), 0u, (3 * n + 3) / 4, s);
+    } else {
+        launch_fill_u32(counters, 0u, 4, s);  // kernels, not memsets: the call may be captured
+        launch_rays(grid_set(target), target->tri64.p, dpts, nullptr, nullptr, (uint32_t)n, allAxes, bits, lists,
+                    counters, s);
+        HIP_TRY(hipMemcpyAsync(host, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    }
+    launch_vote(bits, nullptr, (uint32_t)n, votes, s);
+    if (allAxes)
+        launch_interleave_axes(bits, (uint32_t)n, axes, s);
+    TRY(check_launch("allocation-free rays"));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (host[0] & 2u) {
+        target->gridReady = false;  // the next allocating call rebuilds it at its exact size
+        return fail(SBG_ERR_STATE, "the target's ray grid was truncated: classify against it once with an "
+                                   "allocating call (sbg_classify_points) first");
+    }
+    if (host[0])
+        return fail(SBG_ERR_OVERFLOW, "a ray has more distinct hits than its key list holds");
+    if (inside)
+        HIP_TRY(hipMemcpy(inside, votes, n, hipMemcpyDeviceToHost));
+    if (axis_inside)
+        HIP_TRY(hipMemcpy(axis_inside, axes, 3 * n, hipMemcpyDeviceToHost));
+    return SBG_OK;
 }
 
 }  // extern "C"

@@ -147,6 +147,8 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
 // truncated, cells[ncells] > cap).  tri64 records carry the FP64 normal in
 // their pad slots.
 size_t ray_list_u32(size_t n);
+// per-axis bits [axis][n] -> [n][3]
+void launch_interleave_axes(const uint8_t *axisBits, uint32_t n, uint8_t *out, hipStream_t s);
 void launch_rays(const GridSet &gs, const double *tri64, const double *pts, const uint32_t *order,
                  const uint8_t *skip, uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists,
                  uint32_t *counters, hipStream_t s);

@@ -32,7 +32,8 @@ EXPORTS = [
     "sbg_context_stream", "sbg_mesh_create", "sbg_mesh_prepare", "sbg_mesh_destroy", "sbg_mesh_counts",
     "sbg_mesh_normals", "sbg_mesh_boxes", "sbg_mesh_bvh", "sbg_find_pairs", "sbg_pairs_create", "sbg_pairs_count",
     "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
-    "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_triangles", "sbg_classify_pair", "sbg_components",
+    "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_points_noalloc", "sbg_classify_triangles",
+    "sbg_classify_pair", "sbg_components",
     "sbg_half_edges",
     "sbg_context_set_timing",
     "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_bench_rays", "sbg_calibrate_copy", "sbg_ray_certify",
@@ -122,6 +123,7 @@ def load(path: str = LIB_PATH):
         "sbg_pairs_hits": [_P, _PSZ, _P, _P],
         "sbg_tri_tri_batch": [_P, _D, _D, _SZ, _U8, _D],
         "sbg_classify_points": [_P, _D, _SZ, _P, _U8],
+        "sbg_classify_points_noalloc": [_P, _D, _SZ, _P, _P],
         "sbg_classify_triangles": [_P, _P, _P, C.c_int, _U8],
         "sbg_classify_pair": [_P, _P, _P, _U8, _U8],
         "sbg_components": [_P, _P, _U32],
