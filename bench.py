@@ -15,12 +15,14 @@ One step = the whole GPU hot path on mesh data already in HBM:
 the Guigue-Devillers kernel (FP32-filtered), i.e. candidates per step divided
 by the kernel's HIP-event time in a repeat of the timed steps with stage
 events on.  `ms_per_step` is the second: end-to-end hot-path milliseconds,
-wall clock over the timed steps (no events recorded).  The plain-FP64 kernel, the
+wall clock over the timed steps (no events recorded).  The plain-FP64 step
+(FP64 Guigue-Devillers and FP64 ray tests) and the filtered batched step, the
 saturated (2^24-test) kernel rate, the stage breakdown, the roofline of the
 step's dominant kernel (`roofline`: the x/y ray kernel, timed alone with HIP
-events) and of the GD kernel (`roofline_gd`), and the CPU baseline (the
-compiled reference, or the oracle port, on this host) are reported in the
-same JSON line.
+events) and of the GD kernel over the 4,096-pair batch's candidates
+(`batch.roofline_gd`), the whole boolean through the drop-in, and the CPU
+baseline (the compiled reference, or the oracle port, on this host) are
+reported in the same JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -41,9 +43,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_VECTOR_PEAK_TFLOPS = 78.6
 
-# algorithmic bytes ("Kernels and their bounds")
-TRI64_B = 96  # FP64 triangle record
-TRI32_B = 48  # FP32 triangle record
+# algorithmic bytes in the units of the measurement plan: a triangle's
+# coordinates are 72 B in FP64 and 36 B in FP32 (the records the kernels read
+# are 96 B and 48 B: a normal or error-bound pads ride along)
+TRI64_B = 72
+TRI32_B = 36
 PAIR_B = 8
 STATUS_B = 1
 SEG_B = 48
@@ -61,22 +65,23 @@ def sha(*arrays) -> str:
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_1310720_pair.md")
-RAYS_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_rays_1310720_pair_density{}.md")
+RAYS_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_rays_1310720_pair.md")
+GD_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_gd_4096_batch.md")
 
-# algorithmic bytes of the x/y ray kernel (k_rays_grid): per lane the query
-# record (x, y, z, tag) and its output index; per cast ray the cell range and
-# the result bit; per scanned grid entry its code words; per entry passing the
-# code test its triangle index and FP64 record; per deferred ray a list slot
-RAY_LANE_B = 32 + 4
-RAY_CAST_B = 8 + 1
+# algorithmic bytes of the x/y ray kernel (k_rays_grid) in the units of the
+# ray-parity row of the measurement plan: per ray its 24-B query point and
+# 1-B result; per candidate visited (a scanned grid entry, which stands in for
+# a node visit) its 8 B of box codes; per triangle tested its coordinates,
+# 72 B in FP64 or 36 B in FP32
+RAY_QUERY_B = 24 + 1
 RAY_ENTRY_B = 8
-RAY_PASS_B = 4 + TRI64_B
-RAY_DEFER_B = 4
+RAY_TEST64_B = 72
+RAY_TEST32_B = 36
 
 
-def ray_bytes(lanes: int, work: dict) -> float:
-    return (lanes * RAY_LANE_B + work["rays"] * RAY_CAST_B + work["scanned"] * RAY_ENTRY_B
-            + work["passed"] * RAY_PASS_B + work["deferred"] * RAY_DEFER_B)
+def ray_bytes(work: dict, filtered: bool) -> float:
+    return (work["rays"] * RAY_QUERY_B + work["scanned"] * RAY_ENTRY_B
+            + work["passed"] * (RAY_TEST32_B if filtered else RAY_TEST64_B))
 
 
 def measured_traffic(kernels, grid: int, profile: str = TRAFFIC_PROFILE):
@@ -125,7 +130,7 @@ def cpu_baseline(a, b, pairs_ref, budget_s: float = 12.0):
 
         def classify(mesh, other, inter, n_sample):
             keep = np.nonzero(~inter)[0]
-            sample = keep[:: max(1, len(keep) // n_sample)]
+            sample = keep if n_sample is None else keep[:: max(1, len(keep) // n_sample)]
             c = (mesh.vertices[mesh.triangles[sample, 0]] + mesh.vertices[mesh.triangles[sample, 1]]
                  + mesh.vertices[mesh.triangles[sample, 2]]) / 3.0
             t = 0.0
@@ -150,7 +155,7 @@ def cpu_baseline(a, b, pairs_ref, budget_s: float = 12.0):
 
         def classify(mesh, other, inter, n_sample):
             keep = np.nonzero(~inter)[0]
-            sample = keep[:: max(1, len(keep) // n_sample)]
+            sample = keep if n_sample is None else keep[:: max(1, len(keep) // n_sample)]
             c = (mesh.vertices[mesh.triangles[sample, 0]] + mesh.vertices[mesh.triangles[sample, 1]]
                  + mesh.vertices[mesh.triangles[sample, 2]]) / 3.0
             t0 = time.perf_counter()
@@ -161,8 +166,11 @@ def cpu_baseline(a, b, pairs_ref, budget_s: float = 12.0):
     ib = np.zeros(b.nt, bool)
     ia[pairs[hit, 0]] = True
     ib[pairs[hit, 1]] = True
-    cls_a, sa = classify(a, rb, ia, 20000)
-    cls_b, sb = classify(b, ra, ib, 20000)
+    # every non-intersected triangle of both meshes with the reference's code
+    # (about 11 s on one core); the oracle port, much slower, is sampled
+    n_sample = None if kind == "reference" else 20000
+    cls_a, sa = classify(a, rb, ia, n_sample)
+    cls_b, sb = classify(b, ra, ib, n_sample)
     total_s = prep_s + search_s + gd_s + cls_a + cls_b
     return {
         "value": len(pairs) / gd_s,
@@ -174,24 +182,33 @@ def cpu_baseline(a, b, pairs_ref, budget_s: float = 12.0):
                      "classify": (cls_a + cls_b) * 1e3},
         "sample": (f"same 1.31M-triangle pair on this host, 1 thread: full prepare of both meshes, full candidate "
                    f"search, Guigue-Devillers over all {len(pairs)} candidates (median of repeats), and the "
-                   f"3-ray vote of {sa}+{sb} sampled non-intersected triangles scaled to all of them"),
+                   f"3-ray vote of {sa}+{sb} non-intersected triangles"
+                   + (" (all of them)" if n_sample is None else " sampled, scaled to all of them")),
         "host_cpu": _cpu_model(),
         "nproc": os.cpu_count(),
     }
 
 
 def fp64_vs_batched(ctx, a, b, step, N, steps: int, graph: bool = True) -> dict:
-    """The 1.31M-triangle pair end to end two ways: the plain FP64 path (single
-    pair API, FP64 Guigue-Devillers) and the FP32-filtered batched path (the
+    """The 1.31M-triangle pair end to end three ways: the plain FP64 path
+    (single-pair API, FP64 Guigue-Devillers and FP64 ray tests), the
+    FP32-filtered single-pair path, and the FP32-filtered batched path (the
     batch API with this one pair, replaying its captured graph)."""
-    for _ in range(2):
-        step(N.FP64)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        step(N.FP64)
-    ctx.synchronize()
-    plain = (time.perf_counter() - t0) / steps * 1e3
+    def timed(precision):
+        saved = ctx.tuning()
+        ctx.set_tuning(**{**saved, "ray_precision": precision})
+        for _ in range(2):
+            step(precision)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(precision)
+        ctx.synchronize()
+        ctx.set_tuning(**saved)
+        return (time.perf_counter() - t0) / steps * 1e3
+
+    plain = timed(N.FP64)
+    single = timed(N.FP32_FILTERED)
     batch = ctx.batch([(a, b)])
     if not graph:
         batch.set_tuning(use_graph=0)
@@ -201,10 +218,12 @@ def fp64_vs_batched(ctx, a, b, step, N, steps: int, graph: bool = True) -> dict:
     for _ in range(steps):
         dev.append(batch.run(N.FP32_FILTERED))
     wall = (time.perf_counter() - t0) / steps * 1e3
-    out = {"plain_fp64_ms": plain, "filtered_batched_ms": wall, "filtered_batched_device_ms": float(np.mean(dev)),
-           "speedup": plain / wall}
+    out = {"plain_fp64_ms": plain, "filtered_ms": single, "filtered_batched_ms": wall,
+           "filtered_batched_device_ms": float(np.mean(dev)), "speedup_filtered": plain / single,
+           "speedup_filtered_batched": plain / wall}
     batch.close()
-    log(f"1.31M pair: plain FP64 {plain:.2f} ms, filtered batched {wall:.2f} ms (device {out['filtered_batched_device_ms']:.2f})")
+    log(f"1.31M pair: plain FP64 {plain:.2f} ms, filtered {single:.2f} ms, filtered batched {wall:.2f} ms "
+        f"(device {out['filtered_batched_device_ms']:.2f})")
     return out
 
 
@@ -361,6 +380,37 @@ def batch_section(ctx, mg, N, steps: int, graph: bool = True) -> dict:
         res[name] = (wall, float(np.mean(dev)))
     ncand, nhits = batch.counts()
     wall, dev = res["filtered"]
+    # the Guigue-Devillers kernel alone over the batch's 1.15M distinct
+    # candidates (the last run's list): its roofline in the plan's units
+    gd = {}
+    for name, prec in (("filtered", N.FP32_FILTERED), ("fp64", N.FP64)):
+        gd[name] = batch.bench_tri_tri(prec, 50)
+    gms, gst = gd["filtered"]
+    gbytes = gd_bytes(ncand, gst["hits"], gst["fallback_pairs"], 1)
+    gtraffic = measured_traffic(("k_gd_filter", "k_gd_list"), ncand, GD_TRAFFIC_PROFILE)
+    roof_gd = {
+        "kernel": "k_gd_filter + k_gd_list (FP32-filtered Guigue-Devillers) over the 4,096-pair batch's candidates",
+        "bound": "hbm",
+        "achieved": gbytes / (gms * 1e-3) / 1e9,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": gbytes / (gms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "traffic": gtraffic,
+        "traffic_frac": gtraffic / (gms * 1e-3) / 1e9 / HBM_PEAK_GBS if gtraffic else None,
+        "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/hbm_traffic_pmc_gd_4096_batch.md",
+        "bytes_per_launch": gbytes,
+        "bytes_units": "per pair 2 x 36 B FP32 coordinates + 8 B ids + 1 B status; per FP64 re-run 2 x 72 B; "
+                       "per hit 48 B segment",
+        "avg_launch_ms": gms,
+        "pairs": ncand,
+        "fallback_pairs": gst["fallback_pairs"],
+        "fp64_avg_launch_ms": gd["fp64"][0],
+        "filtered_speedup_vs_fp64": gd["fp64"][0] / gms,
+        "tests_per_s": ncand / (gms * 1e-3),
+        "fp64_tests_per_s": ncand / (gd["fp64"][0] * 1e-3),
+    }
+    log(f"batch GD kernel: filtered {gms * 1e3:.1f} us, fp64 {gd['fp64'][0] * 1e3:.1f} us per launch over {ncand} "
+        f"candidates, fallback {gst['fallback_pairs']}")
     out = {
         "workload": "4,096 icosphere pairs, levels 1-6 with P ~ 4^-level, radii U[0.5, 2], offsets U[-0.8, 0.8]^3 "
                     "x radius, seeded rotations (seed 1), " + ("one captured graph per run" if graph else
@@ -376,6 +426,7 @@ def batch_section(ctx, mg, N, steps: int, graph: bool = True) -> dict:
         "fp64_ms_per_batch": res["fp64"][0],
         "filtered_speedup_vs_fp64": res["fp64"][0] / wall,
         "tuning": tuned,
+        "roofline_gd": roof_gd,
     }
     batch.close()
     log(f"batch: {wall:.2f} ms per batch (device {dev:.2f}), fp64 {res['fp64'][0]:.2f} ms")
@@ -409,6 +460,9 @@ def main():
     ap.add_argument("--rays-only", type=int, default=0,
                     help="profiling aid: one step, then N launches of the x/y ray kernel and a 1 GiB copy")
     ap.add_argument("--grid-density", type=int, default=0, help="grid density for --step-only / --rays-only")
+    ap.add_argument("--batch-only", type=int, default=0,
+                    help="profiling aid: the 4,096-pair batch, N runs per precision, N GD launches over its "
+                         "candidates per precision, and a 1 GiB copy")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the hot path of one mesh pair "
@@ -417,6 +471,22 @@ def main():
 
     from solidboolean_amd import meshgen as mg
     from solidboolean_amd import native as N
+
+    if args.batch_only:
+        ctx = N.Context(0)
+        batch = ctx.batch(mg.batch_pairs(4096, seed=1))
+        if args.no_graph:
+            batch.set_tuning(use_graph=0)
+        for name, prec in (("filtered", N.FP32_FILTERED), ("fp64", N.FP64)):
+            ms = [batch.run(prec) for _ in range(args.batch_only)]
+            gms, gst = batch.bench_tri_tri(prec, args.batch_only)
+            log(f"batch-only {name}: {np.median(ms):.3f} ms per run (device), GD {gms * 1e3:.1f} us per launch over "
+                f"{gst['tested']} candidates, fallback {gst['fallback_pairs']}")
+        cms = ctx.calibrate_copy(1 << 30, 3)
+        log(f"copy 1 GiB {cms:.3f} ms")
+        batch.close()
+        ctx.close()
+        return
 
     t0 = time.perf_counter()
     a, b = mg.large_pair(args.seed, args.level)
@@ -451,10 +521,13 @@ def main():
         ctx.set_tuning(grid_density=args.grid_density)
     if args.rays_only:
         P, _, _ = step()
-        ms, work = A.bench_rays(B, P, 0, args.rays_only)
+        for rp in (N.FP64, N.FP32_FILTERED):
+            ctx.set_tuning(ray_precision=rp)
+            ms, work = A.bench_rays(B, P, 0, args.rays_only)
+            log(f"rays-only: grid density {ctx.tuning()['grid_density']}, ray precision {rp}, x/y ray kernel "
+                f"{ms * 1e3:.1f} us per launch, {work}")
         cms = ctx.calibrate_copy(1 << 30, 5)
-        log(f"rays-only: grid density {ctx.tuning()['grid_density']}, x/y ray kernel {ms * 1e3:.1f} us per launch, "
-            f"{work}; copy 1 GiB {cms:.3f} ms")
+        log(f"copy 1 GiB {cms:.3f} ms")
         return
 
     if args.step_only:
@@ -462,6 +535,8 @@ def main():
             step()
             ctx.synchronize()
             log(f"step {i + 1}/{args.step_only} done")
+        cms = ctx.calibrate_copy(1 << 30, 3)
+        log(f"copy 1 GiB {cms:.3f} ms")
         return
 
     for i in range(args.warmup):
@@ -518,39 +593,41 @@ def main():
     # classification stage, timed alone with HIP events on the context stream
     dom = max(("records", "sort", "bvh", "traverse", "pair_sort", "tri_tri", "compact", "classify", "grid"),
               key=lambda k: stages[k])
-    ray_ms, ray_work = A.bench_rays(B, P, 0, 20)
     lanes = 2 * A.nt
-    rbytes = ray_bytes(lanes, ray_work)
     density = ctx.tuning()["grid_density"]
+    rays = {}
+    for name, rp in (("fp64", N.FP64), ("filtered", N.FP32_FILTERED)):
+        saved = ctx.tuning()
+        ctx.set_tuning(**{**saved, "ray_precision": rp})
+        rays[name] = A.bench_rays(B, P, 0, 20)
+        ctx.set_tuning(**saved)
+    ray_ms, ray_work = rays["filtered"]
+    rbytes = ray_bytes(ray_work, True)
+    traffic = measured_traffic(("void k_rays_grid<true, false, true>",), lanes, RAYS_TRAFFIC_PROFILE)
     roof_rays = {
-        "kernel": "k_rays_grid (x and y rays of mesh A's 1,310,720 triangle centroids into mesh B's grid)",
+        "kernel": "k_rays_grid, FP32-filtered (x and y rays of mesh A's 1,310,720 triangle centroids into mesh B's "
+                  "grid)",
         "bound": "hbm",
         "achieved": rbytes / (ray_ms * 1e-3) / 1e9,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": rbytes / (ray_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "traffic": measured_traffic(("void k_rays_grid<true, false>",), lanes, RAYS_TRAFFIC_PROFILE.format(density)),
-        "traffic_source": ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, "
-                           f"profiles/hbm_traffic_pmc_rays_1310720_pair_density{density}.md"),
+        "traffic": traffic,
+        "traffic_frac": traffic / (ray_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if traffic else None,
+        "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/hbm_traffic_pmc_rays_1310720_pair.md",
         "bytes_per_launch": rbytes,
+        "bytes_units": "25 B per ray (query + result), 8 B per grid entry scanned, 36 B per FP32 triangle test",
         "avg_launch_ms": ray_ms,
         "work_per_launch": ray_work,
+        "rays_to_fp64": ray_work["to_fp64"],
+        "rays_to_fp64_uncertain": ray_work["to_fp64"] - rays["fp64"][1]["to_fp64"],
+        "fallback_rate": (ray_work["to_fp64"] - rays["fp64"][1]["to_fp64"]) / max(1, ray_work["rays"]),
+        "fp64_kernel_ms": rays["fp64"][0],
+        "fp64_kernel_bytes": ray_bytes(rays["fp64"][1], False),
         "grid_density": density,
     }
-    log(f"x/y ray kernel: {ray_ms * 1e3:.1f} us per launch, {ray_work}, {rbytes / 1e6:.0f} MB algorithmic")
-    sat_bytes = gd_bytes(n_sat, hits * reps, fb * reps, 1)
-    roof_gd = {
-        "kernel": "k_tritri<FP32-filtered> (saturated list)",
-        "bound": "hbm",
-        "achieved": sat_bytes / (sat_ms * 1e-3) / 1e9,
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": sat_bytes / (sat_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-        "traffic": measured_traffic(("k_gd_filter", "k_gd_list"), 1048576),
-        "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/hbm_traffic_pmc_1310720_pair.md",
-        "bytes_per_launch": sat_bytes,
-        "avg_launch_ms": sat_ms,
-    }
+    log(f"x/y ray kernel: filtered {ray_ms * 1e3:.1f} us, fp64 {rays['fp64'][0] * 1e3:.1f} us per launch, "
+        f"{ray_work}, {rbytes / 1e6:.0f} MB algorithmic")
     # the same step with the inputs handed over as host arrays (sbg_mesh_create
     # uploads them), i.e. including the PCIe copies
     pt = []
@@ -610,8 +687,8 @@ def main():
             "parallelism": "single GPU",
             "tuning": tuning,
         },
-        "end_to_end_ms": elapsed / args.steps * 1e3,
-        "end_to_end_ms_with_uploads": pcie_ms,
+        "hot_path_step_ms": elapsed / args.steps * 1e3,
+        "hot_path_step_ms_with_uploads": pcie_ms,
         "stage_ms": stages,
         "tri_tri": {
             "filtered_us_per_launch": gd_ms * 1e3,
@@ -629,7 +706,6 @@ def main():
         },
         "dominant_stage": dom,
         "roofline": roof_rays,
-        "roofline_gd": roof_gd,
         "components": comp,
         "copy_kernel_gbs": copy_gbs,
         "parity": parity,
@@ -638,6 +714,9 @@ def main():
         line["stress"] = stress_section(ctx, mg, N, args.steps)
         line["dropin_combine"] = dropin_section(mg, args.steps)
         line["dropin_combine_1310720_robust"] = dropin_large_section(a, b)
+        # the whole boolean (hot path plus host weld, re-triangulation and
+        # face groups, meshes handed over as host arrays) beside the hot path
+        line["whole_boolean_ms"] = line["dropin_combine_1310720_robust"]["ms"]
     if not args.no_batch:
         line["fp64_vs_filtered_batched"] = fp64_vs_batched(ctx, a, b, step, N, args.steps, not args.no_graph)
         line["batch"] = batch_section(ctx, mg, N, args.steps, not args.no_graph)

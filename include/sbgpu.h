@@ -192,10 +192,13 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, int precision, in
 
 /* Measurement aid (no counterpart in the reference): sets up the rays of
  * sbg_classify_triangles(self, other, pairs, side) and runs the fast x/y ray
- * kernel alone `iters` times, HIP events around each launch on the context
- * stream; *avg_ms = mean launch time.  work (4 values, may be null) gets the
- * work of one launch: rays cast, grid entries scanned by the fast path,
- * entries passing the code test, rays deferred to the wave path. */
+ * kernel alone `iters` times (in the context's ray precision), HIP events
+ * around each launch on the context stream; *avg_ms = mean launch time.  work
+ * (5 values, may be null) gets the work of one launch: rays cast, grid
+ * entries scanned by the fast path, entries passing the code test, rays whose
+ * cell is too crowded for the fast path, and the rays the timed kernel handed
+ * to the FP64 wave path (crowded cells, more distinct hits than it holds,
+ * and in the FP32-filtered precision uncertain certificates). */
 int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs pairs, int side, int iters, double *avg_ms,
                    uint64_t *work);
 
@@ -228,6 +231,9 @@ typedef struct sbg_tuning {
     int depth_first;  /* 0: breadth-first levels only, with a host check every 8 levels */
     int early_grids;  /* 1: sbg_find_pairs queues both meshes' ray grids on the second stream, beside
                          the search; 0: each grid is built when a classification first needs it */
+    int ray_precision; /* SBG_PRECISION_FP32_FILTERED (default): each ray candidate is decided from
+                          the FP32 record under a certificate, and a ray with an uncertain candidate
+                          runs in FP64; SBG_PRECISION_FP64: every candidate in FP64.  Same results. */
 } sbg_tuning;
 int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t);
 int sbg_context_get_tuning(sbg_context ctx, sbg_tuning *t);
@@ -278,6 +284,10 @@ int sbg_batch_get_tuning(sbg_batch b, sbg_batch_tuning *t);
 int sbg_batch_run(sbg_batch b, int precision);
 /* device time of the last run's work (HIP events), in ms */
 int sbg_batch_last_ms(sbg_batch b, double *ms);
+/* Measurement aid: the Guigue-Devillers kernel alone over the last run's
+ * candidates, `iters` launches between two HIP events; *avg_ms per launch;
+ * stats (may be null) gets the exit and fallback counts of one launch. */
+int sbg_batch_bench_tri_tri(sbg_batch b, int precision, int iters, double *avg_ms, sbg_tri_tri_stats *stats);
 int sbg_batch_counts(sbg_batch b, size_t *candidates, size_t *hits);
 /* candidates of the last run sorted by (a, b) as global triangle indices
  * (2 per candidate), with status (1 hit, 2 coplanar, 0 none) and, for hits,

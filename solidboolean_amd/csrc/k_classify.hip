@@ -23,6 +23,7 @@
 // `overflow` and the call fails: nothing is guessed.
 #include "sbg_device.h"
 #include "sbg_internal.h"
+#include "sbg_rayfilter.h"
 
 #include <cmath>
 
@@ -561,7 +562,14 @@ __device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, con
 // lane (a crowded cell, or more distinct hits) hands (axis, query) to
 // k_rays_wave.  Query i's rays go to mesh target[i] (mesh 0 without a target
 // list); a query without a target mesh (~0u) counts as outside.
-template <bool Q4, bool LIST>
+//
+// FILTER: each candidate passing the code test is decided from its 48-B FP32
+// record by the certificate of sbg_rayfilter.h (a certain miss, or a certain
+// hit at a distance known to within a bound) instead of the 96-B FP64 record
+// and the FP64 test.  Certain hits farther apart than PositionKey's cell are
+// distinct keys; a ray with an uncertain candidate, or with two hits that
+// close, goes to k_rays_wave, which runs it in FP64 like any deferred ray.
+template <bool Q4, bool LIST, bool FILTER>
 __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__restrict__ tri64,
                                                    const double *__restrict__ pts,
                                                    const uint32_t *__restrict__ order, const uint8_t *__restrict__ skip,
@@ -591,6 +599,43 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
     }
     const RayCell c = ray_cell(gs, m, r, (int)axis);
     bool defer = c.range.y - c.range.x > (uint32_t)kFastCands;
+    if (FILTER) {
+        const float4 p32 = make_float4((float)r.p.x, (float)r.p.y, (float)r.p.z, 0.0f);
+        const double pmax = fmax(fmax(__builtin_fabs(r.p.x), __builtin_fabs(r.p.y)), __builtin_fabs(r.p.z));
+        const float pm = f_up(pmax);
+        HitSpans<kFastKeys> hits;
+        for (uint32_t k0 = c.range.x; !defer && k0 < c.range.y; k0 += kCodeGroup) {
+            uint32_t pass = 0;
+#pragma unroll
+            for (int j = 0; j < kCodeGroup; ++j) {
+                const uint32_t k = k0 + j < c.range.y ? k0 + j : c.range.y - 1;
+                if (code_may_hit(gs.gcode[k], c) && k0 + j < c.range.y)
+                    pass |= 1u << j;
+            }
+            while (pass && !defer) {
+                const int j = __builtin_ctz(pass);
+                pass &= pass - 1;
+                const float4 *rec = reinterpret_cast<const float4 *>(gs.tri32 + (size_t)kTri32Stride * gs.gtri[k0 + j]);
+                const float4 rr[3] = {rec[0], rec[1], rec[2]};
+                const RayCert cert = ray_certificate(p32, pm, (int)axis, rr);
+                if (cert.verdict == kRayUnsure) {
+                    defer = true;
+                } else if (cert.verdict == kRayHit) {
+                    hits.add(cert.t, cert.et);
+                    defer = hits.full;
+                }
+            }
+        }
+        if (!defer) {
+            bits[(size_t)axis * n + i] = (uint8_t)(hits.n & 1);
+            return;
+        }
+        const uint32_t slot = atomicAdd(ovfCount, 1u);
+        uint4 *rec = reinterpret_cast<uint4 *>(ovfList) + 2 * (size_t)slot;
+        rec[0] = make_uint4(g, c.range.x, c.range.y, i);
+        rec[1] = make_uint4(c.u, c.v, c.pa, 0u);
+        return;
+    }
     AxisKeys<kFastKeys> keys;
     // codes in groups of kCodeGroup independent loads (indices clamped into the
     // range, the excess masked off), then the FP64 test of the entries passing
@@ -837,16 +882,24 @@ static void rays(const GridSet &gs, const double *tri64, const double *pts, cons
     uint32_t *overflow = counters;
     const uint32_t lanes = allAxes ? 3u * n : 2u * n;
     uint32_t *deferred = lists, *zDeferred = lists + 8 * 2 * (size_t)n, *zLanes = lists + 8 * 3 * (size_t)n;
-    hipLaunchKernelGGL((k_rays_grid<Q4, false>), dim3((lanes + 255) / 256), dim3(256), 0, s, gs, tri64, pts, order,
-                       skip, n, lanes, nullptr, nullptr, axisBits, deferred, counters + 1, overflow);
+    if (gs.tri32)
+        hipLaunchKernelGGL((k_rays_grid<Q4, false, true>), dim3((lanes + 255) / 256), dim3(256), 0, s, gs, tri64, pts,
+                           order, skip, n, lanes, nullptr, nullptr, axisBits, deferred, counters + 1, overflow);
+    else
+        hipLaunchKernelGGL((k_rays_grid<Q4, false, false>), dim3((lanes + 255) / 256), dim3(256), 0, s, gs, tri64, pts,
+                           order, skip, n, lanes, nullptr, nullptr, axisBits, deferred, counters + 1, overflow);
     hipLaunchKernelGGL(k_rays_wave<Q4>, dim3(8192), dim3(64), 0, s, gs, tri64, pts, order, n, axisBits, deferred,
                        counters + 1, overflow);
     if (allAxes)
         return;
     hipLaunchKernelGGL(k_third_axis<Q4>, dim3((n + 255) / 256), dim3(256), 0, s, pts, order, skip, n, axisBits,
                        zLanes, counters + 2);
-    hipLaunchKernelGGL((k_rays_grid<Q4, true>), dim3((n + 255) / 256), dim3(256), 0, s, gs, tri64, pts, order, skip,
-                       n, n, zLanes, counters + 2, axisBits, zDeferred, counters + 3, overflow);
+    if (gs.tri32)
+        hipLaunchKernelGGL((k_rays_grid<Q4, true, true>), dim3((n + 255) / 256), dim3(256), 0, s, gs, tri64, pts, order,
+                           skip, n, n, zLanes, counters + 2, axisBits, zDeferred, counters + 3, overflow);
+    else
+        hipLaunchKernelGGL((k_rays_grid<Q4, true, false>), dim3((n + 255) / 256), dim3(256), 0, s, gs, tri64, pts, order,
+                           skip, n, n, zLanes, counters + 2, axisBits, zDeferred, counters + 3, overflow);
     hipLaunchKernelGGL(k_rays_wave<Q4>, dim3(8192), dim3(64), 0, s, gs, tri64, pts, order, n, axisBits, zDeferred,
                        counters + 3, overflow);
 }
@@ -875,8 +928,12 @@ void launch_rays_xy_ordered(const GridSet &gs, const double *tri64, const double
     if (n == 0)
         return;
     const uint32_t lanes = 2u * n;
-    hipLaunchKernelGGL((k_rays_grid<true, false>), dim3((lanes + 255) / 256), dim3(256), 0, s, gs, tri64, q4, order,
-                       nullptr, n, lanes, nullptr, nullptr, axisBits, lists, counters + 1, counters);
+    if (gs.tri32)
+        hipLaunchKernelGGL((k_rays_grid<true, false, true>), dim3((lanes + 255) / 256), dim3(256), 0, s, gs, tri64, q4,
+                           order, nullptr, n, lanes, nullptr, nullptr, axisBits, lists, counters + 1, counters);
+    else
+        hipLaunchKernelGGL((k_rays_grid<true, false, false>), dim3((lanes + 255) / 256), dim3(256), 0, s, gs, tri64, q4,
+                           order, nullptr, n, lanes, nullptr, nullptr, axisBits, lists, counters + 1, counters);
 }
 
 void launch_ray_work_ordered(const GridSet &gs, const double *q4, const uint32_t *order, uint32_t n,

@@ -1127,9 +1127,12 @@ int sbg_tri_tri_batch(sbg_context ctx, const double *a9, const double *b9, size_
 // ---------------------------------------------------------------------------
 // classification
 // ---------------------------------------------------------------------------
-static GridSet grid_set(sbg_mesh m)
+// m's ray grid as the ray kernels read it; with `filtered`, rays decide
+// their candidates from m's FP32 records under the certificate
+static GridSet grid_set(sbg_mesh m, bool filtered = false)
 {
     GridSet gs = {};
+    gs.tri32 = filtered ? m->tri32.p : nullptr;
     gs.boxes = m->rootBox.p;
     gs.cells = m->gridCells.p;
     gs.gcode = m->gridCode.p;
@@ -1224,6 +1227,7 @@ static int enqueue_rays(sbg_mesh target, RayBufs rb, const double *dpts, const u
                         const uint8_t *dskip, size_t n, bool ordered, bool allAxes, bool votes)
 {
     sbg_context c = target->ctx;
+    const bool filtered = c->tune.ray_precision == SBG_PRECISION_FP32_FILTERED;
     TRY(rb.ovf.ensure(4));
     if (votes)
         TRY(rb.votesHost.ensure(n));
@@ -1239,10 +1243,10 @@ static int enqueue_rays(sbg_mesh target, RayBufs rb, const double *dpts, const u
         Stage st(c, SBG_STAGE_CLASSIFY, rb.s);
         launch_fill_u32(rb.ovf.p, 0u, 4, rb.s);
         if (ordered)
-            launch_rays_ordered(grid_set(target), target->tri64.p, dpts, dorder, (uint32_t)n, allAxes, rb.bits.p,
+            launch_rays_ordered(grid_set(target, filtered), target->tri64.p, dpts, dorder, (uint32_t)n, allAxes, rb.bits.p,
                                 rb.rayList.p, rb.ovf.p, rb.s);
         else
-            launch_rays(grid_set(target), target->tri64.p, dpts, dorder, dskip, (uint32_t)n, allAxes, rb.bits.p,
+            launch_rays(grid_set(target, filtered), target->tri64.p, dpts, dorder, dskip, (uint32_t)n, allAxes, rb.bits.p,
                         rb.rayList.p, rb.ovf.p, rb.s);
         launch_vote(rb.bits.p, dskip, (uint32_t)n, rb.votes.p, rb.s);
         TRY(check_launch("rays"));
@@ -1423,7 +1427,8 @@ int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t)
     if (!t)
         return fail(SBG_ERR_INVALID, "null tuning");
     if (t->bfs_levels < 0 || t->bfs_levels > 64 || t->dfs_waves < 64 || t->dfs_waves > 16384 ||
-        t->grid_density < 1 || t->grid_density > 64)
+        t->grid_density < 1 || t->grid_density > 64 ||
+        (t->ray_precision != SBG_PRECISION_FP64 && t->ray_precision != SBG_PRECISION_FP32_FILTERED))
         return fail(SBG_ERR_INVALID, "tuning out of range");
     ctx->tune = *t;
     ctx->tune.depth_first = t->depth_first ? 1 : 0;
@@ -1634,7 +1639,7 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
     const size_t nt = self->nt;
     *avg_ms = 0.0;
     if (work)
-        for (int k = 0; k < 4; ++k)
+        for (int k = 0; k < 5; ++k)
             work[k] = 0;
     if (!nt || !other->nt)
         return SBG_OK;
@@ -1648,7 +1653,7 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
     launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, c->flags.p, c->stream);
     launch_centroids_ordered(self->tri64.p, c->flags.p, self->sortedIds, nullptr, (uint32_t)nt, c->pts.p, c->stream);
     TRY(ensure_grid(other));
-    const GridSet gs = grid_set(other);
+    const GridSet gs = grid_set(other, c->tune.ray_precision == SBG_PRECISION_FP32_FILTERED);
     std::vector<hipEvent_t> ev(2 * (size_t)iters, nullptr);
     for (auto &e : ev)
         HIP_TRY(hipEventCreate(&e));
@@ -1662,7 +1667,9 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
     launch_fill_u32((uint32_t *)c->workCounts.p, 0u, 8, c->stream);
     launch_ray_work_ordered(gs, c->pts.p, self->sortedIds, (uint32_t)nt, c->workCounts.p, c->stream);
     uint64_t h[4] = {0, 0, 0, 0};
+    uint32_t deferred = 0;  // rays the last timed launch handed to the wave kernel
     HIP_TRY(hipMemcpyAsync(h, c->workCounts.p, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&deferred, c->ovf.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     double total = 0.0;
     for (int i = 0; i < iters; ++i) {
@@ -1673,9 +1680,11 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
     for (auto &e : ev)
         (void)hipEventDestroy(e);
     *avg_ms = total / iters;
-    if (work)
+    if (work) {
         for (int k = 0; k < 4; ++k)
             work[k] = h[k];
+        work[4] = deferred;
+    }
     return check_launch("bench rays");
 }
 
@@ -1692,7 +1701,7 @@ int sbg_ray_certify(const double *pts, const int32_t *axis, const double *tris9,
         float4 r[3];
         make_tri32(tris9 + 9 * i, tris9 + 9 * i + 3, tris9 + 9 * i + 6, r);
         const double *p = pts + 3 * i;
-        const float p32[3] = {(float)p[0], (float)p[1], (float)p[2]};
+        const float4 p32 = make_float4((float)p[0], (float)p[1], (float)p[2], 0.0f);
         double pm = __builtin_fabs(p[0]);
         pm = __builtin_fabs(p[1]) > pm ? __builtin_fabs(p[1]) : pm;
         pm = __builtin_fabs(p[2]) > pm ? __builtin_fabs(p[2]) : pm;
@@ -1833,7 +1842,8 @@ int sbg_classify_points_noalloc(sbg_mesh target, const double *pts, size_t n, ui
         launch_fill_u32(reinterpret_cast<uint32_t *>(bits), 0u, (3 * n + 3) / 4, s);
     } else {
         launch_fill_u32(counters, 0u, 4, s);  // kernels, not memsets: the call may be captured
-        launch_rays(grid_set(target), target->tri64.p, dpts, nullptr, nullptr, (uint32_t)n, allAxes, bits, lists,
+        launch_rays(grid_set(target, c->tune.ray_precision == SBG_PRECISION_FP32_FILTERED), target->tri64.p, dpts,
+                    nullptr, nullptr, (uint32_t)n, allAxes, bits, lists,
                     counters, s);
         HIP_TRY(hipMemcpyAsync(host, counters, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     }

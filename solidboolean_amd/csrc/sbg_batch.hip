@@ -186,6 +186,8 @@ int enqueue(sbg_batch b, int precision)
     gs.ncells = b->ncells;
     gs.order = b->ids.p;
     const hipStream_t aux = b->ctx->aux;
+    GridSet gsRays = gs;  // the ray kernels' view: FP32 certificates in the filtered precision
+    gsRays.tri32 = precision == SBG_PRECISION_FP32_FILTERED ? b->tri32.p : nullptr;
     HIP_TRY(hipEventRecord(b->fork, s));
     HIP_TRY(hipStreamWaitEvent(aux, b->fork, 0));
     launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, aux);
@@ -218,7 +220,8 @@ int enqueue(sbg_batch b, int precision)
     launch_mark_both(sorted, b->status.p, candCount, ccap, b->flags.p, s);
     launch_centroids_ordered(b->tri64.p, b->flags.p, b->ids.p, b->target.p, nt, b->pts.p, s);
     HIP_TRY(hipStreamWaitEvent(s, b->join, 0));
-    launch_rays_ordered(gs, b->tri64.p, b->pts.p, b->ids.p, nt, false, b->bits.p, b->rayList.p, w + kRayFlagW, s);
+    launch_rays_ordered(gsRays, b->tri64.p, b->pts.p, b->ids.p, nt, false, b->bits.p, b->rayList.p, w + kRayFlagW,
+                        s);
     launch_vote(b->bits.p, b->flags.p, nt, b->votes.p, s);
     launch_copy_u32(b->gridCells.p + b->ncells, w + kGridW, 1, s);
     TRY(check_launch("batch classify"));
@@ -466,6 +469,55 @@ int sbg_batch_set_spill(sbg_batch b, uint32_t per_wave)
     drop_graph(b);
     b->ran = false;
     return size_buffers(b);
+}
+
+int sbg_batch_bench_tri_tri(sbg_batch b, int precision, int iters, double *avg_ms, sbg_tri_tri_stats *stats)
+{
+    if (!b || !avg_ms || iters <= 0)
+        return fail(SBG_ERR_INVALID, "bad argument");
+    if (precision != SBG_PRECISION_FP64 && precision != SBG_PRECISION_FP32_FILTERED)
+        return fail(SBG_ERR_INVALID, "unknown precision %d", precision);
+    if (!b->ran)
+        return fail(SBG_ERR_STATE, "no completed run");
+    TRY(use(b->ctx));
+    hipStream_t s = b->ctx->stream;
+    const uint64_t *sorted = b->pairsInAlt ? b->pairsAlt.p : b->pairs.p;
+    const uint32_t n = b->nCand;
+    // the launches rewrite status and segments with the same values
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, s));
+    for (int i = 0; i < iters; ++i)
+        launch_tritri(b->tri64.p, b->tri64.p, b->tri32.p, b->tri32.p, sorted, n, nullptr, precision, b->status.p,
+                      b->seg.p, b->gdWork.p, nullptr, s);
+    HIP_TRY(hipEventRecord(e1, s));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / iters;
+    if (stats) {
+        DevBuf<GdStats> d;
+        TRY(d.ensure(1));
+        launch_fill_u32(reinterpret_cast<uint32_t *>(d.p), 0u, sizeof(GdStats) / 4, s);
+        launch_tritri(b->tri64.p, b->tri64.p, b->tri32.p, b->tri32.p, sorted, n, nullptr, precision, b->status.p,
+                      b->seg.p, b->gdWork.p, d.p, s);
+        GdStats h;
+        HIP_TRY(hipMemcpyAsync(&h, d.p, sizeof(GdStats), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        d.release();
+        stats->tested = n;
+        stats->exit_plane1 = h.exits[0];
+        stats->exit_plane2 = h.exits[1];
+        stats->exit_interval = h.exits[2];
+        stats->hits = h.exits[3];
+        stats->coplanar = h.exits[4];
+        stats->fallback_pairs = h.fallback;
+        stats->filtered_pairs = h.filtered;
+    }
+    return check_launch("batch bench tri_tri");
 }
 
 int sbg_batch_last_ms(sbg_batch b, double *ms)

@@ -112,7 +112,7 @@ struct sbg_context_s {
     DevBuf<uint2> spill;              // depth-first spill regions
     uint32_t spillPerWave = kSpillInit;
     bool depthFirst = true;           // false after the spill regions proved too small
-    sbg_tuning tune = {kBfsLevels, kDfsWaves, 2, 1, 1};
+    sbg_tuning tune = {kBfsLevels, kDfsWaves, 2, 1, 1, SBG_PRECISION_FP32_FILTERED};
     DevBuf<uint32_t> counters;
     size_t frontCap = 0, candCap = 0;
     // candidates expected from the next search (the last count plus a margin):

@@ -133,6 +133,10 @@ struct GridSet {
     const uint2 *gcode;      // per entry: two box code words
     const uint32_t *gtri;    // per entry: the triangle
     const uint32_t *order;  // grid build's triangle visiting order, e.g. Morton (null: index order)
+    // FP32 records of the target meshes: rays decide their candidates under
+    // the FP32 certificate and run in FP64 only when one is uncertain (null:
+    // every candidate in FP64)
+    const float *tri32;
     uint32_t G0, cap, ncells;
 };
 uint32_t grid_dim(uint32_t nt, int density);

@@ -114,9 +114,12 @@ __host__ __device__ __forceinline__ void make_tri32(const double a[3], const dou
     out[2] = make_float4((float)c[0], (float)c[1], (float)c[2], 0.0f);
 }
 
-// p32: p rounded to nearest; pm: an upper bound of p's largest |coordinate|;
-// k: the ray's axis; r: the record.
-__host__ __device__ __forceinline__ RayCert ray_certificate(const float p32[3], float pm, int k, const float4 r[3])
+// component k of (x, y, z), by selects (no indexed register array)
+__host__ __device__ __forceinline__ float comp_of(const float4 &v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+// p32: p rounded to nearest (w unused); pm: an upper bound of p's largest
+// |coordinate|; k: the ray's axis; r: the record.
+__host__ __device__ __forceinline__ RayCert ray_certificate(const float4 &p32, float pm, int k, const float4 r[3])
 {
     RayCert out = {kRayUnsure, 0.0f, 0.0f};
     const int u = k == 2 ? 0 : k + 1, w = k == 0 ? 2 : k - 1;
@@ -128,14 +131,15 @@ __host__ __device__ __forceinline__ RayCert ray_certificate(const float p32[3], 
     const float E = 1.7320511f * 1.000001f * ext;  // edge length bound: sqrt(3) ext
     // differences V_i - p, on the three axes
     float ak[3], au[3], aw[3];
-    const float4 *vr[3] = {&r[0], &r[1], &r[2]};
+    const float pk = comp_of(p32, k), pu = comp_of(p32, u), pw = comp_of(p32, w);
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
-        const float v[3] = {vr[i]->x, vr[i]->y, vr[i]->z};
-        ak[i] = v[k] - p32[k];
-        au[i] = v[u] - p32[u];
-        aw[i] = v[w] - p32[w];
+        ak[i] = comp_of(r[i], k) - pk;
+        au[i] = comp_of(r[i], u) - pu;
+        aw[i] = comp_of(r[i], w) - pw;
     }
     float O[3], eO[3];
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int j = i == 2 ? 0 : i + 1;
         const float x1 = au[i] * aw[j], x2 = aw[i] * au[j];
@@ -160,6 +164,7 @@ __host__ __device__ __forceinline__ RayCert ray_certificate(const float p32[3], 
     const float F = 16.0f * (Delta * E + 16.0f * kU64 * Ed * Ed + kDblEps) * 1.0001f;
     // per O_i: +1 / -1 when its sign is certain and |O_i| >= F, 0 otherwise
     int sg[3];
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         const float lo = __builtin_fabsf(O[i]) - eO[i] * 1.0001f;
         sg[i] = lo >= F ? (O[i] > 0.0f ? 1 : -1) : 0;
@@ -174,6 +179,7 @@ __host__ __device__ __forceinline__ RayCert ray_certificate(const float p32[3], 
     // inside: t from the barycentric weights (all O_i of one sign, so no
     // cancellation in the sum)
     float eN = 0.0f, absN = 0.0f, T = 0.0f;
+#pragma unroll
     for (int i = 0; i < 3; ++i) {
         const int j = i == 2 ? 0 : i + 1;  // weight of vertex i is O_j
         const float ra = 2.0f * eta + 1.0000002f * kE32 * __builtin_fabsf(ak[i]);
@@ -200,6 +206,19 @@ __host__ __device__ __forceinline__ RayCert ray_certificate(const float p32[3], 
     return out;
 }
 
+// Certain hits of one ray, CAP of them in registers: their distances t from
+// the ray's origin and error bounds.  A hit not certainly apart from one
+// already held, or one more than CAP, sets `full` (the ray is then run in
+// FP64).  New hits enter at slot 0 and shift the others up, so every slot
+// index is a compile-time constant.
+template <int CAP>
+struct HitSpans {
+    float t[CAP], e[CAP];
+    int n = 0;
+    bool full = false;
+    __host__ __device__ __forceinline__ void add(float ht, float he);
+};
+
 // Two certain hits of one ray key to different PositionKeys on the ray's axis
 // when their FP64 hit points are more than 2e-5 apart (the truncated
 // products differ by more than 2, zero's double-width cell included).
@@ -207,6 +226,29 @@ __host__ __device__ __forceinline__ bool hits_apart(float ta, float ea, float tb
 {
     const float gap = __builtin_fabsf(ta - tb) * (1.0f - 2.0f * kE32);
     return gap - ea - eb > 2.0001e-5f + 4e-15f * kMaxMag;
+}
+
+template <int CAP>
+__host__ __device__ __forceinline__ void HitSpans<CAP>::add(float ht, float he)
+{
+#pragma unroll
+    for (int j = 0; j < CAP; ++j)
+        if (j < n && !hits_apart(ht, he, t[j], e[j])) {
+            full = true;
+            return;
+        }
+    if (n == CAP) {
+        full = true;
+        return;
+    }
+#pragma unroll
+    for (int j = CAP - 1; j > 0; --j) {
+        t[j] = t[j - 1];
+        e[j] = e[j - 1];
+    }
+    t[0] = ht;
+    e[0] = he;
+    ++n;
 }
 
 }  // namespace sbg

@@ -42,7 +42,7 @@ EXPORTS = [
     "sbg_adjoint",
     "sbg_batch_create", "sbg_batch_set_vertices", "sbg_batch_set_tuning", "sbg_batch_get_tuning", "sbg_batch_run",
     "sbg_batch_last_ms", "sbg_batch_counts", "sbg_batch_candidates", "sbg_batch_sides", "sbg_batch_autotune",
-    "sbg_batch_set_spill",
+    "sbg_batch_set_spill", "sbg_batch_bench_tri_tri",
     "sbg_batch_destroy",
 ]
 
@@ -62,7 +62,7 @@ class BatchTuning(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("bfs_levels", C.c_int), ("dfs_waves", C.c_int), ("grid_density", C.c_int), ("depth_first", C.c_int),
-                ("early_grids", C.c_int)]
+                ("early_grids", C.c_int), ("ray_precision", C.c_int)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -150,6 +150,7 @@ def load(path: str = LIB_PATH):
         "sbg_batch_sides": [_P, _U8],
         "sbg_batch_autotune": [_P, C.c_int, C.c_int, C.POINTER(BatchTuning), C.POINTER(C.c_double)],
         "sbg_batch_set_spill": [_P, C.c_uint32],
+        "sbg_batch_bench_tri_tri": [_P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(TriTriStats)],
         "sbg_batch_destroy": [_P],
     }
     for name, args in sig.items():
@@ -409,9 +410,9 @@ class Mesh:
         """sbg_bench_rays: mean launch ms of the x/y ray kernel of
         classify_triangles, and the work of one launch."""
         ms = C.c_double(0)
-        work = np.zeros(4, np.uint64)
+        work = np.zeros(5, np.uint64)
         _check(self.L.sbg_bench_rays(self.h, other.h, pairs.h, side, iters, C.byref(ms), work))
-        return ms.value, dict(zip(("rays", "scanned", "passed", "deferred"), (int(x) for x in work)))
+        return ms.value, dict(zip(("rays", "scanned", "passed", "deferred", "to_fp64"), (int(x) for x in work)))
 
 
 class Pairs:
@@ -525,6 +526,14 @@ class Batch:
         for k, v in kw.items():
             setattr(t, k, int(v))
         _check(self.L.sbg_batch_set_tuning(self.h, C.byref(t)))
+
+    def bench_tri_tri(self, precision: int, iters: int):
+        """sbg_batch_bench_tri_tri: (ms per GD launch over the last run's
+        candidates, statistics of one launch)."""
+        ms = C.c_double(0)
+        st = TriTriStats()
+        _check(self.L.sbg_batch_bench_tri_tri(self.h, precision, iters, C.byref(ms), C.byref(st)))
+        return ms.value, st.as_dict()
 
     def set_spill(self, per_wave: int):
         """Testing aid: the depth-first search's spill region per wave."""

@@ -16,6 +16,24 @@ def bits(a):
     return np.ascontiguousarray(a, np.float64).view(np.uint64)
 
 
+RAY_PRECISIONS = [N.FP64, N.FP32_FILTERED]
+
+
+class ray_precision:
+    """Runs the enclosed classification calls with the context's rays in the
+    given precision (sbg_tuning.ray_precision), then restores the tuning."""
+
+    def __init__(self, gpu, precision):
+        self.gpu, self.precision = gpu, precision
+
+    def __enter__(self):
+        self.saved = self.gpu.tuning()
+        self.gpu.set_tuning(**{**self.saved, "ray_precision": self.precision})
+
+    def __exit__(self, *exc):
+        self.gpu.set_tuning(**self.saved)
+
+
 def sorted_pairs(p):
     p = np.asarray(p, np.int64).reshape(-1, 2)
     return p[np.lexsort((p[:, 1], p[:, 0]))]
@@ -91,22 +109,26 @@ def test_tri_tri(case, precision):
         assert st["filtered_pairs"] + st["hits"] + st["coplanar"] >= len(ref) - st["fallback_pairs"]
 
 
-def test_rays(case):
+@pytest.mark.parametrize("rays", RAY_PRECISIONS)
+def test_rays(case, gpu, rays):
     name, g, A, B = case
-    axis_a, vote_a = A.classify_points(g["ray_pts_a"])
-    axis_b, vote_b = B.classify_points(g["ray_pts_b"])
+    with ray_precision(gpu, rays):
+        axis_a, vote_a = A.classify_points(g["ray_pts_a"])
+        axis_b, vote_b = B.classify_points(g["ray_pts_b"])
     assert np.array_equal(axis_a, g["ray_bits_a"]), name
     assert np.array_equal(axis_b, g["ray_bits_b"]), name
     assert np.array_equal(vote_a, (g["ray_bits_a"].sum(1) >= 2).astype(np.uint8))
 
 
-def test_classify_triangles(case):
+@pytest.mark.parametrize("rays", RAY_PRECISIONS)
+def test_classify_triangles(case, gpu, rays):
     name, g, A, B = case
     P = A.find_pairs(B)
     P.tri_tri(A, B, N.FP32_FILTERED, stats=False)
-    assert np.array_equal(A.classify_triangles(B, P, 0), g["cls_a"]), name
-    assert np.array_equal(B.classify_triangles(A, P, 1), g["cls_b"]), name
-    ca, cb = A.classify_pair(B, P)
+    with ray_precision(gpu, rays):
+        assert np.array_equal(A.classify_triangles(B, P, 0), g["cls_a"]), name
+        assert np.array_equal(B.classify_triangles(A, P, 1), g["cls_b"]), name
+        ca, cb = A.classify_pair(B, P)
     assert np.array_equal(ca, g["cls_a"]), name
     assert np.array_equal(cb, g["cls_b"]), name
     if "side_a" in g and name != "cube_n8":
@@ -144,7 +166,10 @@ def test_oracle_random_seeds(gpu, seed):
         assert np.array_equal(bits(seg[st == 1]), bits(oseg[ost == 1]))
     ia = np.zeros(a.nt, np.uint8)
     ia[ref[ost == 1, 0]] = 1
-    assert np.array_equal(A.classify_triangles(B, P, 0), OA.classify_triangles(OB, ia))
+    ref_cls = OA.classify_triangles(OB, ia)
+    for rays in RAY_PRECISIONS:
+        with ray_precision(gpu, rays):
+            assert np.array_equal(A.classify_triangles(B, P, 0), ref_cls), rays
 
 
 def _rays_vs_oracle(A, OA, pts):
@@ -175,10 +200,12 @@ def test_large_coordinates(gpu):
     ia[ref[ost == 1, 0]] = 1
     ib = np.zeros(b.nt, np.uint8)
     ib[ref[ost == 1, 1]] = 1
-    assert np.array_equal(A.classify_triangles(B, P, 0), OA.classify_triangles(OB, ia))
-    assert np.array_equal(B.classify_triangles(A, P, 1), OB.classify_triangles(OA, ib))
-    pts = a.vertices[a.triangles].mean(1)[::7]
-    _rays_vs_oracle(B, OB, pts)
+    for rays in RAY_PRECISIONS:
+        with ray_precision(gpu, rays):
+            assert np.array_equal(A.classify_triangles(B, P, 0), OA.classify_triangles(OB, ia))
+            assert np.array_equal(B.classify_triangles(A, P, 1), OB.classify_triangles(OA, ib))
+            pts = a.vertices[a.triangles].mean(1)[::7]
+            _rays_vs_oracle(B, OB, pts)
 
 
 def test_nested_shells_many_hits(gpu):
@@ -202,10 +229,12 @@ def test_nested_shells_many_hits(gpu):
     st, _ = P.results()
     iq = np.zeros(query.nt, np.uint8)
     iq[ref[st == 1, 0]] = 1
-    assert np.array_equal(Q.classify_triangles(Tm, P, 0), OQ.classify_triangles(OT, iq))
     rng = np.random.default_rng(4)
     pts = rng.uniform(-1.1, 1.1, (3000, 3))
-    _rays_vs_oracle(Tm, OT, pts)
+    for rays in RAY_PRECISIONS:
+        with ray_precision(gpu, rays):
+            assert np.array_equal(Q.classify_triangles(Tm, P, 0), OQ.classify_triangles(OT, iq))
+            _rays_vs_oracle(Tm, OT, pts)
 
 
 @pytest.mark.parametrize("name", ["sub6_s1", "sub6_s3"])
@@ -235,13 +264,15 @@ def _check_digests(gpu, g, a, b):
         hp, hs = P.hits()
         assert sha(hp.astype("<u4"), np.ascontiguousarray(hs, "<f8")) == str(g["hits_digest"])
         assert st["coplanar"] == int(g["n_coplanar"])
-    ca = A.classify_triangles(B, P, 0)
-    cb = B.classify_triangles(A, P, 1)
-    assert np.array_equal(np.bincount(ca, minlength=3), g["cls_count_a"])
-    assert np.array_equal(np.bincount(cb, minlength=3), g["cls_count_b"])
-    assert sha(ca) == str(g["cls_digest_a"]) and sha(cb) == str(g["cls_digest_b"])
-    pa, pb = A.classify_pair(B, P)
-    assert sha(pa) == str(g["cls_digest_a"]) and sha(pb) == str(g["cls_digest_b"])
+    for rays in RAY_PRECISIONS:
+        with ray_precision(gpu, rays):
+            ca = A.classify_triangles(B, P, 0)
+            cb = B.classify_triangles(A, P, 1)
+            assert np.array_equal(np.bincount(ca, minlength=3), g["cls_count_a"]), rays
+            assert np.array_equal(np.bincount(cb, minlength=3), g["cls_count_b"]), rays
+            assert sha(ca) == str(g["cls_digest_a"]) and sha(cb) == str(g["cls_digest_b"]), rays
+            pa, pb = A.classify_pair(B, P)
+            assert sha(pa) == str(g["cls_digest_a"]) and sha(pb) == str(g["cls_digest_b"]), rays
     return P, st  # st: statistics of the FP32-filtered run
 
 
