@@ -64,7 +64,7 @@ def sha(*arrays) -> str:
     return h.hexdigest()
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_1310720_pair.md")
+STEP_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kernel_trace_and_traffic_1310720_pair_step.md")
 RAYS_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_rays_1310720_pair.md")
 GD_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_gd_4096_batch.md")
 
@@ -84,17 +84,24 @@ def ray_bytes(work: dict, filtered: bool) -> float:
             + work["passed"] * (RAY_TEST32_B if filtered else RAY_TEST64_B))
 
 
-def measured_traffic(kernels, grid: int, profile: str = TRAFFIC_PROFILE):
-    """HBM bytes per launch of `kernels` at `grid` work-items from a committed
-    rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE passes scaled by the copy
-    kernel's calibration), or None when the summary does not cover them."""
+def measured_traffic(kernels, grid, profile: str = STEP_TRAFFIC_PROFILE):
+    """HBM bytes per launch of `kernels` from a committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE passes scaled by the copy kernel's
+    calibration), or None when the summary does not cover them.  `grid` is
+    the launch's work-item count, or None for each kernel's smallest profiled
+    launch (a lone-kernel launch over the exact item count, where the
+    profiled program also launches it over a larger capacity)."""
     try:
         rows = {}
         for line in open(profile):
             cells = [c.strip().strip("`") for c in line.split("|")]
             if len(cells) == 7 and cells[2].isdigit():
                 rows[(cells[1].replace("sbg::", ""), int(cells[2]))] = float(cells[5]) * 1e6
-        return sum(rows[(k, grid)] for k in kernels)
+        total = 0.0
+        for k in kernels:
+            g = grid if grid is not None else min(gr for (name, gr) in rows if name == k)
+            total += rows[(k, g)]
+        return total
     except (OSError, KeyError, ValueError):
         return None
 
@@ -387,7 +394,7 @@ def batch_section(ctx, mg, N, steps: int, graph: bool = True) -> dict:
         gd[name] = batch.bench_tri_tri(prec, 50)
     gms, gst = gd["filtered"]
     gbytes = gd_bytes(ncand, gst["hits"], gst["fallback_pairs"], 1)
-    gtraffic = measured_traffic(("k_gd_filter", "k_gd_list"), ncand, GD_TRAFFIC_PROFILE)
+    gtraffic = measured_traffic(("k_gd_filter", "k_gd_list"), None, GD_TRAFFIC_PROFILE)
     roof_gd = {
         "kernel": "k_gd_filter + k_gd_list (FP32-filtered Guigue-Devillers) over the 4,096-pair batch's candidates",
         "bound": "hbm",

@@ -8,7 +8,10 @@
 #include "referencetree.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
+#include <deque>
+#include <thread>
 #include <utility>
 
 namespace sbhost {
@@ -31,56 +34,33 @@ struct Box {
     }
 };
 
-}  // namespace
+struct Work {
+    uint32_t node;
+    size_t begin, end;
+    Box box;
+    Vector3 center;
+};
 
-ReferenceTree::ReferenceTree(const std::vector<Vector3> &vertices, const std::vector<std::vector<size_t>> &triangles)
-{
-    const size_t nt = triangles.size();
-    m_leafOf.assign(nt, 0);
-    m_place.assign(nt, 0);
-    // per triangle: box and center (the running sum from 0, divided by 3)
-    std::vector<Box> box(nt);
-    std::vector<Vector3> center(nt);
-    Box all;
-    Vector3 rootCenter(0.0, 0.0, 0.0);
-    for (size_t t = 0; t < nt; ++t) {
-        Vector3 sum(0.0, 0.0, 0.0);
-        for (size_t k = 0; k < 3; ++k) {
-            const Vector3 &p = vertices[triangles[t][k]];
-            box[t].add(p.constData());
-            all.add(p.constData());
-            sum += p;
-        }
-        center[t] = sum /= (double)(float)3;
-        rootCenter += center[t];
-    }
-    if (nt)
-        rootCenter /= (double)(float)nt;
+constexpr size_t kParallelMin = 1u << 16;  // triangles from which subtrees are built on several threads
 
-    // nodes own ranges of `members`; a split rewrites its range as the left
-    // side's members in reverse scan order, then the right side's in order
-    std::vector<uint32_t> members(nt), scratch(nt);
-    for (size_t t = 0; t < nt; ++t)
-        members[t] = (uint32_t)t;
-    struct Work {
-        uint32_t node;
-        size_t begin, end;
-        Box box;
-        Vector3 center;
-    };
-    m_nodes.emplace_back();
-    m_nodes[0].count = (uint32_t)nt;
-    std::vector<Work> stack{{0, 0, nt, all, rootCenter}};
-    while (!stack.empty()) {
-        Work w = stack.back();
-        stack.pop_back();
+// One node's split (src/axisalignedboundingboxtree.cpp:71-141): a leaf records
+// where its members sit and returns false; otherwise the two children are
+// appended to `nodes` and described in child[0..1].
+struct Splitter {
+    const std::vector<Box> &box;
+    const std::vector<Vector3> &center;
+    std::vector<uint32_t> &members, &scratch;
+    std::vector<uint32_t> &leafOf, &place;
+
+    bool split(const Work &w, std::vector<ReferenceTree::Node> &nodes, Work child[2]) const
+    {
         const size_t n = w.end - w.begin;
         if (n <= kLeafMax) {
             for (size_t i = w.begin; i < w.end; ++i) {
-                m_leafOf[members[i]] = w.node;
-                m_place[members[i]] = (uint32_t)(i - w.begin);
+                leafOf[members[i]] = w.node;
+                place[members[i]] = (uint32_t)(i - w.begin);
             }
-            continue;
+            return false;
         }
         // the longest axis of the node's box (spans narrowed to float; the
         // first of equal ones), split at the node's center on it
@@ -113,12 +93,12 @@ ReferenceTree::ReferenceTree(const std::vector<Vector3> &vertices, const std::ve
             cut = n / 2;
         else if (nr == 0)
             cut = n - n / 2;
-        Work child[2];
         const size_t bounds[3] = {w.begin, w.begin + cut, w.end};
         for (int c = 0; c < 2; ++c) {
             Work &x = child[c];
             x.begin = bounds[c];
             x.end = bounds[c + 1];
+            x.box = Box();
             x.center = Vector3(0.0, 0.0, 0.0);
             for (size_t j = x.begin; j < x.end; ++j) {
                 const uint32_t t = members[j];
@@ -127,14 +107,145 @@ ReferenceTree::ReferenceTree(const std::vector<Vector3> &vertices, const std::ve
                 x.center += center[t];
             }
             x.center /= (double)(float)(x.end - x.begin);
-            x.node = (uint32_t)m_nodes.size();
-            m_nodes.emplace_back();
-            m_nodes.back().count = (uint32_t)(x.end - x.begin);
+            x.node = (uint32_t)nodes.size();
+            nodes.emplace_back();
+            nodes.back().count = (uint32_t)(x.end - x.begin);
         }
-        m_nodes[w.node].left = child[0].node;
-        m_nodes[w.node].right = child[1].node;
+        nodes[w.node].left = child[0].node;
+        nodes[w.node].right = child[1].node;
+        return true;
+    }
+};
+
+// the whole subtree below w, depth first, into `nodes`
+void buildSubtree(const Splitter &sp, const Work &w, std::vector<ReferenceTree::Node> &nodes)
+{
+    std::vector<Work> stack{w};
+    while (!stack.empty()) {
+        const Work x = stack.back();
+        stack.pop_back();
+        Work child[2];
+        if (!sp.split(x, nodes, child))
+            continue;
         stack.push_back(child[1]);
         stack.push_back(child[0]);
+    }
+}
+
+}  // namespace
+
+ReferenceTree::ReferenceTree(const std::vector<Vector3> &vertices, const std::vector<std::vector<size_t>> &triangles)
+{
+    const size_t nt = triangles.size();
+    m_leafOf.assign(nt, 0);
+    m_place.assign(nt, 0);
+    // per triangle: box and center (the running sum from 0, divided by 3),
+    // independent per triangle, so spread over threads for large meshes
+    std::vector<Box> box(nt);
+    std::vector<Vector3> center(nt);
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned threads = hw < 16 ? (hw ? hw : 1) : 16;
+    auto prepare = [&](size_t t0, size_t t1) {
+        for (size_t t = t0; t < t1; ++t) {
+            Vector3 sum(0.0, 0.0, 0.0);
+            for (size_t k = 0; k < 3; ++k) {
+                const Vector3 &p = vertices[triangles[t][k]];
+                box[t].add(p.constData());
+                sum += p;
+            }
+            center[t] = sum /= (double)(float)3;
+        }
+    };
+    if (nt < kParallelMin || threads < 2) {
+        prepare(0, nt);
+    } else {
+        std::vector<std::thread> pool;
+        for (unsigned i = 0; i < threads; ++i)
+            pool.emplace_back(prepare, nt * i / threads, nt * (i + 1) / threads);
+        for (auto &t : pool)
+            t.join();
+    }
+    // the root's box (min / max, any order) and center (a sum in triangle order)
+    Box all;
+    Vector3 rootCenter(0.0, 0.0, 0.0);
+    for (size_t t = 0; t < nt; ++t) {
+        all.add(box[t].lo);
+        all.add(box[t].hi);
+        rootCenter += center[t];
+    }
+    if (nt)
+        rootCenter /= (double)(float)nt;
+
+    // nodes own ranges of `members`; a split rewrites its range as the left
+    // side's members in reverse scan order, then the right side's in order
+    std::vector<uint32_t> members(nt), scratch(nt);
+    for (size_t t = 0; t < nt; ++t)
+        members[t] = (uint32_t)t;
+    m_nodes.emplace_back();
+    m_nodes[0].count = (uint32_t)nt;
+    const Splitter sp{box, center, members, scratch, m_leafOf, m_place};
+    const Work root{0, 0, nt, all, rootCenter};
+    if (nt < kParallelMin || threads < 2) {
+        buildSubtree(sp, root, m_nodes);
+    } else {
+        // Every node's sums run over its own members in their order whoever
+        // builds it, so the tree does not depend on how the work is shared:
+        // split breadth-first on this thread until there are enough disjoint
+        // subtrees, then build those on worker threads into local node lists,
+        // and splice them in.
+        std::deque<Work> top{root};
+        std::vector<Work> pending;
+        while (!top.empty() && top.size() + pending.size() < 8 * (size_t)threads) {
+            const Work w = top.front();
+            top.pop_front();
+            Work child[2];
+            if (!sp.split(w, m_nodes, child))
+                continue;
+            for (const Work &c : child) {
+                if (c.end - c.begin > 2 * kLeafMax)
+                    top.push_back(c);
+                else
+                    pending.push_back(c);
+            }
+        }
+        pending.insert(pending.end(), top.begin(), top.end());
+        std::vector<std::vector<Node>> local(pending.size());
+        std::atomic<size_t> next{0};
+        auto worker = [&]() {
+            for (size_t k = next++; k < pending.size(); k = next++) {
+                Work w = pending[k];
+                w.node = 0;
+                local[k].emplace_back();
+                local[k][0].count = (uint32_t)(w.end - w.begin);
+                buildSubtree(sp, w, local[k]);
+            }
+        };
+        std::vector<std::thread> pool;
+        for (unsigned i = 1; i < threads; ++i)
+            pool.emplace_back(worker);
+        worker();
+        for (auto &t : pool)
+            t.join();
+        // local node 0 is the pending node itself, local j >= 1 goes to base + j - 1
+        for (size_t k = 0; k < pending.size(); ++k) {
+            const uint32_t self = pending[k].node;
+            const uint32_t base = (uint32_t)m_nodes.size();
+            auto map = [&](uint32_t j) { return j == 0 ? self : base + j - 1; };
+            for (size_t j = 1; j < local[k].size(); ++j) {
+                Node x = local[k][j];
+                if (!x.leaf()) {
+                    x.left = map(x.left);
+                    x.right = map(x.right);
+                }
+                m_nodes.push_back(x);
+            }
+            if (!local[k][0].leaf()) {
+                m_nodes[self].left = map(local[k][0].left);
+                m_nodes[self].right = map(local[k][0].right);
+            }
+            for (size_t i = pending[k].begin; i < pending[k].end; ++i)
+                m_leafOf[members[i]] = map(m_leafOf[members[i]]);
+        }
     }
     // the turns down to every leaf
     m_paths.assign(m_nodes.size(), std::string());
