@@ -550,7 +550,7 @@ def main():
         step()
         log(f"warmup step {i + 1}/{args.warmup} done")
     tuning, tuned_ms = ctx.autotune_pair(A, B, reps=2)
-    log(f"autotuned single-pair knobs: {tuning} ({tuned_ms:.2f} ms search + GD + classify)")
+    log(f"autotuned single-pair knobs: {tuning} ({tuned_ms:.2f} ms prepare + search + GD + classify)")
     ctx.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):

@@ -222,18 +222,24 @@ int sbg_calibrate_copy(sbg_context ctx, size_t bytes, int iters, double *avg_ms)
 
 /* ---- tuning of the single-pair path -----------------------------------------
  * Knobs that change speed, never results.  sbg_autotune_pair times the chain
- * find_pairs -> tri_tri (FP32-filtered) -> classify_triangles of both meshes
- * over a coordinate search of the knobs and leaves the fastest set. */
+ * prepare both meshes -> find_pairs -> tri_tri (FP32-filtered) -> classify
+ * both meshes over a coordinate search of the knobs and leaves the fastest
+ * set (it re-prepares both meshes). */
 typedef struct sbg_tuning {
     int bfs_levels;   /* breadth-first levels before the persistent depth-first search (0..64) */
     int dfs_waves;    /* one-wave workgroups of the depth-first search (64..16384) */
     int grid_density; /* ray-grid cells per axis plane = triangles * grid_density / 16 (1..64) */
     int depth_first;  /* 0: breadth-first levels only, with a host check every 8 levels */
-    int early_grids;  /* 1: sbg_find_pairs queues both meshes' ray grids on the second stream, beside
-                         the search; 0: each grid is built when a classification first needs it */
+    int early_grids;  /* 2 (default): sbg_mesh_prepare queues the mesh's ray grid on a third
+                         stream after the prepare, so it builds beside the other mesh's prepare
+                         and the pair search; 1: sbg_find_pairs queues both meshes' grids on the
+                         second stream, beside the search; 0: each grid is built when a
+                         classification first needs it */
     int ray_precision; /* SBG_PRECISION_FP32_FILTERED (default): each ray candidate is decided from
                           the FP32 record under a certificate, and a ray with an uncertain candidate
                           runs in FP64; SBG_PRECISION_FP64: every candidate in FP64.  Same results. */
+    int ray_cell_cap;  /* grid entries one lane scans for its ray (16..4096, default 64); a ray whose
+                          cell holds more goes to the kernel that gives each ray a whole wave */
 } sbg_tuning;
 int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t);
 int sbg_context_get_tuning(sbg_context ctx, sbg_tuning *t);

@@ -35,6 +35,7 @@ constexpr int kFastKeys = 3;     // distinct keys a fast lane keeps
 constexpr int kFastCands = 48;   // cell entries a fast lane scans alone
 constexpr int kCodeGroup = 12;   // entry codes a fast lane loads at once
 constexpr int kWaveHits = 256;   // hits a wave gathers for one deferred ray
+constexpr int kWaveLoads = 2;    // cell entries a wave lane tests per round
 
 __global__ __launch_bounds__(256) void k_mark_intersected(const uint64_t *__restrict__ pairs,
                                                           const uint8_t *__restrict__ status, uint32_t n, int side,
@@ -190,12 +191,22 @@ struct AxisKeys {
     }
 };
 
-// candidate triangle t: FP64 box test, plane hit, three-normal inside test;
-// on a hit, `key` is the hit point's PositionKey
-__device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ tri64, const Ray &r, Key3 &key)
+// a triangle's 96-B FP64 record: vertices, with the FP64 normal in the pad slots
+struct TriRec {
+    double2 r0, r1, r2, r3, r4, r5;
+};
+
+__device__ __forceinline__ TriRec load_rec(const double *__restrict__ tri64, uint32_t t)
 {
     const double2 *x = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * t);
-    const double2 r0 = x[0], r1 = x[1], r2 = x[2], r3 = x[3], r4 = x[4], r5 = x[5];
+    return {x[0], x[1], x[2], x[3], x[4], x[5]};
+}
+
+// candidate triangle (its record): FP64 box test, plane hit, three-normal
+// inside test; on a hit, `key` is the hit point's PositionKey
+__device__ __forceinline__ bool ray_hit_rec(const TriRec &x, const Ray &r, Key3 &key)
+{
+    const double2 r0 = x.r0, r1 = x.r1, r2 = x.r2, r3 = x.r3, r4 = x.r4, r5 = x.r5;
     const V3 v0 = {r0.x, r0.y, r1.x}, v1 = {r2.x, r2.y, r3.x}, v2 = {r4.x, r4.y, r5.x};
     const V3 e = r.end();
     if (!box64_overlap(tri_box64(v0, v1, v2), Box64{r.p, e}))
@@ -218,6 +229,11 @@ __device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ t
         return false;
     key = position_key(hit);
     return true;
+}
+
+__device__ __forceinline__ bool ray_hit(uint32_t t, const double *__restrict__ tri64, const Ray &r, Key3 &key)
+{
+    return ray_hit_rec(load_rec(tri64, t), r, key);
 }
 
 // ---- per-axis projection grids of the target meshes ------------------------
@@ -548,20 +564,18 @@ __device__ __forceinline__ bool code_may_hit(const uint2 &e, const RayCell &c)
            (e.y & kCellMax) <= (c.v >> 16) && ((e.y >> 12) & kCellMax) >= (c.v & 0xffffu) && ha >= c.pa;
 }
 
-// code test of grid entry k; on success `t` is the triangle
-__device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, const RayCell &c, uint32_t &t)
-{
-    if (!code_may_hit(gs.gcode[k], c))
-        return false;
-    t = gs.gtri[k];
-    return true;
-}
-
-// One lane per (axis, query).  A lane whose cell holds at most kFastCands
-// entries scans them with up to kFastKeys distinct keys in registers; any other
-// lane (a crowded cell, or more distinct hits) hands (axis, query) to
-// k_rays_wave.  Query i's rays go to mesh target[i] (mesh 0 without a target
-// list); a query without a target mesh (~0u) counts as outside.
+// One lane per (axis, query).  A lane whose cell holds
+// at most gs.fastCands entries (48 when unset) scans them with up to
+// kFastKeys distinct keys in registers; any other lane (a crowded cell, or
+// more distinct hits) hands (axis, query) to k_rays_wave.  Query i's rays go
+// to mesh target[i] (mesh 0 without a target list); a query without a target
+// mesh (~0u) counts as outside.
+//
+// The wave's lanes trace rays from neighbouring points, so they share few
+// cells.  The wave first stages each distinct cell's entries (codes and
+// triangles) in LDS with coalesced loads, once per cell, up to kStage
+// entries; a lane then reads its cell from LDS (a cell that did not fit is
+// read from global memory as before).
 //
 // FILTER: each candidate passing the code test is decided from its 48-B FP32
 // record by the certificate of sbg_rayfilter.h (a certain miss, or a certain
@@ -569,6 +583,31 @@ __device__ __forceinline__ bool entry_may_hit(const GridSet &gs, uint32_t k, con
 // and the FP64 test.  Certain hits farther apart than PositionKey's cell are
 // distinct keys; a ray with an uncertain candidate, or with two hits that
 // close, goes to k_rays_wave, which runs it in FP64 like any deferred ray.
+constexpr int kStage = 384;  // cell entries a wave stages in LDS
+constexpr int kStageRounds = kStage / 64;
+
+// Code test of entries [k0, k0 + 64) of a cell of len entries, as a bit mask,
+// in groups of kCodeGroup independent loads (indices clamped into the cell,
+// the excess masked off).  The whole chunk is tested before any candidate's
+// record is loaded, so a lane's record loads are not held behind its code
+// loads; the verdict does not depend on the order the hits arrive in.
+template <typename CodeAt>
+__device__ __forceinline__ unsigned long long pass_mask(const CodeAt &code_at, const RayCell &c, uint32_t k0,
+                                                        uint32_t len)
+{
+    const uint32_t kn = len - k0 < 64u ? len - k0 : 64u;
+    unsigned long long pass = 0;
+    for (uint32_t j0 = 0; j0 < kn; j0 += kCodeGroup) {
+#pragma unroll
+        for (int j = 0; j < kCodeGroup; ++j) {
+            const uint32_t jj = j0 + j < kn ? j0 + j : kn - 1;
+            if (code_may_hit(code_at(k0 + jj), c) && j0 + j < kn)
+                pass |= 1ull << (j0 + j);
+        }
+    }
+    return pass;
+}
+
 template <bool Q4, bool LIST, bool FILTER>
 __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__restrict__ tri64,
                                                    const double *__restrict__ pts,
@@ -578,44 +617,114 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
                                                    uint8_t *__restrict__ bits, uint32_t *__restrict__ ovfList,
                                                    uint32_t *ovfCount, uint32_t *overflow)
 {
-    const uint32_t idx = blockIdx.x * 256 + threadIdx.x;
-    if (idx >= (LIST ? *laneCount : lanes))
+    // each of the workgroup's four waves stages into its own LDS region
+    __shared__ uint2 s_codeW[4][kStage];
+    __shared__ uint32_t s_triW[4][kStage];
+    __shared__ uint32_t s_baseW[4][64], s_startW[4][64];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint2 *s_code = s_codeW[wave];
+    uint32_t *s_tri = s_triW[wave], *s_base = s_baseW[wave], *s_start = s_startW[wave];
+    const uint32_t total = LIST ? *laneCount : lanes;
+    // every exit before the staging below is taken by the whole workgroup
+    if (blockIdx.x * 256 >= total)
         return;
-    const uint32_t g = LIST ? laneList[idx] : idx;
     if (gs.cells[gs.ncells] > gs.cap) {  // grid truncated: the host rebuilds it larger
-        if ((threadIdx.x & 63) == 0)
+        if (threadIdx.x == 0)
             atomicOr(overflow, 2u);
         return;
     }
-    const uint32_t axis = g / n;
-    const uint32_t q = g - axis * n;
-    uint32_t i, m = 0;
+    const uint32_t idx = blockIdx.x * 256 + threadIdx.x;
+    uint32_t g = 0, axis = 0, i = 0, m = 0;
     Ray r;
-    const int kind = load_query<Q4>(gs, pts, order, skip, q, axis, i, m, r);
-    if (kind != 2) {
+    int kind = 0;
+    if (idx < total) {
+        g = LIST ? laneList[idx] : idx;
+        axis = g / n;
+        const uint32_t q = g - axis * n;
+        kind = load_query<Q4>(gs, pts, order, skip, q, axis, i, m, r);
         if (kind == 1)
             bits[(size_t)axis * n + i] = 0;
-        return;
     }
-    const RayCell c = ray_cell(gs, m, r, (int)axis);
-    bool defer = c.range.y - c.range.x > (uint32_t)kFastCands;
+    RayCell c;
+    c.range = make_uint2(0u, 0u);
+    c.u = c.v = c.pa = 0u;
+    if (kind == 2)
+        c = ray_cell(gs, m, r, (int)axis);
+    const uint32_t len = c.range.y - c.range.x;
+    bool defer = kind == 2 && len > (gs.fastCands ? gs.fastCands : (uint32_t)kFastCands);
+    const bool scan = kind == 2 && !defer && len > 0;
+
+    // distinct cells of the scanning lanes, in order of their first lane:
+    // slot table (LDS offset, first entry), and each lane's LDS offset
+    int myBase = -1;
+    uint32_t nslots = 0, used = 0;
+    for (unsigned long long pending = __ballot(scan); pending;) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const uint32_t lx = __shfl(c.range.x, leader), llen = __shfl(len, leader);
+        const bool mine = scan && c.range.x == lx;  // non-empty cells never share a first entry
+        pending &= ~__ballot(mine);
+        if (used + llen > (uint32_t)kStage)
+            continue;
+        if (mine)
+            myBase = (int)used;
+        if (lane == 0) {
+            s_base[nslots] = used;
+            s_start[nslots] = lx;
+        }
+        ++nslots;
+        used += llen;
+    }
+    __syncthreads();  // the slot tables (every wave of the workgroup passes both barriers)
+    if (nslots) {
+        uint2 code[kStageRounds];
+        uint32_t tri[kStageRounds];
+#pragma unroll
+        for (int rr = 0; rr < kStageRounds; ++rr) {
+            const uint32_t t = lane + 64u * rr;
+            code[rr] = make_uint2(0u, 0u);
+            tri[rr] = 0u;
+            if (t < used) {
+                // the last slot starting at or before t
+                uint32_t lo = 0, hi = nslots - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (s_base[mid] <= t)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                const uint32_t e = s_start[lo] + (t - s_base[lo]);
+                code[rr] = gs.gcode[e];
+                tri[rr] = gs.gtri[e];
+            }
+        }
+#pragma unroll
+        for (int rr = 0; rr < kStageRounds; ++rr) {
+            const uint32_t t = lane + 64u * rr;
+            if (t < used) {
+                s_code[t] = code[rr];
+                s_tri[t] = tri[rr];
+            }
+        }
+    }
+    __syncthreads();  // the staged entries
+    if (kind != 2)
+        return;
+    // entry k (0 <= k < len) of this lane's cell
+    auto code_at = [&](uint32_t k) -> uint2 { return myBase >= 0 ? s_code[myBase + k] : gs.gcode[c.range.x + k]; };
+    auto tri_at = [&](uint32_t k) -> uint32_t { return myBase >= 0 ? s_tri[myBase + k] : gs.gtri[c.range.x + k]; };
+    uint32_t parity = 0;
     if (FILTER) {
         const float4 p32 = make_float4((float)r.p.x, (float)r.p.y, (float)r.p.z, 0.0f);
         const double pmax = fmax(fmax(__builtin_fabs(r.p.x), __builtin_fabs(r.p.y)), __builtin_fabs(r.p.z));
         const float pm = f_up(pmax);
         HitSpans<kFastKeys> hits;
-        for (uint32_t k0 = c.range.x; !defer && k0 < c.range.y; k0 += kCodeGroup) {
-            uint32_t pass = 0;
-#pragma unroll
-            for (int j = 0; j < kCodeGroup; ++j) {
-                const uint32_t k = k0 + j < c.range.y ? k0 + j : c.range.y - 1;
-                if (code_may_hit(gs.gcode[k], c) && k0 + j < c.range.y)
-                    pass |= 1u << j;
-            }
+        for (uint32_t k0 = 0; !defer && k0 < len; k0 += 64) {
+            unsigned long long pass = pass_mask(code_at, c, k0, len);
             while (pass && !defer) {
-                const int j = __builtin_ctz(pass);
+                const uint32_t k = k0 + (uint32_t)__builtin_ctzll(pass);
                 pass &= pass - 1;
-                const float4 *rec = reinterpret_cast<const float4 *>(gs.tri32 + (size_t)kTri32Stride * gs.gtri[k0 + j]);
+                const float4 *rec = reinterpret_cast<const float4 *>(gs.tri32 + (size_t)kTri32Stride * tri_at(k));
                 const float4 rr[3] = {rec[0], rec[1], rec[2]};
                 const RayCert cert = ray_certificate(p32, pm, (int)axis, rr);
                 if (cert.verdict == kRayUnsure) {
@@ -626,47 +735,33 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
                 }
             }
         }
-        if (!defer) {
-            bits[(size_t)axis * n + i] = (uint8_t)(hits.n & 1);
-            return;
-        }
-        const uint32_t slot = atomicAdd(ovfCount, 1u);
-        uint4 *rec = reinterpret_cast<uint4 *>(ovfList) + 2 * (size_t)slot;
-        rec[0] = make_uint4(g, c.range.x, c.range.y, i);
-        rec[1] = make_uint4(c.u, c.v, c.pa, 0u);
-        return;
-    }
-    AxisKeys<kFastKeys> keys;
-    // codes in groups of kCodeGroup independent loads (indices clamped into the
-    // range, the excess masked off), then the FP64 test of the entries passing
-    for (uint32_t k0 = c.range.x; !defer && k0 < c.range.y; k0 += kCodeGroup) {
-        uint32_t pass = 0;
-#pragma unroll
-        for (int j = 0; j < kCodeGroup; ++j) {
-            const uint32_t k = k0 + j < c.range.y ? k0 + j : c.range.y - 1;
-            if (code_may_hit(gs.gcode[k], c) && k0 + j < c.range.y)
-                pass |= 1u << j;
-        }
-        while (pass && !defer) {
-            const int j = __builtin_ctz(pass);
-            pass &= pass - 1;
-            Key3 key;
-            if (ray_hit(gs.gtri[k0 + j], tri64, r, key)) {
-                keys.add(key, (int)axis);
-                defer = keys.full;
+        parity = hits.n & 1;
+    } else {
+        AxisKeys<kFastKeys> keys;
+        for (uint32_t k0 = 0; !defer && k0 < len; k0 += 64) {
+            unsigned long long pass = pass_mask(code_at, c, k0, len);
+            while (pass && !defer) {
+                const uint32_t k = k0 + (uint32_t)__builtin_ctzll(pass);
+                pass &= pass - 1;
+                Key3 key;
+                if (ray_hit(tri_at(k), tri64, r, key)) {
+                    keys.add(key, (int)axis);
+                    defer = keys.full;
+                }
             }
         }
+        parity = keys.n & 1;
     }
-    if (defer) {
-        // slot < lanes of this launch: every lane appends at most once.  The
-        // record carries what the wave path would otherwise recompute.
-        const uint32_t slot = atomicAdd(ovfCount, 1u);
-        uint4 *rec = reinterpret_cast<uint4 *>(ovfList) + 2 * (size_t)slot;
-        rec[0] = make_uint4(g, c.range.x, c.range.y, i);
-        rec[1] = make_uint4(c.u, c.v, c.pa, 0u);
+    if (!defer) {
+        bits[(size_t)axis * n + i] = (uint8_t)parity;
         return;
     }
-    bits[(size_t)axis * n + i] = (uint8_t)(keys.n & 1);
+    // slot < lanes of this launch: every lane appends at most once.  The
+    // record carries what the wave path would otherwise recompute.
+    const uint32_t slot = atomicAdd(ovfCount, 1u);
+    uint4 *rec = reinterpret_cast<uint4 *>(ovfList) + 2 * (size_t)slot;
+    rec[0] = make_uint4(g, c.range.x, c.range.y, i);
+    rec[1] = make_uint4(c.u, c.v, c.pa, 0u);
 }
 
 // One wave per deferred ray (record of 8 u32 written by k_rays_grid: lane
@@ -704,20 +799,38 @@ __global__ __launch_bounds__(64) void k_rays_wave(GridSet gs, const double *__re
         } else {
             r = ray_from({pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]}, axis);
         }
+        // kWaveLoads entries per lane per round, each stage's loads issued
+        // together: codes, then the triangles passing, then their records
         uint32_t nh = 0;
-        for (uint32_t k0 = c.range.x; k0 < c.range.y; k0 += 64) {
-            const uint32_t e = k0 + lane;
-            uint32_t t;
-            Key3 key;
-            const bool hit = e < c.range.y && entry_may_hit(gs, e, c, t) && ray_hit(t, tri64, r, key);
-            const unsigned long long mask = __ballot(hit);
-            const uint32_t pos = nh + (uint32_t)__popcll(mask & below);
-            if (hit && pos < (uint32_t)kWaveHits) {
-                s_key[0][pos] = key.x;
-                s_key[1][pos] = key.y;
-                s_key[2][pos] = key.z;
+        for (uint32_t k0 = c.range.x; k0 < c.range.y; k0 += 64 * kWaveLoads) {
+            bool pass[kWaveLoads];
+#pragma unroll
+            for (int j = 0; j < kWaveLoads; ++j) {
+                const uint32_t e = k0 + lane + 64 * j;
+                pass[j] = e < c.range.y && code_may_hit(gs.gcode[e], c);
             }
-            nh += (uint32_t)__popcll(mask);
+            uint32_t t[kWaveLoads];
+#pragma unroll
+            for (int j = 0; j < kWaveLoads; ++j)
+                t[j] = pass[j] ? gs.gtri[k0 + lane + 64 * j] : 0u;
+            TriRec rec[kWaveLoads];
+#pragma unroll
+            for (int j = 0; j < kWaveLoads; ++j)
+                if (pass[j])
+                    rec[j] = load_rec(tri64, t[j]);
+#pragma unroll
+            for (int j = 0; j < kWaveLoads; ++j) {
+                Key3 key;
+                const bool hit = pass[j] && ray_hit_rec(rec[j], r, key);
+                const unsigned long long mask = __ballot(hit);
+                const uint32_t pos = nh + (uint32_t)__popcll(mask & below);
+                if (hit && pos < (uint32_t)kWaveHits) {
+                    s_key[0][pos] = key.x;
+                    s_key[1][pos] = key.y;
+                    s_key[2][pos] = key.z;
+                }
+                nh += (uint32_t)__popcll(mask);
+            }
         }
         __syncthreads();
         if (nh > (uint32_t)kWaveHits) {
@@ -789,7 +902,7 @@ __global__ __launch_bounds__(256) void k_ray_work(GridSet gs, const double *__re
         if (load_query<Q4>(gs, pts, order, skip, q, axis, i, m, r) == 2) {
             cast = 1;
             const RayCell c = ray_cell(gs, m, r, (int)axis);
-            if (c.range.y - c.range.x > (uint32_t)kFastCands) {
+            if (c.range.y - c.range.x > (gs.fastCands ? gs.fastCands : (uint32_t)kFastCands)) {
                 deferred = 1;
             } else {
                 scanned = c.range.y - c.range.x;

@@ -79,6 +79,7 @@ struct sbg_mesh_s {
     // reads or rewrites the mesh waits for
     hipStream_t stream = nullptr;
     hipEvent_t used = nullptr;
+    hipEvent_t prepDone = nullptr;  // end of the last prepare (the grid stream's start)
     bool pendingOnAux = false;  // `used` was last recorded on the second stream and not yet waited for
     DevBuf<uint32_t> hist;  // the Morton sort's histograms (per mesh: prepares overlap)
     DevBuf<double> V;
@@ -192,13 +193,16 @@ int sbg_context_create(int device, sbg_context *out)
         return fail(SBG_ERR_OOM, "hipHostMalloc failed");
     }
     if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->grids, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->auxReady, hipEventDisableTiming) != hipSuccess) {
+        if (c->grids)
+            (void)hipStreamDestroy(c->grids);
         if (c->aux)
             (void)hipStreamDestroy(c->aux);
         (void)hipHostFree(c->pinned);
         (void)hipStreamDestroy(c->stream);
         delete c;
-        return fail(SBG_ERR_HIP, "second stream creation failed");
+        return fail(SBG_ERR_HIP, "second or third stream creation failed");
     }
     *out = c;
     return SBG_OK;
@@ -234,6 +238,15 @@ static void mesh_free_buffers(sbg_mesh m)
     m->gridTri.release();
 }
 
+static void mesh_free_events(sbg_mesh m)
+{
+    for (hipEvent_t *e : {&m->used, &m->prepDone, &m->gridDone})
+        if (*e) {
+            (void)hipEventDestroy(*e);
+            *e = nullptr;
+        }
+}
+
 // a new mesh, on the buffers of a destroyed one when there is one (a caller
 // that makes and drops meshes per boolean then allocates nothing)
 static sbg_mesh mesh_new(sbg_context c)
@@ -244,17 +257,16 @@ static sbg_mesh mesh_new(sbg_context c)
         // new mesh's uploads do not go through the streams
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->aux);
+        (void)hipStreamSynchronize(c->grids);
         m = c->spareMeshes.back();
         c->spareMeshes.pop_back();
     } else {
         m = new sbg_mesh_s;
     }
     if ((!m->used && hipEventCreateWithFlags(&m->used, hipEventDisableTiming) != hipSuccess) ||
+        (!m->prepDone && hipEventCreateWithFlags(&m->prepDone, hipEventDisableTiming) != hipSuccess) ||
         (!m->gridDone && hipEventCreateWithFlags(&m->gridDone, hipEventDisableTiming) != hipSuccess)) {
-        if (m->used)
-            (void)hipEventDestroy(m->used);
-        if (m->gridDone)
-            (void)hipEventDestroy(m->gridDone);
+        mesh_free_events(m);
         delete m;
         return nullptr;
     }
@@ -361,18 +373,17 @@ void context_free(sbg_context ctx)
         delete ctx->sparePairs;
         ctx->sparePairs = nullptr;
     }
+    (void)hipStreamSynchronize(ctx->aux);
+    (void)hipStreamSynchronize(ctx->grids);
     for (sbg_mesh m : ctx->spareMeshes) {
         mesh_free_buffers(m);
-        if (m->used)
-            (void)hipEventDestroy(m->used);
-        if (m->gridDone)
-            (void)hipEventDestroy(m->gridDone);
+        mesh_free_events(m);
         delete m;
     }
     ctx->spareMeshes.clear();
     (void)hipHostFree(ctx->pinned);
-    (void)hipStreamSynchronize(ctx->aux);
     (void)hipEventDestroy(ctx->auxReady);
+    (void)hipStreamDestroy(ctx->grids);
     (void)hipStreamDestroy(ctx->aux);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -491,11 +502,9 @@ int sbg_mesh_destroy(sbg_mesh m)
     } else {
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->aux);
+        (void)hipStreamSynchronize(c->grids);
         mesh_free_buffers(m);
-        if (m->used)
-            (void)hipEventDestroy(m->used);
-        if (m->gridDone)
-            (void)hipEventDestroy(m->gridDone);
+        mesh_free_events(m);
         delete m;
     }
     context_release(c);
@@ -512,6 +521,8 @@ int sbg_mesh_counts(sbg_mesh m, size_t *nverts, size_t *ntris)
         *ntris = m->nt;
     return SBG_OK;
 }
+
+static int ensure_grid(sbg_mesh m, hipStream_t stream);
 
 int sbg_mesh_prepare(sbg_mesh m)
 {
@@ -587,6 +598,13 @@ int sbg_mesh_prepare(sbg_mesh m)
         m->pendingOnAux = true;
     }
     m->prepared = true;
+    if (c->tune.early_grids == 2) {
+        // the ray grid on the third stream, after this prepare: the pair
+        // search waits for the prepare only, and the grid is built beside it
+        HIP_TRY(hipEventRecord(m->prepDone, s));
+        HIP_TRY(hipStreamWaitEvent(c->grids, m->prepDone, 0));
+        TRY(ensure_grid(m, c->grids));
+    }
     return SBG_OK;
 }
 
@@ -1141,12 +1159,13 @@ static GridSet grid_set(sbg_mesh m, bool filtered = false)
     gs.cap = (uint32_t)m->gridCap;
     gs.ncells = (uint32_t)(grid_cells(m->gridG) - 1);
     gs.order = m->sortedIds;
+    gs.fastCands = (uint32_t)m->ctx->tune.ray_cell_cap;
     return gs;
 }
 
 // Builds m's ray grid on `stream` unless it is current; a reader on another
 // stream than the build's waits for the build.
-static int ensure_grid(sbg_mesh m, hipStream_t stream = nullptr)
+static int ensure_grid(sbg_mesh m, hipStream_t stream)
 {
     sbg_context c = m->ctx;
     const hipStream_t s = stream ? stream : c->stream;
@@ -1427,12 +1446,12 @@ int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t)
     if (!t)
         return fail(SBG_ERR_INVALID, "null tuning");
     if (t->bfs_levels < 0 || t->bfs_levels > 64 || t->dfs_waves < 64 || t->dfs_waves > 16384 ||
-        t->grid_density < 1 || t->grid_density > 64 ||
+        t->grid_density < 1 || t->grid_density > 64 || t->early_grids < 0 || t->early_grids > 2 ||
+        t->ray_cell_cap < 16 || t->ray_cell_cap > 4096 ||
         (t->ray_precision != SBG_PRECISION_FP64 && t->ray_precision != SBG_PRECISION_FP32_FILTERED))
         return fail(SBG_ERR_INVALID, "tuning out of range");
     ctx->tune = *t;
     ctx->tune.depth_first = t->depth_first ? 1 : 0;
-    ctx->tune.early_grids = t->early_grids ? 1 : 0;
     return SBG_OK;
 }
 
@@ -1445,11 +1464,14 @@ int sbg_context_get_tuning(sbg_context ctx, sbg_tuning *t)
     return SBG_OK;
 }
 
-// wall time of find_pairs -> tri_tri -> classify both meshes, `reps` times
+// wall time of prepare both -> find_pairs -> tri_tri -> classify both
+// meshes, `reps` times
 static int time_pair_chain(sbg_mesh a, sbg_mesh b, int reps, double &ms)
 {
     std::vector<uint8_t> ca(a->nt + 1), cb(b->nt + 1);
     auto once = [&]() -> int {
+        TRY(sbg_mesh_prepare(a));
+        TRY(sbg_mesh_prepare(b));
         sbg_pairs p = nullptr;
         TRY(sbg_find_pairs(a, b, &p));
         int rc = sbg_tri_tri(a, b, p, SBG_PRECISION_FP32_FILTERED, nullptr);
@@ -1460,10 +1482,8 @@ static int time_pair_chain(sbg_mesh a, sbg_mesh b, int reps, double &ms)
     };
     TRY(once());  // builds grids and sizes queues for this setting
     const auto t0 = std::chrono::steady_clock::now();
-    for (int r = 0; r < reps; ++r) {
-        a->gridReady = b->gridReady = false;  // a step rebuilds the grids, as a new mesh would
+    for (int r = 0; r < reps; ++r)
         TRY(once());
-    }
     ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
     return SBG_OK;
 }
@@ -1484,6 +1504,7 @@ int sbg_autotune_pair(sbg_mesh a, sbg_mesh b, int reps, sbg_tuning *best, double
     const int levels[] = {8, 12, 16, 20, 24};
     const int waves[] = {256, 512, 1024, 2048, 4096};
     const int density[] = {2, 4, 8, 16};
+    const int cellCap[] = {48, 64, 96, 128};
     auto trial = [&](const sbg_tuning &t) -> int {
         c->tune = t;
         double ms = 0.0;
@@ -1513,9 +1534,18 @@ int sbg_autotune_pair(sbg_mesh a, sbg_mesh b, int reps, sbg_tuning *best, double
                 t.grid_density = v;
                 TRY(trial(t));
             }
-        sbg_tuning t = cur;
-        t.early_grids = !cur.early_grids;
-        TRY(trial(t));
+        for (int v : cellCap)
+            if (v != cur.ray_cell_cap) {
+                sbg_tuning t = cur;
+                t.ray_cell_cap = v;
+                TRY(trial(t));
+            }
+        for (int v = 0; v <= 2; ++v)
+            if (v != cur.early_grids) {
+                sbg_tuning t = cur;
+                t.early_grids = v;
+                TRY(trial(t));
+            }
     }
     c->tune = cur;
     if (best)
@@ -1652,7 +1682,7 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
     launch_fill_u32((uint32_t *)c->flags.p, 0u, (nt + 3) / 4, c->stream);
     launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, c->flags.p, c->stream);
     launch_centroids_ordered(self->tri64.p, c->flags.p, self->sortedIds, nullptr, (uint32_t)nt, c->pts.p, c->stream);
-    TRY(ensure_grid(other));
+    TRY(ensure_grid(other, c->stream));
     const GridSet gs = grid_set(other, c->tune.ray_precision == SBG_PRECISION_FP32_FILTERED);
     std::vector<hipEvent_t> ev(2 * (size_t)iters, nullptr);
     for (auto &e : ev)

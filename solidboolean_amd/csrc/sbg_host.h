@@ -107,12 +107,13 @@ struct sbg_context_s {
     hipStream_t stream = nullptr;
     hipStream_t aux = nullptr;        // second stream (default priority): side 1 of sbg_classify_pair
     hipEvent_t auxReady = nullptr;    // orders the aux stream after the main one
+    hipStream_t grids = nullptr;      // third stream (default priority): ray grids built at prepare time
     uint32_t *pinned = nullptr;  // host-visible counters (kPinnedWords): search, then two ray sides
     DevBuf<uint2> front[2];
     DevBuf<uint2> spill;              // depth-first spill regions
     uint32_t spillPerWave = kSpillInit;
     bool depthFirst = true;           // false after the spill regions proved too small
-    sbg_tuning tune = {kBfsLevels, kDfsWaves, 2, 1, 1, SBG_PRECISION_FP32_FILTERED};
+    sbg_tuning tune = {kBfsLevels, kDfsWaves, 2, 1, 2, SBG_PRECISION_FP32_FILTERED, 64};
     DevBuf<uint32_t> counters;
     size_t frontCap = 0, candCap = 0;
     // candidates expected from the next search (the last count plus a margin):

@@ -138,6 +138,7 @@ struct GridSet {
     // every candidate in FP64)
     const float *tri32;
     uint32_t G0, cap, ncells;
+    uint32_t fastCands;  // cell entries one lane scans before its ray goes to the wave kernel (0: 48)
 };
 uint32_t grid_dim(uint32_t nt, int density);
 size_t grid_cells(uint32_t G);  // 3 G^2 + 1

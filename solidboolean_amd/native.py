@@ -62,7 +62,7 @@ class BatchTuning(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("bfs_levels", C.c_int), ("dfs_waves", C.c_int), ("grid_density", C.c_int), ("depth_first", C.c_int),
-                ("early_grids", C.c_int), ("ray_precision", C.c_int)]
+                ("early_grids", C.c_int), ("ray_precision", C.c_int), ("ray_cell_cap", C.c_int)]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -264,8 +264,10 @@ class Context:
         _check(self.L.sbg_context_set_tuning(self.h, C.byref(t)))
 
     def autotune_pair(self, a: "Mesh", b: "Mesh", reps: int = 3):
-        """Coordinate search of the single-pair knobs (sbg_autotune_pair);
-        leaves the fastest set and returns it with its chain time in ms."""
+        """Coordinate search of the single-pair knobs (sbg_autotune_pair):
+        times prepare both -> find_pairs -> tri_tri -> classify_pair per
+        setting (re-preparing both meshes), leaves the fastest set and
+        returns it with its chain time in ms."""
         t = Tuning()
         ms = C.c_double(0)
         _check(self.L.sbg_autotune_pair(a.h, b.h, reps, C.byref(t), C.byref(ms)))

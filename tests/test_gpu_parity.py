@@ -340,11 +340,12 @@ def test_stream_ordering(gpu):
     from oracle.oracle import normal
     nb = np.array([normal(*b.vertices[t]) for t in b.triangles])
     saved = gpu.tuning()
-    for step in range(4):
-        # odd steps build the ray grids when classification asks for them,
-        # even ones queue them on the second stream beside the pair search
-        gpu.set_tuning(early_grids=1 - step % 2)
-        if step == 2:
+    for step in range(6):
+        # the ray grids built at prepare time on the third stream (early_grids
+        # 2), queued on the second stream beside the pair search (1), or built
+        # when classification asks for them (0)
+        gpu.set_tuning(early_grids=2 - step % 3)
+        if step in (2, 3):
             # a search whose grids are still being built when the meshes
             # prepare again
             A.find_pairs(B)
@@ -444,7 +445,8 @@ def test_determinism(gpu):
 
 def test_tuning_changes_speed_not_results(gpu):
     """Every setting of the single-pair knobs (breadth-first levels, depth-first
-    waves, breadth-first only, ray-grid density) gives the same pairs,
+    waves, breadth-first only, ray-grid density, the cell size past which a
+    ray gets a whole wave, when the grids are built) gives the same pairs,
     segments and classifications."""
     a, b = mg.icosphere_pair(5, 4, (0.3, -0.2, 0.1), noise=1e-3)
     A, B = gpu.mesh(a.vertices, a.triangles), gpu.mesh(b.vertices, b.triangles)
@@ -459,7 +461,9 @@ def test_tuning_changes_speed_not_results(gpu):
     try:
         ref = run()
         for kw in ({"bfs_levels": 0}, {"bfs_levels": 64, "dfs_waves": 64}, {"depth_first": 0},
-                   {"grid_density": 1}, {"grid_density": 64}, {"dfs_waves": 16384, "bfs_levels": 4}):
+                   {"grid_density": 1}, {"grid_density": 64}, {"dfs_waves": 16384, "bfs_levels": 4},
+                   {"ray_cell_cap": 16}, {"ray_cell_cap": 4096, "grid_density": 1},
+                   {"ray_cell_cap": 16, "ray_precision": N.FP64}, {"early_grids": 0}, {"early_grids": 1}):
             gpu.set_tuning(**{**saved, **kw})
             got = run()
             assert all(np.array_equal(x, y) for x, y in zip(ref, got)), kw

@@ -68,7 +68,12 @@ def test_noalloc_matches_allocating_entry(gpu):
 
 def test_noalloc_limits(gpu):
     s = mg.icosphere(3)
-    M = gpu.mesh(s.vertices, s.triangles)
+    saved = gpu.tuning()
+    gpu.set_tuning(early_grids=0)  # no ray grid at prepare time
+    try:
+        M = gpu.mesh(s.vertices, s.triangles)
+    finally:
+        gpu.set_tuning(**saved)
     pts = np.random.default_rng(9).uniform(-1.5, 1.5, (LIMIT + 1, 3))
     rc, _, _ = noalloc(M, pts[:10], True)
     assert rc == N.SBG_ERR_STATE   # no ray grid yet
