@@ -143,10 +143,12 @@ __device__ __forceinline__ uint64_t cand_key(const Expansion &x, int s)
     return ((uint64_t)(p.x & ~kLeafBit) << 32) | (p.y & ~kLeafBit);
 }
 
-__global__ __launch_bounds__(256) void k_expand(TraverseArgs t, const uint2 *__restrict__ in,
-                                                const uint32_t *__restrict__ inCount, uint2 *__restrict__ out,
-                                                uint32_t *outCount, uint32_t outCap, uint64_t *__restrict__ cand,
-                                                uint32_t *candCount, uint32_t candCap, uint32_t *overflow)
+// One breadth-first level over the whole grid: frontier `in` (inCount pairs)
+// to `out`, candidates appended.
+__device__ __forceinline__ void expand_level(const TraverseArgs &t, const uint2 *__restrict__ in,
+                                             const uint32_t *__restrict__ inCount, uint2 *__restrict__ out,
+                                             uint32_t *outCount, uint32_t outCap, uint64_t *__restrict__ cand,
+                                             uint32_t *candCount, uint32_t candCap, uint32_t *overflow)
 {
     // both frontier buffers hold outCap pairs; a count past it has already
     // raised `overflow` and the whole search is repeated with larger queues
@@ -176,6 +178,106 @@ __global__ __launch_bounds__(256) void k_expand(TraverseArgs t, const uint2 *__r
     }
 }
 
+__global__ __launch_bounds__(256) void k_expand(TraverseArgs t, const uint2 *__restrict__ in,
+                                                const uint32_t *__restrict__ inCount, uint2 *__restrict__ out,
+                                                uint32_t *outCount, uint32_t outCap, uint64_t *__restrict__ cand,
+                                                uint32_t *candCount, uint32_t candCap, uint32_t *overflow)
+{
+    expand_level(t, in, inCount, out, outCount, outCap, cand, candCount, candCap, overflow);
+}
+
+// ---- top of the search in one workgroup -------------------------------------
+// The first levels of a tree-vs-tree search hold a handful of node pairs, and a
+// grid-wide launch per level costs a launch's latency for a few lanes of work
+// (and waits for free slots behind any kernel that fills the chip).  One
+// workgroup of 1024 lanes expands level after level instead, frontier in the
+// two global frontier buffers, next-frontier slots handed out from LDS, until
+// a level holds more than topMax pairs, the level cap is reached or the
+// frontier empties.  cnt[l] gets every level's size as the grid-wide kernel
+// would leave it; *levelOut gets the first level not expanded here.
+__global__ __launch_bounds__(1024) void k_search_top(TraverseArgs t, uint2 *__restrict__ front0,
+                                                     uint2 *__restrict__ front1, uint32_t *__restrict__ cnt,
+                                                     uint32_t fcap, uint64_t *__restrict__ cand, uint32_t *candCount,
+                                                     uint32_t candCap, uint32_t *overflow, int maxLevels,
+                                                     uint32_t topMax, uint32_t *levelOut)
+{
+    __shared__ uint32_t s_next;
+    const int lane = threadIdx.x & 63;
+    uint32_t nIn = cnt[0];
+    int level = 0;
+    while (level < maxLevels && nIn > 0 && nIn <= topMax) {
+        const uint2 *in = (level & 1) ? front1 : front0;
+        uint2 *out = (level & 1) ? front0 : front1;
+        if (threadIdx.x == 0)
+            s_next = 0;
+        __syncthreads();
+        for (uint32_t b = 0; b < nIn; b += 1024) {
+            const uint32_t i = b + threadIdx.x;
+            const bool valid = i < nIn;
+            const Expansion x = expand_pair(t, valid ? in[i] : make_uint2(0, 0), valid);
+            // next-frontier slots: a wave prefix, one LDS atomic per wave
+            const uint32_t nn = __popc(x.next);
+            uint32_t incl = nn;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t v = __shfl_up(incl, off, 64);
+                if (lane >= off)
+                    incl += v;
+            }
+            const uint32_t total = __shfl(incl, 63, 64);
+            uint32_t wbase = 0;
+            if (lane == 63 && total)
+                wbase = atomicAdd(&s_next, total);
+            wbase = __shfl(wbase, 63, 64);
+            const uint32_t on = wbase + incl - nn;
+            if (total && wbase + total > fcap && lane == 0)
+                atomicOr(overflow, 1u);
+            uint32_t wb;
+            const uint32_t oc = wave_reserve(__popc(x.cand), candCount, candCap, overflow, wb);
+            uint32_t kn = 0, kc = 0;
+#pragma unroll
+            for (int sl = 0; sl < 4; ++sl) {
+                if ((x.next >> sl) & 1u) {
+                    if (on + kn < fcap)
+                        out[on + kn] = slot_pair(x, sl);
+                    ++kn;
+                }
+                if ((x.cand >> sl) & 1u) {
+                    if (oc + kc < candCap)
+                        cand[oc + kc] = cand_key(x, sl);
+                    ++kc;
+                }
+            }
+        }
+        // the next level reads this one's output: workgroup-scope ordering of
+        // the global frontier writes, and the count from LDS
+        __syncthreads();
+        nIn = s_next;
+        if (threadIdx.x == 0)
+            cnt[level + 1] = nIn;
+        ++level;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *levelOut = (uint32_t)level;
+}
+
+// Breadth-first level *topLevel + j over the whole grid, or nothing when that
+// is at or past lastLevel (the launches after the top kernel are queued before
+// it is known where it stopped).
+__global__ __launch_bounds__(256) void k_expand_at(TraverseArgs t, uint2 *__restrict__ front0,
+                                                   uint2 *__restrict__ front1, uint32_t *__restrict__ cnt,
+                                                   const uint32_t *__restrict__ topLevel, uint32_t j,
+                                                   uint32_t lastLevel, uint32_t fcap, uint64_t *__restrict__ cand,
+                                                   uint32_t *candCount, uint32_t candCap, uint32_t *overflow)
+{
+    const uint32_t level = *topLevel + j;
+    if (level >= lastLevel)
+        return;
+    const uint2 *in = (level & 1) ? front1 : front0;
+    uint2 *out = (level & 1) ? front0 : front1;
+    expand_level(t, in, cnt + level, out, cnt + level + 1, fcap, cand, candCount, candCap, overflow);
+}
+
 // Persistent depth-first stage.  One wave per workgroup keeps a stack of node
 // pairs in LDS (spilling to a per-wave region of global memory when full).
 // Each step pops up to 64 pairs (one per lane), expands them as k_expand
@@ -188,14 +290,25 @@ __global__ __launch_bounds__(256) void k_expand(TraverseArgs t, const uint2 *__r
 constexpr int kDfsLds = 1024;
 constexpr uint32_t kDfsMaxIter = 1u << 24;
 
+// With levelBase, the seeds are the frontier of level min(*levelBase +
+// levelAdd, lastLevel): `seeds` / `seedsAlt` are the even / odd levels'
+// frontier buffers and seedCount the per-level counts.
 __global__ __launch_bounds__(64) void k_traverse_dfs(TraverseArgs t, const uint2 *__restrict__ seeds,
+                                                     const uint2 *__restrict__ seedsAlt,
                                                      const uint32_t *__restrict__ seedCount, uint32_t seedCap,
-                                                     uint32_t *head, uint2 *__restrict__ spill, uint32_t spillPerWave,
-                                                     uint64_t *__restrict__ cand, uint32_t *candCount,
-                                                     uint32_t candCap, uint32_t *overflow)
+                                                     const uint32_t *__restrict__ levelBase, uint32_t levelAdd,
+                                                     uint32_t lastLevel, uint32_t *head, uint2 *__restrict__ spill,
+                                                     uint32_t spillPerWave, uint64_t *__restrict__ cand,
+                                                     uint32_t *candCount, uint32_t candCap, uint32_t *overflow)
 {
     __shared__ uint2 s_stack[kDfsLds];
     const int lane = threadIdx.x;
+    if (levelBase) {
+        uint32_t level = *levelBase + levelAdd;
+        level = level < lastLevel ? level : lastLevel;
+        seeds = (level & 1) ? seedsAlt : seeds;
+        seedCount += level;
+    }
     const uint32_t nseed = *seedCount < seedCap ? *seedCount : seedCap;
     // seeds per claim: enough for every wave to start busy, at most one per lane
     uint32_t grab = nseed / gridDim.x;
@@ -283,8 +396,28 @@ void launch_traverse_dfs(const TraverseArgs &t, const uint2 *seeds, const uint32
                          uint32_t *head, uint2 *spill, uint32_t spillPerWave, int waves, uint64_t *cand,
                          uint32_t *candCount, uint32_t candCap, uint32_t *overflow, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_traverse_dfs, dim3(waves), dim3(64), 0, s, t, seeds, seedCount, seedCap, head, spill,
-                       spillPerWave, cand, candCount, candCap, overflow);
+    hipLaunchKernelGGL(k_traverse_dfs, dim3(waves), dim3(64), 0, s, t, seeds, seeds, seedCount, seedCap,
+                       (const uint32_t *)nullptr, 0u, 0u, head, spill, spillPerWave, cand, candCount, candCap,
+                       overflow);
+}
+
+// The single-pair search after k_seed: the top kernel, `extra` grid-wide
+// levels after it (each a no-op past lastLevel), then the depth-first stage
+// from the level reached.  levelWord: one u32 of scratch.
+void launch_search_levels(const TraverseArgs &t, uint2 *front0, uint2 *front1, uint32_t *cnt, uint32_t fcap,
+                          uint32_t topMax, int lastLevel, int extra, uint32_t *levelWord, uint32_t *head,
+                          uint2 *spill, uint32_t spillPerWave, int waves, uint64_t *cand, uint32_t *candCount,
+                          uint32_t candCap, uint32_t *overflow, int expandGrid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_search_top, dim3(1), dim3(1024), 0, s, t, front0, front1, cnt, fcap, cand, candCount,
+                       candCap, overflow, lastLevel, topMax, levelWord);
+    for (int j = 0; j < extra; ++j)
+        hipLaunchKernelGGL(k_expand_at, dim3(expandGrid), dim3(256), 0, s, t, front0, front1, cnt,
+                           (const uint32_t *)levelWord, (uint32_t)j, (uint32_t)lastLevel, fcap, cand, candCount,
+                           candCap, overflow);
+    hipLaunchKernelGGL(k_traverse_dfs, dim3(waves), dim3(64), 0, s, t, (const uint2 *)front0, (const uint2 *)front1,
+                       (const uint32_t *)cnt, fcap, (const uint32_t *)levelWord, (uint32_t)extra,
+                       (uint32_t)lastLevel, head, spill, spillPerWave, cand, candCount, candCap, overflow);
 }
 
 // root pair: both meshes non-empty; a root may itself be a leaf (1 triangle)

@@ -800,6 +800,7 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
         uint32_t *candCount = cnt + kMaxLevels + 1;    // candidates
         uint32_t *overflow = cnt + kMaxLevels + 2;     // overflow flags
         uint32_t *head = cnt + kMaxLevels + 3;         // depth-first seed claims
+        uint32_t *topLevel = cnt + kMaxLevels + 4;     // first level past the one-workgroup top
         const uint32_t fcap = (uint32_t)std::min<size_t>(c->frontCap, 0xffffffffu);
         const uint32_t ccap = (uint32_t)std::min<size_t>(c->candCap, 0xffffffffu);
         const bool dfs = c->depthFirst && c->tune.depth_first;
@@ -815,13 +816,12 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
             launch_seed(t, a->root, b->root, a->rootBox.p, b->rootBox.p, c->front[0].p, cnt, p->pairs.p, candCount,
                         c->stream);
             if (dfs) {
-                // a fixed number of levels with no host round trip, then one
+                // up to bfs_levels breadth-first levels with no host round
+                // trip (the small top levels in one workgroup), then one
                 // persistent launch over the frontier, then one sync
-                for (; level < c->tune.bfs_levels; ++level)
-                    launch_expand(t, c->front[level & 1].p, cnt + level, c->front[(level + 1) & 1].p, cnt + level + 1,
-                                  fcap, p->pairs.p, candCount, ccap, overflow, kExpandGrid, c->stream);
-                launch_traverse_dfs(t, c->front[level & 1].p, cnt + level, fcap, head, c->spill.p, c->spillPerWave,
-                                    c->tune.dfs_waves, p->pairs.p, candCount, ccap, overflow, c->stream);
+                launch_search_levels(t, c->front[0].p, c->front[1].p, cnt, fcap, kTopMax, c->tune.bfs_levels,
+                                     kExtraLevels, topLevel, head, c->spill.p, c->spillPerWave, c->tune.dfs_waves,
+                                     p->pairs.p, candCount, ccap, overflow, kExpandGrid, c->stream);
                 rc = check_launch("depth-first search");
                 done = true;
             }

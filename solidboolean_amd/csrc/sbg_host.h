@@ -96,6 +96,11 @@ constexpr int kLevelsPerSync = 8;
 // frontier, its wave count, and the initial / largest spill region per wave
 constexpr int kBfsLevels = 24;
 constexpr int kDfsWaves = 512;
+// the single-pair search expands levels of at most kTopMax node pairs in one
+// workgroup, then queues kExtraLevels grid-wide levels (no-ops once
+// bfs_levels is reached) before the depth-first stage
+constexpr uint32_t kTopMax = 2048;
+constexpr int kExtraLevels = 8;
 constexpr uint32_t kSpillInit = 2048, kSpillMax = 1u << 15;
 
 }  // namespace sbg

@@ -93,6 +93,13 @@ void launch_expand(const TraverseArgs &t, const uint2 *in, const uint32_t *inCou
 void launch_traverse_dfs(const TraverseArgs &t, const uint2 *seeds, const uint32_t *seedCount, uint32_t seedCap,
                          uint32_t *head, uint2 *spill, uint32_t spillPerWave, int waves, uint64_t *cand,
                          uint32_t *candCount, uint32_t candCap, uint32_t *overflow, hipStream_t s);
+// the single-pair search after launch_seed: levels in one workgroup while a
+// level holds at most topMax pairs, `extra` grid-wide levels, then the
+// depth-first stage (levels up to lastLevel breadth-first)
+void launch_search_levels(const TraverseArgs &t, uint2 *front0, uint2 *front1, uint32_t *cnt, uint32_t fcap,
+                          uint32_t topMax, int lastLevel, int extra, uint32_t *levelWord, uint32_t *head,
+                          uint2 *spill, uint32_t spillPerWave, int waves, uint64_t *cand, uint32_t *candCount,
+                          uint32_t candCap, uint32_t *overflow, int expandGrid, hipStream_t s);
 void launch_seed(const TraverseArgs &t, uint32_t rootA, uint32_t rootB, const float *rootBoxA,
                  const float *rootBoxB, uint2 *frontier, uint32_t *count, uint64_t *cand,
                  uint32_t *candCount, hipStream_t s);
