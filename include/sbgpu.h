@@ -202,6 +202,14 @@ int sbg_bench_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs pairs, int precision, in
 int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs pairs, int side, int iters, double *avg_ms,
                    uint64_t *work);
 
+/* Measurement aid (no counterpart in the reference): the same set-up and one
+ * launch of the x/y ray kernel, then the FP64 wave kernel alone `iters` times
+ * over the rays that launch deferred (HIP events around each launch on the
+ * context stream); *avg_ms = mean launch time, *rays (may be null) = the rays
+ * it handles. */
+int sbg_bench_rays_wave(sbg_mesh self, sbg_mesh other, sbg_pairs pairs, int side, int iters, double *avg_ms,
+                        uint64_t *rays);
+
 /* Testing aids (no counterpart in the reference), run on the host, no device
  * needed: the FP32 certificate the ray kernels apply to one candidate of
  * isPointInMesh (src/solidboolean.cpp:64-87), for n candidates: point pts

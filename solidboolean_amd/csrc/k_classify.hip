@@ -1049,6 +1049,15 @@ void launch_rays_xy_ordered(const GridSet &gs, const double *tri64, const double
                            order, nullptr, n, lanes, nullptr, nullptr, axisBits, lists, counters + 1, counters);
 }
 
+void launch_rays_wave_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
+                              uint32_t n, uint8_t *axisBits, const uint32_t *lists, uint32_t *counters, hipStream_t s)
+{
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_rays_wave<true>, dim3(8192), dim3(64), 0, s, gs, tri64, q4, order, n, axisBits, lists,
+                       counters + 1, counters);
+}
+
 void launch_ray_work_ordered(const GridSet &gs, const double *q4, const uint32_t *order, uint32_t n,
                              unsigned long long *work, hipStream_t s)
 {

@@ -1648,10 +1648,12 @@ int sbg_calibrate_copy(sbg_context ctx, size_t bytes, int iters, double *avg_ms)
     return check_launch("copy");
 }
 
-int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int iters, double *avg_ms,
-                   uint64_t *work)
+}  // extern "C"
+
+// the argument checks of the ray measurement aids
+static int bench_rays_check(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int iters)
 {
-    if (!self || !other || !p || !avg_ms || iters <= 0)
+    if (!self || !other || !p || iters <= 0)
         return fail(SBG_ERR_INVALID, "bad argument");
     if (self->ctx != other->ctx || self->ctx != p->ctx)
         return fail(SBG_ERR_INVALID, "handles belong to different contexts");
@@ -1663,14 +1665,16 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
         return fail(SBG_ERR_INVALID, "side must be 0 or 1");
     if (p->n && (side == 0 ? p->maxA : p->maxB) >= self->nt)
         return fail(SBG_ERR_INVALID, "pair list does not index this mesh on side %d", side);
+    return use(self->ctx);
+}
+
+// their set-up: the queries of sbg_classify_triangles(self, other, p, side)
+// in c->pts, other's grid built; *empty when there is nothing to cast
+static int bench_rays_setup(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, bool &empty)
+{
+    empty = true;
     sbg_context c = self->ctx;
-    TRY(use(c));
-    MeshUse mu(self, other);
     const size_t nt = self->nt;
-    *avg_ms = 0.0;
-    if (work)
-        for (int k = 0; k < 5; ++k)
-            work[k] = 0;
     if (!nt || !other->nt)
         return SBG_OK;
     TRY(c->flags.ensure(nt + 4));
@@ -1683,6 +1687,29 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
     launch_mark_intersected(p->pairs.p, p->status.p, (uint32_t)p->n, side, c->flags.p, c->stream);
     launch_centroids_ordered(self->tri64.p, c->flags.p, self->sortedIds, nullptr, (uint32_t)nt, c->pts.p, c->stream);
     TRY(ensure_grid(other, c->stream));
+    empty = false;
+    return SBG_OK;
+}
+
+extern "C" {
+
+int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int iters, double *avg_ms,
+                   uint64_t *work)
+{
+    if (!avg_ms)
+        return fail(SBG_ERR_INVALID, "bad argument");
+    *avg_ms = 0.0;
+    if (work)
+        for (int k = 0; k < 5; ++k)
+            work[k] = 0;
+    TRY(bench_rays_check(self, other, p, side, iters));
+    MeshUse mu(self, other);
+    bool empty = true;
+    TRY(bench_rays_setup(self, other, p, side, empty));
+    if (empty)
+        return SBG_OK;
+    sbg_context c = self->ctx;
+    const size_t nt = self->nt;
     const GridSet gs = grid_set(other, c->tune.ray_precision == SBG_PRECISION_FP32_FILTERED);
     std::vector<hipEvent_t> ev(2 * (size_t)iters, nullptr);
     for (auto &e : ev)
@@ -1716,6 +1743,47 @@ int sbg_bench_rays(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int ite
         work[4] = deferred;
     }
     return check_launch("bench rays");
+}
+
+int sbg_bench_rays_wave(sbg_mesh self, sbg_mesh other, sbg_pairs p, int side, int iters, double *avg_ms,
+                        uint64_t *rays)
+{
+    if (!avg_ms)
+        return fail(SBG_ERR_INVALID, "bad argument");
+    *avg_ms = 0.0;
+    if (rays)
+        *rays = 0;
+    TRY(bench_rays_check(self, other, p, side, iters));
+    MeshUse mu(self, other);
+    bool empty = true;
+    TRY(bench_rays_setup(self, other, p, side, empty));
+    if (empty)
+        return SBG_OK;
+    sbg_context c = self->ctx;
+    const uint32_t nt = (uint32_t)self->nt;
+    const GridSet gs = grid_set(other, c->tune.ray_precision == SBG_PRECISION_FP32_FILTERED);
+    launch_fill_u32(c->ovf.p, 0u, 4, c->stream);
+    launch_rays_xy_ordered(gs, other->tri64.p, c->pts.p, self->sortedIds, nt, c->bits.p, c->rayList.p, c->ovf.p,
+                           c->stream);
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < iters; ++i)
+        launch_rays_wave_ordered(gs, other->tri64.p, c->pts.p, self->sortedIds, nt, c->bits.p, c->rayList.p,
+                                 c->ovf.p, c->stream);
+    HIP_TRY(hipEventRecord(e1, c->stream));
+    uint32_t deferred = 0;
+    HIP_TRY(hipMemcpyAsync(&deferred, c->ovf.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = (double)ms / iters;
+    if (rays)
+        *rays = deferred;
+    return check_launch("bench rays wave");
 }
 
 // Testing aid, run on the host: the ray kernels' FP32 certificate for n

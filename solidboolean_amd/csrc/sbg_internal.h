@@ -177,6 +177,10 @@ void launch_rays_ordered(const GridSet &gs, const double *tri64, const double *q
 // entry: rays cast, entries scanned, entries passing the code test, deferred)
 void launch_rays_xy_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
                             uint32_t n, uint8_t *axisBits, uint32_t *lists, uint32_t *counters, hipStream_t s);
+// the wave kernel alone over the rays launch_rays_xy_ordered deferred (same
+// lists and counters)
+void launch_rays_wave_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
+                              uint32_t n, uint8_t *axisBits, const uint32_t *lists, uint32_t *counters, hipStream_t s);
 void launch_ray_work_ordered(const GridSet &gs, const double *q4, const uint32_t *order, uint32_t n,
                              unsigned long long *work, hipStream_t s);
 void launch_vote(const uint8_t *axisBits, const uint8_t *skip, uint32_t n, uint8_t *out, hipStream_t s);

@@ -36,7 +36,7 @@ EXPORTS = [
     "sbg_classify_pair", "sbg_components",
     "sbg_half_edges",
     "sbg_context_set_timing",
-    "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_bench_rays", "sbg_calibrate_copy", "sbg_ray_certify",
+    "sbg_context_stage_ms", "sbg_bench_tri_tri", "sbg_bench_rays", "sbg_bench_rays_wave", "sbg_calibrate_copy", "sbg_ray_certify",
     "sbg_ray_hits_apart",
     "sbg_context_set_tuning", "sbg_context_get_tuning", "sbg_autotune_pair",
     "sbg_adjoint",
@@ -132,6 +132,7 @@ def load(path: str = LIB_PATH):
         "sbg_context_stage_ms": [_P, _D, C.c_int],
         "sbg_bench_tri_tri": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double)],
         "sbg_bench_rays": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double), _U64],
+        "sbg_bench_rays_wave": [_P, _P, _P, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_uint64)],
         "sbg_calibrate_copy": [_P, _SZ, C.c_int, C.POINTER(C.c_double)],
         "sbg_ray_certify": [_D, _P, _D, _SZ, _P, _F, _F],
         "sbg_ray_hits_apart": [_F, _F, _F, _F, _SZ, _U8],
@@ -415,6 +416,14 @@ class Mesh:
         work = np.zeros(5, np.uint64)
         _check(self.L.sbg_bench_rays(self.h, other.h, pairs.h, side, iters, C.byref(ms), work))
         return ms.value, dict(zip(("rays", "scanned", "passed", "deferred", "to_fp64"), (int(x) for x in work)))
+
+    def bench_rays_wave(self, other: "Mesh", pairs: "Pairs", side: int, iters: int):
+        """sbg_bench_rays_wave: mean launch ms of the FP64 wave ray kernel over
+        the rays one x/y launch defers, and their number."""
+        ms = C.c_double(0)
+        rays = C.c_uint64(0)
+        _check(self.L.sbg_bench_rays_wave(self.h, other.h, pairs.h, side, iters, C.byref(ms), C.byref(rays)))
+        return ms.value, int(rays.value)
 
 
 class Pairs:
