@@ -491,12 +491,46 @@ int sbg_mesh_create(sbg_context ctx, const double *xyz, size_t nverts, const uin
     return SBG_OK;
 }
 
+}  // extern "C"
+
+static int ensure_grid(sbg_mesh m, hipStream_t stream);
+
+// Queues the ray grid of every pending mesh except `keep` on the grid stream,
+// each after its last prepare.  A grid already built since (a classification
+// asked for it) is not built again.
+static int queue_pending_grids(sbg_context c, sbg_mesh keep)
+{
+    std::vector<sbg_mesh> rest;
+    for (sbg_mesh m : c->gridsPending) {
+        if (m == keep) {
+            rest.push_back(m);
+            continue;
+        }
+        if (!m->prepared || (m->gridReady && m->gridDensity == c->tune.grid_density))
+            continue;
+        HIP_TRY(hipStreamWaitEvent(c->grids, m->prepDone, 0));
+        TRY(ensure_grid(m, c->grids));
+    }
+    c->gridsPending.swap(rest);
+    return SBG_OK;
+}
+
+static void drop_pending_grid(sbg_mesh m)
+{
+    auto &v = m->ctx->gridsPending;
+    v.erase(std::remove(v.begin(), v.end(), m), v.end());
+}
+
+
+extern "C" {
+
 int sbg_mesh_destroy(sbg_mesh m)
 {
     if (!m)
         return SBG_OK;
     sbg_context c = m->ctx;
     (void)hipSetDevice(c->device);
+    drop_pending_grid(m);
     if (c->spareMeshes.size() < kSpareMeshes && !c->destroyed) {
         c->spareMeshes.push_back(m);  // keeps its buffers for the next mesh
     } else {
@@ -521,8 +555,6 @@ int sbg_mesh_counts(sbg_mesh m, size_t *nverts, size_t *ntris)
         *ntris = m->nt;
     return SBG_OK;
 }
-
-static int ensure_grid(sbg_mesh m, hipStream_t stream);
 
 int sbg_mesh_prepare(sbg_mesh m)
 {
@@ -556,6 +588,7 @@ int sbg_mesh_prepare(sbg_mesh m)
     m->gridStream = nullptr;
     m->prepared = false;
     m->gridReady = false;
+    drop_pending_grid(m);
     if (nt == 0) {
         m->prepared = true;
         return SBG_OK;
@@ -599,11 +632,13 @@ int sbg_mesh_prepare(sbg_mesh m)
     }
     m->prepared = true;
     if (c->tune.early_grids == 2) {
-        // the ray grid on the third stream, after this prepare: the pair
-        // search waits for the prepare only, and the grid is built beside it
+        // the ray grid goes on the third stream after this prepare, so the
+        // pair search waits for the prepare only and the grid builds beside
+        // it; it is queued after the next prepare of another mesh or at the
+        // next search, so that the host queues the two prepares first
         HIP_TRY(hipEventRecord(m->prepDone, s));
-        HIP_TRY(hipStreamWaitEvent(c->grids, m->prepDone, 0));
-        TRY(ensure_grid(m, c->grids));
+        TRY(queue_pending_grids(c, m));
+        c->gridsPending.push_back(m);
     }
     return SBG_OK;
 }
@@ -767,6 +802,7 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
         return fail(SBG_ERR_STATE, "mesh not prepared");
     sbg_context c = a->ctx;
     TRY(use(c));
+    TRY(queue_pending_grids(c, nullptr));
     MeshUse mu(a, b);
     const bool earlyGrids = c->tune.early_grids && a->nt && b->nt;
     if (earlyGrids) {
@@ -1299,7 +1335,6 @@ static int check_rays(sbg_mesh target, RayBufs rb, bool &retry)
 static int run_rays(sbg_mesh target, RayBufs rb, const double *dpts, const uint32_t *dorder, const uint8_t *dskip,
                     size_t n, bool ordered, bool allAxes, uint8_t *hostVotes)
 {
-    sbg_context c = target->ctx;
     for (int attempt = 0; attempt < 2; ++attempt) {
         TRY(enqueue_rays(target, rb, dpts, dorder, dskip, n, ordered, allAxes, hostVotes != nullptr));
         HIP_TRY(hipStreamSynchronize(rb.s));

@@ -156,6 +156,10 @@ struct sbg_context_s {
     // next ones
     struct sbg_pairs_s *sparePairs = nullptr;
     std::vector<struct sbg_mesh_s *> spareMeshes;
+    // meshes prepared with their ray grid still to be queued on the grid
+    // stream (queued after the next prepare of another mesh or at the next
+    // pair search, so the host queues both prepares first)
+    std::vector<struct sbg_mesh_s *> gridsPending;
     unsigned meshCount = 0;  // meshes made so far: even ones prepare on `stream`, odd ones on `aux`
 };
 
