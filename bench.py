@@ -320,13 +320,14 @@ def stress_section(ctx, mg, N, steps: int) -> dict:
         a, b = make()
         A = ctx.mesh(a.vertices, a.triangles, prepare=False)
         B = ctx.mesh(b.vertices, b.triangles, prepare=False)
+        votes = (np.zeros(max(A.nt, 1), np.uint8), np.zeros(max(B.nt, 1), np.uint8))
 
         def step():
             A.prepare()
             B.prepare()
             P = A.find_pairs(B)
             P.tri_tri(A, B, N.FP32_FILTERED, stats=False)
-            A.classify_pair(B, P)
+            A.classify_pair(B, P, out=votes)
             return P
 
         step()
@@ -501,13 +502,15 @@ def main():
     ctx = N.Context(0)
     A = ctx.mesh(a.vertices, a.triangles, prepare=False)
     B = ctx.mesh(b.vertices, b.triangles, prepare=False)
+    # the vote arrays are reused step after step, as a pipeline would
+    votes = (np.zeros(max(A.nt, 1), np.uint8), np.zeros(max(B.nt, 1), np.uint8))
 
     def step(precision=N.FP32_FILTERED):
         A.prepare()
         B.prepare()
         P = A.find_pairs(B)
         P.tri_tri(A, B, precision, stats=False)
-        ca, cb = A.classify_pair(B, P)
+        ca, cb = A.classify_pair(B, P, out=votes)
         return P, ca, cb
 
     if args.gd_only:

@@ -401,11 +401,19 @@ class Mesh:
         _check(self.L.sbg_components(self.h, bl.ctypes.data_as(C.c_void_p) if bl is not None else None, out))
         return out[: self.nt]
 
-    def classify_pair(self, other: "Mesh", pairs: "Pairs"):
+    def classify_pair(self, other: "Mesh", pairs: "Pairs", out=None):
         """sbg_classify_pair: (votes of self's triangles, votes of other's),
-        self being the pairs' first mesh."""
-        oa = np.zeros(max(self.nt, 1), np.uint8)
-        ob = np.zeros(max(other.nt, 1), np.uint8)
+        self being the pairs' first mesh.  out: optional pair of uint8
+        arrays of at least max(nt, 1) entries each, filled and returned (a
+        caller that classifies step after step reuses them)."""
+        if out is None:
+            oa = np.zeros(max(self.nt, 1), np.uint8)
+            ob = np.zeros(max(other.nt, 1), np.uint8)
+        else:
+            oa, ob = out
+            if (oa.dtype != np.uint8 or ob.dtype != np.uint8 or not oa.flags.c_contiguous
+                    or not ob.flags.c_contiguous or oa.size < max(self.nt, 1) or ob.size < max(other.nt, 1)):
+                raise ValueError("out must be two contiguous uint8 arrays of the meshes' sizes")
         _check(self.L.sbg_classify_pair(self.h, other.h, pairs.h, oa, ob))
         return oa[: self.nt], ob[: other.nt]
 
