@@ -358,8 +358,8 @@ __device__ __forceinline__ uint2 span_code(const GridSpan &s, uint32_t cu, uint3
     return pack_code(cell_code(s.lu, cu), cell_code(s.hu, cu), cell_code(s.lv, cv), cell_code(s.hv, cv), s.ha);
 }
 
-// Grid build: a workgroup takes kGridBlockTris consecutive triangles of the
-// visiting order (Morton order, so they are neighbours), files each under its
+// Grid build: a workgroup takes tiles of kGridBlockTris consecutive triangles
+// of the visiting order (Morton order, so they are neighbours), files each under its
 // cells on all three axes (one record load per triangle), and first counts its
 // entries per cell in an LDS hash table; only then does it touch the global
 // cell counters, once per distinct cell.  Device-scope atomics are the cost
@@ -440,85 +440,102 @@ __device__ __forceinline__ uint32_t cell_key(const TriSpans &x, int axis, uint32
     return (uint32_t)(x.base + (size_t)axis * x.G * x.G + (size_t)cu * x.G + cv);
 }
 
+// The tiles of kGridBlockTris triangles a workgroup of the grid kernels
+// takes: a contiguous run, and runs in XCD-aware order, so neighbouring tiles
+// (which share cells) stay in one L2.  The grid is capped (kGridMaxBlocks):
+// the build runs beside the latency-bound pair search, whose small launches
+// then still find free slots on every CU.
+constexpr uint32_t kGridMaxBlocks = 256;
+
+__device__ __forceinline__ uint2 grid_tiles(uint32_t nt)
+{
+    const uint32_t ntiles = (nt + kGridBlockTris - 1) / kGridBlockTris;
+    const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
+    const uint32_t first = xcd_block(blockIdx.x, gridDim.x) * per;
+    return make_uint2(first < ntiles ? first : ntiles, first + per < ntiles ? first + per : ntiles);
+}
+
 // cells: ncells + 1 u32, zero on entry; entries of cell c are counted in cells[c + 1]
 __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const double *__restrict__ tri64, uint32_t nt,
                                                                GridSet gs, uint32_t *__restrict__ cells)
 {
     __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
-    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
-    if ((uint64_t)tile * kGridBlockTris >= nt)
-        return;
-    for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris) {
-        s_key[k] = 0;
-        s_cnt[k] = 0;
-    }
-    __syncthreads();
-    const TriSpans x = tri_spans(tri64, nt, gs, tile * kGridBlockTris + threadIdx.x);
-    if (x.valid) {
+    const uint2 tiles = grid_tiles(nt);
+    for (uint32_t tile = tiles.x; tile < tiles.y; ++tile) {
+        for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris) {
+            s_key[k] = 0;
+            s_cnt[k] = 0;
+        }
+        __syncthreads();
+        const TriSpans x = tri_spans(tri64, nt, gs, tile * kGridBlockTris + threadIdx.x);
+        if (x.valid) {
 #pragma unroll
-        for (int axis = 0; axis < 3; ++axis)
-            for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
-                for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
-                    const uint32_t key = cell_key(x, axis, cu, cv);
-                    const int sl = table_slot(s_key, key);
-                    if (sl >= 0)
-                        atomicAdd(&s_cnt[sl], 1u);
-                    else
-                        atomicAdd(&cells[key], 1u);
-                }
+            for (int axis = 0; axis < 3; ++axis)
+                for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
+                    for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
+                        const uint32_t key = cell_key(x, axis, cu, cv);
+                        const int sl = table_slot(s_key, key);
+                        if (sl >= 0)
+                            atomicAdd(&s_cnt[sl], 1u);
+                        else
+                            atomicAdd(&cells[key], 1u);
+                    }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris)
+            if (s_key[k])
+                atomicAdd(&cells[s_key[k]], s_cnt[k]);
+        __syncthreads();  // the table is cleared for the next tile
     }
-    __syncthreads();
-    for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris)
-        if (s_key[k])
-            atomicAdd(&cells[s_key[k]], s_cnt[k]);
 }
 
-// After the exclusive scan, cells[c + 1] is the start of cell c; the workgroup
-// reserves one range per distinct cell and hands out slots from LDS, so
-// afterwards cell c spans [cells[c], cells[c + 1]).
+// After the exclusive scan, cells[c + 1] is the start of cell c; per tile the
+// workgroup reserves one range per distinct cell and hands out slots from
+// LDS, so afterwards cell c spans [cells[c], cells[c + 1]).
 __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt,
                                                               GridSet gs, uint32_t *__restrict__ cells,
                                                               uint2 *__restrict__ gcode, uint32_t *__restrict__ gtri)
 {
     __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
-    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
-    if ((uint64_t)tile * kGridBlockTris >= nt)
-        return;
-    for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris) {
-        s_key[k] = 0;
-        s_cnt[k] = 0;
-    }
-    __syncthreads();
-    const TriSpans x = tri_spans(tri64, nt, gs, tile * kGridBlockTris + threadIdx.x);
-    if (x.valid) {
+    const uint2 tiles = grid_tiles(nt);
+    for (uint32_t tile = tiles.x; tile < tiles.y; ++tile) {
+        for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris) {
+            s_key[k] = 0;
+            s_cnt[k] = 0;
+        }
+        __syncthreads();
+        const TriSpans x = tri_spans(tri64, nt, gs, tile * kGridBlockTris + threadIdx.x);
+        if (x.valid) {
 #pragma unroll
-        for (int axis = 0; axis < 3; ++axis)
-            for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
-                for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
-                    const int sl = table_slot(s_key, cell_key(x, axis, cu, cv));
-                    if (sl >= 0)
-                        atomicAdd(&s_cnt[sl], 1u);
-                }
-    }
-    __syncthreads();
-    for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris)
-        if (s_key[k])
-            s_cnt[k] = atomicAdd(&cells[s_key[k]], s_cnt[k]);  // the range's first slot
-    __syncthreads();
-    if (!x.valid)
-        return;
+            for (int axis = 0; axis < 3; ++axis)
+                for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
+                    for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
+                        const int sl = table_slot(s_key, cell_key(x, axis, cu, cv));
+                        if (sl >= 0)
+                            atomicAdd(&s_cnt[sl], 1u);
+                    }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris)
+            if (s_key[k])
+                s_cnt[k] = atomicAdd(&cells[s_key[k]], s_cnt[k]);  // the range's first slot
+        __syncthreads();
+        if (x.valid) {
 #pragma unroll
-    for (int axis = 0; axis < 3; ++axis)
-        for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
-            for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
-                const uint32_t key = cell_key(x, axis, cu, cv);
-                const int sl = table_find(s_key, key);
-                const uint32_t slot = sl >= 0 ? atomicAdd(&s_cnt[sl], 1u) : atomicAdd(&cells[key], 1u);
-                if (slot < gs.cap) {
-                    gcode[slot] = span_code(x.s[axis], cu, cv);
-                    gtri[slot] = x.t;
-                }
-            }
+            for (int axis = 0; axis < 3; ++axis)
+                for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
+                    for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
+                        const uint32_t key = cell_key(x, axis, cu, cv);
+                        const int sl = table_find(s_key, key);
+                        const uint32_t slot = sl >= 0 ? atomicAdd(&s_cnt[sl], 1u) : atomicAdd(&cells[key], 1u);
+                        if (slot < gs.cap) {
+                            gcode[slot] = span_code(x.s[axis], cu, cv);
+                            gtri[slot] = x.t;
+                        }
+                    }
+        }
+        __syncthreads();  // the table is cleared for the next tile
+    }
 }
 
 // A ray's cell in its target mesh: the candidate range and the ray's codes
@@ -973,7 +990,8 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
     launch_fill_u32(cells, 0u, m, s);
     if (nt == 0)
         return;
-    const dim3 grid(xcd_grid((nt + kGridBlockTris - 1) / kGridBlockTris));
+    const uint32_t ntiles = (nt + kGridBlockTris - 1) / kGridBlockTris;
+    const dim3 grid(xcd_grid(ntiles < kGridMaxBlocks ? ntiles : kGridMaxBlocks));
     hipLaunchKernelGGL(k_grid_count, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
     hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gcode, gtri);
