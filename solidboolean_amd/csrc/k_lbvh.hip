@@ -145,11 +145,18 @@ __global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__re
     __shared__ float s_box[kRefitTile][2][6];
     __shared__ uint32_t s_flag[kRefitTile];
     __shared__ float s_wave[kRefitTile / 64][6];
+    __shared__ int4 s_d[kRefitTile];  // links of internal nodes [tile0, tile0 + kRefitTile)
     const int tile0 = blockIdx.x * kRefitTile;
     const int tile1 = min(tile0 + kRefitTile, n);
     const int pos = tile0 + threadIdx.x;
     const bool valid = pos < tile1;
     s_flag[threadIdx.x] = 0;
+    // A node whose leaf range lies inside the tile has its index inside the
+    // tile too (internal node i covers leaves [min(i, j), max(i, j)]), so the
+    // walks below read the links of those nodes from one coalesced load here
+    // instead of one dependent global load per level.
+    if (pos < n - 1)
+        s_d[threadIdx.x] = nodes[pos].d;
     __syncthreads();
     float4 lo = make_float4(__builtin_inff(), __builtin_inff(), __builtin_inff(), 0.0f);
     float4 hi = make_float4(-__builtin_inff(), -__builtin_inff(), -__builtin_inff(), 0.0f);
@@ -164,7 +171,7 @@ __global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__re
         // within the tile's depth; the bound only guards malformed input
         for (int step = 0; step < kRefitTile; ++step) {
             BNode *nd = nodes + node;
-            const int4 d = nd->d;
+            const int4 d = (node >= tile0 && node < tile0 + kRefitTile && node < n - 1) ? s_d[node - tile0] : nd->d;
             const int slot = (d.x == child) ? 0 : 1;
             store_child_box(nd, slot, wl, wh);
             const int lo_leaf = min(node, d.w), hi_leaf = max(node, d.w);
