@@ -239,17 +239,20 @@ __global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__re
 // Sparse table over tile boxes: level j holds the union of tiles [t, t + 2^j).
 // One workgroup.  Up to kLdsTiles tiles the levels are built in place in LDS
 // (read, barrier, write, barrier) and copied out as they complete; beyond that
-// each level is a pass over global memory.
+// each level is a pass over global memory.  The LDS is sized to the tiles at
+// launch (24 B per tile): a workgroup asking for the whole 144 KB waits for a
+// CU with nothing else resident, which the ray-grid build beside it delays.
 constexpr uint32_t kLdsTiles = 6144;  // 144 KB of LDS (gfx950 gives a workgroup 160 KB)
 constexpr int kLdsTileRounds = kLdsTiles / 1024;
 
 __global__ __launch_bounds__(1024) void k_tile_table(float *__restrict__ table, uint32_t ntiles, int levels)
 {
-    __shared__ float s_t[6][kLdsTiles];
+    extern __shared__ float s_tiles[];  // 6 x ntiles when ntiles <= kLdsTiles
+    auto s_t = [&](int k, uint32_t t) -> float & { return s_tiles[(size_t)k * ntiles + t]; };
     if (ntiles <= kLdsTiles) {
         for (uint32_t t = threadIdx.x; t < ntiles; t += 1024)
             for (int k = 0; k < 6; ++k)
-                s_t[k][t] = table[6 * (size_t)t + k];
+                s_t(k, t) = table[6 * (size_t)t + k];
         __syncthreads();
         for (int j = 1; j < levels; ++j) {
             const uint32_t half = 1u << (j - 1);
@@ -260,7 +263,7 @@ __global__ __launch_bounds__(1024) void k_tile_table(float *__restrict__ table, 
                 if (t + 2 * half <= ntiles)
 #pragma unroll
                     for (int k = 0; k < 6; ++k)
-                        v[r][k] = k < 3 ? fminf(s_t[k][t], s_t[k][t + half]) : fmaxf(s_t[k][t], s_t[k][t + half]);
+                        v[r][k] = k < 3 ? fminf(s_t(k, t), s_t(k, t + half)) : fmaxf(s_t(k, t), s_t(k, t + half));
             }
             __syncthreads();
 #pragma unroll
@@ -269,7 +272,7 @@ __global__ __launch_bounds__(1024) void k_tile_table(float *__restrict__ table, 
                 if (t + 2 * half <= ntiles)
 #pragma unroll
                     for (int k = 0; k < 6; ++k) {
-                        s_t[k][t] = v[r][k];
+                        s_t(k, t) = v[r][k];
                         table[6 * ((size_t)j * ntiles + t) + k] = v[r][k];
                     }
             }
@@ -420,7 +423,11 @@ void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *no
                        prefix, suffix, table, rootBox);
     if (ntiles < 2)
         return;
-    hipLaunchKernelGGL(k_tile_table, dim3(1), dim3(1024), 0, s, table, ntiles, table_levels(ntiles));
+    const size_t lds = ntiles <= kLdsTiles ? sizeof(float) * 6 * ntiles : 0;
+    if (lds > 65536)  // past the default cap on dynamic LDS (the error, if any, surfaces at the launch check)
+        (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_table),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_tile_table, dim3(1), dim3(1024), lds, s, table, ntiles, table_levels(ntiles));
     hipLaunchKernelGGL(k_refit_top, dim3(256), dim3(256), 0, s, nodes, work + n, work + 2 * (size_t)n, prefix,
                        suffix, table, ntiles, rootBox);
 }
