@@ -250,6 +250,45 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ke
     }
 }
 
+// A list of (first << 32 | second) keys already sorted by their high word:
+// each run of equal high words is sorted by the low word in place, one lane
+// per run (the lane at the run's first entry), by insertion.  A run longer
+// than maxRun is left as it is and raises *flag (the caller then sorts the
+// whole list by both words).  Runs of ~0 keys (padding) are skipped.  Lanes
+// only ever move entries within their own run, so the high word another lane
+// reads at a run's border never changes under it.
+__global__ __launch_bounds__(256) void k_sort_runs(uint64_t *keys, uint32_t n, uint32_t maxRun, uint32_t *flag)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n)
+        return;
+    const uint64_t k = keys[i];
+    const uint32_t hi = (uint32_t)(k >> 32);
+    if (k == ~0ull || (i > 0 && (uint32_t)(keys[i - 1] >> 32) == hi))
+        return;
+    uint32_t j = i + 1;
+    while (j < n && (uint32_t)(keys[j] >> 32) == hi) {
+        if (++j - i > maxRun) {
+            atomicOr(flag, 1u);
+            return;
+        }
+    }
+    for (uint32_t x = i + 1; x < j; ++x) {
+        const uint64_t v = keys[x];
+        uint32_t y = x;
+        for (; y > i && keys[y - 1] > v; --y)
+            keys[y] = keys[y - 1];
+        keys[y] = v;
+    }
+}
+
+void launch_sort_runs(uint64_t *keys, size_t n, uint32_t maxRun, uint32_t *flag, hipStream_t s)
+{
+    if (n)
+        hipLaunchKernelGGL(k_sort_runs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keys, (uint32_t)n, maxRun,
+                           flag);
+}
+
 void launch_exclusive_scan(uint32_t *data, size_t m, hipStream_t s)
 {
     if (m == 0)

@@ -837,6 +837,7 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
         uint32_t *overflow = cnt + kMaxLevels + 2;     // overflow flags
         uint32_t *head = cnt + kMaxLevels + 3;         // depth-first seed claims
         uint32_t *topLevel = cnt + kMaxLevels + 4;     // first level past the one-workgroup top
+        uint32_t *runFlag = cnt + kMaxLevels + 5;      // a first triangle's run too long for the quick sort
         const uint32_t fcap = (uint32_t)std::min<size_t>(c->frontCap, 0xffffffffu);
         const uint32_t ccap = (uint32_t)std::min<size_t>(c->candCap, 0xffffffffu);
         const bool dfs = c->depthFirst && c->tune.depth_first;
@@ -892,8 +893,13 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
             if ((rc = p->alt.ensure(ccap)) == SBG_OK && (rc = c->hist.ensure(sort_hist_entries(ns))) == SBG_OK) {
                 Stage st(c, SBG_STAGE_PAIR_SORT);
                 launch_pad_pairs(p->pairs.p, candCount, (uint32_t)ns, c->stream);
-                const int ranges[4] = {0, ceil_log2(nb), 32, 32 + ceil_log2(na)};
-                preInAlt = radix_sort_u64(p->pairs.p, p->alt.p, ns, ranges, 2, c->hist.p, c->stream) != 0;
+                // by the first triangle (radix passes over its bits only),
+                // then each first triangle's short run by the second; a run
+                // too long for that raises a flag read with the count, and
+                // the list is then sorted the long way
+                const int ranges[2] = {32, 32 + ceil_log2(na)};
+                preInAlt = radix_sort_u64(p->pairs.p, p->alt.p, ns, ranges, 1, c->hist.p, c->stream) != 0;
+                launch_sort_runs(preInAlt ? p->alt.p : p->pairs.p, ns, kSortRunMax, runFlag, c->stream);
                 preSorted = ns;
                 rc = check_launch("pair sort");
             }
@@ -911,7 +917,7 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
             return fail(SBG_ERR_OVERFLOW, "tree-vs-tree search exceeded %d levels", kMaxLevels);
         }
         // all counters, to size a retry or read the candidate count
-        hipError_t e = hipMemcpyAsync(c->pinned, cnt, sizeof(uint32_t) * (kMaxLevels + 3), hipMemcpyDeviceToHost,
+        hipError_t e = hipMemcpyAsync(c->pinned, cnt, sizeof(uint32_t) * (kMaxLevels + 6), hipMemcpyDeviceToHost,
                                       c->stream);
         if (e == hipSuccess)
             e = hipStreamSynchronize(c->stream);
@@ -924,6 +930,17 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
             maxFront = std::max(maxFront, c->pinned[l]);
         nCand = c->pinned[kMaxLevels + 1];
         const uint32_t flags = c->pinned[kMaxLevels + 2];
+        if (preSorted && c->pinned[kMaxLevels + 5]) {
+            // a long run: the list is sorted again by both triangles below,
+            // from the buffer that holds all of it (the quick sort's output
+            // when it covered the whole count; otherwise `pairs`, where the
+            // slots past the quick sort's are untouched and the ones before
+            // hold a permutation of what it read)
+            if (preInAlt && preSorted >= nCand)
+                std::swap(p->pairs, p->alt);
+            preSorted = 0;
+            preInAlt = false;
+        }
         if (!ovf && !flags) {
             found = true;
             break;

@@ -101,6 +101,8 @@ constexpr int kDfsWaves = 512;
 // bfs_levels is reached) before the depth-first stage
 constexpr uint32_t kTopMax = 2048;
 constexpr int kExtraLevels = 8;
+// longest run of one first triangle the quick pair sort orders by insertion
+constexpr uint32_t kSortRunMax = 64;
 constexpr uint32_t kSpillInit = 2048, kSpillMax = 1u << 15;
 
 }  // namespace sbg

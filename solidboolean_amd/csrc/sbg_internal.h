@@ -57,6 +57,9 @@ size_t scan_large_temp(size_t m);
 void launch_scan_large(uint32_t *data, size_t m, uint32_t *temp, hipStream_t s);
 int radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *valsAlt, size_t n,
                    int beginBit, int endBit, uint32_t *hist, hipStream_t s);
+// keys sorted by their high word: each run of equal high words sorted by the
+// low word; a run longer than maxRun raises *flag and is left unsorted
+void launch_sort_runs(uint64_t *keys, size_t n, uint32_t maxRun, uint32_t *flag, hipStream_t s);
 int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRanges, int nRanges,
                    uint32_t *hist, hipStream_t s);
 

@@ -543,3 +543,22 @@ def test_half_edges_vs_dict(gpu):
         opp, qo, first = gpu.half_edges(tris, queries)
         eo, eq, ef = expect(tris, queries)
         assert np.array_equal(opp, eo) and np.array_equal(qo, eq) and first == ef
+
+
+@pytest.mark.parametrize("levels", [(4, 4), (1, 5), (5, 1)])
+def test_pair_sort_runs(gpu, levels):
+    """The quick pair sort (radix passes over the first triangle only, then
+    each first triangle's run by the second) and its fallback when a run is
+    too long (a coarse first mesh: every triangle meets hundreds of the fine
+    one's): every search after the first on a pair queues it, and the list
+    always equals the oracle's in (a, b) order."""
+    la, lb = levels
+    a = mg.icosphere(la, 1.0)
+    b = mg.transform(mg.icosphere(lb, 0.9), None, (0.3, 0.2, -0.1))
+    OA, OB = OracleMesh(a.vertices, a.triangles), OracleMesh(b.vertices, b.triangles)
+    ref = sorted_pairs(OA.find_pairs(OB))
+    A, B = gpu.mesh(a.vertices, a.triangles), gpu.mesh(b.vertices, b.triangles)
+    runs = np.unique(ref[:, 0], return_counts=True)[1]
+    for _ in range(3):
+        P = A.find_pairs(B)
+        assert np.array_equal(P.array().astype(np.int64), ref), (levels, int(runs.max()))
