@@ -1471,24 +1471,26 @@ int sbg_classify_pair(sbg_mesh a, sbg_mesh b, sbg_pairs p, uint8_t *out_a, uint8
         TRY(enqueue_rays(b, ra, ra.pts.p, a->sortedIds, ra.flags.p, a->nt, true, false, true));
     if (b->nt)
         TRY(enqueue_rays(a, rb, rb.pts.p, b->sortedIds, rb.flags.p, b->nt, true, false, true));
+    // each side's votes are copied out as soon as that side is done (the
+    // first side's copy overlaps the second side's GPU work); a side whose
+    // target grid turned out too small runs again on its own afterwards
+    bool retryA = false, retryB = false;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    HIP_TRY(hipStreamSynchronize(c->aux));
-    // a side whose target grid turned out too small runs again on its own
-    bool retry = false;
     if (a->nt) {
-        TRY(check_rays(b, ra, retry));
-        if (retry)
-            TRY(run_rays(b, ra, ra.pts.p, a->sortedIds, ra.flags.p, a->nt, true, false, out_a));
-        else
+        TRY(check_rays(b, ra, retryA));
+        if (!retryA)
             std::memcpy(out_a, ra.votesHost.p, a->nt);
     }
+    HIP_TRY(hipStreamSynchronize(c->aux));
     if (b->nt) {
-        TRY(check_rays(a, rb, retry));
-        if (retry)
-            TRY(run_rays(a, rb, rb.pts.p, b->sortedIds, rb.flags.p, b->nt, true, false, out_b));
-        else
+        TRY(check_rays(a, rb, retryB));
+        if (!retryB)
             std::memcpy(out_b, rb.votesHost.p, b->nt);
     }
+    if (retryA)
+        TRY(run_rays(b, ra, ra.pts.p, a->sortedIds, ra.flags.p, a->nt, true, false, out_a));
+    if (retryB)
+        TRY(run_rays(a, rb, rb.pts.p, b->sortedIds, rb.flags.p, b->nt, true, false, out_b));
     return SBG_OK;
 }
 
