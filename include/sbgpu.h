@@ -85,6 +85,13 @@ int sbg_context_stream(sbg_context ctx, void **hip_stream);
  * (src/solidmesh.h:31-32).  Indices are validated; ntris < 2^31. */
 int sbg_mesh_create(sbg_context ctx, const double *xyz, size_t nverts, const uint32_t *tris, size_t ntris,
                     sbg_mesh *out);
+/* The same with 64-bit vertex indices, as SolidMesh::setTriangles takes them
+ * (std::vector<std::vector<size_t>>, src/solidmesh.h:34); narrowed to the
+ * device's 32 bits while staged, an index >= nverts (or >= 2^32) is an error.
+ * Both entry points stage the copies through page-locked memory on a few
+ * host threads and return once the data is in HBM. */
+int sbg_mesh_create64(sbg_context ctx, const double *xyz, size_t nverts, const uint64_t *tris, size_t ntris,
+                      sbg_mesh *out);
 /* Device SolidMesh::prepare (src/solidmesh.cpp:42-76): FP64 normals
  * (src/vector3.h:155-176), per-triangle boxes and the LBVH.  Asynchronous. */
 int sbg_mesh_prepare(sbg_mesh mesh);

@@ -7,6 +7,8 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "sbg_host.h"
@@ -381,6 +383,16 @@ void context_free(sbg_context ctx)
         delete m;
     }
     ctx->spareMeshes.clear();
+    for (hipStream_t u : ctx->upload)
+        if (u) {
+            (void)hipStreamSynchronize(u);
+            (void)hipStreamDestroy(u);
+        }
+    for (auto &slots : ctx->stageDone)
+        for (hipEvent_t e : slots)
+            if (e)
+                (void)hipEventDestroy(e);
+    ctx->staging.release();
     (void)hipHostFree(ctx->pinned);
     (void)hipEventDestroy(ctx->auxReady);
     (void)hipStreamDestroy(ctx->grids);
@@ -388,6 +400,161 @@ void context_free(sbg_context ctx)
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
+}  // namespace sbg
+
+namespace sbg {
+
+// The upload streams, the staging slots' events and the page-locked staging,
+// made on the context's first mesh upload.
+static int upload_staging(sbg_context c)
+{
+    for (hipStream_t &u : c->upload)
+        if (!u && hipStreamCreateWithFlags(&u, hipStreamNonBlocking) != hipSuccess) {
+            u = nullptr;
+            return fail(SBG_ERR_HIP, "upload stream creation failed");
+        }
+    for (auto &slots : c->stageDone)
+        for (hipEvent_t &e : slots)
+            if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                e = nullptr;
+                return fail(SBG_ERR_HIP, "upload event creation failed");
+            }
+    return c->staging.ensure(2 * kUploadThreads * kStageSlot);
+}
+
+// One upload thread's share, src[first, last) into dst[first, last): the
+// thread converts a slot's worth of elements into one of its two staging
+// slots while the DMA of the other runs on its upload stream, and keeps the
+// largest element of an integer source (the triangles' vertex indices).
+template <typename S, typename D>
+static void upload_share(sbg_context c, int t, D *dst, const S *src, size_t first, size_t last, int *status,
+                         uint64_t *top)
+{
+    *status = SBG_OK;
+    *top = 0;
+    if (hipSetDevice(c->device) != hipSuccess) {
+        *status = SBG_ERR_HIP;
+        return;
+    }
+    const size_t per = kStageSlot / sizeof(D);
+    S hi = 0;
+    int k = 0;
+    for (size_t i = first; i < last; i += per, k ^= 1) {
+        const size_t n = std::min(per, last - i);
+        D *slot = (D *)(c->staging.p + (size_t)(2 * t + k) * kStageSlot);
+        if (hipEventSynchronize(c->stageDone[t][k]) != hipSuccess) {  // its last DMA is done
+            *status = SBG_ERR_HIP;
+            return;
+        }
+        if constexpr (std::is_integral<S>::value) {
+            for (size_t j = 0; j < n; ++j) {
+                const S v = src[i + j];
+                slot[j] = (D)v;
+                hi = v > hi ? v : hi;
+            }
+        } else {
+            std::memcpy(slot, src + i, n * sizeof(D));
+        }
+        const hipStream_t u = c->upload[t % kUploadStreams];
+        if (hipMemcpyAsync(dst + i, slot, n * sizeof(D), hipMemcpyHostToDevice, u) != hipSuccess ||
+            hipEventRecord(c->stageDone[t][k], u) != hipSuccess) {
+            *status = SBG_ERR_HIP;
+            return;
+        }
+    }
+    *top = (uint64_t)hi;
+}
+
+// The mesh's two host-to-device copies, vertices and triangles (narrowed from
+// 64-bit indices when S is), through the page-locked staging: up to
+// kUploadThreads host threads (one per kUploadShare bytes), each taking one
+// contiguous share of both arrays, the calling thread the first.  Returns
+// when every copy has landed, with the largest vertex index in *top.
+template <typename S>
+static int upload_mesh(sbg_context c, sbg_mesh m, const double *xyz, const S *tris, uint64_t *top)
+{
+    *top = 0;
+    const size_t nx = 3 * m->nv, ni = 3 * m->nt;
+    if (nx + ni == 0)
+        return SBG_OK;
+    TRY(upload_staging(c));
+    const size_t bytes = sizeof(double) * nx + sizeof(uint32_t) * ni;
+    const int shares = (int)std::max<size_t>(1, std::min<size_t>(kUploadThreads, bytes / kUploadShare));
+    int status[kUploadThreads][2];
+    uint64_t tops[kUploadThreads][2];
+    auto share = [&](int t) {
+        const size_t sx = (nx + shares - 1) / shares, si = (ni + shares - 1) / shares;
+        const size_t x0 = std::min(nx, t * sx), i0 = std::min(ni, t * si);
+        upload_share<double, double>(c, t, m->V.p, xyz, x0, std::min(nx, x0 + sx), &status[t][0], &tops[t][0]);
+        upload_share<S, uint32_t>(c, t, m->T.p, tris, i0, std::min(ni, i0 + si), &status[t][1], &tops[t][1]);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < shares; ++t) {
+        try {
+            pool.emplace_back(share, t);
+        } catch (...) {  // no thread to be had: this share on the calling thread
+            share(t);
+        }
+    }
+    share(0);
+    for (std::thread &th : pool)
+        th.join();
+    hipError_t e = hipSuccess;
+    for (hipStream_t u : c->upload) {
+        const hipError_t eu = hipStreamSynchronize(u);
+        e = e != hipSuccess ? e : eu;
+    }
+    for (int t = 0; t < shares; ++t) {
+        if (status[t][0] != SBG_OK || status[t][1] != SBG_OK || e != hipSuccess)
+            return fail(SBG_ERR_HIP, "mesh upload failed: %s", hipGetErrorString(e != hipSuccess ? e : hipGetLastError()));
+        *top = std::max(*top, tops[t][1]);
+    }
+    return SBG_OK;
+}
+
+// sbg_mesh_create and sbg_mesh_create64: the indices are checked against
+// nverts (and against 32 bits, which the device stores) after the upload,
+// whose threads find their largest value on the way.
+template <typename S>
+static int mesh_create(sbg_context ctx, const double *xyz, size_t nverts, const S *tris, size_t ntris, sbg_mesh *out)
+{
+    if (!out)
+        return fail(SBG_ERR_INVALID, "null output");
+    *out = nullptr;
+    TRY(use(ctx));
+    if ((nverts && !xyz) || (ntris && !tris))
+        return fail(SBG_ERR_INVALID, "null vertex or triangle array");
+    if (ntris >= (size_t)kLeafBit)
+        return fail(SBG_ERR_INVALID, "too many triangles (%zu)", ntris);
+    sbg_mesh m = mesh_new(ctx);
+    if (!m)
+        return fail(SBG_ERR_HIP, "event creation failed");
+    m->nv = nverts;
+    m->nt = ntris;
+    int r = m->V.ensure(3 * nverts);
+    if (r == SBG_OK)
+        r = m->T.ensure(3 * ntris);
+    uint64_t top = 0;
+    if (r == SBG_OK)
+        r = upload_mesh(ctx, m, xyz, tris, &top);
+    if (r == SBG_OK && ntris && (top >= nverts || top > 0xffffffffull)) {
+        // the first offending index for the message
+        r = fail(SBG_ERR_INVALID, "vertex index %llu beyond 32 bits", (unsigned long long)top);
+        for (size_t i = 0; i < 3 * ntris; ++i)
+            if ((uint64_t)tris[i] >= nverts) {
+                r = fail(SBG_ERR_INVALID, "triangle %zu references vertex %llu of %zu", i / 3,
+                         (unsigned long long)tris[i], nverts);
+                break;
+            }
+    }
+    if (r != SBG_OK) {
+        sbg_mesh_destroy(m);
+        return r;
+    }
+    *out = m;
+    return SBG_OK;
+}
+
 }  // namespace sbg
 
 extern "C" {
@@ -450,45 +617,13 @@ int sbg_context_stage_ms(sbg_context ctx, double *ms, int reset)
 int sbg_mesh_create(sbg_context ctx, const double *xyz, size_t nverts, const uint32_t *tris, size_t ntris,
                     sbg_mesh *out)
 {
-    if (!out)
-        return fail(SBG_ERR_INVALID, "null output");
-    *out = nullptr;
-    TRY(use(ctx));
-    if ((nverts && !xyz) || (ntris && !tris))
-        return fail(SBG_ERR_INVALID, "null vertex or triangle array");
-    if (ntris >= (size_t)kLeafBit)
-        return fail(SBG_ERR_INVALID, "too many triangles (%zu)", ntris);
-    // the largest index first (a loop without an early exit vectorizes), the
-    // first offending one only when there is one
-    uint32_t top = 0;
-    for (size_t i = 0; i < 3 * ntris; ++i)
-        top = tris[i] > top ? tris[i] : top;
-    if (ntris && (size_t)top >= nverts)
-        for (size_t i = 0; i < 3 * ntris; ++i)
-            if ((size_t)tris[i] >= nverts)
-                return fail(SBG_ERR_INVALID, "triangle %zu references vertex %u of %zu", i / 3, tris[i], nverts);
-    sbg_mesh m = mesh_new(ctx);
-    if (!m)
-        return fail(SBG_ERR_HIP, "event creation failed");
-    m->nv = nverts;
-    m->nt = ntris;
-    int r = m->V.ensure(3 * nverts);
-    if (r == SBG_OK)
-        r = m->T.ensure(3 * ntris);
-    if (r != SBG_OK) {
-        sbg_mesh_destroy(m);
-        return r;
-    }
-    if (nverts && hipMemcpy(m->V.p, xyz, sizeof(double) * 3 * nverts, hipMemcpyHostToDevice) != hipSuccess) {
-        sbg_mesh_destroy(m);
-        return fail(SBG_ERR_HIP, "vertex upload failed");
-    }
-    if (ntris && hipMemcpy(m->T.p, tris, sizeof(uint32_t) * 3 * ntris, hipMemcpyHostToDevice) != hipSuccess) {
-        sbg_mesh_destroy(m);
-        return fail(SBG_ERR_HIP, "triangle upload failed");
-    }
-    *out = m;
-    return SBG_OK;
+    return mesh_create(ctx, xyz, nverts, tris, ntris, out);
+}
+
+int sbg_mesh_create64(sbg_context ctx, const double *xyz, size_t nverts, const uint64_t *tris, size_t ntris,
+                      sbg_mesh *out)
+{
+    return mesh_create(ctx, xyz, nverts, tris, ntris, out);
 }
 
 }  // extern "C"

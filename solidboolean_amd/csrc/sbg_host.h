@@ -104,6 +104,14 @@ constexpr int kExtraLevels = 8;
 // longest run of one first triangle the quick pair sort orders by insertion
 constexpr uint32_t kSortRunMax = 64;
 constexpr uint32_t kSpillInit = 2048, kSpillMax = 1u << 15;
+// sbg_mesh_create's uploads: host threads that fill page-locked staging (two
+// slots each), the slot size, and the smallest share worth a thread of its own
+constexpr int kUploadThreads = 8;
+constexpr size_t kStageSlot = (size_t)2 << 20;
+constexpr size_t kUploadShare = (size_t)2 << 20;
+// the threads' copies alternate between this many streams, so the DMA of
+// one waits for no other's
+constexpr int kUploadStreams = 2;
 
 }  // namespace sbg
 
@@ -116,6 +124,11 @@ struct sbg_context_s {
     hipEvent_t auxReady = nullptr;    // orders the aux stream after the main one
     hipStream_t grids = nullptr;      // third stream (default priority): ray grids built at prepare time
     uint32_t *pinned = nullptr;  // host-visible counters (kPinnedWords): search, then two ray sides
+    // mesh uploads: their own streams, page-locked staging of 2 x kUploadThreads
+    // slots and the event after each slot's last copy (made on first use)
+    hipStream_t upload[kUploadStreams] = {};
+    PinnedBuf staging;
+    hipEvent_t stageDone[kUploadThreads][2] = {};
     DevBuf<uint2> front[2];
     DevBuf<uint2> spill;              // depth-first spill regions
     uint32_t spillPerWave = kSpillInit;

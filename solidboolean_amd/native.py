@@ -29,7 +29,7 @@ STAGES = ["records", "sort", "bvh", "traverse", "pair_sort", "tri_tri", "compact
 # every symbol include/sbgpu.h declares
 EXPORTS = [
     "sbg_last_error", "sbg_device_count", "sbg_context_create", "sbg_context_destroy", "sbg_context_synchronize",
-    "sbg_context_stream", "sbg_mesh_create", "sbg_mesh_prepare", "sbg_mesh_destroy", "sbg_mesh_counts",
+    "sbg_context_stream", "sbg_mesh_create", "sbg_mesh_create64", "sbg_mesh_prepare", "sbg_mesh_destroy", "sbg_mesh_counts",
     "sbg_mesh_normals", "sbg_mesh_boxes", "sbg_mesh_bvh", "sbg_find_pairs", "sbg_pairs_create", "sbg_pairs_count",
     "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
     "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_points_noalloc", "sbg_classify_triangles",
@@ -107,6 +107,7 @@ def load(path: str = LIB_PATH):
         "sbg_context_synchronize": [_P],
         "sbg_context_stream": [_P, _PP],
         "sbg_mesh_create": [_P, _D, _SZ, _U32, _SZ, _PP],
+        "sbg_mesh_create64": [_P, _D, _SZ, _U64, _SZ, _PP],
         "sbg_mesh_prepare": [_P],
         "sbg_mesh_destroy": [_P],
         "sbg_mesh_counts": [_P, _PSZ, _PSZ],
@@ -326,19 +327,21 @@ class Mesh:
         self.L = ctx.L
         v = np.ascontiguousarray(vertices, np.float64).reshape(-1, 3)
         t = np.asarray(triangles).reshape(-1, 3)
-        if t.size and t.dtype != np.uint32:
-            if t.dtype.kind in "iu" and t.dtype.itemsize == 8:
-                # one pass: a negative or >= 2^32 value sets a high bit of the OR
-                wide = (int(np.bitwise_or.reduce(t.view(np.uint64), axis=None)) >> 32) != 0
-            else:
-                wide = t.min() < 0 or t.max() >= 2 ** 32
-            if wide:
+        if t.size and t.dtype.kind in "iu" and t.dtype.itemsize == 8:
+            # 64-bit indices (SolidMesh's size_t) go as they are: the library
+            # narrows them while it stages the upload and refuses any index
+            # >= nverts, a negative one included (as uint64 it is huge)
+            t = np.ascontiguousarray(t).view(np.uint64)
+            create = self.L.sbg_mesh_create64
+        else:
+            if t.size and t.dtype != np.uint32 and (t.min() < 0 or t.max() >= 2 ** 32):
                 raise ValueError("triangle index out of uint32 range")
-        t = np.ascontiguousarray(t, np.uint32)
+            t = np.ascontiguousarray(t, np.uint32)
+            create = self.L.sbg_mesh_create
         self.nv, self.nt = v.shape[0], t.shape[0]
         h = C.c_void_p()
-        _check(self.L.sbg_mesh_create(ctx.h, v.reshape(-1) if v.size else np.zeros(3), self.nv,
-                                      t.reshape(-1) if t.size else np.zeros(3, np.uint32), self.nt, C.byref(h)))
+        _check(create(ctx.h, v.reshape(-1) if v.size else np.zeros(3), self.nv,
+                      t.reshape(-1) if t.size else np.zeros(3, np.uint32), self.nt, C.byref(h)))
         self.h = h
 
     def close(self):

@@ -25,8 +25,10 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_mesh_index_range_checked_before_the_library():
-    """Indices that do not fit the ABI's uint32 are refused in Python, before
-    any library call (the stand-in context has no library)."""
+    """Indices of 32-bit and floating types that do not fit the ABI's uint32
+    are refused in Python, before any library call (the stand-in context has
+    no library); 64-bit integer indices go to sbg_mesh_create64, which checks
+    them itself."""
     import types
 
     import numpy as np
@@ -34,9 +36,9 @@ def test_mesh_index_range_checked_before_the_library():
 
     ctx = types.SimpleNamespace(L=None, h=None)
     v = np.zeros((4, 3))
-    for dtype in (np.int64, np.uint64, np.int32, np.float64):
+    for dtype in (np.int32, np.float64):
         for bad in (-1, 2 ** 32, 2 ** 40):
-            if np.dtype(dtype).kind == "u" and bad < 0 or np.dtype(dtype).itemsize == 4 and bad >= 2 ** 31:
+            if np.dtype(dtype).itemsize == 4 and bad >= 2 ** 31:
                 continue
             t = np.array([[0, 1, 2], [1, 2, 3], [0, 2, bad]], dtype)
             with pytest.raises(ValueError, match="uint32 range"):

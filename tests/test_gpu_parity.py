@@ -428,6 +428,34 @@ def test_host_pair_list_validation(gpu):
     t[3, 0] = a.nv
     with pytest.raises(N.SbgError, match=f"triangle 3 references vertex {a.nv} of {a.nv}"):
         gpu.mesh(a.vertices, t)
+    # 64-bit indices go to sbg_mesh_create64, which refuses negative and wide
+    # ones in the library (a negative int64 reads as a huge uint64)
+    for dtype, bad in ((np.int64, -1), (np.int64, 2 ** 32), (np.uint64, 2 ** 40), (np.int64, a.nv)):
+        t = np.array(a.triangles, dtype)
+        t[5, 1] = bad
+        with pytest.raises(N.SbgError, match=f"triangle 5 references vertex {int(np.uint64(t[5, 1]))} of {a.nv}"):
+            gpu.mesh(a.vertices, t)
+
+
+def test_staged_uploads_both_index_widths(gpu):
+    """sbg_mesh_create (uint32 indices) and sbg_mesh_create64 (the reference's
+    size_t indices, narrowed while staged) copy through page-locked staging on
+    up to four host threads, each share cycling through two slots: a mesh of
+    1,500,001 random triangles over 800,003 vertices (four shares per array,
+    ragged ends) gives the oracle's normals and boxes bit for bit from both,
+    and so does a one-triangle mesh (one share, one slot)."""
+    from oracle.oracle import prepare
+
+    rng = np.random.default_rng(11)
+    for nv, nt in ((800_003, 1_500_001), (3, 1)):
+        v = rng.standard_normal((nv, 3))
+        t = rng.integers(0, nv, (nt, 3), dtype=np.int64)
+        n, box = prepare(v, t)
+        for tt in (t, t.astype(np.uint32)):
+            M = gpu.mesh(v, tt)
+            assert np.array_equal(bits(M.normals()), bits(n)), (nt, tt.dtype)
+            assert np.array_equal(bits(M.boxes()), bits(box[:, :6])), (nt, tt.dtype)
+            M.close()
 
 
 def test_determinism(gpu):
