@@ -237,6 +237,33 @@ def test_nested_shells_many_hits(gpu):
             _rays_vs_oracle(Tm, OT, pts)
 
 
+def test_crowded_cells_stacked_copies(gpu):
+    """Sixty stacked copies of one closed surface (a level-1 icosphere, 4,800
+    triangles over the same 80 places): every cell is crowded, so the rays go
+    to the wave kernel, each with sixty hits per face crossed, all at the same
+    PositionKeys.  Per-axis answers and votes equal the oracle's, which counts
+    each crossing once (distinct keys), at two per-lane cell limits and in
+    both ray precisions, for scattered points and for clusters whose rays
+    share cells."""
+    s = mg.icosphere(1, 0.8)
+    k = 60
+    V = np.concatenate([s.vertices] * k)
+    T = np.concatenate([s.triangles + j * s.vertices.shape[0] for j in range(k)])
+    M, OM = gpu.mesh(V, T), OracleMesh(V, T)
+    rng = np.random.default_rng(12)
+    # scattered points, and clusters whose rays share cells
+    pts = np.concatenate([rng.uniform(-1.0, 1.0, (400, 3)),
+                          rng.uniform(-0.02, 0.02, (6, 40, 3)).reshape(-1, 3) + rng.uniform(-0.6, 0.6, (6, 1, 3)).repeat(40, 0).reshape(-1, 3)])
+    saved = gpu.tuning()
+    try:
+        for rays in RAY_PRECISIONS:
+            for cap in (16, 64):
+                gpu.set_tuning(**{**saved, "ray_precision": rays, "ray_cell_cap": cap})
+                _rays_vs_oracle(M, OM, pts)
+    finally:
+        gpu.set_tuning(**saved)
+
+
 @pytest.mark.parametrize("name", ["sub6_s1", "sub6_s3"])
 def test_sub6_digests(gpu, name):
     g = load_golden(name)
