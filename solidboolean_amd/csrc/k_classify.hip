@@ -442,10 +442,9 @@ __device__ __forceinline__ uint32_t cell_key(const TriSpans &x, int axis, uint32
 
 // The tiles of kGridBlockTris triangles a workgroup of the grid kernels
 // takes: a contiguous run, and runs in XCD-aware order, so neighbouring tiles
-// (which share cells) stay in one L2.  The grid is capped (kGridMaxBlocks):
-// the build runs beside the latency-bound pair search, whose small launches
-// then still find free slots on every CU.
-constexpr uint32_t kGridMaxBlocks = 256;
+// (which share cells) stay in one L2.  The launch may cap its workgroups: a
+// build beside the latency-bound pair search leaves that search's small
+// launches free slots on every CU.
 
 __device__ __forceinline__ uint2 grid_tiles(uint32_t nt)
 {
@@ -984,14 +983,14 @@ size_t grid_cells(uint32_t G) { return 3 * (size_t)G * G + 1; }
 
 
 void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
-                       uint2 *gcode, uint32_t *gtri, hipStream_t s)
+                       uint2 *gcode, uint32_t *gtri, uint32_t maxBlocks, hipStream_t s)
 {
     const size_t m = (size_t)gs.ncells + 1;
     launch_fill_u32(cells, 0u, m, s);
     if (nt == 0)
         return;
     const uint32_t ntiles = (nt + kGridBlockTris - 1) / kGridBlockTris;
-    const dim3 grid(xcd_grid(ntiles < kGridMaxBlocks ? ntiles : kGridMaxBlocks));
+    const dim3 grid(xcd_grid(maxBlocks && ntiles > maxBlocks ? maxBlocks : ntiles));
     hipLaunchKernelGGL(k_grid_count, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
     hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gcode, gtri);

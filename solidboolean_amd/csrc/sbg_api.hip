@@ -1382,7 +1382,7 @@ static int ensure_grid(sbg_mesh m, hipStream_t stream)
     {
         Stage st(c, SBG_STAGE_GRID, s);
         launch_grid_build(m->tri64.p, (uint32_t)m->nt, grid_set(m), m->gridCells.p, m->gridTemp.p, m->gridCode.p,
-                          m->gridTri.p, s);
+                          m->gridTri.p, kGridBesideSearch, s);
         TRY(check_launch("ray grid"));
     }
     m->gridStream = s;

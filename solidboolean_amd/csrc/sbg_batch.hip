@@ -23,6 +23,10 @@ constexpr uint32_t kBatchSpillInit = 2048, kBatchSpillMax = 1u << 15;
 constexpr int kCandW = kBatchMaxLevels + 1, kFlagW = kBatchMaxLevels + 2, kHeadW = kBatchMaxLevels + 3;
 constexpr int kRayFlagW = kBatchMaxLevels + 4, kGridW = kBatchMaxLevels + 8;
 constexpr int kWords = kBatchMaxLevels + 10;
+// the batch's ray-grid build: one workgroup per tile of 512 triangles (the
+// batch's grids are millions of triangles, and the branch beside them is the
+// batch's own search)
+constexpr uint32_t kBatchGridBlocks = 0;
 
 int bits_for(uint64_t n)
 {
@@ -190,7 +194,7 @@ int enqueue(sbg_batch b, int precision)
     gsRays.tri32 = precision == SBG_PRECISION_FP32_FILTERED ? b->tri32.p : nullptr;
     HIP_TRY(hipEventRecord(b->fork, s));
     HIP_TRY(hipStreamWaitEvent(aux, b->fork, 0));
-    launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, aux);
+    launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, kBatchGridBlocks, aux);
     HIP_TRY(hipEventRecord(b->join, aux));
     TRY(check_launch("batch grid"));
     // candidate search: every pair's root pair, breadth-first levels, then the

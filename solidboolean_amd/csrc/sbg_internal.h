@@ -152,8 +152,12 @@ struct GridSet {
 };
 uint32_t grid_dim(uint32_t nt, int density);
 size_t grid_cells(uint32_t G);  // 3 G^2 + 1
+// maxBlocks: workgroups at most, each taking a contiguous run of tiles of 512
+// triangles (0: one per tile)
 void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
-                       uint2 *gcode, uint32_t *gtri, hipStream_t s);
+                       uint2 *gcode, uint32_t *gtri, uint32_t maxBlocks, hipStream_t s);
+// the single-pair build's cap: it runs beside the pair search
+constexpr uint32_t kGridBesideSearch = 256;
 // order (may be null): query q uses point order[q].  allAxes: cast x, y and z
 // for every query; otherwise x and y, and z only where they disagree (the
 // vote's answer is the same; z's bits are then valid only there).  lists:
