@@ -320,7 +320,7 @@ def stress_section(ctx, mg, N, steps: int) -> dict:
         a, b = make()
         A = ctx.mesh(a.vertices, a.triangles, prepare=False)
         B = ctx.mesh(b.vertices, b.triangles, prepare=False)
-        votes = (np.zeros(max(A.nt, 1), np.uint8), np.zeros(max(B.nt, 1), np.uint8))
+        votes = (N.host_array(A.nt), N.host_array(B.nt))  # page-locked: the votes arrive by DMA
 
         def step():
             A.prepare()
@@ -503,7 +503,7 @@ def main():
     A = ctx.mesh(a.vertices, a.triangles, prepare=False)
     B = ctx.mesh(b.vertices, b.triangles, prepare=False)
     # the vote arrays are reused step after step, as a pipeline would
-    votes = (np.zeros(max(A.nt, 1), np.uint8), np.zeros(max(B.nt, 1), np.uint8))
+    votes = (N.host_array(A.nt), N.host_array(B.nt))  # page-locked: the votes arrive by DMA
 
     def step(precision=N.FP32_FILTERED):
         A.prepare()

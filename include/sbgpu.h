@@ -92,6 +92,12 @@ int sbg_mesh_create(sbg_context ctx, const double *xyz, size_t nverts, const uin
  * host threads and return once the data is in HBM. */
 int sbg_mesh_create64(sbg_context ctx, const double *xyz, size_t nverts, const uint64_t *tris, size_t ntris,
                       sbg_mesh *out);
+/* Page-locked host memory, for result arrays: the classification entry
+ * points copy their votes straight into page-locked output arrays by DMA
+ * (any page-locked memory qualifies; other memory gets a staged copy).  No
+ * counterpart in the reference. */
+int sbg_host_alloc(size_t bytes, void **out);
+int sbg_host_free(void *p);
 /* Device SolidMesh::prepare (src/solidmesh.cpp:42-76): FP64 normals
  * (src/vector3.h:155-176), per-triangle boxes and the LBVH.  Asynchronous. */
 int sbg_mesh_prepare(sbg_mesh mesh);

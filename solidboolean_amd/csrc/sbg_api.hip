@@ -626,6 +626,26 @@ int sbg_mesh_create64(sbg_context ctx, const double *xyz, size_t nverts, const u
     return mesh_create(ctx, xyz, nverts, tris, ntris, out);
 }
 
+int sbg_host_alloc(size_t bytes, void **out)
+{
+    if (!out)
+        return fail(SBG_ERR_INVALID, "null output");
+    *out = nullptr;
+    if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        *out = nullptr;
+        (void)hipGetLastError();
+        return fail(SBG_ERR_OOM, "hipHostMalloc of %zu bytes failed", bytes);
+    }
+    return SBG_OK;
+}
+
+int sbg_host_free(void *p)
+{
+    if (p && hipHostFree(p) != hipSuccess)
+        return fail(SBG_ERR_INVALID, "not a block from sbg_host_alloc");
+    return SBG_OK;
+}
+
 }  // extern "C"
 
 static int ensure_grid(sbg_mesh m, hipStream_t stream);
@@ -1425,18 +1445,34 @@ static RayBufs ray_bufs(sbg_context c, int set)
             c->stream, c->votesHost};
 }
 
+// Page-locked host memory (hipHostMalloc, sbg_host_alloc, or registered by
+// the caller): a device-to-host copy into it is plain DMA.
+static bool page_locked(const void *p)
+{
+    hipPointerAttribute_t at;
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();  // pageable memory is not an error here
+        return false;
+    }
+    return at.type == hipMemoryTypeHost;
+}
+
 // Queues the rays of n queries into target's grid, the vote into rb.votes,
-// and the readbacks (overflow flags and the grid's entry count into rb.host,
-// with `votes` the votes into rb.votesHost); nothing waits.  ordered: dpts is the
-// visiting-order (x, y, z, tag) layout over dorder; otherwise xyz triples by
-// index with optional dorder / dskip.
+// and the readbacks (overflow flags and the grid's entry count into rb.host;
+// with hostVotes the votes, straight into hostVotes when it is page-locked,
+// else into rb.votesHost for the caller to copy out); nothing waits.  *direct
+// says which.  ordered: dpts is the visiting-order (x, y, z, tag) layout over
+// dorder; otherwise xyz triples by index with optional dorder / dskip.
 static int enqueue_rays(sbg_mesh target, RayBufs rb, const double *dpts, const uint32_t *dorder,
-                        const uint8_t *dskip, size_t n, bool ordered, bool allAxes, bool votes)
+                        const uint8_t *dskip, size_t n, bool ordered, bool allAxes, uint8_t *hostVotes,
+                        bool *direct)
 {
     sbg_context c = target->ctx;
     const bool filtered = c->tune.ray_precision == SBG_PRECISION_FP32_FILTERED;
+    const bool votes = hostVotes != nullptr;
+    *direct = votes && page_locked(hostVotes);
     TRY(rb.ovf.ensure(4));
-    if (votes)
+    if (votes && !*direct)
         TRY(rb.votesHost.ensure(n));
     TRY(rb.rayList.ensure(ray_list_u32(n)));
     rb.host[0] = rb.host[1] = 0;
@@ -1462,7 +1498,7 @@ static int enqueue_rays(sbg_mesh target, RayBufs rb, const double *dpts, const u
                                sizeof(uint32_t), hipMemcpyDeviceToHost, rb.s));
     }
     if (votes)
-        HIP_TRY(hipMemcpyAsync(rb.votesHost.p, rb.votes.p, n, hipMemcpyDeviceToHost, rb.s));
+        HIP_TRY(hipMemcpyAsync(*direct ? hostVotes : rb.votesHost.p, rb.votes.p, n, hipMemcpyDeviceToHost, rb.s));
     return SBG_OK;
 }
 
@@ -1488,12 +1524,13 @@ static int run_rays(sbg_mesh target, RayBufs rb, const double *dpts, const uint3
                     size_t n, bool ordered, bool allAxes, uint8_t *hostVotes)
 {
     for (int attempt = 0; attempt < 2; ++attempt) {
-        TRY(enqueue_rays(target, rb, dpts, dorder, dskip, n, ordered, allAxes, hostVotes != nullptr));
+        bool direct = false;
+        TRY(enqueue_rays(target, rb, dpts, dorder, dskip, n, ordered, allAxes, hostVotes, &direct));
         HIP_TRY(hipStreamSynchronize(rb.s));
         bool retry = false;
         TRY(check_rays(target, rb, retry));
         if (!retry) {
-            if (hostVotes)
+            if (hostVotes && !direct)
                 std::memcpy(hostVotes, rb.votesHost.p, n);
             return SBG_OK;
         }
@@ -1602,24 +1639,26 @@ int sbg_classify_pair(sbg_mesh a, sbg_mesh b, sbg_pairs p, uint8_t *out_a, uint8
         TRY(enqueue_queries(a, p, 0, ra));
     if (b->nt)
         TRY(enqueue_queries(b, p, 1, rb));
+    bool directA = false, directB = false;
     if (a->nt)
-        TRY(enqueue_rays(b, ra, ra.pts.p, a->sortedIds, ra.flags.p, a->nt, true, false, true));
+        TRY(enqueue_rays(b, ra, ra.pts.p, a->sortedIds, ra.flags.p, a->nt, true, false, out_a, &directA));
     if (b->nt)
-        TRY(enqueue_rays(a, rb, rb.pts.p, b->sortedIds, rb.flags.p, b->nt, true, false, true));
-    // each side's votes are copied out as soon as that side is done (the
-    // first side's copy overlaps the second side's GPU work); a side whose
+        TRY(enqueue_rays(a, rb, rb.pts.p, b->sortedIds, rb.flags.p, b->nt, true, false, out_b, &directB));
+    // each side's votes land in page-locked output arrays directly, or are
+    // copied out of the staging as soon as that side is done (the first
+    // side's copy overlapping the second side's GPU work); a side whose
     // target grid turned out too small runs again on its own afterwards
     bool retryA = false, retryB = false;
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (a->nt) {
         TRY(check_rays(b, ra, retryA));
-        if (!retryA)
+        if (!retryA && !directA)
             std::memcpy(out_a, ra.votesHost.p, a->nt);
     }
     HIP_TRY(hipStreamSynchronize(c->aux));
     if (b->nt) {
         TRY(check_rays(a, rb, retryB));
-        if (!retryB)
+        if (!retryB && !directB)
             std::memcpy(out_b, rb.votesHost.p, b->nt);
     }
     if (retryA)

@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -29,7 +30,7 @@ STAGES = ["records", "sort", "bvh", "traverse", "pair_sort", "tri_tri", "compact
 # every symbol include/sbgpu.h declares
 EXPORTS = [
     "sbg_last_error", "sbg_device_count", "sbg_context_create", "sbg_context_destroy", "sbg_context_synchronize",
-    "sbg_context_stream", "sbg_mesh_create", "sbg_mesh_create64", "sbg_mesh_prepare", "sbg_mesh_destroy", "sbg_mesh_counts",
+    "sbg_context_stream", "sbg_mesh_create", "sbg_mesh_create64", "sbg_host_alloc", "sbg_host_free", "sbg_mesh_prepare", "sbg_mesh_destroy", "sbg_mesh_counts",
     "sbg_mesh_normals", "sbg_mesh_boxes", "sbg_mesh_bvh", "sbg_find_pairs", "sbg_pairs_create", "sbg_pairs_count",
     "sbg_pairs_copy", "sbg_pairs_destroy", "sbg_tri_tri", "sbg_pairs_results", "sbg_pairs_hits",
     "sbg_tri_tri_batch", "sbg_classify_points", "sbg_classify_points_noalloc", "sbg_classify_triangles",
@@ -108,6 +109,8 @@ def load(path: str = LIB_PATH):
         "sbg_context_stream": [_P, _PP],
         "sbg_mesh_create": [_P, _D, _SZ, _U32, _SZ, _PP],
         "sbg_mesh_create64": [_P, _D, _SZ, _U64, _SZ, _PP],
+        "sbg_host_alloc": [_SZ, _PP],
+        "sbg_host_free": [_P],
         "sbg_mesh_prepare": [_P],
         "sbg_mesh_destroy": [_P],
         "sbg_mesh_counts": [_P, _PSZ, _PSZ],
@@ -321,6 +324,21 @@ class Context:
         return status[:n], seg[:n]
 
 
+def host_array(n: int, dtype=np.uint8) -> np.ndarray:
+    """A NumPy array of n elements in page-locked host memory (sbg_host_alloc),
+    freed when the array is collected.  Classification results written into
+    such arrays arrive by DMA, without the staged copy other memory gets."""
+    L = load()
+    dt = np.dtype(dtype)
+    nbytes = max(int(n), 1) * dt.itemsize
+    ptr = C.c_void_p()
+    _check(L.sbg_host_alloc(nbytes, C.byref(ptr)))
+    buf = (C.c_uint8 * nbytes).from_address(ptr.value)
+    arr = np.frombuffer(buf, dtype=dt, count=max(int(n), 1))
+    weakref.finalize(buf, L.sbg_host_free, C.c_void_p(ptr.value))
+    return arr
+
+
 class Mesh:
     def __init__(self, ctx: Context, vertices, triangles):
         self.ctx = ctx
@@ -391,8 +409,13 @@ class Mesh:
         _check(self.L.sbg_classify_points(self.h, pts.reshape(-1) if n else np.zeros(3), n, _ptr(axis), inside))
         return axis[:n], inside[:n]
 
-    def classify_triangles(self, other: "Mesh", pairs: "Pairs", side: int) -> np.ndarray:
-        out = np.zeros(max(self.nt, 1), np.uint8)
+    def classify_triangles(self, other: "Mesh", pairs: "Pairs", side: int, out=None) -> np.ndarray:
+        """sbg_classify_triangles; out: optional contiguous uint8 array of at
+        least max(nt, 1) entries, filled and returned."""
+        if out is None:
+            out = np.zeros(max(self.nt, 1), np.uint8)
+        elif out.dtype != np.uint8 or not out.flags.c_contiguous or out.size < max(self.nt, 1):
+            raise ValueError("out must be a contiguous uint8 array of the mesh's size")
         _check(self.L.sbg_classify_triangles(self.h, other.h, pairs.h, side, out))
         return out[: self.nt]
 

@@ -237,6 +237,25 @@ def test_nested_shells_many_hits(gpu):
             _rays_vs_oracle(Tm, OT, pts)
 
 
+def test_votes_into_page_locked_arrays(gpu):
+    """Votes written into page-locked arrays (sbg_host_alloc) arrive by DMA
+    and equal those staged into ordinary arrays, for classify_pair (both
+    sides, two calls in a row with the arrays reused) and classify_triangles."""
+    a, b = mg.icosphere_pair(5, 2, (0.3, 0.2, -0.1), noise=1e-2, radius_b=0.9)
+    A, B = gpu.mesh(a.vertices, a.triangles), gpu.mesh(b.vertices, b.triangles)
+    P = A.find_pairs(B)
+    P.tri_tri(A, B, N.FP32_FILTERED, stats=False)
+    ref_a, ref_b = A.classify_pair(B, P)
+    out = (N.host_array(A.nt), N.host_array(B.nt))
+    for _ in range(2):
+        out[0][:] = 7
+        out[1][:] = 7
+        ca, cb = A.classify_pair(B, P, out=out)
+        assert np.array_equal(ca, ref_a) and np.array_equal(cb, ref_b)
+    t = N.host_array(B.nt)
+    assert np.array_equal(B.classify_triangles(A, P, 1, out=t), ref_b)
+
+
 def test_crowded_cells_stacked_copies(gpu):
     """Sixty stacked copies of one closed surface (a level-1 icosphere, 4,800
     triangles over the same 80 places): every cell is crowded, so the rays go
