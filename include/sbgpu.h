@@ -11,7 +11,7 @@
  * thread at a time; distinct contexts are independent.
  *
  * Reference interfaces replaced (huxingyi/solidboolean):
- *   SolidMesh::setVertices / setTriangles / prepare   src/solidmesh.h:31-32, src/solidmesh.cpp:42-76
+ *   SolidMesh::setVertices / setTriangles / prepare   src/solidmesh.h:31-34, src/solidmesh.cpp:42-76
  *   SolidMesh::triangleNormals / triangleAxisAlignedBoundingBoxes  src/solidmesh.h:46-59
  *   SolidBoolean::searchPotentialIntersectedPairs      src/solidboolean.cpp:94-101
  *   SolidBoolean::intersectTwoFaces (+ the pair loop)  src/solidboolean.cpp:103-122, 315-341
