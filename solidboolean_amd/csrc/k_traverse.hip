@@ -143,8 +143,46 @@ __device__ __forceinline__ uint64_t cand_key(const Expansion &x, int s)
     return ((uint64_t)(p.x & ~kLeafBit) << 32) | (p.y & ~kLeafBit);
 }
 
+// Offsets of this lane's na next-frontier pairs and nb candidates in their
+// queues, for a workgroup of 256 lanes: the two counts travel packed in one
+// 32-bit prefix sum (at most 4 each per lane, so a workgroup's totals stay
+// below 2^16), and the workgroup takes one atomic per queue.  Every lane of
+// the workgroup calls it.
+__device__ __forceinline__ uint2 group_reserve2(uint32_t na, uint32_t nb, uint32_t *countA, uint32_t capA,
+                                                uint32_t *countB, uint32_t capB, uint32_t *overflow)
+{
+    __shared__ uint32_t s_wave[4], s_pre[4], s_base[2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t mine = na | nb << 16;
+    uint32_t incl = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t v = __shfl_up(incl, off, 64);
+        if (lane >= off)
+            incl += v;
+    }
+    if (lane == 63)
+        s_wave[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < 4; ++w) {
+            s_pre[w] = run;
+            run += s_wave[w];
+        }
+        const uint32_t ta = run & 0xffffu, tb = run >> 16;
+        const uint32_t ba = ta ? atomicAdd(countA, ta) : 0u, bb = tb ? atomicAdd(countB, tb) : 0u;
+        if ((ta && ba + ta > capA) || (tb && bb + tb > capB))
+            atomicOr(overflow, 1u);
+        s_base[0] = ba;
+        s_base[1] = bb;
+    }
+    __syncthreads();
+    const uint32_t off = s_pre[wave] + incl - mine;
+    return make_uint2(s_base[0] + (off & 0xffffu), s_base[1] + (off >> 16));
+}
+
 // One breadth-first level over the whole grid: frontier `in` (inCount pairs)
-// to `out`, candidates appended.
+// to `out`, candidates appended.  Workgroups of 256 lanes.
 __device__ __forceinline__ void expand_level(const TraverseArgs &t, const uint2 *__restrict__ in,
                                              const uint32_t *__restrict__ inCount, uint2 *__restrict__ out,
                                              uint32_t *outCount, uint32_t outCap, uint64_t *__restrict__ cand,
@@ -154,13 +192,13 @@ __device__ __forceinline__ void expand_level(const TraverseArgs &t, const uint2 
     // raised `overflow` and the whole search is repeated with larger queues
     const uint32_t n = *inCount < outCap ? *inCount : outCap;
     const uint32_t stride = gridDim.x * 256;
-    for (uint32_t base = blockIdx.x * 256; base < n; base += stride) {
+    for (uint32_t base = blockIdx.x * 256; base < n; base += stride) {  // uniform over the workgroup
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < n;
         const Expansion x = expand_pair(t, valid ? in[i] : make_uint2(0, 0), valid);
-        uint32_t wb;
-        const uint32_t on = wave_reserve(__popc(x.next), outCount, outCap, overflow, wb);
-        const uint32_t oc = wave_reserve(__popc(x.cand), candCount, candCap, overflow, wb);
+        const uint2 o = group_reserve2(__popc(x.next), __popc(x.cand), outCount, outCap, candCount, candCap,
+                                       overflow);
+        const uint32_t on = o.x, oc = o.y;
         uint32_t kn = 0, kc = 0;
 #pragma unroll
         for (int sl = 0; sl < 4; ++sl) {
