@@ -143,15 +143,20 @@ __device__ __forceinline__ uint64_t cand_key(const Expansion &x, int s)
     return ((uint64_t)(p.x & ~kLeafBit) << 32) | (p.y & ~kLeafBit);
 }
 
+// Workgroup size of the grid-wide breadth-first levels (the launches' grids
+// are given in units of 256 lanes).
+constexpr int kExpandThreads = 256;
+constexpr int kExpandWaves = kExpandThreads / 64;
+
 // Offsets of this lane's na next-frontier pairs and nb candidates in their
-// queues, for a workgroup of 256 lanes: the two counts travel packed in one
-// 32-bit prefix sum (at most 4 each per lane, so a workgroup's totals stay
-// below 2^16), and the workgroup takes one atomic per queue.  Every lane of
-// the workgroup calls it.
+// queues, for a workgroup of kExpandThreads lanes: the two counts travel
+// packed in one 32-bit prefix sum (at most 4 each per lane, so a workgroup's
+// totals stay below 2^16), and the workgroup takes one atomic per queue.
+// Every lane of the workgroup calls it.
 __device__ __forceinline__ uint2 group_reserve2(uint32_t na, uint32_t nb, uint32_t *countA, uint32_t capA,
                                                 uint32_t *countB, uint32_t capB, uint32_t *overflow)
 {
-    __shared__ uint32_t s_wave[4], s_pre[4], s_base[2];
+    __shared__ uint32_t s_wave[kExpandWaves], s_pre[kExpandWaves], s_base[2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t mine = na | nb << 16;
     uint32_t incl = mine;
@@ -165,7 +170,7 @@ __device__ __forceinline__ uint2 group_reserve2(uint32_t na, uint32_t nb, uint32
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t run = 0;
-        for (int w = 0; w < 4; ++w) {
+        for (int w = 0; w < kExpandWaves; ++w) {
             s_pre[w] = run;
             run += s_wave[w];
         }
@@ -182,7 +187,7 @@ __device__ __forceinline__ uint2 group_reserve2(uint32_t na, uint32_t nb, uint32
 }
 
 // One breadth-first level over the whole grid: frontier `in` (inCount pairs)
-// to `out`, candidates appended.  Workgroups of 256 lanes.
+// to `out`, candidates appended.  Workgroups of kExpandThreads lanes.
 __device__ __forceinline__ void expand_level(const TraverseArgs &t, const uint2 *__restrict__ in,
                                              const uint32_t *__restrict__ inCount, uint2 *__restrict__ out,
                                              uint32_t *outCount, uint32_t outCap, uint64_t *__restrict__ cand,
@@ -191,8 +196,8 @@ __device__ __forceinline__ void expand_level(const TraverseArgs &t, const uint2 
     // both frontier buffers hold outCap pairs; a count past it has already
     // raised `overflow` and the whole search is repeated with larger queues
     const uint32_t n = *inCount < outCap ? *inCount : outCap;
-    const uint32_t stride = gridDim.x * 256;
-    for (uint32_t base = blockIdx.x * 256; base < n; base += stride) {  // uniform over the workgroup
+    const uint32_t stride = gridDim.x * kExpandThreads;
+    for (uint32_t base = blockIdx.x * kExpandThreads; base < n; base += stride) {  // uniform over the workgroup
         const uint32_t i = base + threadIdx.x;
         const bool valid = i < n;
         const Expansion x = expand_pair(t, valid ? in[i] : make_uint2(0, 0), valid);
@@ -216,7 +221,7 @@ __device__ __forceinline__ void expand_level(const TraverseArgs &t, const uint2 
     }
 }
 
-__global__ __launch_bounds__(256) void k_expand(TraverseArgs t, const uint2 *__restrict__ in,
+__global__ __launch_bounds__(kExpandThreads) void k_expand(TraverseArgs t, const uint2 *__restrict__ in,
                                                 const uint32_t *__restrict__ inCount, uint2 *__restrict__ out,
                                                 uint32_t *outCount, uint32_t outCap, uint64_t *__restrict__ cand,
                                                 uint32_t *candCount, uint32_t candCap, uint32_t *overflow)
@@ -302,7 +307,7 @@ __global__ __launch_bounds__(1024) void k_search_top(TraverseArgs t, uint2 *__re
 // Breadth-first level *topLevel + j over the whole grid, or nothing when that
 // is at or past lastLevel (the launches after the top kernel are queued before
 // it is known where it stopped).
-__global__ __launch_bounds__(256) void k_expand_at(TraverseArgs t, uint2 *__restrict__ front0,
+__global__ __launch_bounds__(kExpandThreads) void k_expand_at(TraverseArgs t, uint2 *__restrict__ front0,
                                                    uint2 *__restrict__ front1, uint32_t *__restrict__ cnt,
                                                    const uint32_t *__restrict__ topLevel, uint32_t j,
                                                    uint32_t lastLevel, uint32_t fcap, uint64_t *__restrict__ cand,
@@ -450,7 +455,8 @@ void launch_search_levels(const TraverseArgs &t, uint2 *front0, uint2 *front1, u
     hipLaunchKernelGGL(k_search_top, dim3(1), dim3(1024), 0, s, t, front0, front1, cnt, fcap, cand, candCount,
                        candCap, overflow, lastLevel, topMax, levelWord);
     for (int j = 0; j < extra; ++j)
-        hipLaunchKernelGGL(k_expand_at, dim3(expandGrid), dim3(256), 0, s, t, front0, front1, cnt,
+        hipLaunchKernelGGL(k_expand_at, dim3(expandGrid * 256 / kExpandThreads), dim3(kExpandThreads), 0, s, t,
+                           front0, front1, cnt,
                            (const uint32_t *)levelWord, (uint32_t)j, (uint32_t)lastLevel, fcap, cand, candCount,
                            candCap, overflow);
     hipLaunchKernelGGL(k_traverse_dfs, dim3(waves), dim3(64), 0, s, t, (const uint2 *)front0, (const uint2 *)front1,
@@ -485,8 +491,8 @@ void launch_expand(const TraverseArgs &t, const uint2 *in, const uint32_t *inCou
                    uint32_t *outCount, uint32_t outCap, uint64_t *cand, uint32_t *candCount,
                    uint32_t candCap, uint32_t *overflow, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_expand, dim3(grid), dim3(256), 0, s, t, in, inCount, out, outCount, outCap, cand,
-                       candCount, candCap, overflow);
+    hipLaunchKernelGGL(k_expand, dim3(grid * 256 / kExpandThreads), dim3(kExpandThreads), 0, s, t, in, inCount,
+                       out, outCount, outCap, cand, candCount, candCap, overflow);
 }
 
 void launch_seed(const TraverseArgs &t, uint32_t rootA, uint32_t rootB, const float *rootBoxA,
