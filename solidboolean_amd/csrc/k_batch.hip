@@ -26,9 +26,11 @@ __device__ __forceinline__ float float_of(int o) { return __int_as_float(o >= 0 
 // (INT_MAX x 3, INT_MIN x 3) per mesh on entry.  Pass 1: a wave whose
 // triangles all belong to one mesh (all but the waves that straddle a mesh
 // boundary) stores its reduced bounds in its own slot of `waves`; a straddling
-// wave applies per-lane atomics.  Pass 2: one workgroup per mesh reduces the
-// slots of the waves inside the mesh and applies 6 atomics.  No address takes
-// more than a handful of atomics, whatever the mesh sizes.
+// wave reduces each run of one mesh's lanes (a segmented scan: a mesh's
+// triangles are contiguous) and the run's last lane applies 6 atomics.  Pass
+// 2: one workgroup per mesh reduces the slots of the waves inside the mesh
+// and applies 6 atomics.  No address takes more than a handful of atomics,
+// whatever the mesh sizes.
 __global__ __launch_bounds__(256) void k_batch_bounds(const float *__restrict__ box32,
                                                       const uint32_t *__restrict__ meshOf, uint32_t nt,
                                                       int *__restrict__ mb, int *__restrict__ waves)
@@ -59,11 +61,29 @@ __global__ __launch_bounds__(256) void k_batch_bounds(const float *__restrict__ 
         if (lane < 6)
             waves[6 * (size_t)(t / 64) + lane] = lane == 0 ? v[0] : lane == 1 ? v[1] : lane == 2 ? v[2]
                                                  : lane == 3 ? v[3] : lane == 4 ? v[4] : v[5];
-    } else if (valid) {
+    } else {
+        const int lane = threadIdx.x & 63;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t mu = __shfl_up(m, off, 64);
+            int u[6];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            atomicMin(mb + 6 * (size_t)m + k, v[k]);
-            atomicMax(mb + 6 * (size_t)m + 3 + k, v[3 + k]);
+            for (int k = 0; k < 6; ++k)
+                u[k] = __shfl_up(v[k], off, 64);
+            if (lane >= off && mu == m) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    v[k] = min(v[k], u[k]);
+                    v[3 + k] = max(v[3 + k], u[3 + k]);
+                }
+            }
+        }
+        const uint32_t mn = __shfl_down(m, 1, 64);
+        if (valid && (lane == 63 || mn != m)) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                atomicMin(mb + 6 * (size_t)m + k, v[k]);
+                atomicMax(mb + 6 * (size_t)m + 3 + k, v[3 + k]);
+            }
         }
     }
 }
