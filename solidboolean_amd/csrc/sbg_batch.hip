@@ -285,6 +285,12 @@ int sbg_batch_create(sbg_context ctx, const double *xyz, const uint32_t *tris, c
     b->localBits = bits_for(maxTris);
     b->meshBits = std::max(1, bits_for(nmeshes));
     b->mortonBits = std::min(30, 64 - b->localBits - b->meshBits);
+    // the tree sort runs 8-bit passes over mesh id + Morton code: drop the
+    // Morton bits that would cost a pass of their own, down to 8 per axis
+    // (the single-mesh keys' precision)
+    const int rounded = 8 * ((b->meshBits + b->mortonBits) / 8) - b->meshBits;
+    if (rounded >= 24 && rounded < b->mortonBits)
+        b->mortonBits = rounded;
     b->triBits = std::max(1, bits_for(nt));
     int rc = SBG_OK;
     auto bail = [&](int r) {
