@@ -67,6 +67,9 @@ def sha(*arrays) -> str:
 STEP_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "kernel_trace_and_traffic_1310720_pair_step.md")
 RAYS_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_rays_1310720_pair.md")
 GD_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_pmc_gd_4096_batch.md")
+# the per-lane cell limit (sbg_tuning.ray_cell_cap) that summary was taken at:
+# the library default, which `bench.py --rays-only` runs with
+RAYS_PROFILE_CELL_CAP = 64
 
 # algorithmic bytes of the x/y ray kernel (k_rays_grid) in the units of the
 # ray-parity row of the measurement plan: per ray its 24-B query point and
@@ -605,10 +608,14 @@ def main():
               key=lambda k: stages[k])
     lanes = 2 * A.nt
     density = ctx.tuning()["grid_density"]
+    # timed at the per-lane cell limit the committed rocprofv3 summary was
+    # taken at (the library default), whatever the autotuner chose for the
+    # step, so that the two describe the same launch
+    step_cap = ctx.tuning()["ray_cell_cap"]
     rays = {}
     for name, rp in (("fp64", N.FP64), ("filtered", N.FP32_FILTERED)):
         saved = ctx.tuning()
-        ctx.set_tuning(**{**saved, "ray_precision": rp})
+        ctx.set_tuning(**{**saved, "ray_precision": rp, "ray_cell_cap": RAYS_PROFILE_CELL_CAP})
         rays[name] = A.bench_rays(B, P, 0, 20)
         ctx.set_tuning(**saved)
     ray_ms, ray_work = rays["filtered"]
@@ -635,6 +642,8 @@ def main():
         "fp64_kernel_ms": rays["fp64"][0],
         "fp64_kernel_bytes": ray_bytes(rays["fp64"][1], False),
         "grid_density": density,
+        "ray_cell_cap": RAYS_PROFILE_CELL_CAP,
+        "step_ray_cell_cap": step_cap,
     }
     log(f"x/y ray kernel: filtered {ray_ms * 1e3:.1f} us, fp64 {rays['fp64'][0] * 1e3:.1f} us per launch, "
         f"{ray_work}, {rbytes / 1e6:.0f} MB algorithmic")

@@ -1232,7 +1232,8 @@ static int run_tri_tri(sbg_mesh a, sbg_mesh b, sbg_pairs p, int precision, GdSta
     }
     {
         Stage st(c, SBG_STAGE_COMPACT);
-        launch_fill_u32(p->nHits.p, 0u, 1, c->stream);
+        if (n == 0)  // otherwise the compaction's last block writes the count
+            launch_fill_u32(p->nHits.p, 0u, 1, c->stream);
         launch_compact_hits(p->pairs.p, p->status.p, p->seg.p, n, p->blockCounts.p, p->hitPairs.p, p->hitSeg.p,
                             p->nHits.p, c->stream);
         TRY(check_launch("compact"));
