@@ -406,6 +406,14 @@ def test_stream_ordering(gpu):
         assert np.array_equal(ca, cls_a) and np.array_equal(cb, cls_b), step
         assert np.array_equal(A.classify_triangles(B, P, 0), cls_a), step
     gpu.set_tuning(**saved)
+    # two pair lists alive at once, classified in either order
+    P1 = A.find_pairs(B)
+    P1.tri_tri(A, B, N.FP32_FILTERED)
+    P2 = A.find_pairs(B)
+    P2.tri_tri(A, B, N.FP32_FILTERED)
+    for Pk in (P1, P2, P2):
+        ca, cb = A.classify_pair(B, Pk)
+        assert np.array_equal(ca, cls_a) and np.array_equal(cb, cls_b)
     # a mesh made later lands on the other stream; its answers follow too
     C = gpu.mesh(b.vertices, b.triangles)
     P = A.find_pairs(C)
@@ -530,7 +538,9 @@ def test_tuning_changes_speed_not_results(gpu):
         P = A.find_pairs(B)
         P.tri_tri(A, B, N.FP32_FILTERED, stats=False)
         st, seg = P.results()
-        return P.array(), st, bits(seg[st == 1]), A.classify_triangles(B, P, 0), B.classify_triangles(A, P, 1)
+        ca, cb = A.classify_pair(B, P)
+        return (P.array(), st, bits(seg[st == 1]), A.classify_triangles(B, P, 0), B.classify_triangles(A, P, 1),
+                ca, cb)
 
     try:
         ref = run()
