@@ -413,7 +413,12 @@ struct TriSpans {
     bool valid;
 };
 
-__device__ __forceinline__ TriSpans tri_spans(const double *__restrict__ tri64, uint32_t nt, const GridSet &gs,
+// The spans come from the triangle's FP32 leaf box, rounded outward from its
+// FP64 box: it contains the FP64 box, and every cell and code function is
+// monotone, so the cells it is filed under and its codes cover what the FP64
+// box would give (the ray's code test stays a superset of the FP64 box test),
+// for a 32-B read instead of the 96-B FP64 record.
+__device__ __forceinline__ TriSpans tri_spans(const float *__restrict__ box32, uint32_t nt, const GridSet &gs,
                                               uint32_t pos)
 {
     TriSpans x;
@@ -425,9 +430,9 @@ __device__ __forceinline__ TriSpans tri_spans(const double *__restrict__ tri64, 
     const uint2 d = grid_desc(gs, m);
     x.G = d.x;
     x.base = (size_t)d.y + 1;
-    const double2 *r = reinterpret_cast<const double2 *>(tri64 + (size_t)kTri64Stride * x.t);
-    const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4], r5 = r[5];
-    const Box64 b = tri_box64({r0.x, r0.y, r1.x}, {r2.x, r2.y, r3.x}, {r4.x, r4.y, r5.x});
+    const float4 *r = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * x.t);
+    const float4 lo = r[0], hi = r[1];
+    const Box64 b = {{lo.x, lo.y, lo.z}, {hi.x, hi.y, hi.z}};
     const float *bx = gs.boxes + 6 * (size_t)m;
 #pragma unroll
     for (int axis = 0; axis < 3; ++axis)
@@ -455,7 +460,7 @@ __device__ __forceinline__ uint2 grid_tiles(uint32_t nt)
 }
 
 // cells: ncells + 1 u32, zero on entry; entries of cell c are counted in cells[c + 1]
-__global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const double *__restrict__ tri64, uint32_t nt,
+__global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const float *__restrict__ box32, uint32_t nt,
                                                                GridSet gs, uint32_t *__restrict__ cells)
 {
     __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
@@ -466,7 +471,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const double *__r
             s_cnt[k] = 0;
         }
         __syncthreads();
-        const TriSpans x = tri_spans(tri64, nt, gs, tile * kGridBlockTris + threadIdx.x);
+        const TriSpans x = tri_spans(box32, nt, gs, tile * kGridBlockTris + threadIdx.x);
         if (x.valid) {
 #pragma unroll
             for (int axis = 0; axis < 3; ++axis)
@@ -491,7 +496,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const double *__r
 // After the exclusive scan, cells[c + 1] is the start of cell c; per tile the
 // workgroup reserves one range per distinct cell and hands out slots from
 // LDS, so afterwards cell c spans [cells[c], cells[c + 1]).
-__global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__restrict__ tri64, uint32_t nt,
+__global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const float *__restrict__ box32, uint32_t nt,
                                                               GridSet gs, uint32_t *__restrict__ cells,
                                                               uint2 *__restrict__ gcode, uint32_t *__restrict__ gtri)
 {
@@ -503,7 +508,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const double *__re
             s_cnt[k] = 0;
         }
         __syncthreads();
-        const TriSpans x = tri_spans(tri64, nt, gs, tile * kGridBlockTris + threadIdx.x);
+        const TriSpans x = tri_spans(box32, nt, gs, tile * kGridBlockTris + threadIdx.x);
         if (x.valid) {
 #pragma unroll
             for (int axis = 0; axis < 3; ++axis)
@@ -982,7 +987,7 @@ uint32_t grid_dim(uint32_t nt, int density)
 size_t grid_cells(uint32_t G) { return 3 * (size_t)G * G + 1; }
 
 
-void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
+void launch_grid_build(const float *box32, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
                        uint2 *gcode, uint32_t *gtri, uint32_t maxBlocks, hipStream_t s)
 {
     const size_t m = (size_t)gs.ncells + 1;
@@ -991,9 +996,9 @@ void launch_grid_build(const double *tri64, uint32_t nt, const GridSet &gs, uint
         return;
     const uint32_t ntiles = (nt + kGridBlockTris - 1) / kGridBlockTris;
     const dim3 grid(xcd_grid(maxBlocks && ntiles > maxBlocks ? maxBlocks : ntiles));
-    hipLaunchKernelGGL(k_grid_count, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells);
+    hipLaunchKernelGGL(k_grid_count, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
-    hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, tri64, nt, gs, cells, gcode, gtri);
+    hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells, gcode, gtri);
 }
 
 // Rays of n queries.  allAxes: every query casts x, y and z; otherwise x and

@@ -1402,7 +1402,7 @@ static int ensure_grid(sbg_mesh m, hipStream_t stream)
     m->gridG = G;
     {
         Stage st(c, SBG_STAGE_GRID, s);
-        launch_grid_build(m->tri64.p, (uint32_t)m->nt, grid_set(m), m->gridCells.p, m->gridTemp.p, m->gridCode.p,
+        launch_grid_build(m->box32.p, (uint32_t)m->nt, grid_set(m), m->gridCells.p, m->gridTemp.p, m->gridCode.p,
                           m->gridTri.p, kGridBesideSearch, s);
         TRY(check_launch("ray grid"));
     }

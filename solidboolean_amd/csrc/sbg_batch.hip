@@ -194,7 +194,7 @@ int enqueue(sbg_batch b, int precision)
     gsRays.tri32 = precision == SBG_PRECISION_FP32_FILTERED ? b->tri32.p : nullptr;
     HIP_TRY(hipEventRecord(b->fork, s));
     HIP_TRY(hipStreamWaitEvent(aux, b->fork, 0));
-    launch_grid_build(b->tri64.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, kBatchGridBlocks, aux);
+    launch_grid_build(b->box32.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, kBatchGridBlocks, aux);
     HIP_TRY(hipEventRecord(b->join, aux));
     TRY(check_launch("batch grid"));
     // candidate search: every pair's root pair, breadth-first levels, then the
