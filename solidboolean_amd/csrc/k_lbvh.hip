@@ -39,11 +39,27 @@ __device__ __forceinline__ int delta(const uint64_t *__restrict__ codes, int n, 
 // range crosses a tile boundary is a "top" node, finished by k_refit_top.
 constexpr int kRefitTile = 1024;  // leaves per workgroup, one per lane
 
+// Scratch shared by the hierarchy and the refit (2n + 1 u32): the top-node
+// count, then per sorted leaf position two bytes saying whether a top node's
+// range starts there (its suffix union is needed) or ends there (its prefix
+// union is needed), then the top-node list.  Only the first two parts start
+// at zero.
+struct TopWork {
+    uint32_t *count;
+    uint8_t *need;  // [2 pos]: a range starts at pos, [2 pos + 1]: one ends at pos
+    uint32_t *list;
+};
+__host__ __device__ inline TopWork top_work(uint32_t *work, uint32_t n)
+{
+    return {work, reinterpret_cast<uint8_t *>(work + 1), work + n};
+}
+__host__ __device__ inline size_t top_work_zeroed(uint32_t n) { return 1 + ((size_t)n + 1) / 2; }
+
 template <typename K>
 __global__ __launch_bounds__(256) void k_hierarchy(const K *__restrict__ codes,
                                                    const uint32_t *__restrict__ ids, int n,
                                                    BNode *__restrict__ nodes, uint32_t *__restrict__ leafParent,
-                                                   uint32_t *__restrict__ topList, uint32_t *topCount)
+                                                   TopWork top)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n - 1)
@@ -89,8 +105,11 @@ __global__ __launch_bounds__(256) void k_hierarchy(const K *__restrict__ codes,
     nodes[i].d.w = j;  // the node covers sorted leaves [min(i, j), max(i, j)]
     if (i == 0)
         nodes[0].d.z = -1;
-    if (lo / kRefitTile != hi / kRefitTile)
-        topList[atomicAdd(topCount, 1u)] = (uint32_t)i;
+    if (lo / kRefitTile != hi / kRefitTile) {
+        top.list[atomicAdd(top.count, 1u)] = (uint32_t)i;
+        top.need[2 * (size_t)lo] = 1;
+        top.need[2 * (size_t)hi + 1] = 1;
+    }
 }
 
 __device__ __forceinline__ void store_child_box(BNode *nd, int slot, float4 lo, float4 hi)
@@ -124,10 +143,10 @@ __device__ __forceinline__ void write_root(float *rootBox, float4 lo, float4 hi)
 // device-wide fence is needed.  At a node whose range leaves the tile (a
 // "top" node) the lane stores its box in the node and stops.
 //
-// The workgroup also writes, for every leaf position, the union of the tile's
-// leaf boxes up to it (prefix) and from it (suffix), and the tile's box.  Box
-// unions are exact (min / max), so stage 2 can assemble any top node's box
-// from these pieces in any order.
+// The workgroup also writes the union of the tile's leaf boxes up to a leaf
+// position (prefix) where a top node's range ends and from it (suffix) where
+// one starts, and the tile's box.  Box unions are exact (min / max), so stage
+// 2 can assemble any top node's box from these pieces in any order.
 __device__ __forceinline__ void put6(float *dst, const float v[6])
 {
 #pragma unroll
@@ -139,6 +158,7 @@ __global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__re
                                                             const float *__restrict__ box32, int n,
                                                             BNode *__restrict__ nodes,
                                                             const uint32_t *__restrict__ leafParent,
+                                                            const uint8_t *__restrict__ need,
                                                             float *__restrict__ prefix, float *__restrict__ suffix,
                                                             float *__restrict__ tileBox, float *rootBox)
 {
@@ -229,8 +249,11 @@ __global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__re
                 sv[k] = k < 3 ? fminf(sv[k], s_wave[w][k]) : fmaxf(sv[k], s_wave[w][k]);
     }
     if (valid) {
-        put6(prefix + 6 * (size_t)pos, pv);
-        put6(suffix + 6 * (size_t)pos, sv);
+        const uchar2 nd2 = reinterpret_cast<const uchar2 *>(need)[pos];
+        if (nd2.y)
+            put6(prefix + 6 * (size_t)pos, pv);
+        if (nd2.x)
+            put6(suffix + 6 * (size_t)pos, sv);
     }
     if (threadIdx.x == kRefitTile - 1)
         put6(tileBox + 6 * (size_t)blockIdx.x, pv);
@@ -382,9 +405,9 @@ void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BN
 {
     if (n < 2)
         return;
-    launch_fill_u32(work, 0u, (size_t)2 * n + 1, s);
+    launch_fill_u32(work, 0u, top_work_zeroed(n), s);
     hipLaunchKernelGGL(k_hierarchy<uint32_t>, dim3((n - 1 + 255) / 256), dim3(256), 0, s, codes, ids, (int)n, nodes,
-                       leafParent, work + n, work + 2 * (size_t)n);
+                       leafParent, top_work(work, n));
 }
 
 void launch_hierarchy64(const uint64_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
@@ -392,9 +415,9 @@ void launch_hierarchy64(const uint64_t *codes, const uint32_t *ids, uint32_t n, 
 {
     if (n < 2)
         return;
-    launch_fill_u32(work, 0u, (size_t)2 * n + 1, s);
+    launch_fill_u32(work, 0u, top_work_zeroed(n), s);
     hipLaunchKernelGGL(k_hierarchy<uint64_t>, dim3((n - 1 + 255) / 256), dim3(256), 0, s, codes, ids, (int)n, nodes,
-                       leafParent, work + n, work + 2 * (size_t)n);
+                       leafParent, top_work(work, n));
 }
 
 static int table_levels(uint32_t ntiles)
@@ -419,8 +442,9 @@ void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *no
         return;
     const uint32_t ntiles = (n + kRefitTile - 1) / kRefitTile;
     float *prefix = scratch, *suffix = scratch + 6 * (size_t)n, *table = scratch + 12 * (size_t)n;
+    const TopWork top = top_work(work, n);
     hipLaunchKernelGGL(k_refit_tiles, dim3(ntiles), dim3(kRefitTile), 0, s, ids, box32, (int)n, nodes, leafParent,
-                       prefix, suffix, table, rootBox);
+                       (const uint8_t *)top.need, prefix, suffix, table, rootBox);
     if (ntiles < 2)
         return;
     const size_t lds = ntiles <= kLdsTiles ? sizeof(float) * 6 * ntiles : 0;
@@ -428,8 +452,8 @@ void launch_refit(const uint32_t *ids, const float *box32, uint32_t n, BNode *no
         (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tile_table),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_tile_table, dim3(1), dim3(1024), lds, s, table, ntiles, table_levels(ntiles));
-    hipLaunchKernelGGL(k_refit_top, dim3(256), dim3(256), 0, s, nodes, work + n, work + 2 * (size_t)n, prefix,
-                       suffix, table, ntiles, rootBox);
+    hipLaunchKernelGGL(k_refit_top, dim3(256), dim3(256), 0, s, nodes, (const uint32_t *)top.list,
+                       (const uint32_t *)top.count, prefix, suffix, table, ntiles, rootBox);
 }
 
 }  // namespace sbg

@@ -64,7 +64,8 @@ int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRa
                    uint32_t *hist, hipStream_t s);
 
 // k_lbvh.hip
-// work: 2n + 1 u32 of scratch shared by the two (top-node list and count);
+// work: 2n + 1 u32 of scratch shared by the two (top-node count, the leaf
+// positions where top nodes' ranges start and end, top-node list);
 // refit scratch: refit_scratch_floats(n) floats (per-tile prefix / suffix box
 // unions and the tile sparse table)
 size_t refit_scratch_floats(uint32_t n);
