@@ -154,11 +154,6 @@ __global__ __launch_bounds__(256) void k_batch_codes(const uint64_t *__restrict_
     ids[pos] = triBegin[m] + (uint32_t)(key & ((1ull << localBits) - 1ull));
 }
 
-__device__ __forceinline__ uint32_t mesh_at(const uint64_t *__restrict__ codes, int mortonBits, uint32_t pos)
-{
-    return (uint32_t)(codes[pos] >> mortonBits);
-}
-
 __device__ __forceinline__ void put_box(float *dst, float4 lo, float4 hi)
 {
     dst[0] = lo.x;
@@ -169,49 +164,44 @@ __device__ __forceinline__ void put_box(float *dst, float4 lo, float4 hi)
     dst[5] = hi.z;
 }
 
-// Subtree root of every mesh and its FP32 box.  A mesh's sorted positions form
-// one run; a one-triangle run is its own root (a leaf), otherwise the root is
-// the internal node whose leaf range is exactly the run (the keys of the run
-// share the mesh-id prefix, so the radix tree has such a node).  meshRoot holds
-// ~0u on entry, which stays for empty meshes.
+// Subtree root of every mesh and its FP32 box, one lane per mesh.  A mesh's
+// sorted positions form one run, [triBegin[m], triBegin[m + 1]) (the mesh id
+// leads the key); a one-triangle run is its own root (a leaf), otherwise the
+// root is the internal node whose leaf range is exactly the run (the keys of
+// the run share the mesh-id prefix, so the radix tree has such a node), and
+// an internal node's index is one end of its range, so it is node `first`
+// or node `last`.  meshRoot holds ~0u on entry, which stays for empty meshes.
 __global__ __launch_bounds__(256) void k_mesh_roots(const BNode *__restrict__ nodes,
-                                                    const uint64_t *__restrict__ codes,
-                                                    const uint32_t *__restrict__ ids, uint32_t n, int mortonBits,
-                                                    const float *__restrict__ box32,
+                                                    const uint32_t *__restrict__ ids,
+                                                    const uint32_t *__restrict__ triBegin, uint32_t nmeshes,
+                                                    uint32_t n, const float *__restrict__ box32,
                                                     const float *__restrict__ rootBox, uint32_t *__restrict__ meshRoot,
                                                     float *__restrict__ meshBox)
 {
-    const uint32_t pos = blockIdx.x * 256 + threadIdx.x;
-    if (pos >= n)
+    const uint32_t m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= nmeshes)
         return;
-    const uint32_t m = mesh_at(codes, mortonBits, pos);
-    const bool first = pos == 0 || mesh_at(codes, mortonBits, pos - 1) != m;
-    const bool last = pos == n - 1 || mesh_at(codes, mortonBits, pos + 1) != m;
-    if (first && last) {
-        const uint32_t t = ids[pos];
+    const uint32_t first = triBegin[m], end = triBegin[m + 1];
+    if (first == end)
+        return;
+    const uint32_t last = end - 1;
+    float *dst = meshBox + 6 * (size_t)m;
+    if (first == last) {
+        const uint32_t t = ids[first];
         meshRoot[m] = kLeafBit | t;
         const float4 *bx = reinterpret_cast<const float4 *>(box32 + (size_t)kBox32Stride * t);
-        put_box(meshBox + 6 * (size_t)m, bx[0], bx[1]);
+        put_box(dst, bx[0], bx[1]);
+        return;
     }
-    if (pos + 1 >= n)
-        return;
-    const uint32_t i = pos;  // internal node i
-    const int4 d = nodes[i].d;
-    const uint32_t lo = i < (uint32_t)d.w ? i : (uint32_t)d.w;
-    const uint32_t hi = i < (uint32_t)d.w ? (uint32_t)d.w : i;
-    const uint32_t ml = mesh_at(codes, mortonBits, lo);
-    if (ml != mesh_at(codes, mortonBits, hi))
-        return;
-    if ((lo > 0 && mesh_at(codes, mortonBits, lo - 1) == ml) || (hi + 1 < n && mesh_at(codes, mortonBits, hi + 1) == ml))
-        return;
-    meshRoot[ml] = i;
-    float *dst = meshBox + 6 * (size_t)ml;
+    // node `last` exists only below n - 1; a run ending there is node `first`'s
+    const uint32_t i = (last >= n - 1 || nodes[first].d.w == (int)last) ? first : last;
+    meshRoot[m] = i;
     if (i == 0) {
         for (int k = 0; k < 6; ++k)
             dst[k] = rootBox[k];
         return;
     }
-    const BNode *pn = nodes + d.z;
+    const BNode *pn = nodes + nodes[i].d.z;
     const float4 a = pn->a, b = pn->b, c = pn->c;
     if (pn->d.x == (int)i)
         put_box(dst, make_float4(a.x, a.y, a.z, 0.0f), make_float4(a.w, b.x, b.y, 0.0f));
@@ -325,12 +315,13 @@ void launch_batch_codes(const uint64_t *keys, uint32_t n, int localBits, int mor
                            triBegin, codes, ids);
 }
 
-void launch_mesh_roots(const BNode *nodes, const uint64_t *codes, const uint32_t *ids, uint32_t n, int mortonBits,
-                       const float *box32, const float *rootBox, uint32_t *meshRoot, float *meshBox, hipStream_t s)
+void launch_mesh_roots(const BNode *nodes, const uint32_t *ids, const uint32_t *triBegin, uint32_t nmeshes,
+                       uint32_t n, const float *box32, const float *rootBox, uint32_t *meshRoot, float *meshBox,
+                       hipStream_t s)
 {
-    if (n)
-        hipLaunchKernelGGL(k_mesh_roots, dim3(blocks_for(n)), dim3(256), 0, s, nodes, codes, ids, n, mortonBits, box32,
-                           rootBox, meshRoot, meshBox);
+    if (n && nmeshes)
+        hipLaunchKernelGGL(k_mesh_roots, dim3(blocks_for(nmeshes)), dim3(256), 0, s, nodes, ids, triBegin, nmeshes, n,
+                           box32, rootBox, meshRoot, meshBox);
 }
 
 void launch_batch_seed(const TraverseArgs &t, const uint32_t *meshRoot, const float *meshBox, uint32_t npairs,

@@ -171,8 +171,8 @@ int enqueue(sbg_batch b, int precision)
                  s);
     launch_fill_u32(b->meshRoot.p, ~0u, b->nmeshes, s);
     launch_fill_u32(reinterpret_cast<uint32_t *>(b->meshBox.p), 0u, 6 * b->nmeshes, s);  // empty meshes: zero box
-    launch_mesh_roots(b->nodes.p, b->codes.p, b->ids.p, nt, b->mortonBits, b->box32.p, b->rootBox.p, b->meshRoot.p,
-                      b->meshBox.p, s);
+    launch_mesh_roots(b->nodes.p, b->ids.p, b->triBegin.p, (uint32_t)b->nmeshes, nt, b->box32.p, b->rootBox.p,
+                      b->meshRoot.p, b->meshBox.p, s);
     TRY(check_launch("batch tree"));
     // The ray grids depend on the meshes alone: they are built on a branch
     // (the context's second stream, default priority) beside the search and

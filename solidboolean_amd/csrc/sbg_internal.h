@@ -215,8 +215,9 @@ void launch_batch_keys(const float *box32, const uint32_t *meshOf, const uint32_
                        uint32_t nt, int localBits, int mortonBits, uint64_t *keys, hipStream_t s);
 void launch_batch_codes(const uint64_t *keys, uint32_t n, int localBits, int mortonBits, const uint32_t *triBegin,
                         uint64_t *codes, uint32_t *ids, hipStream_t s);
-void launch_mesh_roots(const BNode *nodes, const uint64_t *codes, const uint32_t *ids, uint32_t n, int mortonBits,
-                       const float *box32, const float *rootBox, uint32_t *meshRoot, float *meshBox, hipStream_t s);
+void launch_mesh_roots(const BNode *nodes, const uint32_t *ids, const uint32_t *triBegin, uint32_t nmeshes,
+                       uint32_t n, const float *box32, const float *rootBox, uint32_t *meshRoot, float *meshBox,
+                       hipStream_t s);
 void launch_batch_seed(const TraverseArgs &t, const uint32_t *meshRoot, const float *meshBox, uint32_t npairs,
                        uint2 *frontier, uint32_t *frontCount, uint32_t frontCap, uint64_t *cand, uint32_t *candCount,
                        uint32_t candCap, uint32_t *overflow, hipStream_t s);
