@@ -249,15 +249,6 @@ __global__ __launch_bounds__(256) void k_batch_seed(TraverseArgs t, const uint32
         atomicOr(overflow, 1u);
 }
 
-// entries [*count, cap) of the candidate list sort last
-__global__ __launch_bounds__(256) void k_pad_pairs(uint64_t *__restrict__ pairs, const uint32_t *__restrict__ count,
-                                                   uint32_t cap)
-{
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < cap && i >= *count)
-        pairs[i] = ~0ull;
-}
-
 // both triangles of every hit are intersected
 __global__ __launch_bounds__(256) void k_mark_both(const uint64_t *__restrict__ pairs,
                                                    const uint8_t *__restrict__ status,
@@ -331,12 +322,6 @@ void launch_batch_seed(const TraverseArgs &t, const uint32_t *meshRoot, const fl
     if (npairs)
         hipLaunchKernelGGL(k_batch_seed, dim3(blocks_for(npairs)), dim3(256), 0, s, t, meshRoot, meshBox, npairs,
                            frontier, frontCount, frontCap, cand, candCount, candCap, overflow);
-}
-
-void launch_pad_pairs(uint64_t *pairs, const uint32_t *count, uint32_t cap, hipStream_t s)
-{
-    if (cap)
-        hipLaunchKernelGGL(k_pad_pairs, dim3(blocks_for(cap)), dim3(256), 0, s, pairs, count, cap);
 }
 
 void launch_mark_both(const uint64_t *pairs, const uint8_t *status, const uint32_t *count, uint32_t cap,

@@ -25,10 +25,20 @@ __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask)
     return (uint32_t)(key >> shift) & mask;
 }
 
-template <typename K, int ITEMS>
-__global__ __launch_bounds__(kSortBlock) void k_hist(const K *__restrict__ keys, size_t n, int shift,
-                                                     uint32_t mask, uint32_t *__restrict__ hist, int nblocks)
+// count (may be null): the keys are the first min(n, *count) of the n the
+// launch covers (a list whose length is known on the device only); the rest
+// are neither read nor moved
+__device__ __forceinline__ size_t sort_len(size_t n, const uint32_t *count)
 {
+    return count && *count < n ? *count : n;
+}
+
+template <typename K, int ITEMS>
+__global__ __launch_bounds__(kSortBlock) void k_hist(const K *__restrict__ keys, size_t n,
+                                                     const uint32_t *__restrict__ count, int shift, uint32_t mask,
+                                                     uint32_t *__restrict__ hist, int nblocks)
+{
+    n = sort_len(n, count);
     __shared__ uint32_t s_cnt[256];
     s_cnt[threadIdx.x] = 0;
     __syncthreads();
@@ -173,10 +183,11 @@ __global__ __launch_bounds__(256) void k_scan_rows(uint32_t *__restrict__ hist, 
 template <typename K, bool HAS_VAL, int ITEMS>
 __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ keys, const uint32_t *__restrict__ vals,
                                                         K *__restrict__ keysOut, uint32_t *__restrict__ valsOut,
-                                                        size_t n, int shift, uint32_t mask,
-                                                        const uint32_t *__restrict__ offsets, int nblocks,
-                                                        const uint32_t *__restrict__ rowTotal)
+                                                        size_t n, const uint32_t *__restrict__ count, int shift,
+                                                        uint32_t mask, const uint32_t *__restrict__ offsets,
+                                                        int nblocks, const uint32_t *__restrict__ rowTotal)
 {
+    n = sort_len(n, count);
     __shared__ uint32_t s_base[256];
     __shared__ uint32_t s_wcnt[kWaves][256];
     // the tile first, so that its loads overlap the offset set-up
@@ -257,8 +268,10 @@ __global__ __launch_bounds__(kSortBlock) void k_scatter(const K *__restrict__ ke
 // whole list by both words).  Runs of ~0 keys (padding) are skipped.  Lanes
 // only ever move entries within their own run, so the high word another lane
 // reads at a run's border never changes under it.
-__global__ __launch_bounds__(256) void k_sort_runs(uint64_t *keys, uint32_t n, uint32_t maxRun, uint32_t *flag)
+__global__ __launch_bounds__(256) void k_sort_runs(uint64_t *keys, uint32_t n, const uint32_t *__restrict__ count,
+                                                   uint32_t maxRun, uint32_t *flag)
 {
+    n = (uint32_t)sort_len(n, count);
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n)
         return;
@@ -282,11 +295,12 @@ __global__ __launch_bounds__(256) void k_sort_runs(uint64_t *keys, uint32_t n, u
     }
 }
 
-void launch_sort_runs(uint64_t *keys, size_t n, uint32_t maxRun, uint32_t *flag, hipStream_t s)
+void launch_sort_runs(uint64_t *keys, size_t n, const uint32_t *count, uint32_t maxRun, uint32_t *flag,
+                      hipStream_t s)
 {
     if (n)
-        hipLaunchKernelGGL(k_sort_runs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keys, (uint32_t)n, maxRun,
-                           flag);
+        hipLaunchKernelGGL(k_sort_runs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keys, (uint32_t)n, count,
+                           maxRun, flag);
 }
 
 void launch_exclusive_scan(uint32_t *data, size_t m, hipStream_t s)
@@ -305,29 +319,29 @@ int sort_blocks(size_t n)
 size_t sort_hist_entries(size_t n) { return (size_t)256 * (size_t)(sort_blocks(n) > 0 ? sort_blocks(n) : 1) + 256; }
 
 template <typename K, bool HAS_VAL>
-static void one_pass(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout, size_t n, int shift,
-                     int bits, uint32_t *hist, hipStream_t s)
+static void one_pass(const K *kin, const uint32_t *vin, K *kout, uint32_t *vout, size_t n, const uint32_t *count,
+                     int shift, int bits, uint32_t *hist, hipStream_t s)
 {
     const int nb = sort_blocks(n);
     const uint32_t mask = (1u << bits) - 1u;
     uint32_t *rowTotal = hist + (size_t)256 * nb;
     const int items = sort_items(n);
     if (items == 16)
-        hipLaunchKernelGGL((k_hist<K, 16>), dim3(nb), dim3(kSortBlock), 0, s, kin, n, shift, mask, hist, nb);
+        hipLaunchKernelGGL((k_hist<K, 16>), dim3(nb), dim3(kSortBlock), 0, s, kin, n, count, shift, mask, hist, nb);
     else if (items == 8)
-        hipLaunchKernelGGL((k_hist<K, 8>), dim3(nb), dim3(kSortBlock), 0, s, kin, n, shift, mask, hist, nb);
+        hipLaunchKernelGGL((k_hist<K, 8>), dim3(nb), dim3(kSortBlock), 0, s, kin, n, count, shift, mask, hist, nb);
     else
-        hipLaunchKernelGGL((k_hist<K, 4>), dim3(nb), dim3(kSortBlock), 0, s, kin, n, shift, mask, hist, nb);
+        hipLaunchKernelGGL((k_hist<K, 4>), dim3(nb), dim3(kSortBlock), 0, s, kin, n, count, shift, mask, hist, nb);
     hipLaunchKernelGGL(k_scan_rows, dim3(256), dim3(256), 0, s, hist, nb, rowTotal);
     if (items == 16)
         hipLaunchKernelGGL((k_scatter<K, HAS_VAL, 16>), dim3(nb), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
-                           shift, mask, hist, nb, rowTotal);
+                           count, shift, mask, hist, nb, rowTotal);
     else if (items == 8)
         hipLaunchKernelGGL((k_scatter<K, HAS_VAL, 8>), dim3(nb), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
-                           shift, mask, hist, nb, rowTotal);
+                           count, shift, mask, hist, nb, rowTotal);
     else
         hipLaunchKernelGGL((k_scatter<K, HAS_VAL, 4>), dim3(nb), dim3(kSortBlock), 0, s, kin, vin, kout, vout, n,
-                           shift, mask, hist, nb, rowTotal);
+                           count, shift, mask, hist, nb, rowTotal);
 }
 
 int radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *valsAlt, size_t n,
@@ -341,16 +355,16 @@ int radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *
         uint32_t *ki = flip ? keysAlt : keys, *ko = flip ? keys : keysAlt;
         uint32_t *vi = flip ? valsAlt : vals, *vo = flip ? vals : valsAlt;
         if (vals)
-            one_pass<uint32_t, true>(ki, vi, ko, vo, n, shift, bits, hist, s);
+            one_pass<uint32_t, true>(ki, vi, ko, vo, n, nullptr, shift, bits, hist, s);
         else
-            one_pass<uint32_t, false>(ki, nullptr, ko, nullptr, n, shift, bits, hist, s);
+            one_pass<uint32_t, false>(ki, nullptr, ko, nullptr, n, nullptr, shift, bits, hist, s);
         flip ^= 1;
     }
     return flip;
 }
 
 int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRanges, int nRanges,
-                   uint32_t *hist, hipStream_t s)
+                   uint32_t *hist, hipStream_t s, const uint32_t *count)
 {
     if (n <= 1)
         return 0;
@@ -360,7 +374,7 @@ int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRa
         for (int shift = beginBit; shift < endBit; shift += 8) {
             const int bits = endBit - shift < 8 ? endBit - shift : 8;
             uint64_t *ki = flip ? keysAlt : keys, *ko = flip ? keys : keysAlt;
-            one_pass<uint64_t, false>(ki, nullptr, ko, nullptr, n, shift, bits, hist, s);
+            one_pass<uint64_t, false>(ki, nullptr, ko, nullptr, n, count, shift, bits, hist, s);
             flip ^= 1;
         }
     }

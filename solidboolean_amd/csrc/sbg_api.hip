@@ -1038,8 +1038,9 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
                     done = true;
             }
         }
-        // queue the pair sort before the count comes back (over sortHint
-        // padded slots), so the host's launches overlap the search
+        // queue the pair sort before the count comes back (launched over
+        // sortHint slots, sorting as many as the count on the device says),
+        // so the host's launches overlap the search
         preSorted = 0;
         preInAlt = false;
         if (rc == SBG_OK && dfs && c->sortHint > 1 && c->sortHint <= ccap) {
@@ -1047,14 +1048,13 @@ int sbg_find_pairs(sbg_mesh a, sbg_mesh b, sbg_pairs *out)
             // alt as large as pairs: the two swap, and neither may shrink
             if ((rc = p->alt.ensure(ccap)) == SBG_OK && (rc = c->hist.ensure(sort_hist_entries(ns))) == SBG_OK) {
                 Stage st(c, SBG_STAGE_PAIR_SORT);
-                launch_pad_pairs(p->pairs.p, candCount, (uint32_t)ns, c->stream);
                 // by the first triangle (radix passes over its bits only),
                 // then each first triangle's short run by the second; a run
                 // too long for that raises a flag read with the count, and
                 // the list is then sorted the long way
                 const int ranges[2] = {32, 32 + ceil_log2(na)};
-                preInAlt = radix_sort_u64(p->pairs.p, p->alt.p, ns, ranges, 1, c->hist.p, c->stream) != 0;
-                launch_sort_runs(preInAlt ? p->alt.p : p->pairs.p, ns, kSortRunMax, runFlag, c->stream);
+                preInAlt = radix_sort_u64(p->pairs.p, p->alt.p, ns, ranges, 1, c->hist.p, c->stream, candCount) != 0;
+                launch_sort_runs(preInAlt ? p->alt.p : p->pairs.p, ns, candCount, kSortRunMax, runFlag, c->stream);
                 preSorted = ns;
                 rc = check_launch("pair sort");
             }

@@ -211,10 +211,10 @@ int enqueue(sbg_batch b, int precision)
     launch_traverse_dfs(t, b->front[level & 1].p, w + level, fcap, w + kHeadW, b->spill.p, b->spillPerWave,
                         b->tune.dfs_waves, b->pairs.p, candCount, ccap, flagsW, s);
     TRY(check_launch("batch search"));
-    // canonical (a, b) order over the whole capacity (unused slots sort last)
-    launch_pad_pairs(b->pairs.p, candCount, ccap, s);
+    // canonical (a, b) order of the candidates (their count read on the device)
     const int pairRanges[4] = {0, b->triBits, 32, 32 + b->triBits};
-    b->pairsInAlt = radix_sort_u64(b->pairs.p, b->pairsAlt.p, b->candCap, pairRanges, 2, b->hist.p, s) != 0;
+    b->pairsInAlt =
+        radix_sort_u64(b->pairs.p, b->pairsAlt.p, b->candCap, pairRanges, 2, b->hist.p, s, candCount) != 0;
     const uint64_t *sorted = b->pairsInAlt ? b->pairsAlt.p : b->pairs.p;
     launch_tritri(b->tri64.p, b->tri64.p, b->tri32.p, b->tri32.p, sorted, ccap, candCount, precision, b->status.p,
                   b->seg.p, b->gdWork.p, nullptr, s);

@@ -58,10 +58,13 @@ void launch_scan_large(uint32_t *data, size_t m, uint32_t *temp, hipStream_t s);
 int radix_sort_u32(uint32_t *keys, uint32_t *vals, uint32_t *keysAlt, uint32_t *valsAlt, size_t n,
                    int beginBit, int endBit, uint32_t *hist, hipStream_t s);
 // keys sorted by their high word: each run of equal high words sorted by the
-// low word; a run longer than maxRun raises *flag and is left unsorted
-void launch_sort_runs(uint64_t *keys, size_t n, uint32_t maxRun, uint32_t *flag, hipStream_t s);
+// low word; a run longer than maxRun raises *flag and is left unsorted.
+// count (may be null, here and below): only the first min(n, *count) keys,
+// a length known on the device, are read and moved
+void launch_sort_runs(uint64_t *keys, size_t n, const uint32_t *count, uint32_t maxRun, uint32_t *flag,
+                      hipStream_t s);
 int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRanges, int nRanges,
-                   uint32_t *hist, hipStream_t s);
+                   uint32_t *hist, hipStream_t s, const uint32_t *count = nullptr);
 
 // k_lbvh.hip
 // work: 2n + 1 u32 of scratch shared by the two (top-node count, the leaf
@@ -221,7 +224,6 @@ void launch_mesh_roots(const BNode *nodes, const uint32_t *ids, const uint32_t *
 void launch_batch_seed(const TraverseArgs &t, const uint32_t *meshRoot, const float *meshBox, uint32_t npairs,
                        uint2 *frontier, uint32_t *frontCount, uint32_t frontCap, uint64_t *cand, uint32_t *candCount,
                        uint32_t candCap, uint32_t *overflow, hipStream_t s);
-void launch_pad_pairs(uint64_t *pairs, const uint32_t *count, uint32_t cap, hipStream_t s);
 void launch_mark_both(const uint64_t *pairs, const uint8_t *status, const uint32_t *count, uint32_t cap,
                       uint8_t *flags, hipStream_t s);
 
