@@ -399,13 +399,14 @@ void launch_fill_u32(uint32_t *p, uint32_t value, size_t n, hipStream_t s)
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, value, n);
 }
 
-// work: 2n + 1 u32 = unused [0, n), top list [n, 2n-1), top count [2n]
+uint32_t hierarchy_zeroed_u32(uint32_t n) { return (uint32_t)top_work_zeroed(n); }
+
+// work: top_work(work, n), its first top_work_zeroed(n) u32 already zero
 void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
                       uint32_t *leafParent, uint32_t *work, hipStream_t s)
 {
     if (n < 2)
         return;
-    launch_fill_u32(work, 0u, top_work_zeroed(n), s);
     hipLaunchKernelGGL(k_hierarchy<uint32_t>, dim3((n - 1 + 255) / 256), dim3(256), 0, s, codes, ids, (int)n, nodes,
                        leafParent, top_work(work, n));
 }

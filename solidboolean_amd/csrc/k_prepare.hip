@@ -108,9 +108,12 @@ __global__ __launch_bounds__(256) void k_bounds_reduce(const float *__restrict__
 
 __global__ __launch_bounds__(256) void k_morton(const float *__restrict__ box32, const float *__restrict__ bounds,
                                                 uint32_t nt, uint32_t *__restrict__ keys,
-                                                uint32_t *__restrict__ vals)
+                                                uint32_t *__restrict__ vals, uint32_t *__restrict__ zero,
+                                                uint32_t nzero)
 {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < nzero)
+        zero[i] = 0u;
     if (i >= nt)
         return;
     // 8 bits per axis: a 3-pass sort, and the tree's order is as good at
@@ -181,11 +184,13 @@ void launch_bounds_reduce(const float *blockBounds, int nblocks, float *bounds, 
 }
 
 void launch_morton(const float *box32, const float *bounds, uint32_t nt, uint32_t *keys, uint32_t *vals,
-                   hipStream_t s)
+                   uint32_t *zero, uint32_t nzero, hipStream_t s)
 {
     if (nt == 0)
         return;
-    hipLaunchKernelGGL(k_morton, dim3((nt + 255) / 256), dim3(256), 0, s, box32, bounds, nt, keys, vals);
+    const uint32_t threads = nt > nzero ? nt : nzero;
+    hipLaunchKernelGGL(k_morton, dim3((threads + 255) / 256), dim3(256), 0, s, box32, bounds, nt, keys, vals, zero,
+                       nzero);
 }
 
 }  // namespace sbg

@@ -753,7 +753,8 @@ int sbg_mesh_prepare(sbg_mesh m)
         launch_records(m->V.p, m->T.p, (uint32_t)nt, m->tri64.p, m->tri32.p, m->box32.p,
                        m->blockBounds.p, nb, s);
         launch_bounds_reduce(m->blockBounds.p, nb, m->bounds.p, s);
-        launch_morton(m->box32.p, m->bounds.p, (uint32_t)nt, m->keys.p, m->vals.p, s);
+        launch_morton(m->box32.p, m->bounds.p, (uint32_t)nt, m->keys.p, m->vals.p, m->flags.p,
+                      nt > 1 ? hierarchy_zeroed_u32((uint32_t)nt) : 0u, s);
         TRY(check_launch("records"));
     }
     uint32_t *sk = m->keys.p, *sv = m->vals.p;

@@ -40,8 +40,9 @@ void launch_normals64(const double *tri64, uint32_t nt, double *out, hipStream_t
 int records_blocks(uint32_t nt);
 void launch_boxes64(const double *tri64, uint32_t nt, double *out, hipStream_t s);
 void launch_bounds_reduce(const float *blockBounds, int nblocks, float *bounds, hipStream_t s);
+// also clears zero[0, nzero) (the hierarchy's scratch, one launch fewer)
 void launch_morton(const float *box32, const float *bounds, uint32_t nt, uint32_t *keys, uint32_t *vals,
-                   hipStream_t s);
+                   uint32_t *zero, uint32_t nzero, hipStream_t s);
 
 // k_sort.hip: stable LSD radix sort; keys (and optional u32 values) ping-pong
 // between the two buffers; returns 1 when the result ended in the alt buffers.
@@ -72,6 +73,9 @@ int radix_sort_u64(uint64_t *keys, uint64_t *keysAlt, size_t n, const int *bitRa
 // refit scratch: refit_scratch_floats(n) floats (per-tile prefix / suffix box
 // unions and the tile sparse table)
 size_t refit_scratch_floats(uint32_t n);
+// launch_hierarchy expects the first hierarchy_zeroed_u32(n) u32 of work to
+// be zero (launch_morton clears them); launch_hierarchy64 clears them itself
+uint32_t hierarchy_zeroed_u32(uint32_t n);
 void launch_hierarchy(const uint32_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
                       uint32_t *leafParent, uint32_t *work, hipStream_t s);
 void launch_hierarchy64(const uint64_t *codes, const uint32_t *ids, uint32_t n, BNode *nodes,
