@@ -63,9 +63,10 @@ __global__ __launch_bounds__(256) void k_centroids(const double *__restrict__ tr
 
 // The same centroids in visiting order: query q (triangle order[q]) gets
 // (x, y, z, tag) at q4 + 4q, with tag = the target mesh (target[t], 0 without
-// a target list), kTagSkip for a triangle flagged as intersected, kTagNone for
-// one without a target mesh.  A ray lane then reads its point without first
-// waiting for order[q].
+// a target list), kTagSkip for a triangle flagged as intersected (with flags;
+// without, the queries do not depend on the intersections and the rays read
+// the flags themselves), kTagNone for one without a target mesh.  A ray lane
+// then reads its point without first waiting for order[q].
 constexpr double kTagSkip = -1.0, kTagNone = -2.0;
 
 __global__ __launch_bounds__(256) void k_centroids_ordered(const double *__restrict__ tri64,
@@ -113,8 +114,9 @@ __device__ __forceinline__ Ray ray_from(const V3 &p, uint32_t axis)
 
 // Lane g = axis * n + q: its ray, output index i and target mesh m.  Q4: the
 // points are (x, y, z, tag) records in visiting order; otherwise xyz triples
-// by output index, with optional order / skip / gs.target.  Returns 0 for a
-// skipped query (no output), 1 for one without a target (outside), 2 to cast.
+// by output index, with optional order / gs.target.  Either way an optional
+// skip flag per output index.  Returns 0 for a skipped query (no output), 1
+// for one without a target (outside), 2 to cast.
 template <bool Q4>
 __device__ __forceinline__ int load_query(const GridSet &gs, const double *__restrict__ pts,
                                           const uint32_t *__restrict__ order, const uint8_t *__restrict__ skip,
@@ -124,7 +126,7 @@ __device__ __forceinline__ int load_query(const GridSet &gs, const double *__res
         const double2 *x = reinterpret_cast<const double2 *>(pts) + 2 * (size_t)q;
         const double2 a = x[0], b = x[1];
         i = order[q];
-        if (b.y == kTagSkip)
+        if (b.y == kTagSkip || (skip && skip[i]))
             return 0;
         if (b.y == kTagNone)
             return 1;
@@ -888,8 +890,8 @@ __global__ __launch_bounds__(256) void k_third_axis(const double *__restrict__ p
     bool need = false;
     if (q < n) {
         const uint32_t i = order ? order[q] : q;
-        const bool skipped = Q4 ? reinterpret_cast<const double2 *>(pts)[2 * (size_t)q + 1].y == kTagSkip
-                                : (skip && skip[i]);
+        const bool skipped = (Q4 && reinterpret_cast<const double2 *>(pts)[2 * (size_t)q + 1].y == kTagSkip) ||
+                             (skip && skip[i]);
         need = !skipped && bits[i] != bits[(size_t)n + i];
     }
     const unsigned long long mask = __ballot(need);
@@ -1091,10 +1093,10 @@ void launch_ray_work_ordered(const GridSet &gs, const double *q4, const uint32_t
 }
 
 void launch_rays_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
-                         uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists, uint32_t *counters,
-                         hipStream_t s)
+                         const uint8_t *skip, uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists,
+                         uint32_t *counters, hipStream_t s)
 {
-    rays<true>(gs, tri64, q4, order, nullptr, n, allAxes, axisBits, lists, counters, s);
+    rays<true>(gs, tri64, q4, order, skip, n, allAxes, axisBits, lists, counters, s);
 }
 
 // per-axis bits [axis][n] -> per-point triples [n][3]

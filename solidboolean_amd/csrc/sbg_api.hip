@@ -1488,7 +1488,7 @@ static int enqueue_rays(sbg_mesh target, RayBufs rb, const double *dpts, const u
         Stage st(c, SBG_STAGE_CLASSIFY, rb.s);
         launch_fill_u32(rb.ovf.p, 0u, 4, rb.s);
         if (ordered)
-            launch_rays_ordered(grid_set(target, filtered), target->tri64.p, dpts, dorder, (uint32_t)n, allAxes, rb.bits.p,
+            launch_rays_ordered(grid_set(target, filtered), target->tri64.p, dpts, dorder, nullptr, (uint32_t)n, allAxes, rb.bits.p,
                                 rb.rayList.p, rb.ovf.p, rb.s);
         else
             launch_rays(grid_set(target, filtered), target->tri64.p, dpts, dorder, dskip, (uint32_t)n, allAxes, rb.bits.p,

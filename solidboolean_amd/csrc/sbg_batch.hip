@@ -195,6 +195,9 @@ int enqueue(sbg_batch b, int precision)
     HIP_TRY(hipEventRecord(b->fork, s));
     HIP_TRY(hipStreamWaitEvent(aux, b->fork, 0));
     launch_grid_build(b->box32.p, nt, gs, b->gridCells.p, b->gridTemp.p, b->gridCode.p, b->gridTri.p, kBatchGridBlocks, aux);
+    // the ray queries (centroids in tree order, target mesh as tag) do not
+    // depend on the intersections: the rays skip intersected triangles by flag
+    launch_centroids_ordered(b->tri64.p, nullptr, b->ids.p, b->target.p, nt, b->pts.p, aux);
     HIP_TRY(hipEventRecord(b->join, aux));
     TRY(check_launch("batch grid"));
     // candidate search: every pair's root pair, breadth-first levels, then the
@@ -222,10 +225,9 @@ int enqueue(sbg_batch b, int precision)
     // classification of every non-intersected triangle against its partner
     launch_fill_u32((uint32_t *)b->flags.p, 0u, (nt + 3) / 4, s);
     launch_mark_both(sorted, b->status.p, candCount, ccap, b->flags.p, s);
-    launch_centroids_ordered(b->tri64.p, b->flags.p, b->ids.p, b->target.p, nt, b->pts.p, s);
     HIP_TRY(hipStreamWaitEvent(s, b->join, 0));
-    launch_rays_ordered(gsRays, b->tri64.p, b->pts.p, b->ids.p, nt, false, b->bits.p, b->rayList.p, w + kRayFlagW,
-                        s);
+    launch_rays_ordered(gsRays, b->tri64.p, b->pts.p, b->ids.p, b->flags.p, nt, false, b->bits.p, b->rayList.p,
+                        w + kRayFlagW, s);
     launch_vote(b->bits.p, b->flags.p, nt, b->votes.p, s);
     launch_copy_u32(b->gridCells.p + b->ncells, w + kGridW, 1, s);
     TRY(check_launch("batch classify"));

@@ -180,13 +180,14 @@ void launch_rays(const GridSet &gs, const double *tri64, const double *pts, cons
                  const uint8_t *skip, uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists,
                  uint32_t *counters, hipStream_t s);
 // Visiting-order queries: q4 holds 4 nt doubles, (x, y, z, tag) of triangle
-// order[q] at 4q (tag: target mesh, or skip / no target); the rays write
-// axisBits by triangle like launch_rays.
+// order[q] at 4q (tag: target mesh, or skip where flags is given / no
+// target); the rays write axisBits by triangle like launch_rays and leave out
+// the queries tagged skip or flagged in skip (by triangle; may be null).
 void launch_centroids_ordered(const double *tri64, const uint8_t *flags, const uint32_t *order,
                               const uint32_t *target, uint32_t nt, double *q4, hipStream_t s);
 void launch_rays_ordered(const GridSet &gs, const double *tri64, const double *q4, const uint32_t *order,
-                         uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists, uint32_t *counters,
-                         hipStream_t s);
+                         const uint8_t *skip, uint32_t n, bool allAxes, uint8_t *axisBits, uint32_t *lists,
+                         uint32_t *counters, hipStream_t s);
 // measurement: the x/y fast ray kernel alone (lists: ray_list_u32(n) u32, counters: 4 u32
 // zero on entry), and the work counts of one such launch (work: 4 u64, zero on
 // entry: rays cast, entries scanned, entries passing the code test, deferred)
