@@ -187,17 +187,26 @@ __global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__re
         float4 wl = lo, wh = hi;
         int node = (int)leafParent[pos];
         int child = (int)(kLeafBit | tri);
-        // each step moves to a strictly larger leaf range, so a walk ends
-        // within the tile's depth; the bound only guards malformed input
+        // Each step moves to a strictly larger leaf range, so a walk ends
+        // within the tile's depth; the bound only guards malformed input.
+        // The walk reads and writes LDS only: a node inside the tile gets its
+        // child boxes from LDS after the walk, and the one global store is the
+        // box the walk carries into its top node.  (A global store inside
+        // the loop would hold every step's fence until the store lands.)
+        bool outside = false;  // the walk stopped at a top node with no links in LDS
         for (int step = 0; step < kRefitTile; ++step) {
-            BNode *nd = nodes + node;
-            const int4 d = (node >= tile0 && node < tile0 + kRefitTile && node < n - 1) ? s_d[node - tile0] : nd->d;
-            const int slot = (d.x == child) ? 0 : 1;
-            store_child_box(nd, slot, wl, wh);
-            const int lo_leaf = min(node, d.w), hi_leaf = max(node, d.w);
-            if (lo_leaf < tile0 || hi_leaf >= tile1 || (lo_leaf / kRefitTile) != (hi_leaf / kRefitTile))
+            if (node < tile0 || node >= tile0 + kRefitTile || node >= n - 1) {
+                outside = true;
                 break;
+            }
             const int local = node - tile0;
+            const int4 d = s_d[local];
+            const int slot = (d.x == child) ? 0 : 1;
+            const int lo_leaf = min(node, d.w), hi_leaf = max(node, d.w);
+            if (lo_leaf < tile0 || hi_leaf >= tile1 || (lo_leaf / kRefitTile) != (hi_leaf / kRefitTile)) {
+                store_child_box(nodes + node, slot, wl, wh);
+                break;
+            }
             float *mine = s_box[local][slot];
             mine[0] = wl.x;
             mine[1] = wl.y;
@@ -209,7 +218,7 @@ __global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__re
             if (atomicAdd(&s_flag[local], 1u) == 0)
                 break;
             __threadfence_block();
-            const volatile float *o = s_box[local][1 - slot];
+            const float *o = s_box[local][1 - slot];
             wl = make_float4(fminf(wl.x, o[0]), fminf(wl.y, o[1]), fminf(wl.z, o[2]), 0.0f);
             wh = make_float4(fmaxf(wh.x, o[3]), fmaxf(wh.y, o[4]), fmaxf(wh.z, o[5]), 0.0f);
             child = node;
@@ -218,6 +227,23 @@ __global__ __launch_bounds__(kRefitTile) void k_refit_tiles(const uint32_t *__re
                 write_root(rootBox, wl, wh);
                 break;
             }
+        }
+        if (outside) {
+            BNode *nd = nodes + node;
+            store_child_box(nd, nd->d.x == child ? 0 : 1, wl, wh);
+        }
+    }
+    __syncthreads();  // every walk's LDS boxes
+    // nodes inside the tile: both child boxes, from LDS
+    if (pos < n - 1) {
+        const int4 d = s_d[threadIdx.x];
+        const int lo_leaf = min(pos, d.w), hi_leaf = max(pos, d.w);
+        if (lo_leaf >= tile0 && hi_leaf < tile1 && (lo_leaf / kRefitTile) == (hi_leaf / kRefitTile)) {
+            const float *b0 = s_box[threadIdx.x][0], *b1 = s_box[threadIdx.x][1];
+            BNode *nd = nodes + pos;
+            nd->a = make_float4(b0[0], b0[1], b0[2], b0[3]);
+            nd->b = make_float4(b0[4], b0[5], b1[0], b1[1]);
+            nd->c = make_float4(b1[2], b1[3], b1[4], b1[5]);
         }
     }
     // prefix / suffix unions over the tile's positions: wave scans, then the
