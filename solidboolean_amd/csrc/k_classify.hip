@@ -733,7 +733,10 @@ __global__ __launch_bounds__(256) void k_rays_grid(GridSet gs, const double *__r
     __syncthreads();  // the staged entries
     if (kind != 2)
         return;
-    // entry k (0 <= k < len) of this lane's cell
+    // entry k (0 <= k < len) of this lane's cell.  The per-lane choice
+    // between LDS and global memory compiles to flat loads, one instruction
+    // for a wave that mixes staged and unstaged cells; separate LDS and
+    // global paths (divergent, both run) made the kernel 10 % slower.
     auto code_at = [&](uint32_t k) -> uint2 { return myBase >= 0 ? s_code[myBase + k] : gs.gcode[c.range.x + k]; };
     auto tri_at = [&](uint32_t k) -> uint32_t { return myBase >= 0 ? s_tri[myBase + k] : gs.gtri[c.range.x + k]; };
     uint32_t parity = 0;
