@@ -39,9 +39,53 @@ def sorted_pairs(p):
     return p[np.lexsort((p[:, 1], p[:, 0]))]
 
 
+def f32_down(x):
+    """Largest float32 at or below each float64 value."""
+    y = np.asarray(x, np.float64).astype(np.float32)
+    return np.where(y.astype(np.float64) > x, np.nextafter(y, np.float32(-np.inf)), y)
+
+
+def f32_up(x):
+    """Smallest float32 at or above each float64 value."""
+    y = np.asarray(x, np.float64).astype(np.float32)
+    return np.where(y.astype(np.float64) < x, np.nextafter(y, np.float32(np.inf)), y)
+
+
+def check_bvh_tight(nodes, root, rbox, boxes64):
+    """Every stored child box is exactly the union of the FP32 outward leaf
+    boxes below it (and the root box the union of all): the refit is exact,
+    not merely covering, whatever tile or top node computed a box."""
+    lo32, hi32 = f32_down(boxes64[:, :3]), f32_up(boxes64[:, 3:])
+    if root & 0x80000000:
+        t = root & 0x7FFFFFFF
+        assert np.array_equal(rbox, np.concatenate([lo32[t], hi32[t]]))
+        return
+    order, stack = [], [root]
+    while stack:
+        r = stack.pop()
+        order.append(r)
+        for c in (int(nodes[r]["c0"]), int(nodes[r]["c1"])):
+            if not c & 0x80000000:
+                stack.append(c)
+    union = {}
+    for r in reversed(order):  # children before parents
+        nd = nodes[r]
+        u = None
+        for c, stored in ((int(nd["c0"]), nd["box0"]), (int(nd["c1"]), nd["box1"])):
+            if c & 0x80000000:
+                t = c & 0x7FFFFFFF
+                want = np.concatenate([lo32[t], hi32[t]])
+            else:
+                want = union[c]
+            assert np.array_equal(stored, want), f"node {r}: child {c:#x} box {stored} != {want}"
+            u = want if u is None else np.concatenate([np.minimum(u[:3], want[:3]), np.maximum(u[3:], want[3:])])
+        union[r] = u
+    assert np.array_equal(rbox, union[root])
+
+
 def check_bvh(mesh, boxes64):
     """Every LBVH node's child boxes contain the boxes of the triangles below
-    them, and every triangle is a leaf exactly once."""
+    them (exactly their union), and every triangle is a leaf exactly once."""
     nodes, root, rbox = mesh.bvh()
     nt = mesh.nt
     if nt == 0:
@@ -62,6 +106,7 @@ def check_bvh(mesh, boxes64):
             assert np.all(b[:3] >= box[:3]) and np.all(b[3:] <= box[3:]), "child box outside parent box"
             stack.append((int(c), b))
     assert np.all(seen == 1)
+    check_bvh_tight(nodes, root, rbox, boxes64)
 
 
 @pytest.fixture(scope="module", params=FULL_CASES)
@@ -80,6 +125,20 @@ def test_prepare_normals_boxes_bvh(case):
     assert np.array_equal(bits(B.boxes()), bits(g["boxes_b"][:, :6])), name
     check_bvh(A, g["boxes_a"][:, :6])
     check_bvh(B, g["boxes_b"][:, :6])
+
+
+@pytest.mark.parametrize("level", [3, 4, 5, 6])
+def test_bvh_exact_across_refit_tiles(gpu, level):
+    """Icospheres of 1,280 to 81,920 triangles: leaf counts that are not
+    multiples of the refit's 1,024-leaf tiles, so ranges inside a tile, top
+    nodes and a partial last tile all occur; every stored box is checked
+    against the exact union of its leaves."""
+    m = mg.radial_noise(mg.icosphere(level), 1e-3, mg.SplitMix64(level))
+    mesh = gpu.mesh(m.vertices, m.triangles)
+    corners = np.asarray(m.vertices, np.float64)[np.asarray(m.triangles, np.int64)]  # (nt, 3, 3)
+    boxes64 = np.concatenate([corners.min(axis=1), corners.max(axis=1)], axis=1)
+    assert np.array_equal(bits(mesh.boxes()), bits(boxes64))
+    check_bvh(mesh, boxes64)
 
 
 def test_pair_set(case):
