@@ -477,7 +477,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the hot path of one mesh pair "
-              f"is measured on one GPU's share; sharding is described in DESIGN.md)", file=sys.stderr)
+              f"is measured on one GPU's share; sharding across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
 
     from solidboolean_amd import meshgen as mg
