@@ -370,39 +370,43 @@ __device__ __forceinline__ uint2 span_code(const GridSpan &s, uint32_t cu, uint3
 // no slot within kProbe probes falls back to per-entry global atomics; both
 // paths only ever advance a cell's counter, so they mix freely.
 constexpr int kGridBlockTris = 512;  // one per lane
-constexpr int kGridSlotBits = 12;
-constexpr int kGridSlots = 1 << kGridSlotBits;
+// The LDS table has 2^B slots: 2,048 for a build capped beside the
+// single-pair search (less LDS held beside that search's kernels) and 8,192
+// for the batch's uncapped build (fewer cells spilling to global atomics).
+constexpr int kGridSlotBitsBeside = 11, kGridSlotBitsWide = 13;
 constexpr int kProbe = 64;
 
 // slot holding `key` in the LDS table (inserting it), or -1.  Slots only fill,
 // so a key that once found no slot never finds one later in the same pass.  A
 // plain read comes first: most entries find their cell already there, and
 // reads of one slot by many lanes do not serialise the way atomics do.
+template <int B>
 __device__ __forceinline__ int table_slot(uint32_t *keys, uint32_t key)
 {
-    uint32_t h = (key * 2654435761u) >> (32 - kGridSlotBits);
+    uint32_t h = (key * 2654435761u) >> (32 - B);
     for (int probe = 0; probe < kProbe; ++probe) {
         uint32_t cur = __hip_atomic_load(&keys[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (cur == 0u)
             cur = atomicCAS(&keys[h], 0u, key);
         if (cur == 0u || cur == key)
             return (int)h;
-        h = (h + 1) & (kGridSlots - 1);
+        h = (h + 1) & ((1u << B) - 1);
     }
     return -1;
 }
 
 // slot of `key` in a table that no longer changes, or -1 when it is not there
+template <int B>
 __device__ __forceinline__ int table_find(const uint32_t *keys, uint32_t key)
 {
-    uint32_t h = (key * 2654435761u) >> (32 - kGridSlotBits);
+    uint32_t h = (key * 2654435761u) >> (32 - B);
     for (int probe = 0; probe < kProbe; ++probe) {
         const uint32_t cur = keys[h];
         if (cur == key)
             return (int)h;
         if (cur == 0u)
             return -1;
-        h = (h + 1) & (kGridSlots - 1);
+        h = (h + 1) & ((1u << B) - 1);
     }
     return -1;
 }
@@ -462,13 +466,15 @@ __device__ __forceinline__ uint2 grid_tiles(uint32_t nt)
 }
 
 // cells: ncells + 1 u32, zero on entry; entries of cell c are counted in cells[c + 1]
+template <int B>
 __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const float *__restrict__ box32, uint32_t nt,
                                                                GridSet gs, uint32_t *__restrict__ cells)
 {
-    __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
+    constexpr int kSlots = 1 << B;
+    __shared__ uint32_t s_key[kSlots], s_cnt[kSlots];
     const uint2 tiles = grid_tiles(nt);
     for (uint32_t tile = tiles.x; tile < tiles.y; ++tile) {
-        for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris) {
+        for (int k = threadIdx.x; k < kSlots; k += kGridBlockTris) {
             s_key[k] = 0;
             s_cnt[k] = 0;
         }
@@ -480,7 +486,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const float *__re
                 for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
                     for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
                         const uint32_t key = cell_key(x, axis, cu, cv);
-                        const int sl = table_slot(s_key, key);
+                        const int sl = table_slot<B>(s_key, key);
                         if (sl >= 0)
                             atomicAdd(&s_cnt[sl], 1u);
                         else
@@ -488,7 +494,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const float *__re
                     }
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris)
+        for (int k = threadIdx.x; k < kSlots; k += kGridBlockTris)
             if (s_key[k])
                 atomicAdd(&cells[s_key[k]], s_cnt[k]);
         __syncthreads();  // the table is cleared for the next tile
@@ -498,14 +504,16 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const float *__re
 // After the exclusive scan, cells[c + 1] is the start of cell c; per tile the
 // workgroup reserves one range per distinct cell and hands out slots from
 // LDS, so afterwards cell c spans [cells[c], cells[c + 1]).
+template <int B>
 __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const float *__restrict__ box32, uint32_t nt,
                                                               GridSet gs, uint32_t *__restrict__ cells,
                                                               uint2 *__restrict__ gcode, uint32_t *__restrict__ gtri)
 {
-    __shared__ uint32_t s_key[kGridSlots], s_cnt[kGridSlots];
+    constexpr int kSlots = 1 << B;
+    __shared__ uint32_t s_key[kSlots], s_cnt[kSlots];
     const uint2 tiles = grid_tiles(nt);
     for (uint32_t tile = tiles.x; tile < tiles.y; ++tile) {
-        for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris) {
+        for (int k = threadIdx.x; k < kSlots; k += kGridBlockTris) {
             s_key[k] = 0;
             s_cnt[k] = 0;
         }
@@ -516,13 +524,13 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const float *__res
             for (int axis = 0; axis < 3; ++axis)
                 for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
                     for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
-                        const int sl = table_slot(s_key, cell_key(x, axis, cu, cv));
+                        const int sl = table_slot<B>(s_key, cell_key(x, axis, cu, cv));
                         if (sl >= 0)
                             atomicAdd(&s_cnt[sl], 1u);
                     }
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < kGridSlots; k += kGridBlockTris)
+        for (int k = threadIdx.x; k < kSlots; k += kGridBlockTris)
             if (s_key[k])
                 s_cnt[k] = atomicAdd(&cells[s_key[k]], s_cnt[k]);  // the range's first slot
         __syncthreads();
@@ -532,7 +540,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const float *__res
                 for (uint32_t cu = x.s[axis].u0; cu <= x.s[axis].u1; ++cu)
                     for (uint32_t cv = x.s[axis].v0; cv <= x.s[axis].v1; ++cv) {
                         const uint32_t key = cell_key(x, axis, cu, cv);
-                        const int sl = table_find(s_key, key);
+                        const int sl = table_find<B>(s_key, key);
                         const uint32_t slot = sl >= 0 ? atomicAdd(&s_cnt[sl], 1u) : atomicAdd(&cells[key], 1u);
                         if (slot < gs.cap) {
                             gcode[slot] = span_code(x.s[axis], cu, cv);
@@ -1001,9 +1009,18 @@ void launch_grid_build(const float *box32, uint32_t nt, const GridSet &gs, uint3
         return;
     const uint32_t ntiles = (nt + kGridBlockTris - 1) / kGridBlockTris;
     const dim3 grid(xcd_grid(maxBlocks && ntiles > maxBlocks ? maxBlocks : ntiles));
-    hipLaunchKernelGGL(k_grid_count, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells);
+    const bool wide = maxBlocks == 0;  // an uncapped build: the wide table
+    if (wide)
+        hipLaunchKernelGGL(k_grid_count<kGridSlotBitsWide>, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells);
+    else
+        hipLaunchKernelGGL(k_grid_count<kGridSlotBitsBeside>, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells);
     launch_scan_large(cells, m, scanTemp, s);
-    hipLaunchKernelGGL(k_grid_fill, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells, gcode, gtri);
+    if (wide)
+        hipLaunchKernelGGL(k_grid_fill<kGridSlotBitsWide>, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells,
+                           gcode, gtri);
+    else
+        hipLaunchKernelGGL(k_grid_fill<kGridSlotBitsBeside>, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells,
+                           gcode, gtri);
 }
 
 // Rays of n queries.  allAxes: every query casts x, y and z; otherwise x and
