@@ -374,7 +374,7 @@ constexpr int kGridBlockTris = 512;  // one per lane
 // single-pair search (less LDS held beside that search's kernels) and 8,192
 // for the batch's uncapped build (fewer cells spilling to global atomics).
 constexpr int kGridSlotBitsBeside = 11, kGridSlotBitsWide = 13;
-constexpr int kProbe = 64;
+constexpr int kProbe = 32;
 
 // slot holding `key` in the LDS table (inserting it), or -1.  Slots only fill,
 // so a key that once found no slot never finds one later in the same pass.  A
