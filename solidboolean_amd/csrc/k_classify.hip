@@ -360,7 +360,7 @@ __device__ __forceinline__ uint2 span_code(const GridSpan &s, uint32_t cu, uint3
     return pack_code(cell_code(s.lu, cu), cell_code(s.hu, cu), cell_code(s.lv, cv), cell_code(s.hv, cv), s.ha);
 }
 
-// Grid build: a workgroup takes tiles of kGridBlockTris consecutive triangles
+// Grid build: a workgroup takes tiles of T consecutive triangles
 // of the visiting order (Morton order, so they are neighbours), files each under its
 // cells on all three axes (one record load per triangle), and first counts its
 // entries per cell in an LDS hash table; only then does it touch the global
@@ -369,11 +369,12 @@ __device__ __forceinline__ uint2 span_code(const GridSpan &s, uint32_t cu, uint3
 // triangles share most cells, so this removes most of them.  A cell that finds
 // no slot within kProbe probes falls back to per-entry global atomics; both
 // paths only ever advance a cell's counter, so they mix freely.
-constexpr int kGridBlockTris = 512;  // one per lane
-// The LDS table has 2^B slots: 2,048 for a build capped beside the
-// single-pair search (less LDS held beside that search's kernels) and 8,192
-// for the batch's uncapped build (fewer cells spilling to global atomics).
-constexpr int kGridSlotBitsBeside = 11, kGridSlotBitsWide = 13;
+// A build capped beside the single-pair search takes tiles of 512 triangles
+// with an LDS table of 2,048 slots (less LDS held beside that search's
+// kernels); the batch's uncapped build takes tiles of 1,024 with 8,192 slots
+// (fewer cells spilling to global atomics).  One triangle per lane.
+constexpr int kGridSlotBitsBeside = 11, kGridTrisBeside = 512;
+constexpr int kGridSlotBitsWide = 13, kGridTrisWide = 1024;
 constexpr int kProbe = 32;
 
 // slot holding `key` in the LDS table (inserting it), or -1.  Slots only fill,
@@ -451,35 +452,35 @@ __device__ __forceinline__ uint32_t cell_key(const TriSpans &x, int axis, uint32
     return (uint32_t)(x.base + (size_t)axis * x.G * x.G + (size_t)cu * x.G + cv);
 }
 
-// The tiles of kGridBlockTris triangles a workgroup of the grid kernels
-// takes: a contiguous run, and runs in XCD-aware order, so neighbouring tiles
-// (which share cells) stay in one L2.  The launch may cap its workgroups: a
-// build beside the latency-bound pair search leaves that search's small
-// launches free slots on every CU.
-
+// The tiles of T triangles a workgroup of the grid kernels takes: a
+// contiguous run, and runs in XCD-aware order, so neighbouring tiles (which
+// share cells) stay in one L2.  The launch may cap its workgroups: a build
+// beside the latency-bound pair search leaves that search's small launches
+// free slots on every CU.
+template <int T>
 __device__ __forceinline__ uint2 grid_tiles(uint32_t nt)
 {
-    const uint32_t ntiles = (nt + kGridBlockTris - 1) / kGridBlockTris;
+    const uint32_t ntiles = (nt + T - 1) / T;
     const uint32_t per = (ntiles + gridDim.x - 1) / gridDim.x;
     const uint32_t first = xcd_block(blockIdx.x, gridDim.x) * per;
     return make_uint2(first < ntiles ? first : ntiles, first + per < ntiles ? first + per : ntiles);
 }
 
 // cells: ncells + 1 u32, zero on entry; entries of cell c are counted in cells[c + 1]
-template <int B>
-__global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const float *__restrict__ box32, uint32_t nt,
+template <int B, int T>
+__global__ __launch_bounds__(T) void k_grid_count(const float *__restrict__ box32, uint32_t nt,
                                                                GridSet gs, uint32_t *__restrict__ cells)
 {
     constexpr int kSlots = 1 << B;
     __shared__ uint32_t s_key[kSlots], s_cnt[kSlots];
-    const uint2 tiles = grid_tiles(nt);
+    const uint2 tiles = grid_tiles<T>(nt);
     for (uint32_t tile = tiles.x; tile < tiles.y; ++tile) {
-        for (int k = threadIdx.x; k < kSlots; k += kGridBlockTris) {
+        for (int k = threadIdx.x; k < kSlots; k += T) {
             s_key[k] = 0;
             s_cnt[k] = 0;
         }
         __syncthreads();
-        const TriSpans x = tri_spans(box32, nt, gs, tile * kGridBlockTris + threadIdx.x);
+        const TriSpans x = tri_spans(box32, nt, gs, tile * T + threadIdx.x);
         if (x.valid) {
 #pragma unroll
             for (int axis = 0; axis < 3; ++axis)
@@ -494,7 +495,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const float *__re
                     }
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < kSlots; k += kGridBlockTris)
+        for (int k = threadIdx.x; k < kSlots; k += T)
             if (s_key[k])
                 atomicAdd(&cells[s_key[k]], s_cnt[k]);
         __syncthreads();  // the table is cleared for the next tile
@@ -504,21 +505,21 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_count(const float *__re
 // After the exclusive scan, cells[c + 1] is the start of cell c; per tile the
 // workgroup reserves one range per distinct cell and hands out slots from
 // LDS, so afterwards cell c spans [cells[c], cells[c + 1]).
-template <int B>
-__global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const float *__restrict__ box32, uint32_t nt,
+template <int B, int T>
+__global__ __launch_bounds__(T) void k_grid_fill(const float *__restrict__ box32, uint32_t nt,
                                                               GridSet gs, uint32_t *__restrict__ cells,
                                                               uint2 *__restrict__ gcode, uint32_t *__restrict__ gtri)
 {
     constexpr int kSlots = 1 << B;
     __shared__ uint32_t s_key[kSlots], s_cnt[kSlots];
-    const uint2 tiles = grid_tiles(nt);
+    const uint2 tiles = grid_tiles<T>(nt);
     for (uint32_t tile = tiles.x; tile < tiles.y; ++tile) {
-        for (int k = threadIdx.x; k < kSlots; k += kGridBlockTris) {
+        for (int k = threadIdx.x; k < kSlots; k += T) {
             s_key[k] = 0;
             s_cnt[k] = 0;
         }
         __syncthreads();
-        const TriSpans x = tri_spans(box32, nt, gs, tile * kGridBlockTris + threadIdx.x);
+        const TriSpans x = tri_spans(box32, nt, gs, tile * T + threadIdx.x);
         if (x.valid) {
 #pragma unroll
             for (int axis = 0; axis < 3; ++axis)
@@ -530,7 +531,7 @@ __global__ __launch_bounds__(kGridBlockTris) void k_grid_fill(const float *__res
                     }
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < kSlots; k += kGridBlockTris)
+        for (int k = threadIdx.x; k < kSlots; k += T)
             if (s_key[k])
                 s_cnt[k] = atomicAdd(&cells[s_key[k]], s_cnt[k]);  // the range's first slot
         __syncthreads();
@@ -1000,6 +1001,17 @@ uint32_t grid_dim(uint32_t nt, int density)
 size_t grid_cells(uint32_t G) { return 3 * (size_t)G * G + 1; }
 
 
+template <int B, int T>
+static void grid_build(const float *box32, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
+                       uint2 *gcode, uint32_t *gtri, uint32_t maxBlocks, hipStream_t s)
+{
+    const uint32_t ntiles = (nt + T - 1) / T;
+    const dim3 grid(xcd_grid(maxBlocks && ntiles > maxBlocks ? maxBlocks : ntiles));
+    hipLaunchKernelGGL((k_grid_count<B, T>), grid, dim3(T), 0, s, box32, nt, gs, cells);
+    launch_scan_large(cells, (size_t)gs.ncells + 1, scanTemp, s);
+    hipLaunchKernelGGL((k_grid_fill<B, T>), grid, dim3(T), 0, s, box32, nt, gs, cells, gcode, gtri);
+}
+
 void launch_grid_build(const float *box32, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
                        uint2 *gcode, uint32_t *gtri, uint32_t maxBlocks, hipStream_t s)
 {
@@ -1007,20 +1019,10 @@ void launch_grid_build(const float *box32, uint32_t nt, const GridSet &gs, uint3
     launch_fill_u32(cells, 0u, m, s);
     if (nt == 0)
         return;
-    const uint32_t ntiles = (nt + kGridBlockTris - 1) / kGridBlockTris;
-    const dim3 grid(xcd_grid(maxBlocks && ntiles > maxBlocks ? maxBlocks : ntiles));
-    const bool wide = maxBlocks == 0;  // an uncapped build: the wide table
-    if (wide)
-        hipLaunchKernelGGL(k_grid_count<kGridSlotBitsWide>, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells);
+    if (maxBlocks)
+        grid_build<kGridSlotBitsBeside, kGridTrisBeside>(box32, nt, gs, cells, scanTemp, gcode, gtri, maxBlocks, s);
     else
-        hipLaunchKernelGGL(k_grid_count<kGridSlotBitsBeside>, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells);
-    launch_scan_large(cells, m, scanTemp, s);
-    if (wide)
-        hipLaunchKernelGGL(k_grid_fill<kGridSlotBitsWide>, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells,
-                           gcode, gtri);
-    else
-        hipLaunchKernelGGL(k_grid_fill<kGridSlotBitsBeside>, grid, dim3(kGridBlockTris), 0, s, box32, nt, gs, cells,
-                           gcode, gtri);
+        grid_build<kGridSlotBitsWide, kGridTrisWide>(box32, nt, gs, cells, scanTemp, gcode, gtri, 0, s);
 }
 
 // Rays of n queries.  allAxes: every query casts x, y and z; otherwise x and

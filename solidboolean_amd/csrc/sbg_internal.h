@@ -161,7 +161,8 @@ struct GridSet {
 uint32_t grid_dim(uint32_t nt, int density);
 size_t grid_cells(uint32_t G);  // 3 G^2 + 1
 // maxBlocks: workgroups at most, each taking a contiguous run of tiles of 512
-// triangles (0: one per tile)
+// triangles (0: one workgroup per tile of 1,024 triangles, with a larger LDS
+// table)
 void launch_grid_build(const float *box32, uint32_t nt, const GridSet &gs, uint32_t *cells, uint32_t *scanTemp,
                        uint2 *gcode, uint32_t *gtri, uint32_t maxBlocks, hipStream_t s);
 // the single-pair build's cap: it runs beside the pair search
