@@ -260,7 +260,8 @@ typedef struct sbg_tuning {
                           the FP32 record under a certificate, and a ray with an uncertain candidate
                           runs in FP64; SBG_PRECISION_FP64: every candidate in FP64.  Same results. */
     int ray_cell_cap;  /* grid entries one lane scans for its ray (16..4096, default 64); a ray whose
-                          cell holds more goes to the kernel that gives each ray a whole wave */
+                          cell holds more goes to the kernel that gives each ray a whole wave.  The
+                          batch's rays use it too (as it is when a batch run is queued). */
 } sbg_tuning;
 int sbg_context_set_tuning(sbg_context ctx, const sbg_tuning *t);
 int sbg_context_get_tuning(sbg_context ctx, sbg_tuning *t);
